@@ -1,0 +1,125 @@
+"""Workload definitions for the Baraff-Witkin cloth step.
+
+Builds the reference-format cloth JSON (keys of the reference's
+config/his_config.json, read by cBaraffCloth::Init, BaraffCloth.cpp:22-83)
+for the benchmark configurations, and the seeded initial out-of-plane jitter.
+
+Jitter RNG: splitmix64 (Steele, Lea, Flood 2014).  Vertex v of a cloth with
+seed s draws u = (splitmix64(s + (v+1)*0x9E3779B97F4A7C15) >> 11) * 2**-53 and
+gets z += (2u - 1) * amplitude.  The reference's std::default_random_engine
+is platform-dependent (Rand.h:26), so inputs are always injected as an
+explicit initial position vector (SetPos + SetPrePos after Init).
+"""
+import copy
+import math
+
+import numpy as np
+
+GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def splitmix64_stream(seed, n):
+    """n consecutive splitmix64 outputs for a given seed (uint64 array)."""
+    with np.errstate(over="ignore"):
+        idx = np.arange(1, n + 1, dtype=np.uint64)
+        z = np.uint64(seed) + idx * GOLDEN
+        z = (z ^ (z >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform01(seed, n):
+    z = splitmix64_stream(seed, n)
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+BASE_CLOTH = {
+    "object_name": "cloth",
+    "object_type": "Cloth",
+    "cloth_type": "fem",
+    "geometry_type": "regular_triangle",
+    "cloth_size": [0.5, 0.5],
+    "subdivision": [20, 20],
+    "cloth_init_pos": [0, 0, 0],
+    "cloth_init_orientation": [0, 0, 0, 0],
+    "cloth_uv_rotation": 0,
+    "enable_cloth_from_obj": False,
+    "cloth_density": 0.2,
+    "cloth_stretch_stiffness": [1e3, 1e3, 1e2],
+    "cloth_bending_stiffness": [1e-4, 1e-4],
+    # "Rayleigh damping 1e-3" read as stiffness-proportional damping with the
+    # reference's sign convention (W = (1 + h a) M + h (b - h) K; b <= 0 damps,
+    # BaraffCloth.cpp:274-275, ImGui range :338-339)
+    "rayleigh_damping_a": 0,
+    "rayleigh_damping_b": -1e-3,
+    "damping": 0,
+    "default_timestep": 0.01,
+    "max_iters": 500,
+    "enable_gravity": True,
+    "constraints": {"static_point": [[-0.25, 0.25], [0.25, 0.25]]},
+}
+
+
+def grid_config(side, size=0.5, dt=0.01, stiffness_scale=1.0, **over):
+    """Square cloth with `side` vertices per side, top corners pinned."""
+    c = copy.deepcopy(BASE_CLOTH)
+    c["subdivision"] = [side - 1, side - 1]
+    c["cloth_size"] = [size, size]
+    c["default_timestep"] = dt
+    h = size / 2
+    c["constraints"] = {"static_point": [[-h, h], [h, h]]}
+    if stiffness_scale != 1.0:
+        c["cloth_stretch_stiffness"] = [k * stiffness_scale for k in c["cloth_stretch_stiffness"]]
+        c["cloth_bending_stiffness"] = [k * stiffness_scale for k in c["cloth_bending_stiffness"]]
+    c.update(over)
+    return c
+
+
+def grid_rest_positions(conf):
+    """Rest positions of a regular_triangle grid with identity transform
+    (Triangulator.cpp:395-457): vertex i*(hd+1)+j at (i*uw - w/2, j*uh - h/2, 0)."""
+    hd, wd = conf["subdivision"][:2]
+    height, width = conf["cloth_size"][:2]
+    uh, uw = height / hd, width / wd
+    i = np.repeat(np.arange(wd + 1), hd + 1).astype(np.float64)
+    j = np.tile(np.arange(hd + 1), wd + 1).astype(np.float64)
+    x = np.zeros((len(i), 3))
+    x[:, 0] = i * uw - width / 2
+    x[:, 1] = j * uh - height / 2
+    return x.reshape(-1)
+
+
+def jittered_positions(conf, seed, amplitude=1e-3):
+    x = grid_rest_positions(conf).reshape(-1, 3).copy()
+    u = uniform01(seed, x.shape[0])
+    x[:, 2] += (2.0 * u - 1.0) * amplitude
+    return x.reshape(-1)
+
+
+# --- benchmark configurations -------------------------------------------
+
+SINGLE_SMALL = dict(side=21, dt=0.01, steps=100, seed=1)     # 21x21, CPU-runnable case
+SINGLE_LARGE = dict(side=257, dt=0.005, steps=200, seed=2)   # 257x257 single cloth
+BATCH64 = dict(count=64, side_lo=17, side_hi=129, dt=0.01, steps=100, seed=3)
+
+
+def batch64_members(seed=BATCH64["seed"], count=BATCH64["count"],
+                    lo=BATCH64["side_lo"], hi=BATCH64["side_hi"]):
+    """[(side, stiffness_scale, jitter_seed)] of the 64-cloth batch:
+    side uniform in {lo..hi}, stiffness scale log-uniform in [0.1, 10]."""
+    u = uniform01(seed, 2 * count)
+    out = []
+    for k in range(count):
+        side = lo + min(int(u[2 * k] * (hi - lo + 1)), hi - lo)
+        scale = 10.0 ** (2.0 * u[2 * k + 1] - 1.0)
+        out.append((side, scale, seed * 1000 + k + 1))
+    return out
+
+
+def batch64_configs(**kw):
+    members = batch64_members(**kw)
+    return [(grid_config(s, dt=BATCH64["dt"], stiffness_scale=sc), seed)
+            for (s, sc, seed) in members]

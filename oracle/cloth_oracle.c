@@ -1,0 +1,1180 @@
+/*
+ * cloth_oracle.c -- CPU restatement of the reference's plain implicit cloth
+ * substep, cBaraffCloth::UpdatePos (src/sim/cloth/BaraffCloth.cpp:90-163).
+ *
+ * TEST INFRASTRUCTURE ONLY (the parity oracle).  Only tests/, the smoke
+ * check and the benchmark's CPU-baseline leg load this library, as the
+ * checker or the timed CPU baseline; the product never does.
+ *
+ * Arithmetic follows the reference expression by expression, FP64 with the
+ * reference's FP32 roundings:
+ *   - stretch coefficients a,b,c,d cast to float   BaraffMaterial.cpp:18-32
+ *   - triangle areas returned as float             MathUtil2.cpp:344-357
+ *   - UV coordinates stored as float               Primitives.h:11, Triangulator.cpp:621-658
+ *   - stiffness, density, Rayleigh a/b as float    BaraffCloth.h:25-27, BaseCloth.h:53
+ *   - dihedral rest data float                     DihedralBending.h:23-25
+ *   - dihedral 12x12 Hessian stored as float       DihedralBending.cpp:437
+ * and keeps the reference's behaviour where it is a bug:
+ *   - Dv of the stretch Hessian scales with |Fu|   BaraffMaterial.cpp:142
+ *   - dihedral force carries /2, Hessian does not  DihedralBending.cpp:425-426
+ * Reductions in the solver follow Eigen 3.3.9's SSE2 redux order (four
+ * partial sums), see oc_dot().
+ *
+ * The solver has two modes: the reference's own (exactly max_iters-1 PCG
+ * iterations, BaraffCloth.cpp:529) and a tolerance mode (stop when the
+ * recursive residual on free dofs is <= tol * ||b_free||), which is the
+ * stopping rule of the HIP product.
+ */
+#include "cloth_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------ */
+/* small vector helpers, written in the reference's evaluation order   */
+/* ------------------------------------------------------------------ */
+
+static void v3sub(const double *a, const double *b, double *o)
+{
+    o[0] = a[0] - b[0];
+    o[1] = a[1] - b[1];
+    o[2] = a[2] - b[2];
+}
+/* Eigen cross(): (a1 b2 - a2 b1, a2 b0 - a0 b2, a0 b1 - a1 b0) */
+static void v3cross(const double *a, const double *b, double *o)
+{
+    double t0 = a[1] * b[2] - a[2] * b[1];
+    double t1 = a[2] * b[0] - a[0] * b[2];
+    double t2 = a[0] * b[1] - a[1] * b[0];
+    o[0] = t0;
+    o[1] = t1;
+    o[2] = t2;
+}
+/* fixed-size 3-vector redux: ((x0 + x1) + x2) */
+static double v3dot(const double *a, const double *b)
+{
+    return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+}
+static double v3sqnorm(const double *a) { return v3dot(a, a); }
+static double v3norm(const double *a) { return sqrt(v3sqnorm(a)); }
+/* Eigen normalized(): n / sqrt(|n|^2) if |n|^2 > 0 else n (Dot.h:124-134) */
+static void v3normalized(const double *a, double *o)
+{
+    double z = v3sqnorm(a);
+    if (z > 0)
+    {
+        double s = sqrt(z);
+        o[0] = a[0] / s;
+        o[1] = a[1] / s;
+        o[2] = a[2] / s;
+    }
+    else
+    {
+        o[0] = a[0];
+        o[1] = a[1];
+        o[2] = a[2];
+    }
+}
+
+/* Eigen 3.3.9 dot()/sum() of two aligned dynamic double vectors under SSE2:
+   redux_impl<..., LinearVectorizedTraversal, NoUnrolling> (Redux.h) keeps two
+   2-wide packet accumulators, folds them, then adds the scalar tail. */
+double oc_dot(const double *a, const double *b, long n)
+{
+    const long ps = 2;
+    long aligned2 = (n / (2 * ps)) * (2 * ps);
+    long aligned = (n / ps) * ps;
+    if (aligned == 0)
+    {
+        if (n == 0)
+            return 0.0;
+        double r = a[0] * b[0];
+        for (long i = 1; i < n; i++)
+            r = r + a[i] * b[i];
+        return r;
+    }
+    double p00 = a[0] * b[0], p01 = a[1] * b[1];
+    if (aligned > ps)
+    {
+        double p10 = a[2] * b[2], p11 = a[3] * b[3];
+        for (long i = 2 * ps; i < aligned2; i += 2 * ps)
+        {
+            p00 = p00 + a[i] * b[i];
+            p01 = p01 + a[i + 1] * b[i + 1];
+            p10 = p10 + a[i + 2] * b[i + 2];
+            p11 = p11 + a[i + 3] * b[i + 3];
+        }
+        p00 = p00 + p10;
+        p01 = p01 + p11;
+        if (aligned > aligned2)
+        {
+            p00 = p00 + a[aligned2] * b[aligned2];
+            p01 = p01 + a[aligned2 + 1] * b[aligned2 + 1];
+        }
+    }
+    double r = p00 + p01;
+    for (long i = aligned; i < n; i++)
+        r = r + a[i] * b[i];
+    return r;
+}
+
+/* ------------------------------------------------------------------ */
+/* state                                                               */
+/* ------------------------------------------------------------------ */
+
+struct oc_cloth
+{
+    int nv, nt, ne, npin;
+    double *xrest; /* 3nv, after the init transform */
+    double *x, *xpre;
+    float *uv;     /* 2nv */
+    int *tri;      /* 3nt */
+    int *tri_e;    /* 3nt edge ids (mEId0..2) */
+    int *edge;     /* 5ne: id0, id1, tri0, tri1, boundary */
+    int *stencil;  /* 4ne, -1 for boundary edges */
+    int *pins;
+    unsigned char *pinned;
+    /* rest data */
+    double *mass;    /* nv */
+    double *gravity; /* 3nv */
+    double *coef;    /* 12nt: cu_ww, cv_ww, cu_bias, cv_bias */
+    double *area;    /* nt, float values */
+    float *edge_len; /* ne */
+    float *heron;    /* nt */
+    float *height;   /* 2ne */
+    double k_ww[2], k_bias, edge_k;
+    float ray_a, ray_b;
+    double dt;
+    int max_iters;
+    /* block pattern */
+    int nnzb;
+    int *row_ptr, *cols, *diag;
+    /* per-step */
+    double *Ks, *Kb, *K, *W; /* 9 nnzb each */
+    double *fs, *fb, *f, *b, *dx;
+};
+
+static void *xcalloc(size_t n, size_t s)
+{
+    void *p = calloc(n ? n : 1, s);
+    return p;
+}
+
+/* ------------------------------------------------------------------ */
+/* geometry: Triangulator.cpp:141-390 (cells), 395-457 (vertices),     */
+/* 11-31 (triangle edge ids), 116-131 (transform, UV)                  */
+/* ------------------------------------------------------------------ */
+
+typedef struct
+{
+    int a, b, id;
+} epair;
+static int epair_cmp(const void *p, const void *q)
+{
+    const epair *x = p, *y = q;
+    if (x->a != y->a)
+        return x->a < y->a ? -1 : 1;
+    if (x->b != y->b)
+        return x->b < y->b ? -1 : 1;
+    return 0;
+}
+static int find_edge(const epair *e, int ne, int a, int b)
+{
+    epair key;
+    key.a = a < b ? a : b;
+    key.b = a < b ? b : a;
+    const epair *f = bsearch(&key, e, ne, sizeof(epair), epair_cmp);
+    return f ? f->id : -1;
+}
+
+/* rotation used by cRotUtil::TransformMat with XYZ order (RotUtil.cpp:1014-1046,
+   1252-1258): R = Rz * Ry * Rx */
+static void euler_xyz(const double *e, double R[3][3])
+{
+    double cx = cos(e[0]), sx = sin(e[0]);
+    double cy = cos(e[1]), sy = sin(e[1]);
+    double cz = cos(e[2]), sz = sin(e[2]);
+    double Rx[3][3] = {{1, 0, 0}, {0, cx, -sx}, {0, sx, cx}};
+    double Ry[3][3] = {{cy, 0, sy}, {0, 1, 0}, {-sy, 0, cy}};
+    double Rz[3][3] = {{cz, -sz, 0}, {sz, cz, 0}, {0, 0, 1}};
+    double T[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            T[i][j] = (Rz[i][0] * Ry[0][j] + Rz[i][1] * Ry[1][j]) + Rz[i][2] * Ry[2][j];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            R[i][j] = (T[i][0] * Rx[0][j] + T[i][1] * Rx[1][j]) + T[i][2] * Rx[2][j];
+}
+
+static int build_grid(oc_cloth *c, const oc_params *p)
+{
+    double height = p->cloth_size[0], width = p->cloth_size[1];
+    int hd = p->subdivision[0], wd = p->subdivision[1];
+    if (hd < 1 || wd < 1)
+        return -1;
+    double uh = height / hd, uw = width / wd;
+    int hl = hd + 1, wl = wd + 1;
+    c->nv = wl * hl;
+    c->nt = 2 * hd * wd;
+    c->ne = wl * hd + hl * wd + hd * wd;
+    c->xrest = xcalloc(3 * c->nv, sizeof(double));
+    c->uv = xcalloc(2 * c->nv, sizeof(float));
+    c->tri = xcalloc(3 * c->nt, sizeof(int));
+    c->tri_e = xcalloc(3 * c->nt, sizeof(int));
+    c->edge = xcalloc(5 * c->ne, sizeof(int));
+    for (int k = 0; k < 5 * c->ne; k++)
+        c->edge[k] = -2; /* unvisited marker */
+
+    /* vertices: id = i * (hd + 1) + j at (i*uw, j*uh) minus the centre */
+    for (int i = 0; i < wl; i++)
+        for (int j = 0; j < hl; j++)
+        {
+            int v = i * hl + j;
+            c->xrest[3 * v + 0] = i * uw - width / 2;
+            c->xrest[3 * v + 1] = j * uh - height / 2;
+            c->xrest[3 * v + 2] = 0.0 - 0.0;
+        }
+
+    int per_row = 2 * hd + hl;
+    int tcount = 0;
+    for (int row = 0; row < wd; row++)
+        for (int col = 0; col < hd; col++)
+        {
+            int lu = row * hl + col, ru = lu + 1, ld = lu + hl, rd = ld + 1;
+            int even = ((row + col) % 2) == 0;
+            int top = row * per_row + col;
+            int left = per_row * row + hd + col * 2;
+            int skew = left + 1, right = skew + 1, bottom = top + per_row;
+            int *t = c->tri + 3 * tcount;
+            if (even)
+            {
+                t[0] = lu; t[1] = ld; t[2] = rd;
+                t[3] = lu; t[4] = rd; t[5] = ru;
+            }
+            else
+            {
+                t[0] = lu; t[1] = ld; t[2] = ru;
+                t[3] = ld; t[4] = rd; t[5] = ru;
+            }
+            tcount += 2;
+            int *e;
+            /* skew edge */
+            e = c->edge + 5 * skew;
+            e[0] = even ? lu : ru;
+            e[1] = even ? rd : ld;
+            e[2] = tcount - 2;
+            e[3] = tcount - 1;
+            e[4] = 0;
+            /* bottom edge */
+            e = c->edge + 5 * bottom;
+            e[0] = ld;
+            e[1] = rd;
+            e[2] = even ? tcount - 2 : tcount - 1;
+            if (row == wd - 1)
+            {
+                e[3] = -1;
+                e[4] = 1;
+            }
+            else
+            {
+                e[3] = e[2] + hd * 2;
+                e[4] = 0;
+            }
+            /* right edge */
+            e = c->edge + 5 * right;
+            e[0] = ru;
+            e[1] = rd;
+            e[2] = tcount - 1;
+            e[4] = (col == hd - 1);
+            e[3] = (col == hd - 1) ? -1 : e[2] + 1;
+            if (row == 0)
+            { /* top edge */
+                e = c->edge + 5 * top;
+                e[0] = lu;
+                e[1] = ru;
+                e[2] = even ? tcount - 1 : tcount - 2;
+                e[3] = -1;
+                e[4] = 1;
+            }
+            if (col == 0)
+            { /* left edge */
+                e = c->edge + 5 * left;
+                e[0] = lu;
+                e[1] = ld;
+                e[2] = tcount - 2;
+                e[3] = -1;
+                e[4] = 1;
+            }
+        }
+    for (int k = 0; k < c->ne; k++)
+        if (c->edge[5 * k] == -2)
+            return -2;
+
+    /* BuildTriangleEdgeId: (id0,id1), (id1,id2), (id2,id0) */
+    epair *ep = xcalloc(c->ne, sizeof(epair));
+    for (int k = 0; k < c->ne; k++)
+    {
+        int a = c->edge[5 * k], b = c->edge[5 * k + 1];
+        ep[k].a = a < b ? a : b;
+        ep[k].b = a < b ? b : a;
+        ep[k].id = k;
+    }
+    qsort(ep, c->ne, sizeof(epair), epair_cmp);
+    for (int t = 0; t < c->nt; t++)
+    {
+        int *v = c->tri + 3 * t;
+        c->tri_e[3 * t + 0] = find_edge(ep, c->ne, v[0], v[1]);
+        c->tri_e[3 * t + 1] = find_edge(ep, c->ne, v[1], v[2]);
+        c->tri_e[3 * t + 2] = find_edge(ep, c->ne, v[2], v[0]);
+    }
+    free(ep);
+
+    /* init transform: mPos = TransformMat(pos, euler) * mPos */
+    int identity = p->init_pos[0] == 0 && p->init_pos[1] == 0 && p->init_pos[2] == 0 &&
+                   p->init_euler_xyz[0] == 0 && p->init_euler_xyz[1] == 0 &&
+                   p->init_euler_xyz[2] == 0;
+    double R[3][3];
+    euler_xyz(p->init_euler_xyz, R);
+    double flatxy[2];
+    for (int v = 0; v < c->nv; v++)
+    {
+        double *q = c->xrest + 3 * v;
+        flatxy[0] = q[0];
+        flatxy[1] = q[1];
+        if (!identity)
+        {
+            double o[3];
+            for (int a = 0; a < 3; a++)
+                o[a] = ((R[a][0] * q[0] + R[a][1] * q[1]) + R[a][2] * q[2]) + p->init_pos[a];
+            q[0] = o[0];
+            q[1] = o[1];
+            q[2] = o[2];
+        }
+        /* UV = rot(uv_rotation) * float(inverse-transformed xy); the inverse
+           transform returns the untransformed grid coordinates. */
+        float ang = (float)(p->uv_rotation_deg / 180.0 * M_PI);
+        float cr = (float)cos((double)ang), sr = (float)sin((double)ang);
+        float px = (float)flatxy[0], py = (float)flatxy[1];
+        float m01 = -sr;
+        c->uv[2 * v + 0] = cr * px + m01 * py;
+        c->uv[2 * v + 1] = sr * px + cr * py;
+    }
+    return 0;
+}
+
+/* SelectAnotherVertex (BaseBendingMaterial.cpp:65-74): the vertex of tri
+   other than a and b */
+static int other_vertex(const int *t, int a, int b)
+{
+    for (int k = 0; k < 3; k++)
+        if (t[k] != a && t[k] != b)
+            return t[k];
+    return -1;
+}
+
+/* ------------------------------------------------------------------ */
+/* block pattern: per row, sorted union of {v}, triangle pairs and     */
+/* bending stencils                                                    */
+/* ------------------------------------------------------------------ */
+
+static int int_cmp(const void *a, const void *b)
+{
+    int x = *(const int *)a, y = *(const int *)b;
+    return (x > y) - (x < y);
+}
+
+/* rows[v] gets candidates; returns sorted unique CSR */
+static void build_csr(int nv, int ncand, const int *cand_row, const int *cand_col,
+                      int **row_ptr_out, int **cols_out, int *nnz_out)
+{
+    int *cnt = xcalloc(nv + 1, sizeof(int));
+    for (int k = 0; k < ncand; k++)
+        cnt[cand_row[k] + 1]++;
+    for (int v = 0; v < nv; v++)
+        cnt[v + 1] += cnt[v];
+    int *tmp = xcalloc(ncand, sizeof(int));
+    int *fill = xcalloc(nv, sizeof(int));
+    for (int k = 0; k < ncand; k++)
+    {
+        int r = cand_row[k];
+        tmp[cnt[r] + fill[r]++] = cand_col[k];
+    }
+    int *row_ptr = xcalloc(nv + 1, sizeof(int));
+    int *cols = xcalloc(ncand, sizeof(int));
+    int nnz = 0;
+    for (int v = 0; v < nv; v++)
+    {
+        int *s = tmp + cnt[v];
+        int n = cnt[v + 1] - cnt[v];
+        qsort(s, n, sizeof(int), int_cmp);
+        row_ptr[v] = nnz;
+        for (int k = 0; k < n; k++)
+            if (k == 0 || s[k] != s[k - 1])
+                cols[nnz++] = s[k];
+    }
+    row_ptr[nv] = nnz;
+    free(cnt);
+    free(tmp);
+    free(fill);
+    *row_ptr_out = row_ptr;
+    *cols_out = cols;
+    *nnz_out = nnz;
+}
+
+static int slot_of(const oc_cloth *c, int r, int col)
+{
+    int lo = c->row_ptr[r], hi = c->row_ptr[r + 1] - 1;
+    while (lo <= hi)
+    {
+        int mid = (lo + hi) / 2;
+        if (c->cols[mid] == col)
+            return mid;
+        if (c->cols[mid] < col)
+            lo = mid + 1;
+        else
+            hi = mid - 1;
+    }
+    return -1;
+}
+
+static void build_pattern(oc_cloth *c)
+{
+    int ncand = c->nv + 9 * c->nt + 16 * c->ne;
+    int *cr = xcalloc(ncand, sizeof(int)), *cc = xcalloc(ncand, sizeof(int));
+    int n = 0;
+    for (int v = 0; v < c->nv; v++)
+    {
+        cr[n] = v;
+        cc[n++] = v;
+    }
+    for (int t = 0; t < c->nt; t++)
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+            {
+                cr[n] = c->tri[3 * t + i];
+                cc[n++] = c->tri[3 * t + j];
+            }
+    for (int e = 0; e < c->ne; e++)
+    {
+        if (c->edge[5 * e + 4])
+            continue;
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++)
+            {
+                cr[n] = c->stencil[4 * e + i];
+                cc[n++] = c->stencil[4 * e + j];
+            }
+    }
+    build_csr(c->nv, n, cr, cc, &c->row_ptr, &c->cols, &c->nnzb);
+    free(cr);
+    free(cc);
+    c->diag = xcalloc(c->nv, sizeof(int));
+    for (int v = 0; v < c->nv; v++)
+        c->diag[v] = slot_of(c, v, v);
+}
+
+int oc_get_ref_ell(const oc_cloth *c, int *row_ptr, int *cols)
+{
+    /* BaraffClothGpu.cpp:438-500: std::set per vertex of every interior
+       stencil containing it, plus the vertex itself (triangles add only
+       the diagonal, :460-469) */
+    int ncand = c->nv + 16 * c->ne;
+    int *cr = xcalloc(ncand, sizeof(int)), *cc = xcalloc(ncand, sizeof(int));
+    int n = 0;
+    for (int e = 0; e < c->ne; e++)
+    {
+        if (c->edge[5 * e + 4])
+            continue;
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++)
+            {
+                cr[n] = c->stencil[4 * e + i];
+                cc[n++] = c->stencil[4 * e + j];
+            }
+    }
+    for (int v = 0; v < c->nv; v++)
+    {
+        cr[n] = v;
+        cc[n++] = v;
+    }
+    int *rp, *cl, nnz;
+    build_csr(c->nv, n, cr, cc, &rp, &cl, &nnz);
+    if (row_ptr)
+        memcpy(row_ptr, rp, (c->nv + 1) * sizeof(int));
+    if (cols)
+        memcpy(cols, cl, nnz * sizeof(int));
+    free(rp);
+    free(cl);
+    free(cr);
+    free(cc);
+    return nnz;
+}
+
+/* ------------------------------------------------------------------ */
+/* rest precompute                                                     */
+/* ------------------------------------------------------------------ */
+
+/* cMathUtil::CalcTriangleArea3d: float(0.5 * |(p1-p0) x (p2-p0)|) */
+static float tri_area_f(const double *p0, const double *p1, const double *p2)
+{
+    double a[3], b[3], cr[3];
+    v3sub(p1, p0, a);
+    v3sub(p2, p0, b);
+    v3cross(a, b, cr);
+    return (float)(0.5 * v3norm(cr));
+}
+
+/* CalcBaraffStretchCoef (BaraffMaterial.cpp:18-32) */
+static void stretch_coef(const double m[2][2], double *cu, double *cv)
+{
+    float a = (float)m[0][0], b = (float)m[0][1], c = (float)m[1][0], d = (float)m[1][1];
+    cu[0] = (double)(-a - c);
+    cu[1] = (double)a;
+    cu[2] = (double)c;
+    cv[0] = (double)(-b - d);
+    cv[1] = (double)b;
+    cv[2] = (double)d;
+}
+
+static void rest_precompute(oc_cloth *c, const oc_params *p)
+{
+    int nv = c->nv, nt = c->nt, ne = c->ne;
+    float rho = (float)p->density;
+    /* cBaraffCloth::InitMass (BaraffCloth.cpp:209-236) */
+    double *mdiag = xcalloc(3 * nv, sizeof(double));
+    c->area = xcalloc(nt, sizeof(double));
+    for (int t = 0; t < nt; t++)
+    {
+        const int *v = c->tri + 3 * t;
+        double A = (double)tri_area_f(c->xrest + 3 * v[0], c->xrest + 3 * v[1],
+                                      c->xrest + 3 * v[2]);
+        c->area[t] = A;
+        double add = A / 3 * (double)rho;
+        for (int k = 0; k < 3; k++)
+            for (int a = 0; a < 3; a++)
+                mdiag[3 * v[k] + a] = mdiag[3 * v[k] + a] + add;
+    }
+    c->mass = xcalloc(nv, sizeof(double));
+    c->gravity = xcalloc(3 * nv, sizeof(double));
+    for (int v = 0; v < nv; v++)
+    {
+        c->mass[v] = mdiag[3 * v];
+        /* BaraffCloth.cpp:73-77 */
+        c->gravity[3 * v + 1] = -c->mass[v] * 9.8 * (p->enable_gravity ? 1 : 0);
+    }
+    free(mdiag);
+
+    /* cBaraffMaterial::Init (BaraffMaterial.cpp:33-76) */
+    c->coef = xcalloc(12 * nt, sizeof(double));
+    double cs = cos(M_PI / 4), sn = sin(M_PI / 4);
+    /* R45 = RotMat2D(pi/4) = [[cs, -sn], [sn, cs]]; R45^T = [[cs, sn], [-sn, cs]] */
+    double RT[2][2] = {{cs, sn}, {-sn, cs}};
+    for (int t = 0; t < nt; t++)
+    {
+        const int *v = c->tri + 3 * t;
+        const float *u0 = c->uv + 2 * v[0], *u1 = c->uv + 2 * v[1], *u2 = c->uv + 2 * v[2];
+        double m[2][2];
+        m[0][0] = (double)(u1[0] - u0[0]);
+        m[1][0] = (double)(u1[1] - u0[1]);
+        m[0][1] = (double)(u2[0] - u0[0]);
+        m[1][1] = (double)(u2[1] - u0[1]);
+        double det = m[0][0] * m[1][1] - m[1][0] * m[0][1];
+        double invdet = 1.0 / det;
+        double inv[2][2];
+        inv[0][0] = m[1][1] * invdet;
+        inv[1][0] = -m[1][0] * invdet;
+        inv[0][1] = -m[0][1] * invdet;
+        inv[1][1] = m[0][0] * invdet;
+        double *cf = c->coef + 12 * t;
+        stretch_coef(inv, cf + 0, cf + 3);
+        double r[2][2];
+        for (int i = 0; i < 2; i++)
+            for (int j = 0; j < 2; j++)
+                r[i][j] = inv[i][0] * RT[0][j] + inv[i][1] * RT[1][j];
+        stretch_coef(r, cf + 6, cf + 9);
+    }
+    c->k_ww[0] = (double)(float)p->stretch_k[0];
+    c->k_ww[1] = (double)(float)p->stretch_k[1];
+    c->k_bias = (double)(float)p->stretch_k[2];
+
+    /* cDihedralMaterial::Init (DihedralBending.cpp:23-89) */
+    c->edge_len = xcalloc(ne, sizeof(float));
+    c->heron = xcalloc(nt, sizeof(float));
+    c->height = xcalloc(2 * ne, sizeof(float));
+    for (int e = 0; e < ne; e++)
+    {
+        const double *a = c->xrest + 3 * c->edge[5 * e], *b = c->xrest + 3 * c->edge[5 * e + 1];
+        /* 4-vector norm with w = 1 - 1; SSE2 order (x^2 + z^2) + (y^2 + w^2) */
+        double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2], dw = 0.0;
+        c->edge_len[e] = (float)sqrt((dx * dx + dz * dz) + (dy * dy + dw * dw));
+    }
+    for (int t = 0; t < nt; t++)
+    {
+        double e0 = c->edge_len[c->tri_e[3 * t]], e1 = c->edge_len[c->tri_e[3 * t + 1]],
+               e2 = c->edge_len[c->tri_e[3 * t + 2]];
+        double s = (e0 + e1 + e2) / 2;
+        c->heron[t] = (float)sqrt(s * (s - e0) * (s - e1) * (s - e2));
+    }
+    for (int e = 0; e < ne; e++)
+    {
+        float h0 = 0, h1 = 0;
+        if (!c->edge[5 * e + 4])
+        {
+            /* height = 2 * tVector2f(a0, a1) / e_length: float arithmetic,
+               the divisor converted to float (Eigen scalar promotion) */
+            float a0 = c->heron[c->edge[5 * e + 2]], a1 = c->heron[c->edge[5 * e + 3]];
+            float el = (float)(double)c->edge_len[e];
+            h0 = (2.0f * a0) / el;
+            h1 = (2.0f * a1) / el;
+        }
+        c->height[2 * e] = h0;
+        c->height[2 * e + 1] = h1;
+    }
+    /* mEdgeK = mean of the 3-vector filled with float(k) (BaseBendingMaterial.cpp:84) */
+    double kb = (double)(float)p->bending_k;
+    c->edge_k = ((kb + kb) + kb) / 3;
+
+    c->ray_a = (float)p->rayleigh_a;
+    c->ray_b = (float)p->rayleigh_b;
+    c->dt = p->dt;
+    c->max_iters = p->max_iters;
+}
+
+/* FindVerticesIdFromUV (BaseCloth.cpp:151-189) */
+static void find_pins(oc_cloth *c, const oc_params *p)
+{
+    c->npin = p->num_pins;
+    c->pins = xcalloc(c->npin, sizeof(int));
+    c->pinned = xcalloc(c->nv, 1);
+    for (int j = 0; j < c->npin; j++)
+    {
+        float tu = (float)p->pin_uv[2 * j], tv = (float)p->pin_uv[2 * j + 1];
+        double best = NAN;
+        int id = -1;
+        for (int v = 0; v < c->nv; v++)
+        {
+            float du = c->uv[2 * v] - tu, dv = c->uv[2 * v + 1] - tv;
+            double dist = (double)sqrtf(du * du + dv * dv);
+            if (isnan(best) || dist < best)
+            {
+                id = v;
+                best = dist;
+            }
+        }
+        c->pins[j] = id;
+        if (id >= 0)
+            c->pinned[id] = 1;
+    }
+}
+
+oc_cloth *oc_create_grid(const oc_params *p)
+{
+    oc_cloth *c = xcalloc(1, sizeof(oc_cloth));
+    if (build_grid(c, p) != 0)
+    {
+        oc_destroy(c);
+        return NULL;
+    }
+    /* stencils (BaseCloth.cpp:383-404) */
+    c->stencil = xcalloc(4 * c->ne, sizeof(int));
+    for (int e = 0; e < c->ne; e++)
+    {
+        int *s = c->stencil + 4 * e;
+        if (c->edge[5 * e + 4])
+        {
+            s[0] = s[1] = s[2] = s[3] = -1;
+            continue;
+        }
+        int v0 = c->edge[5 * e], v1 = c->edge[5 * e + 1];
+        s[0] = v0;
+        s[1] = v1;
+        s[2] = other_vertex(c->tri + 3 * c->edge[5 * e + 2], v0, v1);
+        s[3] = other_vertex(c->tri + 3 * c->edge[5 * e + 3], v0, v1);
+    }
+    rest_precompute(c, p);
+    find_pins(c, p);
+    build_pattern(c);
+    size_t nb = 9 * (size_t)c->nnzb;
+    c->Ks = xcalloc(nb, sizeof(double));
+    c->Kb = xcalloc(nb, sizeof(double));
+    c->K = xcalloc(nb, sizeof(double));
+    c->W = xcalloc(nb, sizeof(double));
+    c->fs = xcalloc(3 * c->nv, sizeof(double));
+    c->fb = xcalloc(3 * c->nv, sizeof(double));
+    c->f = xcalloc(3 * c->nv, sizeof(double));
+    c->b = xcalloc(3 * c->nv, sizeof(double));
+    c->dx = xcalloc(3 * c->nv, sizeof(double));
+    c->x = xcalloc(3 * c->nv, sizeof(double));
+    c->xpre = xcalloc(3 * c->nv, sizeof(double));
+    memcpy(c->x, c->xrest, 3 * c->nv * sizeof(double));
+    memcpy(c->xpre, c->xrest, 3 * c->nv * sizeof(double));
+    return c;
+}
+
+void oc_destroy(oc_cloth *c)
+{
+    if (!c)
+        return;
+    free(c->xrest); free(c->x); free(c->xpre); free(c->uv); free(c->tri);
+    free(c->tri_e); free(c->edge); free(c->stencil); free(c->pins); free(c->pinned);
+    free(c->mass); free(c->gravity); free(c->coef); free(c->area);
+    free(c->edge_len); free(c->heron); free(c->height);
+    free(c->row_ptr); free(c->cols); free(c->diag);
+    free(c->Ks); free(c->Kb); free(c->K); free(c->W);
+    free(c->fs); free(c->fb); free(c->f); free(c->b); free(c->dx);
+    free(c);
+}
+
+/* ------------------------------------------------------------------ */
+/* elements                                                            */
+/* ------------------------------------------------------------------ */
+
+/* one direction set of cBaraffMaterial::Update (BaraffMaterial.cpp:205-247):
+   CalcFuAndFv :89-95, CalcFint :113-125, CalcHessian :135-172 */
+static void stretch_pair(const double *x0, const double *x1, const double *x2,
+                         const double *cu, const double *cv, double A, double Ku,
+                         double Kv, double f[3][3], double H[9][3][3])
+{
+    double Fu[3], Fv[3];
+    for (int a = 0; a < 3; a++)
+    {
+        Fu[a] = (cu[0] * x0[a] + cu[1] * x1[a]) + cu[2] * x2[a];
+        Fv[a] = (cv[0] * x0[a] + cv[1] * x1[a]) + cv[2] * x2[a];
+    }
+    double Cu = v3norm(Fu) - 1;
+    double Cv = v3norm(Fv) - 1;
+    double nu[3], nv[3];
+    v3normalized(Fu, nu);
+    v3normalized(Fv, nv);
+    /* CalcFint */
+    double su = ((-A) * Ku) * Cu, sv = ((-A) * Kv) * Cv;
+    double ub[3] = {su * nu[0], su * nu[1], su * nu[2]};
+    double vb[3] = {sv * nv[0], sv * nv[1], sv * nv[2]};
+    for (int i = 0; i < 3; i++)
+        for (int a = 0; a < 3; a++)
+            f[i][a] = ub[a] * cu[i] + vb[a] * cv[i];
+    /* CalcHessian; Fv_norm = Fu.norm() is the reference's (bug :142) */
+    double Fu_norm = v3norm(Fu);
+    double Fv_norm = v3norm(Fu);
+    double Du[3][3], Dv[3][3];
+    double iu = 1.0 / Fu_norm, iv = 1.0 / Fv_norm;
+    double lu[3] = {iu * nu[0], iu * nu[1], iu * nu[2]};
+    double lv[3] = {iv * nv[0], iv * nv[1], iv * nv[2]};
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++)
+        {
+            Du[a][b] = nu[b] * lu[a];
+            Dv[a][b] = nv[b] * lv[a];
+        }
+    if (Fu_norm > 1)
+    {
+        double add = 1 - 1.0 / Fu_norm;
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++)
+                Du[a][b] = Du[a][b] + (a == b ? add : add * 0.0);
+    }
+    if (Fv_norm > 1)
+    {
+        double add = 1 - 1.0 / Fv_norm;
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++)
+                Dv[a][b] = Dv[a][b] + (a == b ? add : add * 0.0);
+    }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+        {
+            double su2 = (Ku * cu[i]) * cu[j];
+            double sv2 = (Kv * cv[i]) * cv[j];
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++)
+                    H[3 * i + j][a][b] = (-A) * (su2 * Du[a][b] + sv2 * Dv[a][b]);
+        }
+}
+
+void oc_stretch_element(const double *x0, const double *x1, const double *x2,
+                        const double *coef12, double area, const double *k3,
+                        double *f9, double *H81)
+{
+    double f1[3][3], f2[3][3], H1[9][3][3], H2[9][3][3];
+    stretch_pair(x0, x1, x2, coef12 + 0, coef12 + 3, area, k3[0], k3[1], f1, H1);
+    stretch_pair(x0, x1, x2, coef12 + 6, coef12 + 9, area, k3[2], k3[2], f2, H2);
+    for (int i = 0; i < 3; i++)
+        for (int a = 0; a < 3; a++)
+            f9[3 * i + a] = (0.0 + f1[i][a]) + f2[i][a];
+    for (int k = 0; k < 9; k++)
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++)
+                H81[9 * k + 3 * a + b] = (0.0 + H1[k][a][b]) + H2[k][a][b];
+}
+
+/* CalcIprime (DihedralBending.cpp:115-119): (I - n n^T) / |v| */
+static void iprime(const double *vec, double M[3][3])
+{
+    double n[3];
+    v3normalized(vec, n);
+    double len = v3norm(vec);
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++)
+            M[a][b] = ((a == b ? 1.0 : 0.0) - n[b] * n[a]) / len;
+}
+
+/* CalcBeta + CalcDBetaDx (DihedralBending.cpp:120-271) */
+void oc_dbeta_dx(const double *v0, const double *v1, const double *v2,
+                 const double *v3, double *beta_out, double *g)
+{
+    double t0[3], t1[3], n0[3], n1[3], e[3];
+    v3sub(v1, v0, t0);
+    v3sub(v2, v1, t1);
+    v3cross(t0, t1, n0);
+    v3sub(v0, v1, t0);
+    v3sub(v3, v0, t1);
+    v3cross(t0, t1, n1);
+    v3sub(v1, v0, e);
+    double n0b[3], n1b[3], eb[3], c01[3];
+    v3normalized(n0, n0b);
+    v3normalized(n1, n1b);
+    v3normalized(e, eb);
+    double x = v3dot(n0b, n1b);
+    v3cross(n0b, n1b, c01);
+    double y = v3dot(eb, c01);
+    double beta = atan2(y, x);
+    /* CalcDBetaDnormal_and_De :163-223 */
+    double sign_sinb = sin(beta) > 0 ? 1.0 : -1.0;
+    double I0[3][3], I1[3][3];
+    iprime(n0, I0);
+    iprime(n1, I1);
+    double m = 1 - x * x;
+    double deno = sqrt(m > 0 ? m : 0);
+    if (deno < 1e-10)
+        deno = 1e-10;
+    double factor = -sign_sinb * 1.0 / deno;
+    double dn0[3], dn1[3];
+    for (int a = 0; a < 3; a++)
+    {
+        /* (factor * I^T) * n_bar, coefficient-wise lazy product */
+        dn0[a] = ((factor * I0[0][a]) * n1b[0] + (factor * I0[1][a]) * n1b[1]) +
+                 (factor * I0[2][a]) * n1b[2];
+        dn1[a] = ((factor * I1[0][a]) * n0b[0] + (factor * I1[1][a]) * n0b[1]) +
+                 (factor * I1[2][a]) * n0b[2];
+    }
+    double a0[3], a1[3], c0[3], c1[3];
+    /* dbeta/dv0 = (v1 - v2) x dn0 + (v3 - v1) x dn1 */
+    v3sub(v1, v2, a0);
+    v3sub(v3, v1, a1);
+    v3cross(a0, dn0, c0);
+    v3cross(a1, dn1, c1);
+    for (int k = 0; k < 3; k++)
+        g[k] = c0[k] + c1[k];
+    /* dbeta/dv1 = (v2 - v0) x dn0 + (v0 - v3) x dn1 */
+    v3sub(v2, v0, a0);
+    v3sub(v0, v3, a1);
+    v3cross(a0, dn0, c0);
+    v3cross(a1, dn1, c1);
+    for (int k = 0; k < 3; k++)
+        g[3 + k] = c0[k] + c1[k];
+    /* dbeta/dv2 = (v0 - v1) x dn0 ; dbeta/dv3 = (v1 - v0) x dn1 */
+    v3sub(v0, v1, a0);
+    v3cross(a0, dn0, g + 6);
+    v3sub(v1, v0, a1);
+    v3cross(a1, dn1, g + 9);
+    *beta_out = beta;
+}
+
+/* ------------------------------------------------------------------ */
+/* assembly                                                            */
+/* ------------------------------------------------------------------ */
+
+void oc_assemble(oc_cloth *c)
+{
+    int nv = c->nv;
+    size_t nb = 9 * (size_t)c->nnzb;
+    memset(c->Ks, 0, nb * sizeof(double));
+    memset(c->Kb, 0, nb * sizeof(double));
+    memset(c->fs, 0, 3 * nv * sizeof(double));
+    memset(c->fb, 0, 3 * nv * sizeof(double));
+    double k3[3] = {c->k_ww[0], c->k_ww[1], c->k_bias};
+    /* stretch: triplets in triangle order, summed by setFromTriplets
+       (BaraffMaterial.cpp:297-351); force CalcTotalForce :267-282 */
+    for (int t = 0; t < c->nt; t++)
+    {
+        const int *v = c->tri + 3 * t;
+        double f9[9], H[81];
+        oc_stretch_element(c->x + 3 * v[0], c->x + 3 * v[1], c->x + 3 * v[2],
+                           c->coef + 12 * t, c->area[t], k3, f9, H);
+        for (int i = 0; i < 3; i++)
+        {
+            for (int a = 0; a < 3; a++)
+                c->fs[3 * v[i] + a] = c->fs[3 * v[i] + a] + f9[3 * i + a];
+            for (int j = 0; j < 3; j++)
+            {
+                int s = slot_of(c, v[i], v[j]);
+                double *blk = c->Ks + 9 * (size_t)s;
+                for (int k = 0; k < 9; k++)
+                    blk[k] = blk[k] + H[9 * (3 * i + j) + k];
+            }
+        }
+    }
+    /* dihedral bending (DihedralBending.cpp:378-489) */
+    for (int e = 0; e < c->ne; e++)
+    {
+        if (c->edge[5 * e + 4])
+            continue;
+        const int *s4 = c->stencil + 4 * e;
+        double beta, g[12];
+        oc_dbeta_dx(c->x + 3 * s4[0], c->x + 3 * s4[1], c->x + 3 * s4[2], c->x + 3 * s4[3],
+                    &beta, g);
+        double el = (double)c->edge_len[e];
+        float hs = c->height[2 * e] + c->height[2 * e + 1];
+        if (hs < 1e-7f)
+            hs = 1e-7f;
+        double factor = -c->edge_k * el / (double)hs;
+        double fb = factor * beta;
+        for (int i = 0; i < 4; i++)
+        {
+            int vi = s4[i];
+            for (int a = 0; a < 3; a++)
+                c->fb[3 * vi + a] = c->fb[3 * vi + a] + (fb * g[3 * i + a]) / 2;
+            for (int j = 0; j < 4; j++)
+            {
+                int s = slot_of(c, vi, s4[j]);
+                double *blk = c->Kb + 9 * (size_t)s;
+                for (int a = 0; a < 3; a++)
+                {
+                    double l = factor * g[3 * i + a];
+                    for (int b = 0; b < 3; b++)
+                    {
+                        /* mEleKLst = float(-H); K_b = -1 * sum(double(float(-H))) */
+                        float hf = (float)(-(l * g[3 * j + b]));
+                        blk[3 * a + b] = blk[3 * a + b] + (double)hf;
+                    }
+                }
+            }
+        }
+    }
+    for (size_t k = 0; k < nb; k++)
+    {
+        c->Kb[k] = -1 * c->Kb[k];
+        c->K[k] = c->Ks[k] + c->Kb[k];
+    }
+    /* f = f_stretch + f_bend (BaraffCloth.cpp:202-207) */
+    for (int k = 0; k < 3 * nv; k++)
+        c->f[k] = c->fs[k] + c->fb[k];
+    /* W = (1 + dt a) M + dt (b - dt) K ; rhs = dt^2 (g + u + f) + dt M (x - xpre) / dt
+       (BaraffCloth.cpp:274-283) */
+    double dt = c->dt;
+    double sk = dt * ((double)c->ray_b - dt);
+    double sm = 1 + dt * (double)c->ray_a;
+    for (int v = 0; v < nv; v++)
+        for (int s = c->row_ptr[v]; s < c->row_ptr[v + 1]; s++)
+        {
+            double *w = c->W + 9 * (size_t)s;
+            const double *k = c->K + 9 * (size_t)s;
+            for (int q = 0; q < 9; q++)
+                w[q] = sk * k[q];
+            if (c->cols[s] == v)
+            {
+                double mm = sm * c->mass[v];
+                w[0] = mm + w[0];
+                w[4] = mm + w[4];
+                w[8] = mm + w[8];
+            }
+        }
+    double dt2 = dt * dt;
+    for (int v = 0; v < nv; v++)
+        for (int a = 0; a < 3; a++)
+        {
+            int q = 3 * v + a;
+            double t1 = dt2 * ((c->gravity[q] + 0.0) + c->f[q]);
+            double t2 = ((dt * c->mass[v]) * (c->x[q] - c->xpre[q])) / dt;
+            c->b[q] = t1 + t2;
+        }
+}
+
+/* ApplyMatmul (BaraffCloth.cpp:469-506): rows and columns of pinned vertices
+   skipped, row sums in ascending column order */
+static void apply_matmul(const oc_cloth *c, const double *x, double *res)
+{
+    for (int v = 0; v < c->nv; v++)
+        for (int a = 0; a < 3; a++)
+        {
+            double acc = 0.0;
+            if (!c->pinned[v])
+                for (int s = c->row_ptr[v]; s < c->row_ptr[v + 1]; s++)
+                {
+                    int col = c->cols[s];
+                    if (c->pinned[col])
+                        continue;
+                    const double *w = c->W + 9 * (size_t)s + 3 * a;
+                    acc = acc + x[3 * col + 0] * w[0];
+                    acc = acc + x[3 * col + 1] * w[1];
+                    acc = acc + x[3 * col + 2] * w[2];
+                }
+            res[3 * v + a] = acc;
+        }
+}
+
+/* cBaraffCloth::Solve (BaraffCloth.cpp:507-611) and the tolerance variant */
+static int solve(oc_cloth *c, int mode, double tol, int max_iters, oc_stats *st)
+{
+    int n = 3 * c->nv;
+    double *x = c->dx;
+    double *r = xcalloc(n, sizeof(double)), *Ad = xcalloc(n, sizeof(double));
+    double *d = xcalloc(n, sizeof(double)), *P = xcalloc(n, sizeof(double));
+    double *Pr = xcalloc(n, sizeof(double)), *rn = xcalloc(n, sizeof(double));
+    double *Prn = xcalloc(n, sizeof(double)), *bf = xcalloc(n, sizeof(double));
+    /* warm start x = xcur - xpre (:284) */
+    for (int k = 0; k < n; k++)
+        x[k] = c->x[k] - c->xpre[k];
+    apply_matmul(c, x, Ad);
+    for (int k = 0; k < n; k++)
+        r[k] = c->b[k] - Ad[k];
+    for (int j = 0; j < c->npin; j++)
+        for (int a = 0; a < 3; a++)
+        {
+            r[3 * c->pins[j] + a] = 0.0;
+            x[3 * c->pins[j] + a] = 0.0;
+        }
+    for (int v = 0; v < c->nv; v++)
+        for (int a = 0; a < 3; a++)
+        {
+            const double *w = c->W + 9 * (size_t)c->diag[v];
+            P[3 * v + a] = 1.0 / w[4 * a];
+            bf[3 * v + a] = c->pinned[v] ? 0.0 : c->b[3 * v + a];
+        }
+    double bnorm = sqrt(oc_dot(bf, bf, n));
+    for (int k = 0; k < n; k++)
+        d[k] = P[k] * r[k];
+    int iters = 1;
+    int done = 0;
+    if (mode == OC_SOLVE_TOLERANCE)
+    {
+        double rn0 = sqrt(oc_dot(r, r, n));
+        if (rn0 <= tol * bnorm)
+            done = 1;
+    }
+    int cap = mode == OC_SOLVE_REFERENCE ? c->max_iters : max_iters + 1;
+    while (!done && iters < cap)
+    {
+        apply_matmul(c, d, Ad);
+        for (int k = 0; k < n; k++)
+            Pr[k] = P[k] * r[k];
+        double deno = oc_dot(d, Ad, n);
+        if (deno == 0)
+            break;
+        double alpha = oc_dot(r, Pr, n) / deno;
+        for (int k = 0; k < n; k++)
+            x[k] = x[k] + alpha * d[k];
+        for (int k = 0; k < n; k++)
+            rn[k] = r[k] - alpha * Ad[k];
+        for (int k = 0; k < n; k++)
+            Prn[k] = P[k] * rn[k];
+        double beta = oc_dot(rn, Prn, n) / oc_dot(r, Pr, n);
+        if (mode == OC_SOLVE_TOLERANCE)
+        {
+            double rr = sqrt(oc_dot(rn, rn, n));
+            if (rr <= tol * bnorm)
+            {
+                memcpy(r, rn, n * sizeof(double));
+                iters += 1;
+                break;
+            }
+        }
+        if (beta == 0)
+            break;
+        for (int k = 0; k < n; k++)
+            d[k] = Prn[k] + beta * d[k];
+        memcpy(r, rn, n * sizeof(double));
+        iters += 1;
+    }
+    if (st)
+    {
+        st->iterations = iters - 1;
+        double rr = sqrt(oc_dot(r, r, n));
+        st->rel_residual = bnorm > 0 ? rr / bnorm : rr;
+        apply_matmul(c, x, Ad);
+        for (int v = 0; v < c->nv; v++)
+            for (int a = 0; a < 3; a++)
+                Ad[3 * v + a] = c->pinned[v] ? 0.0 : c->b[3 * v + a] - Ad[3 * v + a];
+        double tr = sqrt(oc_dot(Ad, Ad, n));
+        st->true_rel_residual = bnorm > 0 ? tr / bnorm : tr;
+    }
+    free(r); free(Ad); free(d); free(P); free(Pr); free(rn); free(Prn); free(bf);
+    return 0;
+}
+
+int oc_step(oc_cloth *c, int mode, double tol, int max_iters, oc_stats *st)
+{
+    oc_assemble(c);
+    solve(c, mode, tol, max_iters, st);
+    int n = 3 * c->nv;
+    /* mXpre = mXcur; mXcur = mXpre + dx (BaraffCloth.cpp:150-153) */
+    for (int k = 0; k < n; k++)
+    {
+        c->xpre[k] = c->x[k];
+        c->x[k] = c->xpre[k] + c->dx[k];
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* accessors                                                           */
+/* ------------------------------------------------------------------ */
+
+int oc_num_vertices(const oc_cloth *c) { return c->nv; }
+int oc_num_triangles(const oc_cloth *c) { return c->nt; }
+int oc_num_edges(const oc_cloth *c) { return c->ne; }
+int oc_num_pins(const oc_cloth *c) { return c->npin; }
+int oc_ell_nnzb(const oc_cloth *c) { return c->nnzb; }
+void oc_get_triangles(const oc_cloth *c, int *o) { memcpy(o, c->tri, 3 * c->nt * sizeof(int)); }
+void oc_get_edges(const oc_cloth *c, int *o) { memcpy(o, c->edge, 5 * c->ne * sizeof(int)); }
+void oc_get_stencils(const oc_cloth *c, int *o) { memcpy(o, c->stencil, 4 * c->ne * sizeof(int)); }
+void oc_get_pins(const oc_cloth *c, int *o) { memcpy(o, c->pins, c->npin * sizeof(int)); }
+void oc_get_uv(const oc_cloth *c, double *o)
+{
+    for (int k = 0; k < 2 * c->nv; k++)
+        o[k] = c->uv[k];
+}
+void oc_get_rest_pos(const oc_cloth *c, double *o) { memcpy(o, c->xrest, 3 * c->nv * sizeof(double)); }
+void oc_get_mass(const oc_cloth *c, double *o) { memcpy(o, c->mass, c->nv * sizeof(double)); }
+void oc_get_gravity(const oc_cloth *c, double *o) { memcpy(o, c->gravity, 3 * c->nv * sizeof(double)); }
+void oc_get_stretch_coef(const oc_cloth *c, double *coef12, double *area)
+{
+    memcpy(coef12, c->coef, 12 * c->nt * sizeof(double));
+    memcpy(area, c->area, c->nt * sizeof(double));
+}
+void oc_get_bending_rest(const oc_cloth *c, double *el, double *h2, double *ek)
+{
+    for (int e = 0; e < c->ne; e++)
+    {
+        el[e] = c->edge_len[e];
+        h2[2 * e] = c->height[2 * e];
+        h2[2 * e + 1] = c->height[2 * e + 1];
+    }
+    *ek = c->edge_k;
+}
+void oc_get_pattern(const oc_cloth *c, int *rp, int *cols)
+{
+    memcpy(rp, c->row_ptr, (c->nv + 1) * sizeof(int));
+    memcpy(cols, c->cols, c->nnzb * sizeof(int));
+}
+void oc_set_state(oc_cloth *c, const double *x, const double *xpre)
+{
+    memcpy(c->x, x, 3 * c->nv * sizeof(double));
+    memcpy(c->xpre, xpre, 3 * c->nv * sizeof(double));
+}
+void oc_get_pos(const oc_cloth *c, double *x) { memcpy(x, c->x, 3 * c->nv * sizeof(double)); }
+void oc_get_pre_pos(const oc_cloth *c, double *x) { memcpy(x, c->xpre, 3 * c->nv * sizeof(double)); }
+void oc_get_K(const oc_cloth *c, double *k9, int which)
+{
+    const double *src = which == 1 ? c->Ks : which == 2 ? c->Kb : c->K;
+    memcpy(k9, src, 9 * (size_t)c->nnzb * sizeof(double));
+}
+void oc_get_W(const oc_cloth *c, double *w9) { memcpy(w9, c->W, 9 * (size_t)c->nnzb * sizeof(double)); }
+void oc_get_force(const oc_cloth *c, double *f3, int which)
+{
+    const double *src = which == 1 ? c->fs : which == 2 ? c->fb : c->f;
+    memcpy(f3, src, 3 * c->nv * sizeof(double));
+}
+void oc_get_rhs(const oc_cloth *c, double *b3) { memcpy(b3, c->b, 3 * c->nv * sizeof(double)); }
+void oc_get_dx(const oc_cloth *c, double *dx3) { memcpy(dx3, c->dx, 3 * c->nv * sizeof(double)); }

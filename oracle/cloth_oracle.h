@@ -1,0 +1,109 @@
+/*
+ * cloth_oracle.h -- CPU restatement of the reference's plain implicit cloth
+ * step (cBaraffCloth, cloth_type "fem"), in plain C, FP64 with the
+ * reference's FP32 roundings.
+ *
+ * TEST INFRASTRUCTURE.  This is the parity oracle: only tests/,
+ * __graft_entry__.smoke() and bench.py's CPU-baseline leg may load it, and
+ * only as the checker or the timed CPU baseline.  The product (the HIP
+ * library under acousticsim_amd/) never links or calls it.
+ */
+#ifndef CLOTH_ORACLE_H
+#define CLOTH_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oc_params
+{
+    /* geometry: regular_triangle grid (Triangulator.cpp:141-457) */
+    double cloth_size[2];       /* "cloth_size": [height(y), width(x)] */
+    int subdivision[2];         /* "subdivision": [height_div, width_div] */
+    double init_pos[3];         /* "cloth_init_pos" */
+    double init_euler_xyz[3];   /* "cloth_init_orientation" (x, y, z) */
+    double uv_rotation_deg;     /* "cloth_uv_rotation" in {0, 45, 90} */
+    /* material (BaraffCloth.cpp:22-83, BaraffCloth.h:25-27) */
+    double density;             /* "cloth_density" (stored as float) */
+    double stretch_k[3];        /* warp, weft, bias (stored as float) */
+    double bending_k;           /* "cloth_bending_stiffness"[0] (float) */
+    double rayleigh_a, rayleigh_b; /* stored as float */
+    double dt;                  /* "default_timestep" */
+    int enable_gravity;
+    int max_iters;              /* "max_iters" */
+    int num_pins;
+    const double *pin_uv;       /* num_pins x 2, "constraints.static_point" */
+} oc_params;
+
+typedef struct oc_cloth oc_cloth;
+
+/* solve modes */
+#define OC_SOLVE_REFERENCE 0 /* exactly max_iters-1 iterations, BaraffCloth.cpp:507-611 */
+#define OC_SOLVE_TOLERANCE 1 /* stop at ||r|| <= tol * ||b_free|| */
+
+typedef struct oc_stats
+{
+    int iterations;
+    double rel_residual;      /* recursive residual / ||b_free|| */
+    double true_rel_residual; /* ||b - W dx|| / ||b_free|| on free dofs */
+} oc_stats;
+
+oc_cloth *oc_create_grid(const oc_params *p);
+void oc_destroy(oc_cloth *c);
+
+int oc_num_vertices(const oc_cloth *c);
+int oc_num_triangles(const oc_cloth *c);
+int oc_num_edges(const oc_cloth *c);
+int oc_num_pins(const oc_cloth *c);
+int oc_ell_nnzb(const oc_cloth *c);     /* blocks in the assembly pattern */
+
+/* topology / rest data readback (reference numbering) */
+void oc_get_triangles(const oc_cloth *c, int *tri3);        /* T x 3 */
+void oc_get_edges(const oc_cloth *c, int *edge5);           /* E x {id0,id1,t0,t1,boundary} */
+void oc_get_stencils(const oc_cloth *c, int *st4);          /* E x 4, -1 on boundary */
+void oc_get_pins(const oc_cloth *c, int *pins);
+void oc_get_uv(const oc_cloth *c, double *uv2);             /* float values */
+void oc_get_rest_pos(const oc_cloth *c, double *x3);
+void oc_get_mass(const oc_cloth *c, double *m);             /* per vertex */
+void oc_get_gravity(const oc_cloth *c, double *g3);
+void oc_get_stretch_coef(const oc_cloth *c, double *coef12, double *area);
+void oc_get_bending_rest(const oc_cloth *c, double *edge_len, double *height2,
+                         double *edge_k);
+/* block pattern (CSR over 3x3 blocks, columns ascending) */
+void oc_get_pattern(const oc_cloth *c, int *row_ptr, int *cols);
+/* the reference GPU path's ELL rule, BaraffClothGpu.cpp:438-500:
+   row v = sorted {v} U stencils(interior e) containing v.  Returns the
+   total count when row_ptr/cols are NULL. */
+int oc_get_ref_ell(const oc_cloth *c, int *row_ptr, int *cols);
+
+/* state */
+void oc_set_state(oc_cloth *c, const double *x, const double *xpre);
+void oc_get_pos(const oc_cloth *c, double *x);
+void oc_get_pre_pos(const oc_cloth *c, double *x);
+
+/* one substep at the current state: assembles, solves, updates x.
+   mode = OC_SOLVE_REFERENCE uses max_iters from params (tol ignored). */
+int oc_step(oc_cloth *c, int mode, double tol, int max_iters, oc_stats *st);
+
+/* stage access: assemble K, f, W, b at the current state without stepping */
+void oc_assemble(oc_cloth *c);
+void oc_get_K(const oc_cloth *c, double *k9, int which); /* 0 total,1 stretch,2 bend */
+void oc_get_W(const oc_cloth *c, double *w9);
+void oc_get_force(const oc_cloth *c, double *f3, int which); /* 0 total,1 stretch,2 bend */
+void oc_get_rhs(const oc_cloth *c, double *b3);
+void oc_get_dx(const oc_cloth *c, double *dx3); /* last solve's dx */
+
+/* element-level pieces for finite-difference and closed-form checks */
+void oc_stretch_element(const double *x0, const double *x1, const double *x2,
+                        const double *coef12, double area, const double *k3,
+                        double *f9, double *H81);
+void oc_dbeta_dx(const double *v0, const double *v1, const double *v2,
+                 const double *v3, double *beta, double *g12);
+
+/* reduction used by the restated solver (Eigen 3.3.9 SSE2 redux order) */
+double oc_dot(const double *a, const double *b, long n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

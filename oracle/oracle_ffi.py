@@ -1,0 +1,262 @@
+"""ctypes binding of the C parity oracle (oracle/liboracle.so).
+
+Test infrastructure: imported only by tests/, __graft_entry__.smoke() and
+bench.py's CPU-baseline leg.  It restates cBaraffCloth (reference
+src/sim/cloth/BaraffCloth.cpp) on the CPU; the product never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+SOLVE_REFERENCE = 0
+SOLVE_TOLERANCE = 1
+
+
+class OcParams(ctypes.Structure):
+    _fields_ = [
+        ("cloth_size", ctypes.c_double * 2),
+        ("subdivision", ctypes.c_int * 2),
+        ("init_pos", ctypes.c_double * 3),
+        ("init_euler_xyz", ctypes.c_double * 3),
+        ("uv_rotation_deg", ctypes.c_double),
+        ("density", ctypes.c_double),
+        ("stretch_k", ctypes.c_double * 3),
+        ("bending_k", ctypes.c_double),
+        ("rayleigh_a", ctypes.c_double),
+        ("rayleigh_b", ctypes.c_double),
+        ("dt", ctypes.c_double),
+        ("enable_gravity", ctypes.c_int),
+        ("max_iters", ctypes.c_int),
+        ("num_pins", ctypes.c_int),
+        ("pin_uv", ctypes.POINTER(ctypes.c_double)),
+    ]
+
+
+class OcStats(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_int),
+        ("rel_residual", ctypes.c_double),
+        ("true_rel_residual", ctypes.c_double),
+    ]
+
+
+_lib = None
+
+
+def build():
+    """Compile oracle/liboracle.so (gcc)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER
+        vp = ctypes.c_void_p
+        dp = P(ctypes.c_double)
+        ip = P(ctypes.c_int)
+        L.oc_create_grid.restype = vp
+        L.oc_create_grid.argtypes = [P(OcParams)]
+        L.oc_destroy.argtypes = [vp]
+        for n in ("oc_num_vertices", "oc_num_triangles", "oc_num_edges",
+                  "oc_num_pins", "oc_ell_nnzb"):
+            getattr(L, n).argtypes = [vp]
+            getattr(L, n).restype = ctypes.c_int
+        for n in ("oc_get_triangles", "oc_get_edges", "oc_get_stencils", "oc_get_pins"):
+            getattr(L, n).argtypes = [vp, ip]
+        for n in ("oc_get_uv", "oc_get_rest_pos", "oc_get_mass", "oc_get_gravity",
+                  "oc_get_pos", "oc_get_pre_pos", "oc_get_W", "oc_get_rhs", "oc_get_dx"):
+            getattr(L, n).argtypes = [vp, dp]
+        L.oc_get_stretch_coef.argtypes = [vp, dp, dp]
+        L.oc_get_bending_rest.argtypes = [vp, dp, dp, dp]
+        L.oc_get_pattern.argtypes = [vp, ip, ip]
+        L.oc_get_ref_ell.argtypes = [vp, ip, ip]
+        L.oc_get_ref_ell.restype = ctypes.c_int
+        L.oc_set_state.argtypes = [vp, dp, dp]
+        L.oc_step.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, P(OcStats)]
+        L.oc_step.restype = ctypes.c_int
+        L.oc_assemble.argtypes = [vp]
+        L.oc_get_K.argtypes = [vp, dp, ctypes.c_int]
+        L.oc_get_force.argtypes = [vp, dp, ctypes.c_int]
+        L.oc_stretch_element.argtypes = [dp, dp, dp, dp, ctypes.c_double, dp, dp, dp]
+        L.oc_dbeta_dx.argtypes = [dp, dp, dp, dp, dp, dp]
+        L.oc_dot.argtypes = [dp, dp, ctypes.c_long]
+        L.oc_dot.restype = ctypes.c_double
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def params_from_config(conf):
+    """OcParams from a reference cloth JSON dict (keys of config/his_config.json)."""
+    p = OcParams()
+    p.cloth_size[:] = [float(v) for v in conf["cloth_size"][:2]]
+    p.subdivision[:] = [int(v) for v in conf["subdivision"][:2]]
+    ip = list(conf.get("cloth_init_pos", [0, 0, 0])) + [0, 0, 0]
+    p.init_pos[:] = [float(v) for v in ip[:3]]
+    io = list(conf.get("cloth_init_orientation", [0, 0, 0])) + [0, 0, 0]
+    p.init_euler_xyz[:] = [float(v) for v in io[:3]]
+    p.uv_rotation_deg = float(conf.get("cloth_uv_rotation", 0))
+    p.density = float(conf["cloth_density"])
+    p.stretch_k[:] = [float(v) for v in conf["cloth_stretch_stiffness"][:3]]
+    p.bending_k = float(conf["cloth_bending_stiffness"][0])
+    p.rayleigh_a = float(conf["rayleigh_damping_a"])
+    p.rayleigh_b = float(conf["rayleigh_damping_b"])
+    p.dt = float(conf["default_timestep"])
+    p.enable_gravity = int(bool(conf["enable_gravity"]))
+    p.max_iters = int(conf["max_iters"])
+    pins = conf.get("constraints", {}).get("static_point", [])
+    arr = np.ascontiguousarray(np.array(pins, dtype=np.float64).reshape(-1, 2))
+    p.num_pins = arr.shape[0]
+    p._pin_keepalive = arr
+    p.pin_uv = _dp(arr)
+    return p
+
+
+class OracleCloth:
+    """One cloth held by the C oracle."""
+
+    def __init__(self, conf):
+        L = lib()
+        self._params = params_from_config(conf)
+        self.h = L.oc_create_grid(ctypes.byref(self._params))
+        if not self.h:
+            raise ValueError("oracle: invalid grid configuration")
+        self.nv = L.oc_num_vertices(self.h)
+        self.nt = L.oc_num_triangles(self.h)
+        self.ne = L.oc_num_edges(self.h)
+        self.npin = L.oc_num_pins(self.h)
+        self.nnzb = L.oc_ell_nnzb(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oc_destroy(self.h)
+            self.h = None
+
+    def _f64(self, fn, n, *extra):
+        out = np.zeros(n, dtype=np.float64)
+        getattr(lib(), fn)(self.h, _dp(out), *extra)
+        return out
+
+    def _i32(self, fn, n):
+        out = np.zeros(n, dtype=np.int32)
+        getattr(lib(), fn)(self.h, _ip(out))
+        return out
+
+    def triangles(self):
+        return self._i32("oc_get_triangles", 3 * self.nt).reshape(-1, 3)
+
+    def edges(self):
+        return self._i32("oc_get_edges", 5 * self.ne).reshape(-1, 5)
+
+    def stencils(self):
+        return self._i32("oc_get_stencils", 4 * self.ne).reshape(-1, 4)
+
+    def pins(self):
+        return self._i32("oc_get_pins", self.npin)
+
+    def uv(self):
+        return self._f64("oc_get_uv", 2 * self.nv).reshape(-1, 2)
+
+    def rest_pos(self):
+        return self._f64("oc_get_rest_pos", 3 * self.nv)
+
+    def mass(self):
+        return self._f64("oc_get_mass", self.nv)
+
+    def gravity(self):
+        return self._f64("oc_get_gravity", 3 * self.nv)
+
+    def stretch_coef(self):
+        coef = np.zeros(12 * self.nt)
+        area = np.zeros(self.nt)
+        lib().oc_get_stretch_coef(self.h, _dp(coef), _dp(area))
+        return coef.reshape(-1, 12), area
+
+    def bending_rest(self):
+        el = np.zeros(self.ne)
+        h2 = np.zeros(2 * self.ne)
+        ek = np.zeros(1)
+        lib().oc_get_bending_rest(self.h, _dp(el), _dp(h2), _dp(ek))
+        return el, h2.reshape(-1, 2), float(ek[0])
+
+    def pattern(self):
+        rp = np.zeros(self.nv + 1, dtype=np.int32)
+        cols = np.zeros(self.nnzb, dtype=np.int32)
+        lib().oc_get_pattern(self.h, _ip(rp), _ip(cols))
+        return rp, cols
+
+    def ref_ell(self):
+        n = lib().oc_get_ref_ell(self.h, None, None)
+        rp = np.zeros(self.nv + 1, dtype=np.int32)
+        cols = np.zeros(n, dtype=np.int32)
+        lib().oc_get_ref_ell(self.h, _ip(rp), _ip(cols))
+        return rp, cols
+
+    def set_state(self, x, xpre):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        xpre = np.ascontiguousarray(xpre, dtype=np.float64)
+        assert x.size == 3 * self.nv and xpre.size == 3 * self.nv
+        lib().oc_set_state(self.h, _dp(x), _dp(xpre))
+
+    def pos(self):
+        return self._f64("oc_get_pos", 3 * self.nv)
+
+    def pre_pos(self):
+        return self._f64("oc_get_pre_pos", 3 * self.nv)
+
+    def step(self, mode=SOLVE_TOLERANCE, tol=1e-6, max_iters=100000):
+        st = OcStats()
+        lib().oc_step(self.h, mode, tol, max_iters, ctypes.byref(st))
+        return {"iterations": st.iterations, "rel_residual": st.rel_residual,
+                "true_rel_residual": st.true_rel_residual}
+
+    def assemble(self):
+        lib().oc_assemble(self.h)
+
+    def K(self, which=0):
+        return self._f64("oc_get_K", 9 * self.nnzb, which).reshape(-1, 3, 3)
+
+    def W(self):
+        return self._f64("oc_get_W", 9 * self.nnzb).reshape(-1, 3, 3)
+
+    def force(self, which=0):
+        return self._f64("oc_get_force", 3 * self.nv, which)
+
+    def rhs(self):
+        return self._f64("oc_get_rhs", 3 * self.nv)
+
+    def dx(self):
+        return self._f64("oc_get_dx", 3 * self.nv)
+
+
+def stretch_element(x0, x1, x2, coef12, area, k3):
+    f = np.zeros(9)
+    H = np.zeros(81)
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in (x0, x1, x2, coef12, k3)]
+    lib().oc_stretch_element(_dp(a[0]), _dp(a[1]), _dp(a[2]), _dp(a[3]), float(area),
+                             _dp(a[4]), _dp(f), _dp(H))
+    return f.reshape(3, 3), H.reshape(9, 3, 3)
+
+
+def dbeta_dx(v0, v1, v2, v3):
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in (v0, v1, v2, v3)]
+    beta = np.zeros(1)
+    g = np.zeros(12)
+    lib().oc_dbeta_dx(_dp(a[0]), _dp(a[1]), _dp(a[2]), _dp(a[3]), _dp(beta), _dp(g))
+    return float(beta[0]), g

@@ -1,0 +1,232 @@
+// Test-infrastructure driver for the reference's own CPU cloth class.
+//
+// This file is ours; it is compiled together with the reference's CPU
+// translation units (sources left where they lie under the reference tree, see
+// oracle/Makefile.ref) into oracle/_ref/ref_cloth.  It never ships in the
+// product: only tests/golden/make_golden.py and bench.py's CPU-baseline leg
+// run it, as the checker.
+//
+// It follows the reference's own call sequence for one cloth
+// (src/sim/SimObjectBuilder.cpp:40-54 -> cBaraffCloth::Init,
+//  src/sim/cloth/BaseCloth.cpp:97-109 SetPos/SetPrePos,
+//  src/sim/cloth/BaraffCloth.cpp:90-163 UpdatePos)
+// and dumps state into a small tagged binary file:
+//   repeated records:  "<name> <dtype> <count>\n" + count raw little-endian values
+//   dtype is "f8" (double) or "i4" (int32).
+//
+// usage: ref_cloth <config.json> <x0.bin|-> <nsteps> <out.bin>
+//                  [--dump-steps a,b,c] [--stages] [--time]
+#include "sim/cloth/BaraffCloth.h"
+#include "sim/cloth/BaraffMaterial.h"
+#include "sim/cloth/BaseBendingMaterial.h"
+#include "geometries/Primitives.h"
+#include "utils/JsonUtil.h"
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <set>
+#include <string>
+#include <vector>
+#include <unistd.h>
+#include <fcntl.h>
+
+namespace
+{
+// Exposes protected state of the reference class for dumping only.
+class cProbeCloth : public cBaraffCloth
+{
+public:
+    explicit cProbeCloth(int id) : cBaraffCloth(id) {}
+    const tSparseMatd &K() const { return mStiffnessMatrix; }
+    const tVectorXd &IntForce() const { return mIntForce; }
+    const tVectorXd &GravityForce() const { return mGravityForce; }
+    const tVectorXd &MassDiag() const { return mMassMatrixDiag; }
+    const tVectorXd &Dx() const { return mDx_buf; }
+    const tVectorXd &Xpre() const { return mXpre; }
+    cBaraffMaterialPtr Stretch() const { return mMaterial; }
+    cBaseBendingMaterialPtr Bending() const { return mBendingMaterial; }
+    const tEigenArr<tVector4i> &Stencils() const { return mEdgeAffectVertexId; }
+};
+
+FILE *gOut = nullptr;
+
+void WriteF8(const std::string &name, const double *p, size_t n)
+{
+    fprintf(gOut, "%s f8 %zu\n", name.c_str(), n);
+    fwrite(p, sizeof(double), n, gOut);
+}
+void WriteI4(const std::string &name, const int *p, size_t n)
+{
+    fprintf(gOut, "%s i4 %zu\n", name.c_str(), n);
+    fwrite(p, sizeof(int), n, gOut);
+}
+void WriteVec(const std::string &name, const tVectorXd &v)
+{
+    WriteF8(name, v.data(), v.size());
+}
+// Sparse matrix as COO over its stored entries (row-major CSR order).
+void WriteSparse(const std::string &name, const tSparseMatd &m)
+{
+    std::vector<int> rows, cols;
+    std::vector<double> vals;
+    for (int k = 0; k < m.outerSize(); ++k)
+        for (tSparseMatd::InnerIterator it(m, k); it; ++it)
+        {
+            rows.push_back(it.row());
+            cols.push_back(it.col());
+            vals.push_back(it.value());
+        }
+    WriteI4(name + "_row", rows.data(), rows.size());
+    WriteI4(name + "_col", cols.data(), cols.size());
+    WriteF8(name + "_val", vals.data(), vals.size());
+}
+
+std::vector<int> ParseList(const char *s)
+{
+    std::vector<int> out;
+    std::string cur;
+    for (const char *c = s;; ++c)
+    {
+        if (*c == ',' || *c == 0)
+        {
+            if (!cur.empty())
+                out.push_back(atoi(cur.c_str()));
+            cur.clear();
+            if (*c == 0)
+                break;
+        }
+        else
+            cur.push_back(*c);
+    }
+    return out;
+}
+} // namespace
+
+int main(int argc, char **argv)
+{
+    if (argc < 5)
+    {
+        fprintf(stderr, "usage: %s config.json x0.bin|- nsteps out.bin "
+                        "[--dump-steps a,b] [--stages] [--time]\n",
+                argv[0]);
+        return 2;
+    }
+    std::string conf_path = argv[1], x0_path = argv[2], out_path = argv[4];
+    int nsteps = atoi(argv[3]);
+    std::set<int> dump_steps;
+    bool stages = false, timing = false;
+    for (int i = 5; i < argc; i++)
+    {
+        if (!strcmp(argv[i], "--dump-steps") && i + 1 < argc)
+        {
+            for (int s : ParseList(argv[++i]))
+                dump_steps.insert(s);
+        }
+        else if (!strcmp(argv[i], "--stages"))
+            stages = true;
+        else if (!strcmp(argv[i], "--time"))
+            timing = true;
+    }
+    dump_steps.insert(nsteps);
+
+    Json::Value conf;
+    if (!cJsonUtil::LoadJson(conf_path, conf))
+        return 3;
+
+    // the reference prints on every step; keep stdout quiet
+    fflush(stdout);
+    int devnull = open("/dev/null", O_WRONLY);
+    int saved_stdout = dup(1);
+    dup2(devnull, 1);
+
+    auto cloth = std::make_shared<cProbeCloth>(0);
+    cloth->Init(conf);
+    int nv = cloth->GetNumOfVertices();
+    tVectorXd x0 = cloth->GetPos();
+    if (x0_path != "-")
+    {
+        FILE *f = fopen(x0_path.c_str(), "rb");
+        if (!f)
+            return 4;
+        size_t got = fread(x0.data(), sizeof(double), 3 * nv, f);
+        fclose(f);
+        if (got != size_t(3 * nv))
+            return 5;
+        cloth->SetPos(x0);
+        cloth->SetPrePos(x0);
+    }
+
+    gOut = fopen(out_path.c_str(), "wb");
+    if (!gOut)
+        return 6;
+
+    // topology and rest data (reference numbering)
+    {
+        std::vector<int> tri, edge, stencil;
+        for (auto &t : cloth->GetTriangleArray())
+        {
+            tri.push_back(t->mId0);
+            tri.push_back(t->mId1);
+            tri.push_back(t->mId2);
+        }
+        for (auto &e : cloth->GetEdgeArray())
+        {
+            edge.push_back(e->mId0);
+            edge.push_back(e->mId1);
+            edge.push_back(e->mTriangleId0);
+            edge.push_back(e->mTriangleId1);
+            edge.push_back(e->mIsBoundary ? 1 : 0);
+        }
+        for (auto &s : cloth->Stencils())
+            for (int k = 0; k < 4; k++)
+                stencil.push_back(s[k]);
+        WriteI4("triangles", tri.data(), tri.size());
+        WriteI4("edges", edge.data(), edge.size());
+        WriteI4("stencils", stencil.data(), stencil.size());
+        const std::vector<int> &pins = cloth->GetConstraintStaticVertices();
+        WriteI4("pins", pins.data(), pins.size());
+        std::vector<double> uv, rest;
+        for (auto &v : cloth->GetVertexArray())
+        {
+            uv.push_back(v->muv[0]);
+            uv.push_back(v->muv[1]);
+        }
+        WriteF8("uv", uv.data(), uv.size());
+        WriteVec("x_init", cloth->GetInitPos());
+        WriteVec("x0", x0);
+        WriteVec("mass_diag", cloth->MassDiag());
+        WriteVec("gravity", cloth->GravityForce());
+    }
+
+    std::vector<double> step_seconds;
+    for (int step = 1; step <= nsteps; step++)
+    {
+        tVectorXd xbefore = cloth->GetPos();
+        auto t0 = std::chrono::steady_clock::now();
+        cloth->UpdatePos(cloth->GetDefaultTimestep());
+        auto t1 = std::chrono::steady_clock::now();
+        step_seconds.push_back(std::chrono::duration<double>(t1 - t0).count());
+        if (stages && step == 1)
+        {
+            // state of the first step: K and f were formed at xbefore
+            WriteVec("s1_x", xbefore);
+            WriteSparse("s1_K", cloth->K());
+            WriteVec("s1_fint", cloth->IntForce());
+            WriteVec("s1_dx", cloth->Dx());
+            WriteVec("s1_f_stretch", cloth->Stretch()->CalcTotalForce());
+            WriteSparse("s1_K_stretch",
+                        cloth->Stretch()->CalcTotalStiffnessMatrix());
+            WriteSparse("s1_K_bend", cloth->Bending()->GetStiffnessMatrix());
+            WriteVec("s1_f_bend", cloth->Bending()->CalcForce(xbefore));
+        }
+        if (dump_steps.count(step))
+            WriteVec("x_step" + std::to_string(step), cloth->GetPos());
+    }
+    if (timing)
+        WriteF8("step_seconds", step_seconds.data(), step_seconds.size());
+    fclose(gOut);
+    fflush(stdout);
+    dup2(saved_stdout, 1);
+    return 0;
+}

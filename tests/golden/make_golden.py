@@ -1,0 +1,140 @@
+"""Generate the golden fixtures from the reference's own CPU cloth class.
+
+Runs oracle/_ref/ref_cloth (the reference's cBaraffCloth compiled from its
+sources by oracle/Makefile.ref, driven by oracle/ref_harness.cpp) on seeded
+inputs and stores inputs and outputs as small .npz fixtures in this folder.
+Run in the build container, where the reference tree exists:
+
+    make -f oracle/Makefile.ref -j8
+    python tests/golden/make_golden.py            # all cases
+    python tests/golden/make_golden.py grid21     # one case
+
+The fixtures are data: configuration, initial positions, pins, and the
+reference's positions after given steps (and, for small meshes, its
+per-stage K, forces and dx of the first step).
+"""
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, HERE)
+
+from acousticsim_amd import workloads  # noqa: E402
+from refdump import read_dump  # noqa: E402
+
+REF_BIN = os.path.join(ROOT, "oracle", "_ref", "ref_cloth")
+
+
+def case_defs():
+    cases = {}
+    c = workloads.grid_config(3, dt=0.01)
+    cases["tiny3"] = dict(conf=c, seed=11, steps=5, dump=[1, 2, 5], stages=True, full_k=True)
+
+    c = workloads.grid_config(5, dt=0.005)
+    c.update({"cloth_uv_rotation": 45, "rayleigh_damping_a": 0.1, "rayleigh_damping_b": -2e-3,
+              "cloth_stretch_stiffness": [500, 800, 50], "cloth_bending_stiffness": [3e-4, 1e-4],
+              "cloth_density": 0.3, "max_iters": 200})
+    cases["grid5_rot45"] = dict(conf=c, seed=12, steps=5, dump=[1, 5], stages=True, full_k=True)
+
+    c = workloads.grid_config(7)
+    c.update({"subdivision": [6, 11], "cloth_size": [0.3, 0.5], "cloth_init_pos": [0, 0.4, 0],
+              "constraints": {"static_point": [[-0.25, 0.15], [0.25, 0.15], [0.0, 0.15]]},
+              "cloth_uv_rotation": 90, "max_iters": 300})
+    cases["rect7x12"] = dict(conf=c, seed=13, steps=8, dump=[1, 8], stages=True, full_k=True)
+
+    c = workloads.grid_config(21, dt=0.01)
+    cases["grid21"] = dict(conf=c, seed=workloads.SINGLE_SMALL["seed"], steps=100,
+                           dump=[1, 10, 50, 100], stages=True, full_k=False)
+
+    c = workloads.grid_config(257, dt=0.005)
+    cases["grid257"] = dict(conf=c, seed=workloads.SINGLE_LARGE["seed"], steps=5,
+                            dump=[1, 5], stages=False, full_k=False, stride=17)
+    return cases
+
+
+def initial_positions(conf, seed):
+    x = workloads.jittered_positions(conf, seed).reshape(-1, 3)
+    x += np.array((list(conf.get("cloth_init_pos", [0, 0, 0])) + [0, 0, 0])[:3])
+    return x.reshape(-1)
+
+
+def run_case(name, d):
+    conf = d["conf"]
+    x0 = initial_positions(conf, d["seed"])
+    with tempfile.TemporaryDirectory() as td:
+        cpath = os.path.join(td, "conf.json")
+        xpath = os.path.join(td, "x0.bin")
+        opath = os.path.join(td, "out.bin")
+        with open(cpath, "w") as f:
+            json.dump(conf, f)
+        x0.astype("<f8").tofile(xpath)
+        cmd = [REF_BIN, cpath, xpath, str(d["steps"]), opath,
+               "--dump-steps", ",".join(str(s) for s in d["dump"]), "--time"]
+        if d["stages"]:
+            cmd.append("--stages")
+        subprocess.run(cmd, check=True)
+        dump = read_dump(opath)
+    # the injected x0 must be a jitter of the reference's own rest positions
+    xi = dump["x_init"]
+    assert np.max(np.abs((x0 - xi).reshape(-1, 3)[:, :2])) == 0.0, name
+    out = {"config_json": np.array(json.dumps(conf)), "seed": np.array(d["seed"]),
+           "steps": np.array(d["dump"], dtype=np.int32)}
+    stride = d.get("stride", 1)
+    sel = np.arange(0, len(xi) // 3, stride)
+    out["vertex_subset"] = sel.astype(np.int32)
+    if stride == 1:
+        out["x0"] = x0
+    else:
+        # large meshes: x0 is regenerated from (config, seed); keep the
+        # subset as a check that the regeneration matches
+        out["x0_subset"] = x0.reshape(-1, 3)[sel].reshape(-1)
+    out["pins"] = dump["pins"]
+    out["mass"] = dump["mass_diag"][0::3][sel]
+    for s in d["dump"]:
+        x = dump["x_step%d" % s].reshape(-1, 3)
+        out["x_step%d" % s] = x[sel].reshape(-1)
+        out["norm_step%d" % s] = np.array(np.linalg.norm(x))
+        out["sum_step%d" % s] = x.sum(axis=0)
+    out["step_seconds"] = dump["step_seconds"]
+    if d["stages"]:
+        out["triangles"] = dump["triangles"]
+        out["edges"] = dump["edges"]
+        out["stencils"] = dump["stencils"]
+        out["uv"] = dump["uv"]
+        for k in ("s1_fint", "s1_dx", "s1_f_stretch", "s1_f_bend"):
+            out[k] = dump[k]
+        if d["full_k"]:
+            for k in ("s1_K", "s1_K_stretch", "s1_K_bend"):
+                for part in ("row", "col", "val"):
+                    out[k + "_" + part] = dump[k + "_" + part]
+        else:
+            # the block pattern only (rows/cols of the scalar CSR, one per block)
+            r, c = dump["s1_K_row"], dump["s1_K_col"]
+            keep = (r % 3 == 0) & (c % 3 == 0)
+            out["s1_K_block_row"] = (r[keep] // 3).astype(np.int32)
+            out["s1_K_block_col"] = (c[keep] // 3).astype(np.int32)
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **out)
+    print("%-12s V=%d steps=%d  ref %.3f s/step  -> %s (%d KB)" % (
+        name, len(xi) // 3, d["steps"], float(np.mean(dump["step_seconds"])), path,
+        os.path.getsize(path) // 1024))
+
+
+def main():
+    if not os.path.exists(REF_BIN):
+        sys.exit("build the reference first: make -f oracle/Makefile.ref -j8")
+    cases = case_defs()
+    names = sys.argv[1:] or list(cases)
+    for n in names:
+        run_case(n, cases[n])
+
+
+if __name__ == "__main__":
+    main()
