@@ -1,0 +1,303 @@
+"""Python binding of the C ABI (include/bw_cloth.h) in libbwcloth.so.
+
+`BaraffClothHip` mirrors the reference's cloth-object interface
+(cBaseCloth, src/sim/cloth/BaseCloth.h:19-80, as implemented by cBaraffCloth):
+Init from the cloth JSON, UpdatePos, GetPos / SetPos / SetPrePos, Reset,
+constraint vertices, plus readback of mesh, ELL structure and step counters.
+`ClothBatch` steps many cloths together.  The library is loaded from the
+package folder; if it is missing the import of this module fails loudly
+(there is no fallback path).
+"""
+import ctypes
+import json
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbwcloth.so")
+
+BW_OK = 0
+
+
+class BwError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("bw error %d: %s" % (code, msg))
+        self.code = code
+
+
+class StepStats(ctypes.Structure):
+    _fields_ = [
+        ("pcg_iterations", ctypes.c_int),
+        ("outer_iterations", ctypes.c_int),
+        ("rel_residual", ctypes.c_double),
+        ("step_us", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+EXPORTS = [
+    "bw_create", "bw_destroy", "bw_update_pos", "bw_reset", "bw_get_pos", "bw_get_pre_pos",
+    "bw_get_velocity", "bw_set_pos", "bw_set_pre_pos", "bw_get_init_pos",
+    "bw_get_default_timestep", "bw_num_vertices", "bw_num_triangles", "bw_num_edges",
+    "bw_get_constraint_static_vertices", "bw_add_constraint_static_vertices",
+    "bw_clear_constraint_static_vertices", "bw_get_triangles", "bw_get_edges",
+    "bw_get_ell_map", "bw_assemble_system", "bw_get_last_stats", "bw_last_error", "bw_batch_create",
+    "bw_batch_destroy", "bw_batch_size", "bw_batch_step", "bw_batch_num_vertices",
+    "bw_batch_get_pos", "bw_batch_set_state", "bw_batch_get_stats", "bw_batch_set_graphs",
+    "bw_batch_time_spmv",
+]
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libbwcloth.so is not built (run python -m acousticsim_amd.build): "
+                          + LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER
+    vp, dp, ip, cp = ctypes.c_void_p, P(ctypes.c_double), P(ctypes.c_int), ctypes.c_char_p
+    i32 = ctypes.c_int
+    sig = {
+        "bw_create": (i32, [cp, dp, P(vp)]),
+        "bw_destroy": (None, [vp]),
+        "bw_update_pos": (i32, [vp, ctypes.c_double]),
+        "bw_reset": (i32, [vp]),
+        "bw_get_pos": (i32, [vp, dp]),
+        "bw_get_pre_pos": (i32, [vp, dp]),
+        "bw_get_velocity": (i32, [vp, dp]),
+        "bw_set_pos": (i32, [vp, dp]),
+        "bw_set_pre_pos": (i32, [vp, dp]),
+        "bw_get_init_pos": (i32, [vp, dp]),
+        "bw_get_default_timestep": (ctypes.c_double, [vp]),
+        "bw_num_vertices": (i32, [vp]),
+        "bw_num_triangles": (i32, [vp]),
+        "bw_num_edges": (i32, [vp]),
+        "bw_get_constraint_static_vertices": (i32, [vp, ip, i32]),
+        "bw_add_constraint_static_vertices": (i32, [vp, ip, i32]),
+        "bw_clear_constraint_static_vertices": (i32, [vp]),
+        "bw_get_triangles": (i32, [vp, ip]),
+        "bw_get_edges": (i32, [vp, ip]),
+        "bw_get_ell_map": (i32, [vp, ip, ip]),
+        "bw_assemble_system": (i32, [vp, dp, dp]),
+        "bw_get_last_stats": (i32, [vp, P(StepStats)]),
+        "bw_last_error": (cp, []),
+        "bw_batch_create": (i32, [P(cp), P(dp), i32, P(vp)]),
+        "bw_batch_destroy": (None, [vp]),
+        "bw_batch_size": (i32, [vp]),
+        "bw_batch_step": (i32, [vp, i32]),
+        "bw_batch_num_vertices": (i32, [vp, i32]),
+        "bw_batch_get_pos": (i32, [vp, i32, dp]),
+        "bw_batch_set_state": (i32, [vp, i32, dp, dp]),
+        "bw_batch_get_stats": (i32, [vp, i32, P(StepStats)]),
+        "bw_batch_set_graphs": (i32, [vp, i32]),
+        "bw_batch_time_spmv": (i32, [vp, i32, dp, dp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def _check(rc):
+    if rc < 0:
+        raise BwError(rc, lib().bw_last_error().decode())
+    return rc
+
+
+def _json(conf):
+    return (conf if isinstance(conf, str) else json.dumps(conf)).encode()
+
+
+class BaraffClothHip:
+    """The plain implicit cloth step on the GPU, cBaseCloth-style interface."""
+
+    def __init__(self, conf, x0=None):
+        self._h = ctypes.c_void_p()
+        x = None
+        if x0 is not None:
+            x = np.ascontiguousarray(x0, dtype=np.float64)
+        _check(lib().bw_create(_json(conf), _dp(x) if x is not None else None,
+                               ctypes.byref(self._h)))
+        self.nv = lib().bw_num_vertices(self._h)
+        if x is not None and x.size != 3 * self.nv:
+            raise ValueError("x0 has %d values, expected %d" % (x.size, 3 * self.nv))
+
+    def close(self):
+        if self._h:
+            lib().bw_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- cBaseCloth names -------------------------------------------------
+    def UpdatePos(self, dt=0.0):
+        _check(lib().bw_update_pos(self._h, dt))
+
+    def Reset(self):
+        _check(lib().bw_reset(self._h))
+
+    def GetPos(self):
+        x = np.empty(3 * self.nv)
+        _check(lib().bw_get_pos(self._h, _dp(x)))
+        return x
+
+    def GetPrePos(self):
+        x = np.empty(3 * self.nv)
+        _check(lib().bw_get_pre_pos(self._h, _dp(x)))
+        return x
+
+    def GetVelocity(self):
+        v = np.empty(3 * self.nv)
+        _check(lib().bw_get_velocity(self._h, _dp(v)))
+        return v
+
+    def SetPos(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        assert x.size == 3 * self.nv
+        _check(lib().bw_set_pos(self._h, _dp(x)))
+
+    def SetPrePos(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        assert x.size == 3 * self.nv
+        _check(lib().bw_set_pre_pos(self._h, _dp(x)))
+
+    def GetInitPos(self):
+        x = np.empty(3 * self.nv)
+        _check(lib().bw_get_init_pos(self._h, _dp(x)))
+        return x
+
+    def GetDefaultTimestep(self):
+        return lib().bw_get_default_timestep(self._h)
+
+    def GetNumOfVertices(self):
+        return self.nv
+
+    def GetNumOfTriangles(self):
+        return _check(lib().bw_num_triangles(self._h))
+
+    def GetNumOfEdges(self):
+        return _check(lib().bw_num_edges(self._h))
+
+    def GetConstraintStaticVertices(self):
+        n = _check(lib().bw_get_constraint_static_vertices(self._h, None, 0))
+        out = np.zeros(n, dtype=np.int32)
+        lib().bw_get_constraint_static_vertices(self._h, _ip(out), n)
+        return out
+
+    def AddConstraintStaticVertices(self, ids):
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        _check(lib().bw_add_constraint_static_vertices(self._h, _ip(ids), ids.size))
+
+    def ClearConstraintStaticVertices(self):
+        _check(lib().bw_clear_constraint_static_vertices(self._h))
+
+    # --- readback ----------------------------------------------------------
+    def triangles(self):
+        t = np.zeros(3 * self.GetNumOfTriangles(), dtype=np.int32)
+        _check(lib().bw_get_triangles(self._h, _ip(t)))
+        return t.reshape(-1, 3)
+
+    def edges(self):
+        e = np.zeros(5 * self.GetNumOfEdges(), dtype=np.int32)
+        _check(lib().bw_get_edges(self._h, _ip(e)))
+        return e.reshape(-1, 5)
+
+    def ell_map(self):
+        n = _check(lib().bw_get_ell_map(self._h, None, None))
+        rp = np.zeros(self.nv + 1, dtype=np.int32)
+        cols = np.zeros(n, dtype=np.int32)
+        _check(lib().bw_get_ell_map(self._h, _ip(rp), _ip(cols)))
+        return rp, cols
+
+    def assemble_system(self):
+        """(W blocks in ell_map order, rhs) assembled at the current state."""
+        rp, cols = self.ell_map()
+        W = np.zeros(9 * cols.size)
+        b = np.zeros(3 * self.nv)
+        _check(lib().bw_assemble_system(self._h, _dp(W), _dp(b)))
+        return W.reshape(-1, 3, 3), b
+
+    def last_stats(self):
+        st = StepStats()
+        _check(lib().bw_get_last_stats(self._h, ctypes.byref(st)))
+        return st.as_dict()
+
+
+class ClothBatch:
+    """Many independent cloths stepped together (one launch chain per phase)."""
+
+    def __init__(self, confs, x0s=None):
+        n = len(confs)
+        texts = [_json(c) for c in confs]
+        arr = (ctypes.c_char_p * n)(*texts)
+        self._x0 = [None] * n
+        ptrs = (ctypes.POINTER(ctypes.c_double) * n)()
+        if x0s is not None:
+            for i, x in enumerate(x0s):
+                if x is not None:
+                    self._x0[i] = np.ascontiguousarray(x, dtype=np.float64)
+                    ptrs[i] = _dp(self._x0[i])
+        self._h = ctypes.c_void_p()
+        _check(lib().bw_batch_create(arr, ptrs, n, ctypes.byref(self._h)))
+        self.n = n
+        self.nv = [lib().bw_batch_num_vertices(self._h, i) for i in range(n)]
+
+    def close(self):
+        if self._h:
+            lib().bw_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, nsteps=1):
+        _check(lib().bw_batch_step(self._h, nsteps))
+
+    def set_graphs(self, on):
+        _check(lib().bw_batch_set_graphs(self._h, 1 if on else 0))
+
+    def get_pos(self, i):
+        x = np.empty(3 * self.nv[i])
+        _check(lib().bw_batch_get_pos(self._h, i, _dp(x)))
+        return x
+
+    def time_spmv(self, reps=50):
+        """(mean us per SpMV launch, algorithmic bytes per launch); call after
+        at least one step so W is assembled."""
+        us = np.zeros(1)
+        nbytes = np.zeros(1)
+        _check(lib().bw_batch_time_spmv(self._h, reps, _dp(us), _dp(nbytes)))
+        return float(us[0]), float(nbytes[0])
+
+    def set_state(self, i, x, xpre):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        xpre = np.ascontiguousarray(xpre, dtype=np.float64)
+        _check(lib().bw_batch_set_state(self._h, i, _dp(x), _dp(xpre)))
+
+    def stats(self, i):
+        st = StepStats()
+        _check(lib().bw_batch_get_stats(self._h, i, ctypes.byref(st)))
+        return st.as_dict()

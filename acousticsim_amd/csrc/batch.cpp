@@ -1,0 +1,553 @@
+#include "batch.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+
+namespace bw
+{
+
+namespace
+{
+void check(hipError_t e, const char *what)
+{
+    if (e != hipSuccess)
+        throw std::runtime_error(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
+}
+} // namespace
+
+template <typename T> T *Batch::dalloc(size_t n)
+{
+    void *p = nullptr;
+    const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    check(hipMalloc(&p, bytes), "hipMalloc");
+    allocations_.push_back(p);
+    check(hipMemset(p, 0, bytes), "hipMemset");
+    return static_cast<T *>(p);
+}
+
+namespace
+{
+template <typename T> void upload(T *dst, const std::vector<T> &src)
+{
+    if (!src.empty())
+        check(hipMemcpy(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice),
+              "hipMemcpy H2D");
+}
+} // namespace
+
+Batch::Batch(std::vector<ClothModel> models) : models_(std::move(models))
+{
+    if (models_.empty())
+        throw std::runtime_error("batch: no cloths");
+    int nmixed = 0;
+    for (auto &m : models_)
+    {
+        const bool mixed = m.cfg.precision == Precision::Mixed &&
+                           m.cfg.solve_mode == SolveMode::Tolerance;
+        nmixed += mixed ? 1 : 0;
+        any_ref_ |= m.cfg.solve_mode == SolveMode::Reference;
+    }
+    if (nmixed != 0 && nmixed != (int)models_.size())
+        throw std::runtime_error("batch: members must share one precision (mixed or fp64)");
+    mixed_ = nmixed != 0;
+    stats_.resize(models_.size());
+    check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    check(hipHostMalloc((void **)&host_state_, sizeof(InstState) * models_.size()),
+          "hipHostMalloc");
+    build_layout();
+}
+
+Batch::~Batch()
+{
+    for (auto &g : graphs_)
+        (void)hipGraphExecDestroy(g.exec);
+    if (stream_)
+        (void)hipStreamSynchronize(stream_);
+    for (void *p : allocations_)
+        (void)hipFree(p);
+    if (host_state_)
+        (void)hipHostFree(host_state_);
+    if (stream_)
+        (void)hipStreamDestroy(stream_);
+}
+
+void Batch::build_layout()
+{
+    const int n = (int)models_.size();
+    row_base_.resize(n);
+    slice_base_.resize(n);
+    nslices_.resize(n);
+    int slices = 0;
+    int nvert = 0, ntri = 0, nie = 0, nblk = 0;
+    std::vector<int> vert_base(n), tri_base(n), ie_base(n), blk_base(n);
+    for (int i = 0; i < n; i++)
+    {
+        const ClothModel &m = models_[i];
+        slice_base_[i] = slices;
+        nslices_[i] = (m.nv + SLICE - 1) / SLICE;
+        row_base_[i] = slices * SLICE;
+        slices += nslices_[i];
+        vert_base[i] = nvert;
+        tri_base[i] = ntri;
+        ie_base[i] = nie;
+        blk_base[i] = nblk;
+        nvert += m.nv;
+        ntri += m.nt;
+        nie += (int)m.interior.size();
+        nblk += m.nnzb();
+    }
+    if ((long long)ntri >= (1LL << 27) || (long long)nie >= (1LL << 27))
+        throw std::runtime_error("batch: too many elements");
+    d_.n_inst = n;
+    d_.n_slices = slices;
+    d_.n_rows = slices * SLICE;
+    d_.n_vert = nvert;
+    d_.n_tri = ntri;
+    d_.n_ie = nie;
+    d_.n_blocks = nblk;
+
+    std::vector<int> slice_off(slices), slice_w(slices), slice_inst(slices);
+    std::vector<int> row_vertex(d_.n_rows, -1), row_w(d_.n_rows, 0);
+    std::vector<uint8_t> row_pinned(d_.n_rows, 0);
+    std::vector<double> row_mass(d_.n_rows, 0.0);
+    long long entries = 0;
+    for (int i = 0; i < n; i++)
+    {
+        const ClothModel &m = models_[i];
+        for (int v = 0; v < m.nv; v++)
+        {
+            const int p = row_base_[i] + v;
+            row_vertex[p] = vert_base[i] + v;
+            row_w[p] = m.row_ptr[v + 1] - m.row_ptr[v];
+            row_pinned[p] = m.pinned[v];
+            row_mass[p] = m.mass[v];
+        }
+        for (int s = 0; s < nslices_[i]; s++)
+        {
+            const int gs = slice_base_[i] + s;
+            int w = 0;
+            for (int l = 0; l < SLICE; l++)
+                w = std::max(w, row_w[gs * SLICE + l]);
+            slice_w[gs] = w;
+            slice_inst[gs] = i;
+            slice_off[gs] = (int)entries;
+            entries += (long long)w * SLICE;
+        }
+    }
+    if (entries >= (1LL << 31) / 9)
+        throw std::runtime_error("batch: matrix too large for 32-bit entry offsets");
+    d_.n_entries = entries;
+    std::vector<int> col(entries, 0);
+    for (int s = 0; s < slices; s++)
+        for (int k = 0; k < slice_w[s]; k++)
+            for (int l = 0; l < SLICE; l++)
+                col[slice_off[s] + (size_t)k * SLICE + l] = s * SLICE + l; // padding: self
+    std::vector<int> vert_row(nvert), blk_ptr(nvert + 1, 0), blk_col(nblk);
+    std::vector<int> s_ptr(nblk + 1, 0), b_ptr(nblk + 1, 0), s_list, b_list;
+    std::vector<int> vt_ptr(nvert + 1, 0), ve_ptr(nvert + 1, 0), vt_list, ve_list;
+    std::vector<int> tri_rows(3 * (size_t)ntri), tri_inst(ntri), ie_rows(4 * (size_t)nie);
+    std::vector<float> tri_coef(12 * (size_t)ntri), tri_area(ntri);
+    std::vector<double> ie_factor(nie);
+    std::vector<InstParams> params(n);
+    for (int i = 0; i < n; i++)
+    {
+        const ClothModel &m = models_[i];
+        const int rb = row_base_[i];
+        for (int v = 0; v < m.nv; v++)
+        {
+            const int vg = vert_base[i] + v;
+            vert_row[vg] = rb + v;
+            blk_ptr[vg + 1] = blk_base[i] + m.row_ptr[v + 1];
+            vt_ptr[vg + 1] = (int)vt_list.size() + m.vt_ptr[v + 1];
+            ve_ptr[vg + 1] = (int)ve_list.size() + m.ve_ptr[v + 1];
+            for (int k = m.row_ptr[v]; k < m.row_ptr[v + 1]; k++)
+            {
+                const int p = rb + v, ks = k - m.row_ptr[v];
+                const int s = p / SLICE, l = p % SLICE;
+                col[slice_off[s] + (size_t)ks * SLICE + l] = rb + m.cols[k];
+            }
+        }
+        if (m.nv > 0)
+        {
+            blk_ptr[vert_base[i]] = blk_base[i];
+            vt_ptr[vert_base[i]] = (int)vt_list.size();
+            ve_ptr[vert_base[i]] = (int)ve_list.size();
+        }
+        for (int code : m.vt_list)
+            vt_list.push_back(((code >> 2) + tri_base[i]) << 2 | (code & 3));
+        for (int code : m.ve_list)
+            ve_list.push_back(((code >> 2) + ie_base[i]) << 2 | (code & 3));
+        const int sb = (int)s_list.size(), bb = (int)b_list.size();
+        for (int k = 0; k < m.nnzb(); k++)
+        {
+            const int gb = blk_base[i] + k;
+            blk_col[gb] = rb + m.cols[k];
+            s_ptr[gb + 1] = sb + m.s_ptr[k + 1];
+            b_ptr[gb + 1] = bb + m.b_ptr[k + 1];
+        }
+        if (m.nnzb() > 0)
+        {
+            s_ptr[blk_base[i]] = sb;
+            b_ptr[blk_base[i]] = bb;
+        }
+        for (int code : m.s_list)
+            s_list.push_back(((code >> 4) + tri_base[i]) << 4 | (code & 15));
+        for (int code : m.b_list)
+            b_list.push_back(((code >> 4) + ie_base[i]) << 4 | (code & 15));
+        for (int t = 0; t < m.nt; t++)
+        {
+            const int tg = tri_base[i] + t;
+            for (int k = 0; k < 3; k++)
+                tri_rows[3 * (size_t)tg + k] = rb + m.tri[t][k];
+            tri_inst[tg] = i;
+            for (int k = 0; k < 12; k++)
+                tri_coef[12 * (size_t)tg + k] = m.coef[12 * (size_t)t + k];
+            tri_area[tg] = m.area[t];
+        }
+        for (size_t ie = 0; ie < m.interior.size(); ie++)
+        {
+            const size_t g = ie_base[i] + ie;
+            for (int k = 0; k < 4; k++)
+                ie_rows[4 * g + k] = rb + m.stencil[m.interior[ie]][k];
+            ie_factor[g] = m.bend_factor[ie];
+        }
+        InstParams &P = params[i];
+        P.dt = m.cfg.dt;
+        P.sk = m.sk;
+        P.sm = m.sm;
+        P.k0 = m.k[0];
+        P.k1 = m.k[1];
+        P.k2 = m.k[2];
+        P.grav = m.cfg.gravity ? 1.0 : 0.0;
+        P.ref_mode = m.cfg.solve_mode == SolveMode::Reference ? 1 : 0;
+        P.tol = P.ref_mode ? 0.0 : m.cfg.tolerance;
+        P.max_iter = P.ref_mode ? m.cfg.max_iters - 1 : m.cfg.solver_max_iters;
+        P.mixed = mixed_ ? 1 : 0;
+        P.max_outer = 30;
+        P.slice_begin = slice_base_[i];
+        P.slice_count = nslices_[i];
+    }
+
+    d_.slice_off = dalloc<int>(slices);
+    d_.slice_w = dalloc<int>(slices);
+    d_.slice_inst = dalloc<int>(slices);
+    d_.row_vertex = dalloc<int>(d_.n_rows);
+    d_.row_w = dalloc<int>(d_.n_rows);
+    d_.row_pinned = dalloc<uint8_t>(d_.n_rows);
+    d_.row_mass = dalloc<double>(d_.n_rows);
+    d_.col = dalloc<int>(entries);
+    d_.W64 = dalloc<double>(9 * (size_t)entries);
+    d_.W32 = dalloc<float>(9 * (size_t)entries);
+    d_.vert_row = dalloc<int>(nvert);
+    d_.blk_ptr = dalloc<int>(nvert + 1);
+    d_.blk_col = dalloc<int>(nblk);
+    d_.s_ptr = dalloc<int>(nblk + 1);
+    d_.s_list = dalloc<int>(s_list.size());
+    d_.b_ptr = dalloc<int>(nblk + 1);
+    d_.b_list = dalloc<int>(b_list.size());
+    d_.vt_ptr = dalloc<int>(nvert + 1);
+    d_.vt_list = dalloc<int>(vt_list.size());
+    d_.ve_ptr = dalloc<int>(nvert + 1);
+    d_.ve_list = dalloc<int>(ve_list.size());
+    d_.tri_rows = dalloc<int>(3 * (size_t)ntri);
+    d_.tri_inst = dalloc<int>(ntri);
+    d_.tri_coef = dalloc<float>(12 * (size_t)ntri);
+    d_.tri_area = dalloc<float>(ntri);
+    d_.ie_rows = dalloc<int>(4 * (size_t)nie);
+    d_.ie_factor = dalloc<double>(nie);
+    d_.tri_ing = dalloc<double>(25 * (size_t)ntri);
+    d_.ie_ing = dalloc<double>(25 * (size_t)nie);
+    const size_t nv3 = 3 * (size_t)d_.n_rows;
+    d_.x = dalloc<double>(nv3);
+    d_.xpre = dalloc<double>(nv3);
+    d_.dx = dalloc<double>(nv3);
+    d_.b = dalloc<double>(nv3);
+    d_.P64 = dalloc<double>(nv3);
+    d_.r64 = dalloc<double>(nv3);
+    if (!mixed_)
+    {
+        d_.d64[0] = dalloc<double>(nv3);
+        d_.d64[1] = dalloc<double>(nv3);
+        d_.q64 = dalloc<double>(nv3);
+    }
+    d_.P32 = dalloc<float>(nv3);
+    if (mixed_)
+    {
+        d_.r32 = dalloc<float>(nv3);
+        d_.e32 = dalloc<float>(nv3);
+        d_.d32[0] = dalloc<float>(nv3);
+        d_.d32[1] = dalloc<float>(nv3);
+        d_.q32 = dalloc<float>(nv3);
+    }
+    d_.partial = dalloc<double>(4 * (size_t)slices);
+    d_.params = dalloc<InstParams>(n);
+    d_.state = dalloc<InstState>(n);
+
+    upload(d_.slice_off, slice_off);
+    upload(d_.slice_w, slice_w);
+    upload(d_.slice_inst, slice_inst);
+    upload(d_.row_vertex, row_vertex);
+    upload(d_.row_w, row_w);
+    upload(d_.row_pinned, row_pinned);
+    upload(d_.row_mass, row_mass);
+    upload(d_.col, col);
+    upload(d_.vert_row, vert_row);
+    upload(d_.blk_ptr, blk_ptr);
+    upload(d_.blk_col, blk_col);
+    upload(d_.s_ptr, s_ptr);
+    upload(d_.s_list, s_list);
+    upload(d_.b_ptr, b_ptr);
+    upload(d_.b_list, b_list);
+    upload(d_.vt_ptr, vt_ptr);
+    upload(d_.vt_list, vt_list);
+    upload(d_.ve_ptr, ve_ptr);
+    upload(d_.ve_list, ve_list);
+    upload(d_.tri_rows, tri_rows);
+    upload(d_.tri_inst, tri_inst);
+    upload(d_.tri_coef, tri_coef);
+    upload(d_.tri_area, tri_area);
+    upload(d_.ie_rows, ie_rows);
+    upload(d_.ie_factor, ie_factor);
+    upload(d_.params, params);
+    // initial state: rest positions, at rest
+    for (int i = 0; i < n; i++)
+    {
+        upload_state(i, models_[i].xrest.data(), d_.x);
+        upload_state(i, models_[i].xrest.data(), d_.xpre);
+    }
+}
+
+void Batch::refresh_pins(int i, const std::vector<int> &pins)
+{
+    ClothModel &m = models_[i];
+    m.pins = pins;
+    m.rebuild_pins_from_list();
+    check(hipMemcpy(d_.row_pinned + row_base_[i], m.pinned.data(), m.nv, hipMemcpyHostToDevice),
+          "hipMemcpy pins");
+}
+
+void Batch::upload_state(int i, const double *src, double *dst_dev)
+{
+    check(hipMemcpy(dst_dev + 3 * (size_t)row_base_[i], src,
+                    3 * (size_t)models_[i].nv * sizeof(double), hipMemcpyHostToDevice),
+          "hipMemcpy state");
+}
+
+void Batch::download(int i, const double *src_dev, double *dst) const
+{
+    check(hipMemcpy(dst, src_dev + 3 * (size_t)row_base_[i],
+                    3 * (size_t)models_[i].nv * sizeof(double), hipMemcpyDeviceToHost),
+          "hipMemcpy D2H");
+}
+
+void Batch::get_pos(int i, double *x) const { download(i, d_.x, x); }
+void Batch::get_pre_pos(int i, double *x) const { download(i, d_.xpre, x); }
+void Batch::set_pos(int i, const double *x) { upload_state(i, x, d_.x); }
+void Batch::set_pre_pos(int i, const double *x) { upload_state(i, x, d_.xpre); }
+
+void Batch::assemble_system(int i, double *W, double *rhs)
+{
+    launch_reset_states(d_, stream_);
+    launch_elements(d_, stream_);
+    launch_assemble_rows(d_, stream_);
+    check(hipGetLastError(), "assemble launch");
+    check(hipStreamSynchronize(stream_), "assemble sync");
+    const ClothModel &m = models_[i];
+    const int s0 = slice_base_[i], s1 = s0 + nslices_[i];
+    std::vector<int> off(nslices_[i]);
+    check(hipMemcpy(off.data(), d_.slice_off + s0, off.size() * sizeof(int),
+                    hipMemcpyDeviceToHost),
+          "hipMemcpy slice_off");
+    long long e0 = off.empty() ? 0 : off[0];
+    long long e1 = e0;
+    {
+        int last_w = 0;
+        check(hipMemcpy(&last_w, d_.slice_w + s1 - 1, sizeof(int), hipMemcpyDeviceToHost),
+              "hipMemcpy slice_w");
+        e1 = (long long)off.back() + (long long)last_w * SLICE;
+    }
+    std::vector<double> vals(9 * (size_t)(e1 - e0));
+    check(hipMemcpy(vals.data(), d_.W64 + 9 * (size_t)e0, vals.size() * sizeof(double),
+                    hipMemcpyDeviceToHost),
+          "hipMemcpy W");
+    for (int v = 0; v < m.nv; v++)
+    {
+        const int p = row_base_[i] + v, s = p / SLICE, l = p % SLICE;
+        const size_t base = 9 * (size_t)(off[s - s0] - e0);
+        for (int k = m.row_ptr[v]; k < m.row_ptr[v + 1]; k++)
+        {
+            const int ks = k - m.row_ptr[v];
+            for (int q = 0; q < 9; q++)
+                W[9 * (size_t)k + q] = vals[base + (size_t)(9 * ks + q) * SLICE + l];
+        }
+    }
+    download(i, d_.b, rhs);
+}
+
+double Batch::time_spmv(int reps, double *alg_bytes)
+{
+    if (reps < 1)
+        throw std::runtime_error("batch: reps must be >= 1");
+    // a nonzero residual everywhere keeps every cloth active (d.Wd > 0)
+    launch_reset_states(d_, stream_);
+    launch_fill_residual(d_, mixed_, stream_);
+    hipEvent_t e0, e1;
+    check(hipEventCreate(&e0), "hipEventCreate");
+    check(hipEventCreate(&e1), "hipEventCreate");
+    launch_spmv_only(d_, mixed_, stream_); // warm
+    check(hipEventRecord(e0, stream_), "hipEventRecord");
+    for (int k = 0; k < reps; k++)
+        launch_spmv_only(d_, mixed_, stream_);
+    check(hipEventRecord(e1, stream_), "hipEventRecord");
+    check(hipEventSynchronize(e1), "hipEventSynchronize");
+    float ms = 0;
+    check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    double nnzb = 0, nv = 0;
+    for (auto &m : models_)
+    {
+        nnzb += m.nnzb();
+        nv += m.nv;
+    }
+    const double per_block = mixed_ ? 40.0 : 76.0, per_row = mixed_ ? 60.0 : 120.0;
+    if (alg_bytes)
+        *alg_bytes = per_block * nnzb + per_row * nv;
+    return 1000.0 * ms / reps;
+}
+
+void Batch::read_states()
+{
+    check(hipMemcpyAsync(host_state_, d_.state, sizeof(InstState) * models_.size(),
+                         hipMemcpyDeviceToHost, stream_),
+          "hipMemcpyAsync state");
+    check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+}
+
+bool Batch::all(bool (*pred)(const InstState &)) const
+{
+    for (size_t i = 0; i < models_.size(); i++)
+        if (!pred(host_state_[i]))
+            return false;
+    return true;
+}
+
+void Batch::run_chunk(bool fp32, int iters)
+{
+    if (!use_graphs_)
+    {
+        for (int k = 0; k < iters; k++)
+            fp32 ? launch_pcg_iter32(d_, stream_) : launch_pcg_iter64(d_, stream_);
+        check(hipGetLastError(), "PCG launch");
+        return;
+    }
+    hipGraphExec_t exec = nullptr;
+    for (auto &g : graphs_)
+        if (g.fp32 == fp32 && g.iters == iters)
+            exec = g.exec;
+    if (!exec)
+    {
+        hipGraph_t graph;
+        check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "BeginCapture");
+        for (int k = 0; k < iters; k++)
+            fp32 ? launch_pcg_iter32(d_, stream_) : launch_pcg_iter64(d_, stream_);
+        check(hipStreamEndCapture(stream_, &graph), "EndCapture");
+        check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "GraphInstantiate");
+        (void)hipGraphDestroy(graph);
+        graphs_.push_back({fp32, iters, exec});
+    }
+    check(hipGraphLaunch(exec, stream_), "GraphLaunch");
+}
+
+namespace
+{
+bool pred_done(const InstState &s) { return s.done != 0; }
+bool pred_outer_done(const InstState &s) { return s.outer_done != 0; }
+
+// chunk sizes are multiples of 16 iterations so few graphs get built
+int chunk_for(int estimate)
+{
+    int c = std::max(16, std::min(estimate, 512));
+    return ((c + 15) / 16) * 16;
+}
+} // namespace
+
+void Batch::step(int nsteps)
+{
+    for (int n = 0; n < nsteps; n++)
+    {
+        const auto t0 = std::chrono::steady_clock::now();
+        launch_reset_states(d_, stream_);
+        launch_elements(d_, stream_);
+        launch_assemble_rows(d_, stream_);
+        int iters_this_step = 0;
+        if (!mixed_)
+        {
+            launch_pcg_init64(d_, stream_);
+            int chunk = chunk_for(last_iters_ > 0 ? last_iters_ : 16);
+            int done_iters = 0;
+            for (;;)
+            {
+                read_states();
+                if (all(pred_done))
+                    break;
+                run_chunk(false, chunk);
+                done_iters += chunk;
+                chunk = 32;
+            }
+            iters_this_step = done_iters;
+            launch_residual64(d_, 0, stream_);
+        }
+        else
+        {
+            launch_residual64(d_, 1, stream_);
+            bool first_pass = true;
+            for (;;)
+            {
+                read_states();
+                if (all(pred_outer_done))
+                    break;
+                int chunk = chunk_for(first_pass && last_iters_ > 0 ? last_iters_ : 16);
+                for (;;)
+                {
+                    if (all(pred_done))
+                        break;
+                    run_chunk(true, chunk);
+                    iters_this_step += chunk;
+                    chunk = 32;
+                    read_states();
+                }
+                first_pass = false;
+                launch_correct(d_, stream_);
+                launch_residual64(d_, 1, stream_);
+            }
+        }
+        launch_update_positions(d_, stream_);
+        check(hipGetLastError(), "step launch");
+        read_states();
+        const double us =
+            std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+        int max_iter = 0;
+        bool bad = false;
+        for (size_t i = 0; i < models_.size(); i++)
+        {
+            const InstState &s = host_state_[i];
+            StepStats &st = stats_[i];
+            st.pcg_iterations = s.total_iter;
+            st.outer_iterations = mixed_ ? s.outer : 1;
+            st.rel_residual = s.bb > 0 ? std::sqrt(s.rr64 / s.bb) : std::sqrt(s.rr64);
+            st.step_us = us;
+            max_iter = std::max(max_iter, s.total_iter);
+            bad |= !std::isfinite(s.rr64);
+        }
+        last_iters_ = max_iter;
+        (void)iters_this_step;
+        if (bad)
+            throw std::runtime_error("step: non-finite residual (NaN/Inf in the state)");
+    }
+}
+
+} // namespace bw

@@ -1,0 +1,421 @@
+// C ABI (include/bw_cloth.h) over the device batch runtime.  Every entry
+// point catches exceptions and returns a status; nothing exits the process.
+#include "../../include/bw_cloth.h"
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "batch.h"
+#include "json_lite.h"
+#include "model.h"
+
+namespace
+{
+thread_local std::string g_error;
+
+int fail(int code, const std::string &msg)
+{
+    g_error = msg;
+    return code;
+}
+
+template <typename F> int guard(F &&f)
+{
+    try
+    {
+        return f();
+    }
+    catch (const std::invalid_argument &e)
+    {
+        return fail(BW_ERR_ARG, e.what());
+    }
+    catch (const std::runtime_error &e)
+    {
+        const std::string m = e.what();
+        if (m.rfind("HIP error", 0) == 0)
+            return fail(BW_ERR_DEVICE, m);
+        if (m.rfind("step:", 0) == 0)
+            return fail(BW_ERR_NUMERIC, m);
+        if (m.rfind("config", 0) == 0 || m.rfind("batch:", 0) == 0)
+            return fail(BW_ERR_ARG, m);
+        return fail(BW_ERR_STATE, m);
+    }
+    catch (const std::bad_alloc &)
+    {
+        return fail(BW_ERR_CAPACITY, "out of host memory");
+    }
+    catch (...)
+    {
+        return fail(BW_ERR_STATE, "unknown error");
+    }
+}
+
+bw::ClothModel model_from_json(const char *json)
+{
+    if (!json)
+        throw std::runtime_error("config: NULL JSON text");
+    bw::Json j = bw::Json::parse(json);
+    if (j.kind != bw::Json::Object)
+        throw std::runtime_error("config: top level must be an object");
+    if (j.has("cloth_type"))
+    {
+        const std::string t = j.at("cloth_type").as_string();
+        if (t != "fem" && t != "fem_hip")
+            throw std::runtime_error("config: cloth_type \"" + t +
+                                     "\" is not the plain implicit method (\"fem\")");
+    }
+    return bw::ClothModel::build(bw::ClothConfig::from_json(j));
+}
+
+bool all_finite(const double *x, size_t n)
+{
+    for (size_t k = 0; k < n; k++)
+        if (!std::isfinite(x[k]))
+            return false;
+    return true;
+}
+} // namespace
+
+struct bw_cloth
+{
+    std::unique_ptr<bw::Batch> batch;
+    std::vector<double> init_pos;
+};
+
+struct bw_batch
+{
+    std::unique_ptr<bw::Batch> batch;
+};
+
+extern "C" {
+
+const char *bw_last_error(void) { return g_error.c_str(); }
+
+int bw_create(const char *json, const double *x0, bw_cloth **out)
+{
+    return guard([&] {
+        if (!out)
+            return fail(BW_ERR_ARG, "out is NULL");
+        *out = nullptr;
+        std::vector<bw::ClothModel> ms;
+        ms.push_back(model_from_json(json));
+        auto c = std::make_unique<bw_cloth>();
+        c->init_pos = ms[0].xrest;
+        const size_t n3 = 3 * (size_t)ms[0].nv;
+        c->batch = std::make_unique<bw::Batch>(std::move(ms));
+        if (x0)
+        {
+            if (!all_finite(x0, n3))
+                return fail(BW_ERR_ARG, "x0 has NaN/Inf");
+            c->batch->set_pos(0, x0);
+            c->batch->set_pre_pos(0, x0);
+        }
+        *out = c.release();
+        return BW_OK;
+    });
+}
+
+void bw_destroy(bw_cloth *c) { delete c; }
+
+int bw_update_pos(bw_cloth *c, double)
+{
+    return guard([&] {
+        if (!c)
+            return fail(BW_ERR_ARG, "NULL cloth");
+        c->batch->step(1);
+        return BW_OK;
+    });
+}
+
+int bw_reset(bw_cloth *c)
+{
+    return guard([&] {
+        if (!c)
+            return fail(BW_ERR_ARG, "NULL cloth");
+        c->batch->set_pos(0, c->init_pos.data());
+        c->batch->set_pre_pos(0, c->init_pos.data());
+        return BW_OK;
+    });
+}
+
+int bw_get_pos(const bw_cloth *c, double *x)
+{
+    return guard([&] {
+        if (!c || !x)
+            return fail(BW_ERR_ARG, "NULL argument");
+        c->batch->get_pos(0, x);
+        return BW_OK;
+    });
+}
+
+int bw_get_pre_pos(const bw_cloth *c, double *x)
+{
+    return guard([&] {
+        if (!c || !x)
+            return fail(BW_ERR_ARG, "NULL argument");
+        c->batch->get_pre_pos(0, x);
+        return BW_OK;
+    });
+}
+
+int bw_get_velocity(const bw_cloth *c, double *v)
+{
+    return guard([&] {
+        if (!c || !v)
+            return fail(BW_ERR_ARG, "NULL argument");
+        const size_t n3 = 3 * (size_t)c->batch->model(0).nv;
+        std::vector<double> xp(n3);
+        c->batch->get_pos(0, v);
+        c->batch->get_pre_pos(0, xp.data());
+        const double dt = c->batch->model(0).cfg.dt;
+        for (size_t k = 0; k < n3; k++)
+            v[k] = (v[k] - xp[k]) / dt;
+        return BW_OK;
+    });
+}
+
+int bw_set_pos(bw_cloth *c, const double *x)
+{
+    return guard([&] {
+        if (!c || !x)
+            return fail(BW_ERR_ARG, "NULL argument");
+        if (!all_finite(x, 3 * (size_t)c->batch->model(0).nv))
+            return fail(BW_ERR_ARG, "positions have NaN/Inf");
+        c->batch->set_pos(0, x);
+        return BW_OK;
+    });
+}
+
+int bw_set_pre_pos(bw_cloth *c, const double *x)
+{
+    return guard([&] {
+        if (!c || !x)
+            return fail(BW_ERR_ARG, "NULL argument");
+        if (!all_finite(x, 3 * (size_t)c->batch->model(0).nv))
+            return fail(BW_ERR_ARG, "positions have NaN/Inf");
+        c->batch->set_pre_pos(0, x);
+        return BW_OK;
+    });
+}
+
+int bw_get_init_pos(const bw_cloth *c, double *x)
+{
+    return guard([&] {
+        if (!c || !x)
+            return fail(BW_ERR_ARG, "NULL argument");
+        std::memcpy(x, c->init_pos.data(), c->init_pos.size() * sizeof(double));
+        return BW_OK;
+    });
+}
+
+double bw_get_default_timestep(const bw_cloth *c) { return c ? c->batch->model(0).cfg.dt : 0.0; }
+int bw_num_vertices(const bw_cloth *c) { return c ? c->batch->model(0).nv : BW_ERR_ARG; }
+int bw_num_triangles(const bw_cloth *c) { return c ? c->batch->model(0).nt : BW_ERR_ARG; }
+int bw_num_edges(const bw_cloth *c) { return c ? c->batch->model(0).ne : BW_ERR_ARG; }
+
+int bw_get_constraint_static_vertices(const bw_cloth *c, int *ids, int cap)
+{
+    if (!c)
+        return fail(BW_ERR_ARG, "NULL cloth");
+    const auto &p = c->batch->model(0).pins;
+    for (int k = 0; k < (int)p.size() && k < cap && ids; k++)
+        ids[k] = p[k];
+    return (int)p.size();
+}
+
+int bw_add_constraint_static_vertices(bw_cloth *c, const int *ids, int n)
+{
+    return guard([&] {
+        if (!c || (n > 0 && !ids) || n < 0)
+            return fail(BW_ERR_ARG, "bad arguments");
+        std::vector<int> pins = c->batch->model(0).pins;
+        for (int k = 0; k < n; k++)
+        {
+            if (ids[k] < 0 || ids[k] >= c->batch->model(0).nv)
+                return fail(BW_ERR_ARG, "vertex id out of range");
+            pins.push_back(ids[k]);
+        }
+        c->batch->refresh_pins(0, pins);
+        return BW_OK;
+    });
+}
+
+int bw_clear_constraint_static_vertices(bw_cloth *c)
+{
+    return guard([&] {
+        if (!c)
+            return fail(BW_ERR_ARG, "NULL cloth");
+        c->batch->refresh_pins(0, {});
+        return BW_OK;
+    });
+}
+
+int bw_get_triangles(const bw_cloth *c, int *tri3)
+{
+    if (!c || !tri3)
+        return fail(BW_ERR_ARG, "NULL argument");
+    const auto &m = c->batch->model(0);
+    for (int t = 0; t < m.nt; t++)
+        for (int k = 0; k < 3; k++)
+            tri3[3 * t + k] = m.tri[t][k];
+    return BW_OK;
+}
+
+int bw_get_edges(const bw_cloth *c, int *e5)
+{
+    if (!c || !e5)
+        return fail(BW_ERR_ARG, "NULL argument");
+    const auto &m = c->batch->model(0);
+    for (int e = 0; e < m.ne; e++)
+    {
+        const auto &ed = m.edges[e];
+        e5[5 * e] = ed.v0;
+        e5[5 * e + 1] = ed.v1;
+        e5[5 * e + 2] = ed.t0;
+        e5[5 * e + 3] = ed.t1;
+        e5[5 * e + 4] = ed.boundary ? 1 : 0;
+    }
+    return BW_OK;
+}
+
+int bw_get_ell_map(const bw_cloth *c, int *row_ptr, int *cols)
+{
+    return guard([&] {
+        if (!c)
+            return fail(BW_ERR_ARG, "NULL cloth");
+        std::vector<int> rp, cl;
+        c->batch->model(0).reference_ell(rp, cl);
+        if (row_ptr && cols)
+        {
+            std::memcpy(row_ptr, rp.data(), rp.size() * sizeof(int));
+            std::memcpy(cols, cl.data(), cl.size() * sizeof(int));
+        }
+        return (int)cl.size();
+    });
+}
+
+int bw_assemble_system(bw_cloth *c, double *W, double *rhs)
+{
+    return guard([&] {
+        if (!c || !W || !rhs)
+            return fail(BW_ERR_ARG, "NULL argument");
+        c->batch->assemble_system(0, W, rhs);
+        return c->batch->model(0).nnzb();
+    });
+}
+
+int bw_get_last_stats(const bw_cloth *c, bw_step_stats *st)
+{
+    if (!c || !st)
+        return fail(BW_ERR_ARG, "NULL argument");
+    const auto &s = c->batch->stats(0);
+    st->pcg_iterations = s.pcg_iterations;
+    st->outer_iterations = s.outer_iterations;
+    st->rel_residual = s.rel_residual;
+    st->step_us = s.step_us;
+    return BW_OK;
+}
+
+int bw_batch_create(const char *const *jsons, const double *const *x0s, int n, bw_batch **out)
+{
+    return guard([&] {
+        if (!out || !jsons || n <= 0)
+            return fail(BW_ERR_ARG, "bad arguments");
+        *out = nullptr;
+        std::vector<bw::ClothModel> ms;
+        for (int i = 0; i < n; i++)
+            ms.push_back(model_from_json(jsons[i]));
+        std::vector<size_t> n3;
+        for (auto &m : ms)
+            n3.push_back(3 * (size_t)m.nv);
+        auto b = std::make_unique<bw_batch>();
+        b->batch = std::make_unique<bw::Batch>(std::move(ms));
+        for (int i = 0; i < n; i++)
+            if (x0s && x0s[i])
+            {
+                if (!all_finite(x0s[i], n3[i]))
+                    return fail(BW_ERR_ARG, "x0 has NaN/Inf");
+                b->batch->set_pos(i, x0s[i]);
+                b->batch->set_pre_pos(i, x0s[i]);
+            }
+        *out = b.release();
+        return BW_OK;
+    });
+}
+
+void bw_batch_destroy(bw_batch *b) { delete b; }
+int bw_batch_size(const bw_batch *b) { return b ? b->batch->size() : BW_ERR_ARG; }
+
+int bw_batch_step(bw_batch *b, int nsteps)
+{
+    return guard([&] {
+        if (!b || nsteps < 0)
+            return fail(BW_ERR_ARG, "bad arguments");
+        b->batch->step(nsteps);
+        return BW_OK;
+    });
+}
+
+int bw_batch_num_vertices(const bw_batch *b, int i)
+{
+    if (!b || i < 0 || i >= b->batch->size())
+        return fail(BW_ERR_ARG, "bad index");
+    return b->batch->model(i).nv;
+}
+
+int bw_batch_get_pos(const bw_batch *b, int i, double *x)
+{
+    return guard([&] {
+        if (!b || !x || i < 0 || i >= b->batch->size())
+            return fail(BW_ERR_ARG, "bad arguments");
+        b->batch->get_pos(i, x);
+        return BW_OK;
+    });
+}
+
+int bw_batch_set_state(bw_batch *b, int i, const double *x, const double *xpre)
+{
+    return guard([&] {
+        if (!b || !x || !xpre || i < 0 || i >= b->batch->size())
+            return fail(BW_ERR_ARG, "bad arguments");
+        b->batch->set_pos(i, x);
+        b->batch->set_pre_pos(i, xpre);
+        return BW_OK;
+    });
+}
+
+int bw_batch_get_stats(const bw_batch *b, int i, bw_step_stats *st)
+{
+    if (!b || !st || i < 0 || i >= b->batch->size())
+        return fail(BW_ERR_ARG, "bad arguments");
+    const auto &s = b->batch->stats(i);
+    st->pcg_iterations = s.pcg_iterations;
+    st->outer_iterations = s.outer_iterations;
+    st->rel_residual = s.rel_residual;
+    st->step_us = s.step_us;
+    return BW_OK;
+}
+
+int bw_batch_time_spmv(bw_batch *b, int reps, double *avg_us, double *alg_bytes)
+{
+    return guard([&] {
+        if (!b || !avg_us)
+            return fail(BW_ERR_ARG, "NULL argument");
+        *avg_us = b->batch->time_spmv(reps, alg_bytes);
+        return BW_OK;
+    });
+}
+
+int bw_batch_set_graphs(bw_batch *b, int enable)
+{
+    if (!b)
+        return fail(BW_ERR_ARG, "NULL batch");
+    b->batch->set_graphs(enable != 0);
+    return BW_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,94 @@
+// Shared host/device declarations: the batched device layout and the kernel
+// launchers.  One "batch" holds any number of independent cloths; every
+// array below is indexed by global (batch-wide) ids.
+//
+// Matrix layout: sliced ELL with slice height SLICE = 64 (one wavefront per
+// slice, one lane per block row).  Rows of one cloth occupy whole slices.
+// Entry (slice s, slot k, lane l) has index e = slice_off[s] + k*64 + l; its
+// column is col[e] and its 3x3 block is stored as nine planes of 64 values:
+// W[9*slice_off[s] + (9*k + q)*64 + l] for q = 3*a + b.  Every wave load is
+// one contiguous 64-value run.
+#pragma once
+#include <cstdint>
+
+#include <hip/hip_runtime.h>
+
+namespace bw
+{
+
+constexpr int SLICE = 64;
+
+struct InstParams
+{
+    double dt, sk, sm, k0, k1, k2, grav;
+    double tol;       // relative residual tolerance
+    int max_iter;     // PCG iteration cap (reference mode: max_iters - 1)
+    int ref_mode;     // 1: reference semantics (no tolerance stop)
+    int mixed;        // 1: FP32 inner PCG + FP64 refinement
+    int max_outer;    // refinement passes (mixed)
+    int slice_begin, slice_count;
+};
+
+struct InstState
+{
+    double rho, alpha, beta, bb, rr, dq, rr64, inner_tol2;
+    int iter, done, outer, outer_done, total_iter, breakdown;
+    unsigned int count;
+    int pad;
+};
+
+struct DeviceBatch
+{
+    int n_inst = 0, n_slices = 0, n_rows = 0;
+    long long n_entries = 0;
+    int n_tri = 0, n_ie = 0, n_vert = 0, n_blocks = 0;
+    // per slice / row
+    int *slice_off = nullptr, *slice_w = nullptr, *slice_inst = nullptr;
+    int *row_vertex = nullptr, *row_w = nullptr;
+    uint8_t *row_pinned = nullptr;
+    double *row_mass = nullptr;
+    // matrix
+    int *col = nullptr;
+    double *W64 = nullptr;
+    float *W32 = nullptr;
+    // assembly inputs (global vertex ids map to physical rows)
+    int *vert_row = nullptr;                       // n_vert
+    int *blk_ptr = nullptr;                        // per vertex, n_vert + 1 (CSR over blocks)
+    int *blk_col = nullptr;                        // physical column row per block
+    int *s_ptr = nullptr, *s_list = nullptr;       // stretch contributions per block
+    int *b_ptr = nullptr, *b_list = nullptr;       // bending contributions per block
+    int *vt_ptr = nullptr, *vt_list = nullptr;     // per vertex incident triangles
+    int *ve_ptr = nullptr, *ve_list = nullptr;     // per vertex incident interior edges
+    int *tri_rows = nullptr;                       // 3 per triangle (physical rows)
+    int *tri_inst = nullptr;
+    float *tri_coef = nullptr, *tri_area = nullptr;
+    int *ie_rows = nullptr;                        // 4 per interior edge
+    double *ie_factor = nullptr;
+    double *tri_ing = nullptr, *ie_ing = nullptr;  // 25 doubles per element
+    // state and vectors (3 per row)
+    double *x = nullptr, *xpre = nullptr, *dx = nullptr, *b = nullptr, *P64 = nullptr;
+    double *r64 = nullptr, *d64[2] = {nullptr, nullptr}, *q64 = nullptr;
+    float *P32 = nullptr, *r32 = nullptr, *e32 = nullptr, *d32[2] = {nullptr, nullptr},
+          *q32 = nullptr;
+    double *partial = nullptr; // 4 per slice
+    InstParams *params = nullptr;
+    InstState *state = nullptr;
+};
+
+// assembly (assemble.hip)
+void launch_elements(const DeviceBatch &d, hipStream_t s);
+void launch_assemble_rows(const DeviceBatch &d, hipStream_t s);
+// solver (pcg.hip)
+void launch_reset_states(const DeviceBatch &d, hipStream_t s);
+void launch_pcg_init64(const DeviceBatch &d, hipStream_t s);
+void launch_pcg_iter64(const DeviceBatch &d, hipStream_t s);
+void launch_residual64(const DeviceBatch &d, int start_inner, hipStream_t s);
+void launch_pcg_iter32(const DeviceBatch &d, hipStream_t s);
+void launch_correct(const DeviceBatch &d, hipStream_t s);
+void launch_update_positions(const DeviceBatch &d, hipStream_t s);
+// measurement helpers: set the solver residual to 1 on every row; one SpMV
+// launch of the solver's precision
+void launch_fill_residual(const DeviceBatch &d, bool fp32, hipStream_t s);
+void launch_spmv_only(const DeviceBatch &d, bool fp32, hipStream_t s);
+
+} // namespace bw

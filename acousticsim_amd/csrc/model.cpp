@@ -1,0 +1,521 @@
+#include "model.h"
+
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <map>
+#include <stdexcept>
+
+namespace bw
+{
+
+// ---------------------------------------------------------------- config --
+
+ClothConfig ClothConfig::from_json(const Json &j)
+{
+    ClothConfig c;
+    c.geometry_type = j.at("geometry_type").as_string();
+    if (c.geometry_type != "regular_triangle")
+        throw std::runtime_error("config: geometry_type \"" + c.geometry_type +
+                                 "\" is not supported (regular_triangle only)");
+    if (j.has("enable_cloth_from_obj") && j.at("enable_cloth_from_obj").as_bool())
+        throw std::runtime_error("config: enable_cloth_from_obj is not supported");
+    auto size = j.at("cloth_size").as_vector();
+    auto sub = j.at("subdivision").as_vector();
+    if (size.size() < 2 || sub.size() < 2)
+        throw std::runtime_error("config: cloth_size and subdivision need 2 values");
+    c.cloth_size[0] = size[0];
+    c.cloth_size[1] = size[1];
+    c.subdivision[0] = (int)sub[0];
+    c.subdivision[1] = (int)sub[1];
+    if (c.subdivision[0] < 1 || c.subdivision[1] < 1 || !(size[0] > 0) || !(size[1] > 0))
+        throw std::runtime_error("config: cloth_size and subdivision must be positive");
+    if (j.has("cloth_init_pos"))
+    {
+        auto p = j.at("cloth_init_pos").as_vector();
+        for (size_t k = 0; k < 3 && k < p.size(); k++)
+            c.init_pos[k] = p[k];
+    }
+    if (j.has("cloth_init_orientation"))
+    {
+        auto p = j.at("cloth_init_orientation").as_vector();
+        for (size_t k = 0; k < 3 && k < p.size(); k++)
+            c.init_euler[k] = p[k];
+    }
+    c.uv_rotation_deg = (float)j.get_double("cloth_uv_rotation", 0);
+    int rot = (int)c.uv_rotation_deg;
+    if (rot != 0 && rot != 45 && rot != 90)
+        throw std::runtime_error("config: cloth_uv_rotation must be 0, 45 or 90");
+    c.density = (float)j.at("cloth_density").as_double();
+    auto sk = j.at("cloth_stretch_stiffness").as_vector();
+    if (sk.size() < 3)
+        throw std::runtime_error("config: cloth_stretch_stiffness needs 3 values");
+    for (int k = 0; k < 3; k++)
+        c.stretch_k[k] = (float)sk[k];
+    auto bk = j.at("cloth_bending_stiffness").as_vector();
+    if (bk.empty())
+        throw std::runtime_error("config: cloth_bending_stiffness is empty");
+    c.bending_k = (float)bk[0];
+    c.rayleigh_a = (float)j.at("rayleigh_damping_a").as_double();
+    c.rayleigh_b = (float)j.at("rayleigh_damping_b").as_double();
+    c.dt = j.at("default_timestep").as_double();
+    if (!(c.dt > 0))
+        throw std::runtime_error("config: default_timestep must be positive");
+    c.gravity = j.at("enable_gravity").as_double() != 0;
+    c.max_iters = j.at("max_iters").as_int();
+    if (j.has("constraints") && j.at("constraints").has("static_point"))
+    {
+        for (auto &p : j.at("constraints").at("static_point").arr)
+        {
+            auto v = p.as_vector();
+            if (v.size() != 2)
+                throw std::runtime_error("config: static_point entries need 2 values");
+            c.pin_uv.push_back({(float)v[0], (float)v[1]});
+        }
+    }
+    std::string prec = j.get_string("precision", "mixed");
+    if (prec == "mixed")
+        c.precision = Precision::Mixed;
+    else if (prec == "fp64")
+        c.precision = Precision::FP64;
+    else
+        throw std::runtime_error("config: precision must be \"mixed\" or \"fp64\"");
+    std::string mode = j.get_string("solver_mode", "tolerance");
+    if (mode == "tolerance")
+        c.solve_mode = SolveMode::Tolerance;
+    else if (mode == "reference")
+        c.solve_mode = SolveMode::Reference;
+    else
+        throw std::runtime_error("config: solver_mode must be \"tolerance\" or \"reference\"");
+    c.tolerance = j.get_double("solver_tolerance", 1e-6);
+    c.solver_max_iters = (int)j.get_double("solver_max_iters", 20000);
+    if (!(c.tolerance >= 0) || c.solver_max_iters < 1)
+        throw std::runtime_error("config: bad solver_tolerance / solver_max_iters");
+    if (c.solve_mode == SolveMode::Reference && c.max_iters < 1)
+        throw std::runtime_error("config: max_iters must be >= 1");
+    return c;
+}
+
+// -------------------------------------------------------------- geometry --
+
+namespace
+{
+
+// Regular grid with alternating diagonals (Triangulator.cpp:141-457).
+// Vertex (i, j) -> id i*(hd+1)+j at (i*uw - width/2, j*uh - height/2).
+// Cell (row, col) -> triangles 2*(row*hd+col)+{0,1}; the diagonal runs
+// lu->rd when (row+col) is even, ru->ld otherwise.  Edge slots per row of
+// cells: hd "top" slots, then left/skew/right interleaved (left of col c+1
+// is the right of col c), then the next row's top slots are this row's
+// bottoms.
+void make_grid(const ClothConfig &c, ClothModel &m)
+{
+    const double height = c.cloth_size[0], width = c.cloth_size[1];
+    const int hd = c.subdivision[0], wd = c.subdivision[1];
+    const double uh = height / hd, uw = width / wd;
+    const int hl = hd + 1, wl = wd + 1;
+    m.nv = wl * hl;
+    m.nt = 2 * hd * wd;
+    m.ne = wl * hd + hl * wd + hd * wd;
+    m.xrest.assign(3 * (size_t)m.nv, 0.0);
+    for (int i = 0; i < wl; i++)
+        for (int j = 0; j < hl; j++)
+        {
+            double *p = &m.xrest[3 * (size_t)(i * hl + j)];
+            p[0] = i * uw - width / 2;
+            p[1] = j * uh - height / 2;
+            p[2] = 0.0;
+        }
+    m.tri.resize(m.nt);
+    m.edges.assign(m.ne, Edge{-1, -1, -1, -1, false});
+    std::vector<char> set(m.ne, 0);
+    auto put = [&](int id, int a, int b, int t0, int t1, bool boundary) {
+        m.edges[id] = Edge{a, b, t0, t1, boundary};
+        set[id] = 1;
+    };
+    const int stride = 3 * hd + 1; // edge slots per row of cells
+    for (int row = 0; row < wd; row++)
+        for (int col = 0; col < hd; col++)
+        {
+            const int lu = row * hl + col, ru = lu + 1, ld = lu + hl, rd = ld + 1;
+            const bool even = ((row + col) & 1) == 0;
+            const int t = 2 * (row * hd + col); // first triangle of the cell
+            if (even)
+            {
+                m.tri[t] = {lu, ld, rd};
+                m.tri[t + 1] = {lu, rd, ru};
+            }
+            else
+            {
+                m.tri[t] = {lu, ld, ru};
+                m.tri[t + 1] = {ld, rd, ru};
+            }
+            const int top = row * stride + col;
+            const int left = row * stride + hd + 2 * col;
+            // diagonal: owned by the two triangles of this cell
+            if (even)
+                put(left + 1, lu, rd, t, t + 1, false);
+            else
+                put(left + 1, ru, ld, t, t + 1, false);
+            // bottom: in the first triangle for even cells, the second for odd
+            {
+                const int own = even ? t : t + 1;
+                const bool last = row == wd - 1;
+                put(top + stride, ld, rd, own, last ? -1 : own + 2 * hd, last);
+            }
+            // right: second triangle of this cell and first of the next cell
+            {
+                const bool last = col == hd - 1;
+                put(left + 2, ru, rd, t + 1, last ? -1 : t + 2, last);
+            }
+            if (row == 0)
+                put(top, lu, ru, even ? t + 1 : t, -1, true);
+            if (col == 0)
+                put(left, lu, ld, t, -1, true);
+        }
+    for (int e = 0; e < m.ne; e++)
+        if (!set[e])
+            throw std::runtime_error("internal: grid edge slot not filled");
+
+    // triangle -> edge ids (Triangulator.cpp:11-31)
+    std::map<std::pair<int, int>, int> key;
+    for (int e = 0; e < m.ne; e++)
+        key[{std::min(m.edges[e].v0, m.edges[e].v1), std::max(m.edges[e].v0, m.edges[e].v1)}] = e;
+    m.tri_e.resize(m.nt);
+    for (int t = 0; t < m.nt; t++)
+        for (int k = 0; k < 3; k++)
+        {
+            int a = m.tri[t][k], b = m.tri[t][(k + 1) % 3];
+            m.tri_e[t][k] = key.at({std::min(a, b), std::max(a, b)});
+        }
+
+    // init transform (cRotUtil::TransformMat, XYZ Euler: Rz*Ry*Rx, then
+    // translation) and UV = rot(uv_rotation) * float(untransformed xy)
+    // (Triangulator.cpp:116-131, 621-658)
+    const bool identity = c.init_pos[0] == 0 && c.init_pos[1] == 0 && c.init_pos[2] == 0 &&
+                          c.init_euler[0] == 0 && c.init_euler[1] == 0 && c.init_euler[2] == 0;
+    double R[3][3];
+    {
+        const double cx = std::cos(c.init_euler[0]), sx = std::sin(c.init_euler[0]);
+        const double cy = std::cos(c.init_euler[1]), sy = std::sin(c.init_euler[1]);
+        const double cz = std::cos(c.init_euler[2]), sz = std::sin(c.init_euler[2]);
+        const double Rx[3][3] = {{1, 0, 0}, {0, cx, -sx}, {0, sx, cx}};
+        const double Ry[3][3] = {{cy, 0, sy}, {0, 1, 0}, {-sy, 0, cy}};
+        const double Rz[3][3] = {{cz, -sz, 0}, {sz, cz, 0}, {0, 0, 1}};
+        double T[3][3];
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++)
+                T[a][b] = (Rz[a][0] * Ry[0][b] + Rz[a][1] * Ry[1][b]) + Rz[a][2] * Ry[2][b];
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++)
+                R[a][b] = (T[a][0] * Rx[0][b] + T[a][1] * Rx[1][b]) + T[a][2] * Rx[2][b];
+    }
+    const float ang = (float)((double)c.uv_rotation_deg / 180.0 * M_PI);
+    const float cr = (float)std::cos((double)ang), sr = (float)std::sin((double)ang);
+    const float msr = -sr;
+    m.uv.resize(2 * (size_t)m.nv);
+    for (int v = 0; v < m.nv; v++)
+    {
+        double *p = &m.xrest[3 * (size_t)v];
+        const float px = (float)p[0], py = (float)p[1];
+        m.uv[2 * v] = cr * px + msr * py;
+        m.uv[2 * v + 1] = sr * px + cr * py;
+        if (!identity)
+        {
+            double o[3];
+            for (int a = 0; a < 3; a++)
+                o[a] = ((R[a][0] * p[0] + R[a][1] * p[1]) + R[a][2] * p[2]) + c.init_pos[a];
+            p[0] = o[0];
+            p[1] = o[1];
+            p[2] = o[2];
+        }
+    }
+}
+
+// Eigen-order helpers used where the reference's rounding must be kept
+inline double dot3(const double *a, const double *b) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
+
+// cMathUtil::CalcTriangleArea returns float (MathUtil2.cpp:344-357)
+float triangle_area(const double *p0, const double *p1, const double *p2)
+{
+    const double a[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+    const double b[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+    const double c[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2],
+                         a[0] * b[1] - a[1] * b[0]};
+    return (float)(0.5 * std::sqrt(dot3(c, c)));
+}
+
+// CalcBaraffStretchCoef (BaraffMaterial.cpp:18-32): the inverse's entries
+// are rounded to float before forming (-a-c, a, c) and (-b-d, b, d)
+void stretch_coef(const double inv[2][2], float *cu, float *cv)
+{
+    const float a = (float)inv[0][0], b = (float)inv[0][1];
+    const float c = (float)inv[1][0], d = (float)inv[1][1];
+    cu[0] = -a - c;
+    cu[1] = a;
+    cu[2] = c;
+    cv[0] = -b - d;
+    cv[1] = b;
+    cv[2] = d;
+}
+
+void rest_data(ClothModel &m)
+{
+    const ClothConfig &c = m.cfg;
+    // lumped mass (cBaraffCloth::InitMass, BaraffCloth.cpp:209-236)
+    m.area.resize(m.nt);
+    m.mass.assign(m.nv, 0.0);
+    for (int t = 0; t < m.nt; t++)
+    {
+        const auto &v = m.tri[t];
+        const float A = triangle_area(&m.xrest[3 * (size_t)v[0]], &m.xrest[3 * (size_t)v[1]],
+                                      &m.xrest[3 * (size_t)v[2]]);
+        m.area[t] = A;
+        const double add = (double)A / 3 * (double)c.density;
+        for (int k = 0; k < 3; k++)
+            m.mass[v[k]] = m.mass[v[k]] + add;
+    }
+    // stretch coefficients (cBaraffMaterial::Init, BaraffMaterial.cpp:33-76)
+    m.coef.resize(12 * (size_t)m.nt);
+    const double cs = std::cos(M_PI / 4), sn = std::sin(M_PI / 4);
+    for (int t = 0; t < m.nt; t++)
+    {
+        const auto &v = m.tri[t];
+        const float *u0 = &m.uv[2 * v[0]], *u1 = &m.uv[2 * v[1]], *u2 = &m.uv[2 * v[2]];
+        const double D[2][2] = {{(double)(u1[0] - u0[0]), (double)(u2[0] - u0[0])},
+                                {(double)(u1[1] - u0[1]), (double)(u2[1] - u0[1])}};
+        const double invdet = 1.0 / (D[0][0] * D[1][1] - D[1][0] * D[0][1]);
+        const double inv[2][2] = {{D[1][1] * invdet, -D[0][1] * invdet},
+                                  {-D[1][0] * invdet, D[0][0] * invdet}};
+        float *cf = &m.coef[12 * (size_t)t];
+        stretch_coef(inv, cf, cf + 3);
+        // Dm^-1 * R45^T with R45 = rot(pi/4)
+        const double rt[2][2] = {{cs, sn}, {-sn, cs}};
+        double r[2][2];
+        for (int a = 0; a < 2; a++)
+            for (int b = 0; b < 2; b++)
+                r[a][b] = inv[a][0] * rt[0][b] + inv[a][1] * rt[1][b];
+        stretch_coef(r, cf + 6, cf + 9);
+    }
+    for (int k = 0; k < 3; k++)
+        m.k[k] = (double)c.stretch_k[k];
+
+    // dihedral rest data (cDihedralMaterial::Init, DihedralBending.cpp:23-89)
+    std::vector<float> elen(m.ne), heron(m.nt);
+    for (int e = 0; e < m.ne; e++)
+    {
+        const double *a = &m.xrest[3 * (size_t)m.edges[e].v0];
+        const double *b = &m.xrest[3 * (size_t)m.edges[e].v1];
+        const double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+        // 4-vector norm (w difference 0), 2-wide packet order
+        elen[e] = (float)std::sqrt((dx * dx + dz * dz) + (dy * dy + 0.0 * 0.0));
+    }
+    for (int t = 0; t < m.nt; t++)
+    {
+        const double a = elen[m.tri_e[t][0]], b = elen[m.tri_e[t][1]], cc = elen[m.tri_e[t][2]];
+        const double p = (a + b + cc) / 2;
+        heron[t] = (float)std::sqrt(p * (p - a) * (p - b) * (p - cc));
+    }
+    const double kb = (double)c.bending_k;
+    const double edge_k = ((kb + kb) + kb) / 3; // mean of a 3-vector of float(k)
+    m.bend_factor.resize(m.interior.size());
+    for (size_t ie = 0; ie < m.interior.size(); ie++)
+    {
+        const Edge &e = m.edges[m.interior[ie]];
+        const float el = elen[m.interior[ie]];
+        const float h0 = (2.0f * heron[e.t0]) / el, h1 = (2.0f * heron[e.t1]) / el;
+        float hs = h0 + h1;
+        if (hs < 1e-7f)
+            hs = 1e-7f;
+        m.bend_factor[ie] = -edge_k * (double)el / (double)hs;
+    }
+    // W = (1 + h a) M + h (b - h) K (BaraffCloth.cpp:274-275)
+    m.sm = 1 + c.dt * (double)c.rayleigh_a;
+    m.sk = c.dt * ((double)c.rayleigh_b - c.dt);
+}
+
+int other_vertex(const std::array<int, 3> &t, int a, int b)
+{
+    for (int v : t)
+        if (v != a && v != b)
+            return v;
+    throw std::runtime_error("internal: degenerate triangle");
+}
+
+} // namespace
+
+void ClothModel::rebuild_pins_from_list()
+{
+    pinned.assign(nv, 0);
+    for (int p : pins)
+        if (p >= 0 && p < nv)
+            pinned[p] = 1;
+}
+
+void ClothModel::reference_ell(std::vector<int> &rp, std::vector<int> &cl) const
+{
+    std::vector<std::vector<int>> rows(nv);
+    for (int v = 0; v < nv; v++)
+        rows[v].push_back(v);
+    for (int e : interior)
+        for (int a : stencil[e])
+            for (int b : stencil[e])
+                rows[a].push_back(b);
+    rp.assign(nv + 1, 0);
+    cl.clear();
+    for (int v = 0; v < nv; v++)
+    {
+        auto &r = rows[v];
+        std::sort(r.begin(), r.end());
+        r.erase(std::unique(r.begin(), r.end()), r.end());
+        cl.insert(cl.end(), r.begin(), r.end());
+        rp[v + 1] = (int)cl.size();
+    }
+}
+
+ClothModel ClothModel::build(const ClothConfig &cfg)
+{
+    ClothModel m;
+    m.cfg = cfg;
+    make_grid(cfg, m);
+    // bending stencils (BaseCloth.cpp:383-404): v2, v3 = the third vertices
+    // of tri0 and tri1
+    m.stencil.resize(m.ne);
+    for (int e = 0; e < m.ne; e++)
+    {
+        const Edge &ed = m.edges[e];
+        if (ed.boundary)
+        {
+            m.stencil[e] = {-1, -1, -1, -1};
+            continue;
+        }
+        m.stencil[e] = {ed.v0, ed.v1, other_vertex(m.tri[ed.t0], ed.v0, ed.v1),
+                        other_vertex(m.tri[ed.t1], ed.v0, ed.v1)};
+        m.interior.push_back(e);
+    }
+    rest_data(m);
+
+    // pins: nearest vertex in UV, first minimum wins (BaseCloth.cpp:151-189)
+    for (auto &target : cfg.pin_uv)
+    {
+        double best = std::numeric_limits<double>::quiet_NaN();
+        int id = -1;
+        for (int v = 0; v < m.nv; v++)
+        {
+            const float du = m.uv[2 * v] - target[0], dv = m.uv[2 * v + 1] - target[1];
+            const double dist = (double)std::sqrt(du * du + dv * dv);
+            if (std::isnan(best) || dist < best)
+            {
+                best = dist;
+                id = v;
+            }
+        }
+        m.pins.push_back(id);
+    }
+    m.rebuild_pins_from_list();
+
+    // block structure: the reference ELL rule plus every triangle pair (the
+    // latter is already inside the rule whenever each triangle has an
+    // interior edge, which holds on every grid)
+    {
+        std::vector<int> rp, cl;
+        m.reference_ell(rp, cl);
+        std::vector<std::vector<int>> rows(m.nv);
+        for (int v = 0; v < m.nv; v++)
+            rows[v].assign(cl.begin() + rp[v], cl.begin() + rp[v + 1]);
+        for (auto &t : m.tri)
+            for (int a : t)
+                for (int b : t)
+                    rows[a].push_back(b);
+        m.row_ptr.assign(m.nv + 1, 0);
+        m.cols.clear();
+        m.diag_slot.resize(m.nv);
+        for (int v = 0; v < m.nv; v++)
+        {
+            auto &r = rows[v];
+            std::sort(r.begin(), r.end());
+            r.erase(std::unique(r.begin(), r.end()), r.end());
+            m.diag_slot[v] = (int)(std::lower_bound(r.begin(), r.end(), v) - r.begin());
+            m.cols.insert(m.cols.end(), r.begin(), r.end());
+            m.row_ptr[v + 1] = (int)m.cols.size();
+        }
+    }
+    auto slot = [&](int r, int c) {
+        auto b = m.cols.begin() + m.row_ptr[r], e = m.cols.begin() + m.row_ptr[r + 1];
+        auto it = std::lower_bound(b, e, c);
+        return (int)(it - m.cols.begin());
+    };
+    // contribution lists (ascending element id inside every block)
+    const int nb = m.nnzb();
+    std::vector<int> scount(nb + 1, 0), bcount(nb + 1, 0);
+    for (int t = 0; t < m.nt; t++)
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                scount[slot(m.tri[t][i], m.tri[t][j]) + 1]++;
+    for (size_t ie = 0; ie < m.interior.size(); ie++)
+    {
+        const auto &s = m.stencil[m.interior[ie]];
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++)
+                bcount[slot(s[i], s[j]) + 1]++;
+    }
+    for (int b = 0; b < nb; b++)
+    {
+        scount[b + 1] += scount[b];
+        bcount[b + 1] += bcount[b];
+    }
+    m.s_ptr = scount;
+    m.b_ptr = bcount;
+    m.s_list.assign(scount[nb], 0);
+    m.b_list.assign(bcount[nb], 0);
+    std::vector<int> sfill(nb, 0), bfill(nb, 0);
+    for (int t = 0; t < m.nt; t++)
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+            {
+                const int s = slot(m.tri[t][i], m.tri[t][j]);
+                m.s_list[m.s_ptr[s] + sfill[s]++] = (t << 4) | (i << 2) | j;
+            }
+    for (size_t ie = 0; ie < m.interior.size(); ie++)
+    {
+        const auto &st = m.stencil[m.interior[ie]];
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++)
+            {
+                const int s = slot(st[i], st[j]);
+                m.b_list[m.b_ptr[s] + bfill[s]++] = ((int)ie << 4) | (i << 2) | j;
+            }
+    }
+    // per-vertex force lists
+    m.vt_ptr.assign(m.nv + 1, 0);
+    m.ve_ptr.assign(m.nv + 1, 0);
+    for (int t = 0; t < m.nt; t++)
+        for (int i = 0; i < 3; i++)
+            m.vt_ptr[m.tri[t][i] + 1]++;
+    for (size_t ie = 0; ie < m.interior.size(); ie++)
+        for (int i = 0; i < 4; i++)
+            m.ve_ptr[m.stencil[m.interior[ie]][i] + 1]++;
+    for (int v = 0; v < m.nv; v++)
+    {
+        m.vt_ptr[v + 1] += m.vt_ptr[v];
+        m.ve_ptr[v + 1] += m.ve_ptr[v];
+    }
+    m.vt_list.assign(m.vt_ptr[m.nv], 0);
+    m.ve_list.assign(m.ve_ptr[m.nv], 0);
+    std::vector<int> tf(m.nv, 0), ef(m.nv, 0);
+    for (int t = 0; t < m.nt; t++)
+        for (int i = 0; i < 3; i++)
+        {
+            const int v = m.tri[t][i];
+            m.vt_list[m.vt_ptr[v] + tf[v]++] = (t << 2) | i;
+        }
+    for (size_t ie = 0; ie < m.interior.size(); ie++)
+        for (int i = 0; i < 4; i++)
+        {
+            const int v = m.stencil[m.interior[ie]][i];
+            m.ve_list[m.ve_ptr[v] + ef[v]++] = ((int)ie << 2) | i;
+        }
+    return m;
+}
+
+} // namespace bw

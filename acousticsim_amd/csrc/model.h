@@ -1,0 +1,96 @@
+// Host-side cloth model: configuration, mesh topology, rest data, the block
+// (ELL) column structure and the per-block contribution lists that let the
+// device assemble K and f by gathering in a fixed order (no atomics).
+#pragma once
+#include <array>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "json_lite.h"
+
+namespace bw
+{
+
+enum class Precision
+{
+    Mixed,
+    FP64
+};
+enum class SolveMode
+{
+    Tolerance,
+    Reference
+};
+
+// Everything cBaraffCloth::Init reads (BaraffCloth.cpp:22-83,
+// BaseCloth.cpp:44-55, Triangulator.cpp:32-139), plus solver keys.
+struct ClothConfig
+{
+    std::string geometry_type = "regular_triangle";
+    double cloth_size[2] = {0.5, 0.5}; // [height (y), width (x)]
+    int subdivision[2] = {20, 20};     // [height_div, width_div]
+    double init_pos[3] = {0, 0, 0};
+    double init_euler[3] = {0, 0, 0};
+    float uv_rotation_deg = 0;
+    float density = 0.2f;    // float member (BaseCloth.h:53)
+    float stretch_k[3] = {}; // tVector3f (BaraffCloth.h:25)
+    float bending_k = 0;     // [0] of the bending vector (BaraffCloth.cpp:38-41)
+    float rayleigh_a = 0, rayleigh_b = 0; // float members (BaraffCloth.h:27)
+    double dt = 0.01;
+    bool gravity = true;
+    int max_iters = 500;
+    std::vector<std::array<float, 2>> pin_uv; // constraints.static_point
+    Precision precision = Precision::Mixed;
+    SolveMode solve_mode = SolveMode::Tolerance;
+    double tolerance = 1e-6;
+    int solver_max_iters = 20000;
+
+    static ClothConfig from_json(const Json &j);
+};
+
+struct Edge
+{
+    int v0, v1, t0, t1;
+    bool boundary;
+};
+
+struct ClothModel
+{
+    ClothConfig cfg;
+    int nv = 0, nt = 0, ne = 0;
+    std::vector<double> xrest;              // 3 nv
+    std::vector<float> uv;                  // 2 nv
+    std::vector<std::array<int, 3>> tri;    // vertex ids
+    std::vector<std::array<int, 3>> tri_e;  // edge ids (id0,id1),(id1,id2),(id2,id0)
+    std::vector<Edge> edges;
+    std::vector<std::array<int, 4>> stencil; // -1 on boundary edges
+    std::vector<int> interior;               // interior edge ids, ascending
+
+    // rest data
+    std::vector<double> mass;  // nv
+    std::vector<float> coef;   // 12 nt: cu, cv (warp/weft), cu, cv (bias)
+    std::vector<float> area;   // nt
+    std::vector<double> bend_factor; // per interior edge: -k_e * e / (h0 + h1)
+    double k[3] = {};          // stiffness as double(float)
+    double sk = 0, sm = 0;     // W = sm M + sk K
+
+    std::vector<int> pins; // static vertices (may repeat)
+    std::vector<uint8_t> pinned;
+
+    // block structure, reference numbering
+    std::vector<int> row_ptr, cols, diag_slot;
+    // per-block contribution lists in ascending element order.  Stretch:
+    // (triangle << 4) | (i << 2) | j ; bending: (interior edge << 4) | (i << 2) | j
+    std::vector<int> s_ptr, s_list, b_ptr, b_list;
+    // per-vertex force lists: (triangle << 2) | i ; (interior edge << 2) | i
+    std::vector<int> vt_ptr, vt_list, ve_ptr, ve_list;
+
+    static ClothModel build(const ClothConfig &cfg);
+    void rebuild_pins_from_list();
+    int nnzb() const { return (int)cols.size(); }
+    // the reference GPU ELL rule (BaraffClothGpu.cpp:438-500)
+    void reference_ell(std::vector<int> &rp, std::vector<int> &cl) const;
+};
+
+} // namespace bw

@@ -1,0 +1,207 @@
+"""Benchmark of the MI355X implicit cloth step.
+
+Workload (default "batch64"): 64 independent square cloths whose vertices per
+side are drawn uniformly from 17..129 (seeded), stiffness scaled
+log-uniformly in [0.1, 10] per cloth, two top corners pinned, +-1 mm seeded
+out-of-plane jitter, dt = 0.01 s, Jacobi-PCG to a 1e-6 relative residual.
+All cloths step together (mixed precision: FP32 ELL SpMV, FP64 reductions,
+FP64 residual refinement).  A "step" is one substep of all 64 cloths; the
+headline value is cloth-steps per second (64 per step).
+
+Reported beside it, in the same run:
+  * step latency p50 / p99 (per batch step)
+  * the plain FP64 form: every cloth stepped on its own, all-FP64, direct
+    launches (the denominator of the batched/mixed speedup)
+  * roofline of the dominant kernel (the SpMV), timed with HIP events on the
+    library's stream: algorithmic bytes per launch / mean launch time vs the
+    8 TB/s HBM peak
+  * the CPU baseline: the C restatement (oracle) of the reference's plain
+    method, one thread, on a bounded sample of the same cloths
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batch64|single257|single21]
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from acousticsim_amd import capi  # noqa: E402
+from acousticsim_amd import workloads as wl  # noqa: E402
+
+METRIC = "cloth steps/s & p50/p99 step latency at 1e-6 PCG residual; SpMV HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def members(workload, precision):
+    if workload == "batch64":
+        out = []
+        for side, scale, seed in wl.batch64_members():
+            c = wl.grid_config(side, dt=wl.BATCH64["dt"], stiffness_scale=scale)
+            c["precision"] = precision
+            out.append((c, wl.jittered_positions(c, seed)))
+        return out
+    if workload == "single257":
+        c = wl.grid_config(257, dt=wl.SINGLE_LARGE["dt"])
+        c["precision"] = precision
+        return [(c, wl.jittered_positions(c, wl.SINGLE_LARGE["seed"]))]
+    if workload == "single21":
+        c = wl.grid_config(21, dt=wl.SINGLE_SMALL["dt"])
+        c["precision"] = precision
+        return [(c, wl.jittered_positions(c, wl.SINGLE_SMALL["seed"]))]
+    raise ValueError(workload)
+
+
+def describe(workload, ms):
+    nv = sum(c["subdivision"][0] + 1 for c, _ in ms)
+    verts = sum((c["subdivision"][0] + 1) * (c["subdivision"][1] + 1) for c, _ in ms)
+    names = {
+        "batch64": "64 cloths, 17..129 vertices/side (seeded), stiffness x[0.1,10], dt 0.01 s",
+        "single257": "one 257x257-vertex cloth, dt 0.005 s",
+        "single21": "one 21x21-vertex cloth, dt 0.01 s",
+    }
+    del nv
+    return {"workload": names[workload], "cloths": len(ms), "vertices": verts,
+            "tolerance": 1e-6, "precision": "mixed (FP32 SpMV, FP64 dots + refinement)"}
+
+
+def time_batch(ms, steps, warmup, graphs=True):
+    b = capi.ClothBatch([c for c, _ in ms], [x for _, x in ms])
+    b.set_graphs(graphs)
+    for _ in range(warmup):
+        b.step(1)
+    lat = []
+    iters = []
+    worst = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s0 = time.perf_counter()
+        b.step(1)
+        lat.append(time.perf_counter() - s0)
+        st = [b.stats(i) for i in range(b.n)]
+        iters.append(np.mean([s["pcg_iterations"] for s in st]))
+        worst = max(worst, max(s["rel_residual"] for s in st))
+    total = time.perf_counter() - t0
+    return b, total, np.array(lat), float(np.mean(iters)), worst
+
+
+def time_plain(ms, steps, warmup):
+    """Every cloth on its own, FP64, direct launches."""
+    objs = []
+    for c, x in ms:
+        c = dict(c, precision="fp64")
+        b = capi.ClothBatch([c], [x])
+        b.set_graphs(False)
+        objs.append(b)
+    for _ in range(warmup):
+        for b in objs:
+            b.step(1)
+    worst = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for b in objs:
+            b.step(1)
+            worst = max(worst, b.stats(0)["rel_residual"])
+    return time.perf_counter() - t0, worst
+
+
+def cpu_baseline(ms, budget_s=20.0):
+    """The oracle (C restatement of the reference's plain method), one thread,
+    tolerance 1e-6, on every 4th cloth: one warm-up step and one timed step
+    each, stopped early if the budget runs out."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_ffi as of
+
+    sample = ms[::4] if len(ms) > 4 else ms
+    t_steps, n_steps = 0.0, 0
+    start = time.perf_counter()
+    for c, x in sample:
+        oc = of.OracleCloth(c)
+        oc.set_state(x, x)
+        oc.step(of.SOLVE_TOLERANCE, 1e-6)
+        t0 = time.perf_counter()
+        oc.step(of.SOLVE_TOLERANCE, 1e-6)
+        t_steps += time.perf_counter() - t0
+        n_steps += 1
+        if time.perf_counter() - start > budget_s:
+            break
+    cpu = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": n_steps / t_steps, "unit": "cloth steps/s", "cores": 1, "kind": "port",
+            "sample": "%d of %d cloths (every 4th), 1 warm-up + 1 timed step each, tol 1e-6, "
+                      "gcc -O2, host %s" % (n_steps, len(ms), cpu)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="batch64", choices=["batch64", "single257", "single21"])
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparison")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build runs on a single MI355X; multi-GPU "
+              "sharding (whole cloths per GPU, no per-iteration communication) is not part "
+              "of it", file=sys.stderr)
+        sys.exit(2)
+
+    ms = members(args.workload, "mixed")
+    print("bench: %s, %d steps after %d warm-up" % (args.workload, args.steps, args.warmup),
+          file=sys.stderr, flush=True)
+    b, total, lat, iters, worst = time_batch(ms, args.steps, args.warmup)
+    n = len(ms)
+    value = n * args.steps / total
+    us, nbytes = b.time_spmv(reps=200)
+    achieved = nbytes / us / 1e3  # GB/s
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cloth steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000 * total / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": describe(args.workload, ms),
+        "p50_ms": 1000 * float(np.percentile(lat, 50)),
+        "p99_ms": 1000 * float(np.percentile(lat, 99)),
+        "pcg_iterations_mean": iters,
+        "max_rel_residual": worst,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_pcg_spmv<float>", "launch_us": us, "alg_bytes": nbytes},
+    }
+    b.close()
+    if not args.no_plain:
+        print("bench: plain FP64 form", file=sys.stderr, flush=True)
+        tp, worst_p = time_plain(ms, args.steps, args.warmup)
+        plain = n * args.steps / tp
+        out["plain_fp64"] = {"value": plain, "unit": "cloth steps/s",
+                             "max_rel_residual": worst_p, "speedup": value / plain}
+    if not args.no_cpu:
+        print("bench: CPU baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(ms)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

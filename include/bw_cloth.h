@@ -1,0 +1,137 @@
+/*
+ * bw_cloth.h -- C ABI of the MI355X-native Baraff-Witkin implicit cloth step.
+ *
+ * Drop-in boundary for the reference's plain cloth method (cloth_type
+ * "fem", class cBaraffCloth).  The reference binds its cloths through the
+ * C++ virtual interface cBaseCloth (src/sim/cloth/BaseCloth.h:19-80) and the
+ * factory BuildCloth (src/sim/cloth/ClothBuilder.cpp:17-54); each entry point
+ * below names the member it replaces.  Plain pointers and sizes only; every
+ * call returns a status (BW_OK = 0) instead of exiting the process, and
+ * bw_last_error() describes the last failure on the calling thread.
+ *
+ * Configuration is the reference's cloth JSON (keys read by cBaraffCloth::Init,
+ * BaraffCloth.cpp:22-83, and cBaseCloth::Init, BaseCloth.cpp:44-55), plus
+ * optional keys of this library:
+ *   "precision":        "mixed" (default; FP32 ELL SpMV, FP64 reductions and
+ *                        FP64 residual refinement) or "fp64" (all FP64)
+ *   "solver_tolerance": relative residual ||b - W dx|| / ||b|| (default 1e-6)
+ *   "solver_max_iters": PCG iteration cap per solve (default 20000)
+ *   "solver_mode":      "tolerance" (default) or "reference" (exactly
+ *                        max_iters-1 FP64 PCG iterations, BaraffCloth.cpp:529)
+ * Positions are 3*V doubles, xyz interleaved, in the reference's vertex order.
+ */
+#ifndef BW_CLOTH_H
+#define BW_CLOTH_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BW_OK 0
+#define BW_ERR_ARG -1      /* bad argument or configuration */
+#define BW_ERR_DEVICE -2   /* HIP runtime error */
+#define BW_ERR_NUMERIC -3  /* NaN/Inf or solver breakdown */
+#define BW_ERR_STATE -4    /* call not valid in this state */
+#define BW_ERR_CAPACITY -5 /* over a size or memory budget */
+
+typedef struct bw_cloth bw_cloth;
+typedef struct bw_batch bw_batch;
+
+typedef struct bw_step_stats
+{
+    int pcg_iterations;   /* PCG iterations of the last step (all passes) */
+    int outer_iterations; /* FP64 residual-refinement passes (1 in fp64) */
+    double rel_residual;  /* final FP64 true residual / ||b|| on free dofs */
+    double step_us;       /* host wall time of the last step */
+} bw_step_stats;
+
+/* ---- single cloth: the cBaseCloth interface --------------------------- */
+
+/* cBaraffCloth::Init(const Json::Value&) (BaraffCloth.cpp:22-83) via the
+   factory key "fem" (ClothBuilder.cpp:17-54).  x0 (3V, may be NULL) is then
+   applied as SetPos(x0) + SetPrePos(x0) (BaseCloth.cpp:97-109). */
+int bw_create(const char *json_config, const double *x0, bw_cloth **out);
+/* destructor of the object (shared_ptr release in the reference) */
+void bw_destroy(bw_cloth *c);
+
+/* cBaraffCloth::UpdatePos(double dt) (BaraffCloth.cpp:90-163); like the
+   reference, dt is ignored and "default_timestep" is used. */
+int bw_update_pos(bw_cloth *c, double dt);
+/* cBaseObject::Reset -> cBaseCloth::Reset (BaseCloth.cpp:57-64) */
+int bw_reset(bw_cloth *c);
+/* cBaseCloth::GetPos (BaseCloth.cpp:112): copies 3V doubles */
+int bw_get_pos(const bw_cloth *c, double *x);
+/* mXpre (BaseCloth.h:63) */
+int bw_get_pre_pos(const bw_cloth *c, double *x);
+/* velocity (mXcur - mXpre) / dt (BaraffCloth.cpp:283-284) */
+int bw_get_velocity(const bw_cloth *c, double *v);
+/* cBaseCloth::SetPos / SetPrePos (BaseCloth.cpp:97-109) */
+int bw_set_pos(bw_cloth *c, const double *x);
+int bw_set_pre_pos(bw_cloth *c, const double *x);
+/* cBaseCloth::GetInitPos (BaseCloth.cpp:570) */
+int bw_get_init_pos(const bw_cloth *c, double *x);
+/* cBaseCloth::GetDefaultTimestep (BaseCloth.h:39) */
+double bw_get_default_timestep(const bw_cloth *c);
+/* cBaseObject::GetNumOfVertices/Triangles/Edges (BaseObject.cpp) */
+int bw_num_vertices(const bw_cloth *c);
+int bw_num_triangles(const bw_cloth *c);
+int bw_num_edges(const bw_cloth *c);
+/* Get/Add/ClearConstraintStaticVertices (BaseCloth.cpp:553-568); the get
+   returns the count and copies at most cap ids */
+int bw_get_constraint_static_vertices(const bw_cloth *c, int *ids, int cap);
+int bw_add_constraint_static_vertices(bw_cloth *c, const int *ids, int n);
+int bw_clear_constraint_static_vertices(bw_cloth *c);
+/* mesh readback: tTriangle ids (T x 3) and tEdge {id0, id1, tri0, tri1,
+   boundary} (E x 5) as built by cTriangulator (Triangulator.cpp:141-390) */
+int bw_get_triangles(const bw_cloth *c, int *tri3);
+int bw_get_edges(const bw_cloth *c, int *edge5);
+/* ELL column structure in reference numbering: row v lists, ascending,
+   {v} U every interior-edge stencil containing v, as
+   cBaraffClothGpu::InitELLMatrix builds it (BaraffClothGpu.cpp:438-500), with
+   no 32-column limit.  row_ptr has V+1 entries; returns the block count, and
+   only the count when row_ptr or cols is NULL. */
+int bw_get_ell_map(const bw_cloth *c, int *row_ptr, int *cols);
+/* counters of the last step (no reference counterpart: gProfRec phase times
+   are ImGui-only there, BaraffCloth.cpp:393-396) */
+int bw_get_last_stats(const bw_cloth *c, bw_step_stats *st);
+
+/* Assemble the implicit system at the current state without solving and
+   copy it out: W (nnzb x 3 x 3, row-major blocks in the order of
+   bw_get_ell_map) and rhs (3V), both FP64.  Rows and columns of static
+   vertices are zero, as the reference's solve filters them
+   (BaraffCloth.cpp:469-506, 515-522).  No reference counterpart: the
+   reference forms W and b inside SolveForDx (BaraffCloth.cpp:262-298) and
+   never exposes them.  Returns nnzb. */
+int bw_assemble_system(bw_cloth *c, double *W, double *rhs);
+
+/* last error message of the calling thread */
+const char *bw_last_error(void);
+
+/* ---- batched stepping (no reference counterpart: many independent cloths
+   of uneven size stepped together, one launch sequence per phase) -------- */
+
+/* n cloths from n JSON configs; x0s[i] may be NULL */
+int bw_batch_create(const char *const *json_configs, const double *const *x0s,
+                    int n, bw_batch **out);
+void bw_batch_destroy(bw_batch *b);
+int bw_batch_size(const bw_batch *b);
+/* nsteps substeps of every member */
+int bw_batch_step(bw_batch *b, int nsteps);
+int bw_batch_num_vertices(const bw_batch *b, int i);
+int bw_batch_get_pos(const bw_batch *b, int i, double *x);
+int bw_batch_set_state(bw_batch *b, int i, const double *x, const double *xpre);
+int bw_batch_get_stats(const bw_batch *b, int i, bw_step_stats *st);
+/* use HIP graph replay for the per-iteration launch chains (default on) */
+int bw_batch_set_graphs(bw_batch *b, int enable);
+/* Measurement (no reference counterpart): time `reps` back-to-back launches
+   of the solver's SpMV kernel (the batch's precision, every cloth active) with
+   HIP events on the batch's stream.  avg_us gets the mean time per launch,
+   alg_bytes the algorithmic bytes one launch moves: 36+4 (FP32) or 72+4
+   (FP64) bytes per true 3x3 block, plus five 3-vectors per row (r, P and the
+   previous direction read; the new direction and W d written). */
+int bw_batch_time_spmv(bw_batch *b, int reps, double *avg_us, double *alg_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,266 @@
+"""GPU parity of the HIP cloth step against the oracle and the reference's
+golden vectors, through the C ABI.
+
+Tolerances (stated per test):
+  * topology, pins, ELL column structure: identical
+  * assembled FP64 W and rhs: equal to the oracle (bit-exact restatement of
+    the reference) to a few ulp; bending Hessian entries are float-rounded in
+    the reference, so they may differ by one float ulp where the device's
+    atan2/sin differ by an ulp
+  * reference-semantics solve (exactly max_iters-1 PCG iterations): positions
+    within 1e-9 of the step displacement of the golden vectors
+  * 1e-6 tolerance solve: one step from any state along the reference
+    trajectory within 1e-4 of that step's displacement; 100 steps of a
+    non-buckling configuration within 1e-4 of the total displacement; every
+    step's FP64 true residual <= 1e-6
+"""
+import copy
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_ffi as of
+from acousticsim_amd import workloads as wl
+from conftest import load_golden
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def capi():
+    from acousticsim_amd import capi as c
+    c.lib()
+    return c
+
+
+def _mode(conf, precision, mode="tolerance", tol=1e-6):
+    c = copy.deepcopy(conf)
+    c["precision"] = precision
+    c["solver_mode"] = mode
+    c["solver_tolerance"] = tol
+    return c
+
+
+@pytest.mark.parametrize("name", ["tiny3", "grid5_rot45", "rect7x12", "grid21"])
+def test_topology_pins_and_ell_structure(capi, name):
+    say("case topology", name)
+    g = load_golden(name)
+    cl = capi.BaraffClothHip(g["config"], g["x0"])
+    assert np.array_equal(cl.triangles().ravel(), g["triangles"])
+    assert np.array_equal(cl.edges().ravel(), g["edges"])
+    assert np.array_equal(cl.GetConstraintStaticVertices(), g["pins"])
+    oc = of.OracleCloth(g["config"])
+    rp, cols = cl.ell_map()
+    orp, ocols = oc.ref_ell()
+    assert np.array_equal(rp, orp) and np.array_equal(cols, ocols)
+    assert np.array_equal(cl.GetInitPos(), oc.rest_pos())
+
+
+@pytest.mark.parametrize("name", ["grid5_rot45", "rect7x12", "grid21"])
+def test_assembled_system_matches_oracle(capi, name):
+    say("case assemble", name)
+    g = load_golden(name)
+    conf = _mode(g["config"], "fp64")
+    cl = capi.BaraffClothHip(conf, g["x0"])
+    W, b = cl.assemble_system()
+    oc = of.OracleCloth(g["config"])
+    oc.set_state(g["x0"], g["x0"])
+    oc.assemble()
+    Wo, bo = oc.W(), oc.rhs()
+    rp, cols = oc.pattern()
+    pinned = np.zeros(oc.nv, bool)
+    pinned[oc.pins()] = True
+    rows = np.repeat(np.arange(oc.nv), np.diff(rp))
+    free = ~pinned[rows] & ~pinned[cols]
+    assert np.all(W[~free] == 0)
+    scale = np.abs(Wo[free]).max()
+    err = np.abs(W[free] - Wo[free]).max()
+    say("  W max abs err %.3e (scale %.3e), exact %d/%d" % (
+        err, scale, int((W[free] == Wo[free]).sum()), W[free].size))
+    assert err <= 1e-12 * scale
+    fr = np.repeat(~pinned, 3)
+    berr = np.abs(b[fr] - bo[fr]).max()
+    assert berr <= 1e-12 * np.abs(bo[fr]).max()
+    assert np.all(b[~fr] == 0)
+
+
+@pytest.mark.parametrize("name", ["tiny3", "grid5_rot45", "rect7x12", "grid21"])
+def test_reference_semantics_trajectory(capi, name):
+    """Exactly max_iters-1 FP64 PCG iterations per step, as the reference.
+    The first step agrees to 1e-9 of the displacement.  Later steps of the
+    21x21 cloth inherit the reference's own sensitivity: there a 1e-15
+    perturbation of x0 grows to ~6e-6 of the displacement by step 10, so
+    rounding-level differences (device atan2, reduction order) are held to
+    1e-4 after the first step."""
+    g = load_golden(name)
+    conf = _mode(g["config"], "fp64", "reference")
+    say("case reference-mode", name)
+    cl = capi.BaraffClothHip(conf, g["x0"])
+    steps = [int(s) for s in g["steps"]]
+    horizon = min(max(steps), 10)
+    for s in range(1, horizon + 1):
+        cl.UpdatePos(0.01)
+        st = cl.last_stats()
+        assert st["pcg_iterations"] <= g["config"]["max_iters"] - 1
+        if s in steps:
+            x = cl.GetPos()
+            xr = g["x_step%d" % s]
+            disp = np.abs(xr - g["x0"]).max()
+            err = np.abs(x - xr).max()
+            say("  step %d err/disp %.3e" % (s, err / disp))
+            assert err <= (1e-9 if s == 1 or name != "grid21" else 1e-4) * disp
+
+
+def test_grid257_reference_semantics_first_step(capi):
+    """257x257: one 499-iteration FP64 step against the reference's own.  The
+    reference's iterate is not converged at this size (relative residual
+    ~1.8e-4), and an unconverged CG iterate depends on summation order at the
+    1e-4 level, so the bound is 1e-3 of the displacement."""
+    g = load_golden("grid257")
+    conf = g["config"]
+    x0 = wl.jittered_positions(conf, int(g["seed"]))
+    sel = g["vertex_subset"]
+    say("case grid257 reference-mode")
+    cl = capi.BaraffClothHip(_mode(conf, "fp64", "reference"), x0)
+    cl.UpdatePos()
+    st = cl.last_stats()
+    assert st["pcg_iterations"] == conf["max_iters"] - 1
+    x = cl.GetPos().reshape(-1, 3)[sel].ravel()
+    xr = g["x_step1"]
+    disp = np.abs(xr - g["x0_subset"]).max()
+    err = np.abs(x - xr).max()
+    say("  err/disp %.3e, residual %.3e" % (err / disp, st["rel_residual"]))
+    assert 1e-5 < st["rel_residual"] < 1e-3
+    assert err <= 1e-3 * disp
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_one_step_along_reference_trajectory(capi, precision):
+    """configs[0] (21x21): from the reference's states at steps 0, 10, 50, 99
+    one 1e-6-tolerance step agrees with the reference's own step."""
+    g = load_golden("grid21")
+    conf = g["config"]
+    ref = of.OracleCloth(conf)
+    x0 = g["x0"]
+    ref.set_state(x0, x0)
+    states = {}
+    for s in range(100):
+        if s in (0, 10, 50, 99):
+            states[s] = (ref.pos(), ref.pre_pos())
+        ref.step(of.SOLVE_REFERENCE)
+        if s in states:
+            states[s] = states[s] + (ref.pos(),)
+    assert np.array_equal(ref.pos(), g["x_step100"])
+    cl = capi.BaraffClothHip(_mode(conf, precision), x0)
+    for s, (x, xp, xr) in states.items():
+        say("case one-step", precision, "from step", s)
+        cl.SetPos(x)
+        cl.SetPrePos(xp)
+        cl.UpdatePos()
+        st = cl.last_stats()
+        y = cl.GetPos()
+        dstep = np.abs(xr - x).max()
+        err = np.abs(y - xr).max()
+        say("  err/|dx| %.3e iters %d outer %d residual %.2e" % (
+            err / dstep, st["pcg_iterations"], st["outer_iterations"], st["rel_residual"]))
+        assert st["rel_residual"] <= 1e-6
+        assert err <= 1e-4 * dstep
+
+
+def _top_edge_variant():
+    c = wl.grid_config(21)
+    c["constraints"] = {"static_point": [[-0.25 + 0.025 * k, 0.25] for k in range(21)]}
+    return c
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_hundred_steps_non_buckling_configuration(capi, precision):
+    """21x21 hanging from its whole top edge (no buckling, so the trajectory is
+    not chaotic): 100 steps at 1e-6 against the reference-semantics oracle."""
+    conf = _top_edge_variant()
+    x0 = wl.jittered_positions(conf, 1)
+    ref = of.OracleCloth(conf)
+    ref.set_state(x0, x0)
+    for _ in range(100):
+        ref.step(of.SOLVE_REFERENCE)
+    say("case 100 steps top-edge", precision)
+    cl = capi.BaraffClothHip(_mode(conf, precision), x0)
+    worst = 0.0
+    for s in range(100):
+        cl.UpdatePos()
+        worst = max(worst, cl.last_stats()["rel_residual"])
+    x = cl.GetPos()
+    xr = ref.pos()
+    disp = np.abs(xr - x0).max()
+    err = np.abs(x - xr).max()
+    say("  err/disp %.3e rel-norm %.3e worst residual %.2e" % (
+        err / disp, np.linalg.norm(x - xr) / np.linalg.norm(xr), worst))
+    assert worst <= 1e-6
+    assert err <= 1e-4 * disp
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_configs0_ten_steps(capi, precision):
+    """configs[0] (21x21, corners pinned) for 10 steps.  Its buckling makes the
+    reference's own trajectory chaotic (a 1e-15 perturbation of x0 reaches
+    ~2e-3 of the positions by step 100), so 1e-6-tolerance trajectories are
+    compared over the first 10 steps, where the oracle's own 1e-6 solve stays
+    within 1.2e-5 of the reference."""
+    g = load_golden("grid21")
+    conf = g["config"]
+    x0 = g["x0"]
+    say("case configs0 10 steps", precision)
+    cl = capi.BaraffClothHip(_mode(conf, precision), x0)
+    for _ in range(10):
+        cl.UpdatePos()
+        assert cl.last_stats()["rel_residual"] <= 1e-6
+    x = cl.GetPos()
+    xr = g["x_step10"]
+    err = np.abs(x - xr).max()
+    disp = np.abs(xr - x0).max()
+    say("  err/disp %.3e" % (err / disp))
+    assert err <= 1e-4 * disp
+
+
+def test_reset_and_velocity(capi):
+    conf = wl.grid_config(9)
+    x0 = wl.jittered_positions(conf, 4)
+    say("case reset/velocity")
+    cl = capi.BaraffClothHip(conf, x0)
+    cl.UpdatePos()
+    cl.UpdatePos()
+    v = cl.GetVelocity()
+    assert np.allclose(v, (cl.GetPos() - cl.GetPrePos()) / conf["default_timestep"])
+    cl.Reset()
+    assert np.array_equal(cl.GetPos(), cl.GetInitPos())
+    assert np.array_equal(cl.GetPrePos(), cl.GetInitPos())
+
+
+def test_constraint_edit_pins_vertex(capi):
+    conf = wl.grid_config(9)
+    conf["constraints"] = {"static_point": []}
+    x0 = wl.jittered_positions(conf, 4)
+    say("case constraint edit")
+    cl = capi.BaraffClothHip(conf, x0)
+    assert cl.GetConstraintStaticVertices().size == 0
+    cl.AddConstraintStaticVertices([8, 80])
+    cl.UpdatePos()
+    x = cl.GetPos().reshape(-1, 3)
+    assert np.array_equal(x[[8, 80]], x0.reshape(-1, 3)[[8, 80]])
+    assert np.abs(x - x0.reshape(-1, 3)).max() > 0
+    cl.ClearConstraintStaticVertices()
+    assert cl.GetConstraintStaticVertices().size == 0
+
+
+def test_bad_config_returns_error_not_exit(capi):
+    conf = wl.grid_config(5)
+    del conf["cloth_density"]
+    with pytest.raises(capi.BwError) as e:
+        capi.BaraffClothHip(conf)
+    assert e.value.code == -1 and "cloth_density" in str(e.value)
