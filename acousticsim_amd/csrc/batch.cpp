@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 
@@ -55,6 +56,11 @@ Batch::Batch(std::vector<ClothModel> models) : models_(std::move(models))
     mixed_ = nmixed != 0;
     stats_.resize(models_.size());
     check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    // BW_GRAPHS=0 launches the iteration chains directly (profilers that do
+    // not follow graph launches)
+    if (const char *g = std::getenv("BW_GRAPHS"))
+        graphs_disabled_ = std::string(g) == "0";
+    use_graphs_ = !graphs_disabled_;
     check(hipHostMalloc((void **)&host_state_, sizeof(InstState) * models_.size()),
           "hipHostMalloc");
     build_layout();
@@ -87,7 +93,7 @@ void Batch::build_layout()
     {
         const ClothModel &m = models_[i];
         slice_base_[i] = slices;
-        nslices_[i] = (m.nv + SLICE - 1) / SLICE;
+        nslices_[i] = ((m.nv + SLICE - 1) / SLICE + BW_SB - 1) / BW_SB * BW_SB;
         row_base_[i] = slices * SLICE;
         slices += nslices_[i];
         vert_base[i] = nvert;
@@ -227,6 +233,9 @@ void Batch::build_layout()
         P.max_iter = P.ref_mode ? m.cfg.max_iters - 1 : m.cfg.solver_max_iters;
         P.mixed = mixed_ ? 1 : 0;
         P.max_outer = 30;
+        P.eta_min = 2e-5;
+        P.blk_begin = slice_base_[i] / BW_SB;
+        P.blk_count = nslices_[i] / BW_SB;
         P.slice_begin = slice_base_[i];
         P.slice_count = nslices_[i];
     }
@@ -234,6 +243,11 @@ void Batch::build_layout()
     d_.slice_off = dalloc<int>(slices);
     d_.slice_w = dalloc<int>(slices);
     d_.slice_inst = dalloc<int>(slices);
+    d_.n_blocks_wg = slices / BW_SB;
+    d_.blk_inst = dalloc<int>(d_.n_blocks_wg);
+    std::vector<int> blk_inst(d_.n_blocks_wg);
+    for (int s = 0; s < slices; s += BW_SB)
+        blk_inst[s / BW_SB] = slice_inst[s];
     d_.row_vertex = dalloc<int>(d_.n_rows);
     d_.row_w = dalloc<int>(d_.n_rows);
     d_.row_pinned = dalloc<uint8_t>(d_.n_rows);
@@ -282,13 +296,14 @@ void Batch::build_layout()
         d_.d32[1] = dalloc<float>(nv3);
         d_.q32 = dalloc<float>(nv3);
     }
-    d_.partial = dalloc<double>(4 * (size_t)slices);
+    d_.partial = dalloc<double>(PARTS * (size_t)d_.n_blocks_wg);
     d_.params = dalloc<InstParams>(n);
     d_.state = dalloc<InstState>(n);
 
     upload(d_.slice_off, slice_off);
     upload(d_.slice_w, slice_w);
     upload(d_.slice_inst, slice_inst);
+    upload(d_.blk_inst, blk_inst);
     upload(d_.row_vertex, row_vertex);
     upload(d_.row_w, row_w);
     upload(d_.row_pinned, row_pinned);
@@ -468,9 +483,20 @@ bool pred_done(const InstState &s) { return s.done != 0; }
 bool pred_outer_done(const InstState &s) { return s.outer_done != 0; }
 
 // chunk sizes are multiples of 16 iterations so few graphs get built
+// BW_MAX_CHUNK caps the iterations queued between host checks (default 512)
+int max_chunk()
+{
+    static const int v = [] {
+        const char *e = std::getenv("BW_MAX_CHUNK");
+        const int c = e ? std::atoi(e) : 512;
+        return std::max(16, std::min(c, 4096));
+    }();
+    return v;
+}
+
 int chunk_for(int estimate)
 {
-    int c = std::max(16, std::min(estimate, 512));
+    int c = std::max(16, std::min(estimate, max_chunk()));
     return ((c + 15) / 16) * 16;
 }
 } // namespace
@@ -496,10 +522,10 @@ void Batch::step(int nsteps)
                     break;
                 run_chunk(false, chunk);
                 done_iters += chunk;
-                chunk = 32;
+                chunk = std::min(32, max_chunk());
             }
             iters_this_step = done_iters;
-            launch_residual64(d_, 0, stream_);
+            launch_residual64(d_, 2, stream_);
         }
         else
         {
@@ -517,7 +543,7 @@ void Batch::step(int nsteps)
                         break;
                     run_chunk(true, chunk);
                     iters_this_step += chunk;
-                    chunk = 32;
+                    chunk = std::min(32, max_chunk());
                     read_states();
                 }
                 first_pass = false;

@@ -41,7 +41,7 @@ public:
     void set_pos(int i, const double *x);
     void set_pre_pos(int i, const double *x);
     const StepStats &stats(int i) const { return stats_[i]; }
-    void set_graphs(bool on) { use_graphs_ = on; }
+    void set_graphs(bool on) { use_graphs_ = on && !graphs_disabled_; }
     // mean microseconds per SpMV launch over reps launches; algorithmic bytes
     double time_spmv(int reps, double *alg_bytes);
 
@@ -65,6 +65,7 @@ private:
     InstState *host_state_ = nullptr; // pinned
     bool mixed_ = false, any_ref_ = false;
     bool use_graphs_ = true;
+    bool graphs_disabled_ = false; // BW_GRAPHS=0 in the environment
     int last_iters_ = 0;
     // cached chunk graphs: key (fp32, iterations)
     struct GraphEntry

@@ -17,6 +17,13 @@ namespace bw
 {
 
 constexpr int SLICE = 64;
+// slices per solver workgroup (SB * 64 threads); a cloth's slices are padded
+// to a multiple of it so that no workgroup spans two cloths
+#ifndef BW_SB
+#define BW_SB 4
+#endif
+// FP64 partial sums per workgroup
+constexpr int PARTS = 5;
 
 struct InstParams
 {
@@ -26,15 +33,19 @@ struct InstParams
     int ref_mode;     // 1: reference semantics (no tolerance stop)
     int mixed;        // 1: FP32 inner PCG + FP64 refinement
     int max_outer;    // refinement passes (mixed)
+    double eta_min;   // deepest relative reduction asked of one inner solve
     int slice_begin, slice_count;
+    int blk_begin, blk_count; // solver workgroups of the cloth
 };
 
+// Per-cloth solver state.  Each field has one writing kernel (see pcg.hip).
 struct InstState
 {
-    double rho, alpha, beta, bb, rr, dq, rr64, inner_tol2;
-    int iter, done, outer, outer_done, total_iter, breakdown;
-    unsigned int count;
-    int pad;
+    double rho[2];      // r.(P r) of iterations of parity 0 / 1
+    double rr, bb, rr64, inner_tol2;
+    int it_a;           // next iteration (written by the update kernel)
+    int it_b;           // current iteration (written by the SpMV kernel)
+    int done, outer, outer_done, total_iter, breakdown, pad;
 };
 
 struct DeviceBatch
@@ -42,7 +53,9 @@ struct DeviceBatch
     int n_inst = 0, n_slices = 0, n_rows = 0;
     long long n_entries = 0;
     int n_tri = 0, n_ie = 0, n_vert = 0, n_blocks = 0;
-    // per slice / row
+    int n_blocks_wg = 0; // solver workgroups
+    // per solver workgroup / slice / row
+    int *blk_inst = nullptr;
     int *slice_off = nullptr, *slice_w = nullptr, *slice_inst = nullptr;
     int *row_vertex = nullptr, *row_w = nullptr;
     uint8_t *row_pinned = nullptr;
@@ -70,7 +83,7 @@ struct DeviceBatch
     double *r64 = nullptr, *d64[2] = {nullptr, nullptr}, *q64 = nullptr;
     float *P32 = nullptr, *r32 = nullptr, *e32 = nullptr, *d32[2] = {nullptr, nullptr},
           *q32 = nullptr;
-    double *partial = nullptr; // 4 per slice
+    double *partial = nullptr; // PARTS per solver workgroup
     InstParams *params = nullptr;
     InstState *state = nullptr;
 };
@@ -82,7 +95,8 @@ void launch_assemble_rows(const DeviceBatch &d, hipStream_t s);
 void launch_reset_states(const DeviceBatch &d, hipStream_t s);
 void launch_pcg_init64(const DeviceBatch &d, hipStream_t s);
 void launch_pcg_iter64(const DeviceBatch &d, hipStream_t s);
-void launch_residual64(const DeviceBatch &d, int start_inner, hipStream_t s);
+// mode 0: FP64 solve start; 1: refinement pass; 2: counters only
+void launch_residual64(const DeviceBatch &d, int mode, hipStream_t s);
 void launch_pcg_iter32(const DeviceBatch &d, hipStream_t s);
 void launch_correct(const DeviceBatch &d, hipStream_t s);
 void launch_update_positions(const DeviceBatch &d, hipStream_t s);

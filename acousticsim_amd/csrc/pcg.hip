@@ -1,21 +1,31 @@
 // Batched Jacobi-PCG on the sliced-ELL matrix (cBaraffCloth::Solve,
 // BaraffCloth.cpp:507-611, restated for the device).
 //
-// One wavefront per 64-row slice, one lane per block row.  Each cloth of the
-// batch owns whole slices and its own scalars (InstState); converged cloths
-// skip their slices.  Dot products are reduced deterministically: each slice
-// reduces its 64 lanes in a fixed shuffle tree and writes one FP64 partial;
-// the last slice of a cloth to arrive sums that cloth's partials in a fixed
-// order and updates its scalars.  A cloth's results therefore do not depend
-// on what else is in the batch.
+// Work decomposition: one workgroup = SB slices of 64 rows (one wavefront
+// per slice, one lane per block row).  Each cloth owns whole workgroups, so
+// workgroups never mix cloths.
 //
-// Per iteration two kernels run:
-//   k_pcg_spmv   d = P r + beta d_prev (formed on the fly for every gathered
-//                column, written for the own row), q = W d, partial d.q;
-//                the last slice sets alpha = rho / (d.q)
-//   k_pcg_update x += alpha d, r -= alpha q, partials r.(P r) and r.r; the
-//                last slice sets beta, rho and the convergence flag
-// The direction buffer alternates between two arrays by iteration parity.
+// No kernel communicates across workgroups.  Every kernel writes one FP64
+// partial sum per workgroup for each dot product it forms (a fixed shuffle
+// tree inside the wavefront, then a fixed order across the SB wavefronts);
+// the NEXT kernel starts by summing the cloth's partials in a fixed order,
+// redundantly in every workgroup of the cloth, and derives the scalars from
+// them.  The kernel boundary makes the partials visible, there are no atomics
+// and no fences, and a cloth's arithmetic does not depend on what else is in
+// the batch.  Per-cloth fields of InstState are each written by one kernel
+// and read only by later kernels (writes come from the cloth's first
+// workgroup).
+//
+// Per PCG iteration k:
+//   k_pcg_spmv(k)   prologue (k >= 1): rho_k = sum r.(P r), rr_k = sum r.r
+//                   from the partials of k_pcg_update(k-1); beta = rho_k /
+//                   rho_{k-1}; stop if converged.  Body: d_k = P r + beta
+//                   d_{k-1} (formed on the fly for every gathered column,
+//                   stored for the own row), q = W d_k, partial d_k.q.
+//   k_pcg_update(k) prologue: alpha = rho_k / sum d.q (stop on d.q == 0, as
+//                   the reference does, :547-551).  Body: x += alpha d,
+//                   r -= alpha q, partials r.(P r) and r.r.
+// Direction buffers alternate by iteration parity.
 //
 // Precision: "fp64" runs everything in FP64.  "mixed" runs the inner PCG on
 // the FP32 copy of W with FP32 vectors but FP64 dots and scalars, inside an
@@ -28,6 +38,16 @@ namespace bw
 namespace
 {
 
+constexpr int NP = PARTS; // partial components per workgroup
+enum Part
+{
+    P_DQ = 0,   // d.Wd
+    P_RZ = 1,   // r.(P r)
+    P_RR = 2,   // r.r (solver precision)
+    P_BB = 3,   // b.b
+    P_RR64 = 4, // FP64 residual r64.r64
+};
+
 __device__ inline double wave_sum(double v)
 {
     for (int off = 32; off > 0; off >>= 1)
@@ -35,26 +55,50 @@ __device__ inline double wave_sum(double v)
     return v;
 }
 
-// last-arrival protocol: returns true in the last slice of the cloth
-__device__ inline bool last_arrival(InstState *st, int nslices)
+// block-wide sums of up to 5 values, fixed order; result in every thread
+template <int SB, int N>
+__device__ inline void block_sum(double (&v)[N], double *lds)
 {
-    __threadfence();
-    unsigned int prev = 0;
-    if (threadIdx.x == 0)
-        prev = atomicAdd(&st->count, 1u);
-    prev = __shfl(prev, 0, 64);
-    const bool last = prev == (unsigned int)(nslices - 1);
-    if (last)
-        __threadfence();
-    return last;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int j = 0; j < N; j++)
+        v[j] = wave_sum(v[j]);
+    if (SB == 1)
+        return;
+    __syncthreads();
+    if (lane == 0)
+        for (int j = 0; j < N; j++)
+            lds[wave * 8 + j] = v[j];
+    __syncthreads();
+    for (int j = 0; j < N; j++)
+    {
+        double s = lds[j];
+        for (int w = 1; w < SB; w++)
+            s += lds[w * 8 + j];
+        v[j] = s;
+    }
+    __syncthreads();
 }
 
-__device__ inline double reduce_partials(const double *part, int comp, int sb, int n)
+// write this workgroup's partials (values already block-summed)
+template <int N>
+__device__ inline void write_partials(double *part, int blk, const int (&comp)[N], const double (&v)[N])
 {
-    double acc = 0.0;
-    for (int i = threadIdx.x; i < n; i += 64)
-        acc += part[4 * (size_t)(sb + i) + comp];
-    return wave_sum(acc);
+    if (threadIdx.x == 0)
+        for (int j = 0; j < N; j++)
+            part[NP * (size_t)blk + comp[j]] = v[j];
+}
+
+// sum the cloth's partials (fixed order), result in every thread
+template <int SB, int N>
+__device__ inline void reduce_cloth(const double *part, int b0, int nb, const int (&comp)[N],
+                                    double (&out)[N], double *lds)
+{
+    for (int j = 0; j < N; j++)
+        out[j] = 0.0;
+    for (int i = threadIdx.x; i < nb; i += SB * 64)
+        for (int j = 0; j < N; j++)
+            out[j] += part[NP * (size_t)(b0 + i) + comp[j]];
+    block_sum<SB, N>(out, lds);
 }
 
 template <typename T>
@@ -63,41 +107,71 @@ __device__ inline T dir_value(T P, T r, T dold, T beta, bool first)
     return first ? P * r : P * r + beta * dold;
 }
 
-template <typename T>
-__global__ void __launch_bounds__(64)
+template <typename T, int SB>
+__global__ void __launch_bounds__(SB * 64)
 k_pcg_spmv(DeviceBatch d, const T *__restrict__ W, const T *__restrict__ P,
            const T *__restrict__ r, T *__restrict__ dbuf0, T *__restrict__ dbuf1,
            T *__restrict__ q)
 {
-    const int s = blockIdx.x;
-    const int inst = d.slice_inst[s];
+    __shared__ double lds[8 * SB];
+    const int blk = blockIdx.x;
+    const int inst = d.blk_inst[blk];
     InstState *st = d.state + inst;
     if (st->done)
         return;
-    const int it = st->iter;
-    const bool first = it == 0;
-    const T beta = (T)st->beta;
-    const T *dold = (it & 1) ? dbuf1 : dbuf0;
-    T *dnew = (it & 1) ? dbuf0 : dbuf1;
-    const int lane = threadIdx.x;
+    const InstParams &ip = d.params[inst];
+    const int k = st->it_a;
+    double beta = 0.0;
+    if (k > 0)
+    {
+        double v[2];
+        const int comp[2] = {P_RZ, P_RR};
+        reduce_cloth<SB, 2>(d.partial, ip.blk_begin, ip.blk_count, comp, v, lds);
+        const double rho = v[0], rr = v[1];
+        beta = rho / st->rho[(k - 1) & 1];
+        bool done;
+        if (ip.ref_mode)
+            done = k >= ip.max_iter || beta == 0.0; // :529, :585-590
+        else
+            done = rr <= st->inner_tol2 || st->total_iter >= ip.max_iter || !isfinite(rr);
+        if (blk == ip.blk_begin && threadIdx.x == 0)
+        {
+            st->rho[k & 1] = rho;
+            st->rr = rr;
+            st->it_b = k;
+            if (done)
+                st->done = 1;
+        }
+        if (done)
+            return;
+    }
+    else if (blk == ip.blk_begin && threadIdx.x == 0)
+        st->it_b = 0;
+    const bool first = k == 0;
+    const T bt = (T)beta;
+    const T *dold = (k & 1) ? dbuf0 : dbuf1;
+    T *dnew = (k & 1) ? dbuf1 : dbuf0;
+    const int lane = threadIdx.x & 63;
+    const int s = blk * SB + (threadIdx.x >> 6);
     const int p = s * SLICE + lane;
     const size_t p3 = 3 * (size_t)p;
     T dn[3];
     for (int a = 0; a < 3; a++)
-        dn[a] = dir_value<T>(P[p3 + a], r[p3 + a], dold[p3 + a], beta, first);
+        dn[a] = dir_value<T>(P[p3 + a], r[p3 + a], dold[p3 + a], bt, first);
     const int w = d.row_w[p];
     const size_t base = (size_t)d.slice_off[s];
     const T *Wl = W + 9 * base + lane;
     const int *cl = d.col + base + lane;
     T y0 = 0, y1 = 0, y2 = 0;
-    for (int k = 0; k < w; k++)
+#pragma unroll 2
+    for (int kk = 0; kk < w; kk++)
     {
-        const int c = cl[k * SLICE];
+        const int c = cl[kk * SLICE];
         const size_t c3 = 3 * (size_t)c;
-        const T v0 = dir_value<T>(P[c3], r[c3], dold[c3], beta, first);
-        const T v1 = dir_value<T>(P[c3 + 1], r[c3 + 1], dold[c3 + 1], beta, first);
-        const T v2 = dir_value<T>(P[c3 + 2], r[c3 + 2], dold[c3 + 2], beta, first);
-        const T *wk = Wl + 9 * (size_t)k * SLICE;
+        const T v0 = dir_value<T>(P[c3], r[c3], dold[c3], bt, first);
+        const T v1 = dir_value<T>(P[c3 + 1], r[c3 + 1], dold[c3 + 1], bt, first);
+        const T v2 = dir_value<T>(P[c3 + 2], r[c3 + 2], dold[c3 + 2], bt, first);
+        const T *wk = Wl + 9 * (size_t)kk * SLICE;
         y0 += wk[0 * SLICE] * v0 + wk[1 * SLICE] * v1 + wk[2 * SLICE] * v2;
         y1 += wk[3 * SLICE] * v0 + wk[4 * SLICE] * v1 + wk[5 * SLICE] * v2;
         y2 += wk[6 * SLICE] * v0 + wk[7 * SLICE] * v1 + wk[8 * SLICE] * v2;
@@ -108,46 +182,52 @@ k_pcg_spmv(DeviceBatch d, const T *__restrict__ W, const T *__restrict__ P,
     q[p3] = y0;
     q[p3 + 1] = y1;
     q[p3 + 2] = y2;
-    double part = (double)dn[0] * (double)y0 + (double)dn[1] * (double)y1 + (double)dn[2] * (double)y2;
-    part = wave_sum(part);
-    if (lane == 0)
-        d.partial[4 * (size_t)s] = part;
-    const InstParams &ip = d.params[inst];
-    if (last_arrival(st, ip.slice_count))
-    {
-        const double dq = reduce_partials(d.partial, 0, ip.slice_begin, ip.slice_count);
-        if (lane == 0)
-        {
-            st->dq = dq;
-            if (dq == 0.0 || !isfinite(dq))
-            {
-                // the reference breaks out of its loop on d.Ad == 0 (:547-551)
-                st->done = 1;
-                st->breakdown = dq == 0.0 ? 0 : 1;
-            }
-            else
-                st->alpha = st->rho / dq;
-            st->count = 0;
-        }
-    }
+    double v[1] = {(double)dn[0] * (double)y0 + (double)dn[1] * (double)y1 +
+                   (double)dn[2] * (double)y2};
+    block_sum<SB, 1>(v, lds);
+    const int comp[1] = {P_DQ};
+    write_partials<1>(d.partial, blk, comp, v);
 }
 
-template <typename T>
-__global__ void __launch_bounds__(64)
+template <typename T, int SB>
+__global__ void __launch_bounds__(SB * 64)
 k_pcg_update(DeviceBatch d, const T *__restrict__ P, T *__restrict__ x, T *__restrict__ r,
              const T *__restrict__ dbuf0, const T *__restrict__ dbuf1, const T *__restrict__ q)
 {
-    const int s = blockIdx.x;
-    const int inst = d.slice_inst[s];
+    __shared__ double lds[8 * SB];
+    const int blk = blockIdx.x;
+    const int inst = d.blk_inst[blk];
     InstState *st = d.state + inst;
     if (st->done)
         return;
-    const int it = st->iter;
-    const double alpha = st->alpha;
-    const T *dn = (it & 1) ? dbuf0 : dbuf1;
-    const int lane = threadIdx.x;
-    const size_t p3 = 3 * (size_t)(s * SLICE + lane);
-    double rz = 0.0, rr = 0.0;
+    const InstParams &ip = d.params[inst];
+    const int k = st->it_b;
+    double v[1];
+    {
+        const int comp[1] = {P_DQ};
+        reduce_cloth<SB, 1>(d.partial, ip.blk_begin, ip.blk_count, comp, v, lds);
+    }
+    const double dq = v[0];
+    const bool lead = blk == ip.blk_begin && threadIdx.x == 0;
+    if (dq == 0.0 || !isfinite(dq))
+    {
+        if (lead)
+        {
+            st->done = 1;
+            st->breakdown = dq == 0.0 ? 0 : 1;
+        }
+        return;
+    }
+    const double alpha = st->rho[k & 1] / dq;
+    if (lead)
+    {
+        st->it_a = k + 1;
+        st->total_iter += 1;
+    }
+    const T *dn = (k & 1) ? dbuf1 : dbuf0;
+    const int s = blk * SB + (threadIdx.x >> 6);
+    const size_t p3 = 3 * (size_t)(s * SLICE + (threadIdx.x & 63));
+    double vals[2] = {0.0, 0.0};
     for (int a = 0; a < 3; a++)
     {
         const T xn = (T)((double)x[p3 + a] + alpha * (double)dn[p3 + a]);
@@ -155,114 +235,27 @@ k_pcg_update(DeviceBatch d, const T *__restrict__ P, T *__restrict__ x, T *__res
         x[p3 + a] = xn;
         r[p3 + a] = rn;
         const T z = P[p3 + a] * rn;
-        rz += (double)rn * (double)z;
-        rr += (double)rn * (double)rn;
+        vals[0] += (double)rn * (double)z;
+        vals[1] += (double)rn * (double)rn;
     }
-    rz = wave_sum(rz);
-    rr = wave_sum(rr);
-    if (lane == 0)
-    {
-        d.partial[4 * (size_t)s + 1] = rz;
-        d.partial[4 * (size_t)s + 2] = rr;
-    }
-    const InstParams &ip = d.params[inst];
-    if (last_arrival(st, ip.slice_count))
-    {
-        const double rz_t = reduce_partials(d.partial, 1, ip.slice_begin, ip.slice_count);
-        const double rr_t = reduce_partials(d.partial, 2, ip.slice_begin, ip.slice_count);
-        if (lane == 0)
-        {
-            const int it1 = it + 1;
-            st->iter = it1;
-            st->total_iter += 1;
-            const double beta = rz_t / st->rho;
-            st->beta = beta;
-            st->rho = rz_t;
-            st->rr = rr_t;
-            bool done;
-            if (ip.ref_mode)
-                done = it1 >= ip.max_iter || beta == 0.0; // :585-590
-            else
-                done = rr_t <= st->inner_tol2 || st->total_iter >= ip.max_iter || !isfinite(rr_t);
-            if (done)
-                st->done = 1;
-            st->count = 0;
-        }
-    }
+    block_sum<SB, 2>(vals, lds);
+    const int comp[2] = {P_RZ, P_RR};
+    write_partials<2>(d.partial, blk, comp, vals);
 }
 
-// r = b - W64 dx (the warm start), first direction data; FP64 mode
-__global__ void __launch_bounds__(64) k_pcg_init64(DeviceBatch d)
+// r64 = b - W64 dx.  mode 0: FP64 solver start (r64 kept, partials rz, rr,
+// bb); mode 1: refinement pass (r32 = float(r64), e32 = 0, partials of r32
+// plus rr64 and bb); mode 2: residual for the step counters only.
+template <int SB> __global__ void __launch_bounds__(SB * 64) k_residual64(DeviceBatch d, int mode)
 {
-    const int s = blockIdx.x;
-    const int inst = d.slice_inst[s];
-    InstState *st = d.state + inst;
-    const int lane = threadIdx.x;
-    const int p = s * SLICE + lane;
-    const size_t p3 = 3 * (size_t)p;
-    const int w = d.row_w[p];
-    const size_t base = (size_t)d.slice_off[s];
-    const double *Wl = d.W64 + 9 * base + lane;
-    const int *cl = d.col + base + lane;
-    double y[3] = {0.0, 0.0, 0.0};
-    for (int k = 0; k < w; k++)
-    {
-        const size_t c3 = 3 * (size_t)cl[k * SLICE];
-        const double *wk = Wl + 9 * (size_t)k * SLICE;
-        for (int a = 0; a < 3; a++)
-            y[a] += wk[(3 * a) * SLICE] * d.dx[c3] + wk[(3 * a + 1) * SLICE] * d.dx[c3 + 1] +
-                    wk[(3 * a + 2) * SLICE] * d.dx[c3 + 2];
-    }
-    double rz = 0.0, rr = 0.0, bb = 0.0;
-    for (int a = 0; a < 3; a++)
-    {
-        const double bv = d.b[p3 + a];
-        const double rv = bv - y[a];
-        d.r64[p3 + a] = rv;
-        rz += rv * (d.P64[p3 + a] * rv);
-        rr += rv * rv;
-        bb += bv * bv;
-    }
-    rz = wave_sum(rz);
-    rr = wave_sum(rr);
-    bb = wave_sum(bb);
-    if (lane == 0)
-    {
-        d.partial[4 * (size_t)s + 1] = rz;
-        d.partial[4 * (size_t)s + 2] = rr;
-        d.partial[4 * (size_t)s + 3] = bb;
-    }
-    const InstParams &ip = d.params[inst];
-    if (last_arrival(st, ip.slice_count))
-    {
-        const double rz_t = reduce_partials(d.partial, 1, ip.slice_begin, ip.slice_count);
-        const double rr_t = reduce_partials(d.partial, 2, ip.slice_begin, ip.slice_count);
-        const double bb_t = reduce_partials(d.partial, 3, ip.slice_begin, ip.slice_count);
-        if (lane == 0)
-        {
-            st->rho = rz_t;
-            st->rr = rr_t;
-            st->bb = bb_t;
-            st->iter = 0;
-            st->beta = 0.0;
-            st->inner_tol2 = ip.tol * ip.tol * bb_t;
-            bool done = ip.ref_mode ? ip.max_iter <= 0 : rr_t <= st->inner_tol2;
-            st->done = done ? 1 : 0;
-            st->count = 0;
-        }
-    }
-}
-
-// FP64 residual r64 = b - W64 dx.  With start_inner it also decides the
-// refinement: converged, or start an inner FP32 solve on r32 = float(r64).
-__global__ void __launch_bounds__(64) k_residual64(DeviceBatch d, int start_inner)
-{
-    const int s = blockIdx.x;
-    const int inst = d.slice_inst[s];
-    InstState *st = d.state + inst;
-    if (start_inner && st->outer_done)
+    __shared__ double lds[8 * SB];
+    const int blk = blockIdx.x;
+    const int inst = d.blk_inst[blk];
+    const InstState *st = d.state + inst;
+    if (mode == 1 && st->outer_done)
         return;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int s = blk * SB + (threadIdx.x >> 6);
     const int p = s * SLICE + lane;
     const size_t p3 = 3 * (size_t)p;
     const int w = d.row_w[p];
@@ -270,81 +263,91 @@ __global__ void __launch_bounds__(64) k_residual64(DeviceBatch d, int start_inne
     const double *Wl = d.W64 + 9 * base + lane;
     const int *cl = d.col + base + lane;
     double y[3] = {0.0, 0.0, 0.0};
-    for (int k = 0; k < w; k++)
+    for (int kk = 0; kk < w; kk++)
     {
-        const size_t c3 = 3 * (size_t)cl[k * SLICE];
-        const double *wk = Wl + 9 * (size_t)k * SLICE;
+        const size_t c3 = 3 * (size_t)cl[kk * SLICE];
+        const double *wk = Wl + 9 * (size_t)kk * SLICE;
+        const double x0 = d.dx[c3], x1 = d.dx[c3 + 1], x2 = d.dx[c3 + 2];
         for (int a = 0; a < 3; a++)
-            y[a] += wk[(3 * a) * SLICE] * d.dx[c3] + wk[(3 * a + 1) * SLICE] * d.dx[c3 + 1] +
-                    wk[(3 * a + 2) * SLICE] * d.dx[c3 + 2];
+            y[a] += wk[(3 * a) * SLICE] * x0 + wk[(3 * a + 1) * SLICE] * x1 +
+                    wk[(3 * a + 2) * SLICE] * x2;
     }
-    double rr64 = 0.0, bb = 0.0, rz = 0.0, rr32 = 0.0;
+    double v[4] = {0.0, 0.0, 0.0, 0.0}; // rz, rr, bb, rr64
     for (int a = 0; a < 3; a++)
     {
         const double bv = d.b[p3 + a];
         const double rv = bv - y[a];
-        rr64 += rv * rv;
-        bb += bv * bv;
-        if (start_inner)
+        v[2] += bv * bv;
+        v[3] += rv * rv;
+        if (mode == 0)
+        {
+            d.r64[p3 + a] = rv;
+            v[0] += rv * (d.P64[p3 + a] * rv);
+            v[1] += rv * rv;
+        }
+        else if (mode == 1)
         {
             const float rf = (float)rv;
             d.r32[p3 + a] = rf;
             d.e32[p3 + a] = 0.0f;
             const float z = d.P32[p3 + a] * rf;
-            rz += (double)rf * (double)z;
-            rr32 += (double)rf * (double)rf;
+            v[0] += (double)rf * (double)z;
+            v[1] += (double)rf * (double)rf;
         }
     }
-    rr64 = wave_sum(rr64);
-    bb = wave_sum(bb);
-    rz = wave_sum(rz);
-    rr32 = wave_sum(rr32);
-    if (lane == 0)
-    {
-        d.partial[4 * (size_t)s + 0] = rr64;
-        d.partial[4 * (size_t)s + 1] = rz;
-        d.partial[4 * (size_t)s + 2] = rr32;
-        d.partial[4 * (size_t)s + 3] = bb;
-    }
+    block_sum<SB, 4>(v, lds);
+    const int comp[4] = {P_RZ, P_RR, P_BB, P_RR64};
+    write_partials<4>(d.partial, blk, comp, v);
+}
+
+// One workgroup per cloth: reduce the residual partials and decide.
+//   mode 0 (FP64 start): rho_0, stop if ||r|| <= tol ||b||
+//   mode 1 (refinement): converged, or start an inner FP32 solve
+//   mode 2 (counters):   record ||r64|| and ||b||
+template <int SB> __global__ void __launch_bounds__(SB * 64) k_decide(DeviceBatch d, int mode)
+{
+    __shared__ double lds[8 * SB];
+    const int inst = blockIdx.x;
+    InstState *st = d.state + inst;
     const InstParams &ip = d.params[inst];
-    if (last_arrival(st, ip.slice_count))
+    if (mode == 1 && st->outer_done)
+        return;
+    double v[4];
+    const int comp[4] = {P_RZ, P_RR, P_BB, P_RR64};
+    reduce_cloth<SB, 4>(d.partial, ip.blk_begin, ip.blk_count, comp, v, lds);
+    if (threadIdx.x != 0)
+        return;
+    const double rz = v[0], rr = v[1], bb = v[2], rr64 = v[3];
+    st->bb = bb;
+    st->rr64 = rr64;
+    if (mode == 2)
+        return;
+    st->it_a = 0;
+    st->it_b = 0;
+    st->rho[0] = rz;
+    st->rr = rr;
+    if (mode == 0)
     {
-        const double rr64_t = reduce_partials(d.partial, 0, ip.slice_begin, ip.slice_count);
-        const double rz_t = reduce_partials(d.partial, 1, ip.slice_begin, ip.slice_count);
-        const double rr32_t = reduce_partials(d.partial, 2, ip.slice_begin, ip.slice_count);
-        const double bb_t = reduce_partials(d.partial, 3, ip.slice_begin, ip.slice_count);
-        if (lane == 0)
-        {
-            st->rr64 = rr64_t;
-            st->bb = bb_t;
-            if (start_inner)
-            {
-                const double tol2 = ip.tol * ip.tol * bb_t;
-                if (rr64_t <= tol2 || st->outer >= ip.max_outer || st->total_iter >= ip.max_iter ||
-                    !isfinite(rr64_t))
-                {
-                    st->outer_done = 1;
-                    st->done = 1;
-                }
-                else
-                {
-                    st->outer += 1;
-                    st->iter = 0;
-                    st->beta = 0.0;
-                    st->rho = rz_t;
-                    st->rr = rr32_t;
-                    // inner target: the outer tolerance with a margin, but no
-                    // deeper than FP32 arithmetic can carry
-                    double eta = 0.5 * ip.tol * sqrt(bb_t / rr64_t);
-                    if (eta < 2e-5)
-                        eta = 2e-5;
-                    st->inner_tol2 = eta * eta * rr32_t;
-                    st->done = rr32_t <= st->inner_tol2 ? 1 : 0;
-                }
-            }
-            st->count = 0;
-        }
+        st->inner_tol2 = ip.tol * ip.tol * bb;
+        st->done = ip.ref_mode ? (ip.max_iter <= 0) : (rr <= st->inner_tol2);
+        return;
     }
+    const double tol2 = ip.tol * ip.tol * bb;
+    if (rr64 <= tol2 || st->outer >= ip.max_outer || st->total_iter >= ip.max_iter ||
+        !isfinite(rr64))
+    {
+        st->outer_done = 1;
+        st->done = 1;
+        return;
+    }
+    st->outer += 1;
+    // inner target: the outer tolerance with a margin, but no deeper than
+    // FP32 arithmetic carries
+    double eta = 0.5 * ip.tol * sqrt(bb / rr64);
+    if (eta < ip.eta_min)
+        eta = ip.eta_min;
+    st->inner_tol2 = eta * eta * rr;
+    st->done = rr <= st->inner_tol2 ? 1 : 0;
 }
 
 // dx += e after an inner solve
@@ -353,8 +356,7 @@ __global__ void __launch_bounds__(256) k_correct(DeviceBatch d)
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= d.n_rows)
         return;
-    const int s = p / SLICE;
-    if (d.state[d.slice_inst[s]].outer_done)
+    if (d.state[d.blk_inst[p / (SLICE * BW_SB)]].outer_done)
         return;
     const size_t p3 = 3 * (size_t)p;
     for (int a = 0; a < 3; a++)
@@ -394,7 +396,53 @@ template <typename T> __global__ void k_fill(T *v, size_t n, T value)
         v[i] = value;
 }
 
+constexpr int SB = BW_SB;
+
 } // namespace
+
+void launch_reset_states(const DeviceBatch &d, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_reset_states, dim3((d.n_inst + 63) / 64), dim3(64), 0, s, d.state,
+                       d.n_inst);
+}
+
+void launch_pcg_init64(const DeviceBatch &d, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_residual64<SB>, dim3(d.n_blocks_wg), dim3(SB * 64), 0, s, d, 0);
+    hipLaunchKernelGGL(k_decide<SB>, dim3(d.n_inst), dim3(SB * 64), 0, s, d, 0);
+}
+
+void launch_pcg_iter64(const DeviceBatch &d, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_pcg_spmv<double, SB>), dim3(d.n_blocks_wg), dim3(SB * 64), 0, s, d,
+                       d.W64, d.P64, d.r64, d.d64[0], d.d64[1], d.q64);
+    hipLaunchKernelGGL((k_pcg_update<double, SB>), dim3(d.n_blocks_wg), dim3(SB * 64), 0, s, d,
+                       d.P64, d.dx, d.r64, d.d64[0], d.d64[1], d.q64);
+}
+
+void launch_pcg_iter32(const DeviceBatch &d, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_pcg_spmv<float, SB>), dim3(d.n_blocks_wg), dim3(SB * 64), 0, s, d,
+                       d.W32, d.P32, d.r32, d.d32[0], d.d32[1], d.q32);
+    hipLaunchKernelGGL((k_pcg_update<float, SB>), dim3(d.n_blocks_wg), dim3(SB * 64), 0, s, d,
+                       d.P32, d.e32, d.r32, d.d32[0], d.d32[1], d.q32);
+}
+
+void launch_residual64(const DeviceBatch &d, int mode, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_residual64<SB>, dim3(d.n_blocks_wg), dim3(SB * 64), 0, s, d, mode);
+    hipLaunchKernelGGL(k_decide<SB>, dim3(d.n_inst), dim3(SB * 64), 0, s, d, mode);
+}
+
+void launch_correct(const DeviceBatch &d, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_correct, dim3((d.n_rows + 255) / 256), dim3(256), 0, s, d);
+}
+
+void launch_update_positions(const DeviceBatch &d, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_update_positions, dim3((d.n_rows + 255) / 256), dim3(256), 0, s, d);
+}
 
 void launch_fill_residual(const DeviceBatch &d, bool fp32, hipStream_t s)
 {
@@ -409,53 +457,11 @@ void launch_fill_residual(const DeviceBatch &d, bool fp32, hipStream_t s)
 void launch_spmv_only(const DeviceBatch &d, bool fp32, hipStream_t s)
 {
     if (fp32)
-        hipLaunchKernelGGL(k_pcg_spmv<float>, dim3(d.n_slices), dim3(64), 0, s, d, d.W32, d.P32,
-                           d.r32, d.d32[0], d.d32[1], d.q32);
+        hipLaunchKernelGGL((k_pcg_spmv<float, SB>), dim3(d.n_blocks_wg), dim3(SB * 64), 0, s, d,
+                           d.W32, d.P32, d.r32, d.d32[0], d.d32[1], d.q32);
     else
-        hipLaunchKernelGGL(k_pcg_spmv<double>, dim3(d.n_slices), dim3(64), 0, s, d, d.W64,
-                           d.P64, d.r64, d.d64[0], d.d64[1], d.q64);
-}
-
-void launch_reset_states(const DeviceBatch &d, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_reset_states, dim3((d.n_inst + 63) / 64), dim3(64), 0, s, d.state,
-                       d.n_inst);
-}
-
-void launch_pcg_init64(const DeviceBatch &d, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_pcg_init64, dim3(d.n_slices), dim3(64), 0, s, d);
-}
-
-void launch_pcg_iter64(const DeviceBatch &d, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_pcg_spmv<double>, dim3(d.n_slices), dim3(64), 0, s, d, d.W64, d.P64,
-                       d.r64, d.d64[0], d.d64[1], d.q64);
-    hipLaunchKernelGGL(k_pcg_update<double>, dim3(d.n_slices), dim3(64), 0, s, d, d.P64, d.dx,
-                       d.r64, d.d64[0], d.d64[1], d.q64);
-}
-
-void launch_pcg_iter32(const DeviceBatch &d, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_pcg_spmv<float>, dim3(d.n_slices), dim3(64), 0, s, d, d.W32, d.P32,
-                       d.r32, d.d32[0], d.d32[1], d.q32);
-    hipLaunchKernelGGL(k_pcg_update<float>, dim3(d.n_slices), dim3(64), 0, s, d, d.P32, d.e32,
-                       d.r32, d.d32[0], d.d32[1], d.q32);
-}
-
-void launch_residual64(const DeviceBatch &d, int start_inner, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_residual64, dim3(d.n_slices), dim3(64), 0, s, d, start_inner);
-}
-
-void launch_correct(const DeviceBatch &d, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_correct, dim3((d.n_rows + 255) / 256), dim3(256), 0, s, d);
-}
-
-void launch_update_positions(const DeviceBatch &d, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_update_positions, dim3((d.n_rows + 255) / 256), dim3(256), 0, s, d);
+        hipLaunchKernelGGL((k_pcg_spmv<double, SB>), dim3(d.n_blocks_wg), dim3(SB * 64), 0, s,
+                           d, d.W64, d.P64, d.r64, d.d64[0], d.d64[1], d.q64);
 }
 
 } // namespace bw
