@@ -33,6 +33,7 @@ SOURCES = [
     ("capi.cpp", ["-ffp-contract=off"]),
     ("assemble.hip", DEVICE + ["-ffp-contract=off"]),
     ("pcg.hip", DEVICE + ["-ffp-contract=fast"]),
+    ("cg.hip", DEVICE + ["-ffp-contract=fast"]),
 ]
 
 

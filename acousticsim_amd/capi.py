@@ -48,7 +48,7 @@ EXPORTS = [
     "bw_get_ell_map", "bw_assemble_system", "bw_get_last_stats", "bw_last_error", "bw_batch_create",
     "bw_batch_destroy", "bw_batch_size", "bw_batch_step", "bw_batch_num_vertices",
     "bw_batch_get_pos", "bw_batch_set_state", "bw_batch_get_stats", "bw_batch_set_graphs",
-    "bw_batch_time_spmv",
+    "bw_batch_time_solver_kernel",
 ]
 
 
@@ -96,7 +96,7 @@ def lib():
         "bw_batch_set_state": (i32, [vp, i32, dp, dp]),
         "bw_batch_get_stats": (i32, [vp, i32, P(StepStats)]),
         "bw_batch_set_graphs": (i32, [vp, i32]),
-        "bw_batch_time_spmv": (i32, [vp, i32, dp, dp]),
+        "bw_batch_time_solver_kernel": (i32, [vp, i32, dp, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -284,12 +284,12 @@ class ClothBatch:
         _check(lib().bw_batch_get_pos(self._h, i, _dp(x)))
         return x
 
-    def time_spmv(self, reps=50):
-        """(mean us per SpMV launch, algorithmic bytes per launch); call after
-        at least one step so W is assembled."""
+    def time_solver_kernel(self, reps=50):
+        """(mean us per launch of the dominant solver kernel, algorithmic
+        bytes per launch); call after at least one step so W is assembled."""
         us = np.zeros(1)
         nbytes = np.zeros(1)
-        _check(lib().bw_batch_time_spmv(self._h, reps, _dp(us), _dp(nbytes)))
+        _check(lib().bw_batch_time_solver_kernel(self._h, reps, _dp(us), _dp(nbytes)))
         return float(us[0]), float(nbytes[0])
 
     def set_state(self, i, x, xpre):

@@ -43,17 +43,16 @@ Batch::Batch(std::vector<ClothModel> models) : models_(std::move(models))
 {
     if (models_.empty())
         throw std::runtime_error("batch: no cloths");
-    int nmixed = 0;
+    auto kind_of = [](const ClothModel &m) {
+        if (m.cfg.solve_mode == SolveMode::Reference)
+            return SolverKind::RefPCG64;
+        return m.cfg.precision == Precision::Mixed ? SolverKind::CG32 : SolverKind::CG64;
+    };
+    kind_ = kind_of(models_[0]);
     for (auto &m : models_)
-    {
-        const bool mixed = m.cfg.precision == Precision::Mixed &&
-                           m.cfg.solve_mode == SolveMode::Tolerance;
-        nmixed += mixed ? 1 : 0;
-        any_ref_ |= m.cfg.solve_mode == SolveMode::Reference;
-    }
-    if (nmixed != 0 && nmixed != (int)models_.size())
-        throw std::runtime_error("batch: members must share one precision (mixed or fp64)");
-    mixed_ = nmixed != 0;
+        if (kind_of(m) != kind_)
+            throw std::runtime_error(
+                "batch: members must share one precision and solver_mode");
     stats_.resize(models_.size());
     check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     // BW_GRAPHS=0 launches the iteration chains directly (profilers that do
@@ -62,6 +61,8 @@ Batch::Batch(std::vector<ClothModel> models) : models_(std::move(models))
         graphs_disabled_ = std::string(g) == "0";
     use_graphs_ = !graphs_disabled_;
     check(hipHostMalloc((void **)&host_state_, sizeof(InstState) * models_.size()),
+          "hipHostMalloc");
+    check(hipHostMalloc((void **)&host_cg_, 2 * sizeof(CgState) * models_.size()),
           "hipHostMalloc");
     build_layout();
 }
@@ -76,6 +77,8 @@ Batch::~Batch()
         (void)hipFree(p);
     if (host_state_)
         (void)hipHostFree(host_state_);
+    if (host_cg_)
+        (void)hipHostFree(host_cg_);
     if (stream_)
         (void)hipStreamDestroy(stream_);
 }
@@ -231,9 +234,17 @@ void Batch::build_layout()
         P.ref_mode = m.cfg.solve_mode == SolveMode::Reference ? 1 : 0;
         P.tol = P.ref_mode ? 0.0 : m.cfg.tolerance;
         P.max_iter = P.ref_mode ? m.cfg.max_iters - 1 : m.cfg.solver_max_iters;
-        P.mixed = mixed_ ? 1 : 0;
-        P.max_outer = 30;
-        P.eta_min = 2e-5;
+        // reliable updates: FP32 recurrences are replaced by the FP64 true
+        // residual each time r.r drops by rel2, or after max_since
+        // iterations; FP64 recurrences are checked only at convergence.
+        // BW_REL2 and BW_MAX_SINCE override them for tuning runs.
+        const bool mixed = kind_ == SolverKind::CG32;
+        P.rel2 = mixed ? 1e-2 : 0.0;
+        P.max_since = mixed ? 2000 : (1 << 30);
+        if (const char *e = std::getenv("BW_REL2"))
+            P.rel2 = std::atof(e);
+        if (const char *e = std::getenv("BW_MAX_SINCE"))
+            P.max_since = std::atoi(e);
         P.blk_begin = slice_base_[i] / BW_SB;
         P.blk_count = nslices_[i] / BW_SB;
         P.slice_begin = slice_base_[i];
@@ -280,25 +291,36 @@ void Batch::build_layout()
     d_.dx = dalloc<double>(nv3);
     d_.b = dalloc<double>(nv3);
     d_.P64 = dalloc<double>(nv3);
-    d_.r64 = dalloc<double>(nv3);
-    if (!mixed_)
+    d_.P32 = dalloc<float>(nv3);
+    d_.params = dalloc<InstParams>(n);
+    if (kind_ == SolverKind::RefPCG64)
     {
+        d_.r64 = dalloc<double>(nv3);
         d_.d64[0] = dalloc<double>(nv3);
         d_.d64[1] = dalloc<double>(nv3);
         d_.q64 = dalloc<double>(nv3);
+        d_.partial = dalloc<double>(PARTS * (size_t)d_.n_blocks_wg);
+        d_.state = dalloc<InstState>(n);
     }
-    d_.P32 = dalloc<float>(nv3);
-    if (mixed_)
+    else
     {
-        d_.r32 = dalloc<float>(nv3);
-        d_.e32 = dalloc<float>(nv3);
-        d_.d32[0] = dalloc<float>(nv3);
-        d_.d32[1] = dalloc<float>(nv3);
-        d_.q32 = dalloc<float>(nv3);
+        if (kind_ == SolverKind::CG32)
+        {
+            d_.r32 = dalloc<float>(nv3);
+            d_.q32 = dalloc<float>(nv3);
+            d_.d32[0] = dalloc<float>(nv3);
+            d_.d32[1] = dalloc<float>(nv3);
+        }
+        else
+        {
+            d_.r64 = dalloc<double>(nv3);
+            d_.q64 = dalloc<double>(nv3);
+            d_.d64[0] = dalloc<double>(nv3);
+            d_.d64[1] = dalloc<double>(nv3);
+        }
+        d_.cg = dalloc<CgState>(2 * (size_t)n);
+        d_.cg_partial = dalloc<double>(2 * PARTS * (size_t)d_.n_blocks_wg);
     }
-    d_.partial = dalloc<double>(PARTS * (size_t)d_.n_blocks_wg);
-    d_.params = dalloc<InstParams>(n);
-    d_.state = dalloc<InstState>(n);
 
     upload(d_.slice_off, slice_off);
     upload(d_.slice_w, slice_w);
@@ -365,7 +387,6 @@ void Batch::set_pre_pos(int i, const double *x) { upload_state(i, x, d_.xpre); }
 
 void Batch::assemble_system(int i, double *W, double *rhs)
 {
-    launch_reset_states(d_, stream_);
     launch_elements(d_, stream_);
     launch_assemble_rows(d_, stream_);
     check(hipGetLastError(), "assemble launch");
@@ -402,88 +423,109 @@ void Batch::assemble_system(int i, double *W, double *rhs)
     download(i, d_.b, rhs);
 }
 
-double Batch::time_spmv(int reps, double *alg_bytes)
+double Batch::time_solver_kernel(int reps, double *alg_bytes)
 {
-    if (reps < 1)
-        throw std::runtime_error("batch: reps must be >= 1");
-    // a nonzero residual everywhere keeps every cloth active (d.Wd > 0)
-    launch_reset_states(d_, stream_);
-    launch_fill_residual(d_, mixed_, stream_);
+    if (reps < 2)
+        throw std::runtime_error("batch: reps must be >= 2");
+    reps &= ~1;
     hipEvent_t e0, e1;
     check(hipEventCreate(&e0), "hipEventCreate");
     check(hipEventCreate(&e1), "hipEventCreate");
-    launch_spmv_only(d_, mixed_, stream_); // warm
-    check(hipEventRecord(e0, stream_), "hipEventRecord");
-    for (int k = 0; k < reps; k++)
-        launch_spmv_only(d_, mixed_, stream_);
-    check(hipEventRecord(e1, stream_), "hipEventRecord");
-    check(hipEventSynchronize(e1), "hipEventSynchronize");
-    float ms = 0;
-    check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     double nnzb = 0, nv = 0;
     for (auto &m : models_)
     {
         nnzb += m.nnzb();
         nv += m.nv;
     }
-    const double per_block = mixed_ ? 40.0 : 76.0, per_row = mixed_ ? 60.0 : 120.0;
+    double bytes = 0;
+    if (kind_ == SolverKind::RefPCG64)
+    {
+        // a nonzero residual everywhere keeps every cloth active (d.Wd > 0)
+        launch_reset_states(d_, stream_);
+        launch_fill_residual(d_, false, stream_);
+        launch_spmv_only(d_, false, stream_); // warm
+        check(hipEventRecord(e0, stream_), "hipEventRecord");
+        for (int k = 0; k < reps; k++)
+            launch_spmv_only(d_, false, stream_);
+        check(hipEventRecord(e1, stream_), "hipEventRecord");
+        // W (72 B) + column (4 B) per block; r, P, d_prev read and d, W d written
+        bytes = 76.0 * nnzb + 120.0 * nv;
+    }
+    else
+    {
+        // k_cg_spmv alone: it reads state slot 0 and writes slot 1, so
+        // repeated launches redo the same SpMV with every cloth active
+        const bool fp32 = kind_ == SolverKind::CG32;
+        launch_cg_timing_setup(d_, fp32, stream_);
+        launch_cg_step(d_, fp32, 0, stream_); // warm
+        check(hipEventRecord(e0, stream_), "hipEventRecord");
+        for (int k = 0; k < reps; k++)
+            launch_cg_step(d_, fp32, 0, stream_);
+        check(hipEventRecord(e1, stream_), "hipEventRecord");
+        // per block: W + column; per row: r, P, d_prev read, d, W d written
+        bytes = fp32 ? 40.0 * nnzb + 60.0 * nv : 76.0 * nnzb + 120.0 * nv;
+    }
+    check(hipEventSynchronize(e1), "hipEventSynchronize");
+    float ms = 0;
+    check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     if (alg_bytes)
-        *alg_bytes = per_block * nnzb + per_row * nv;
+        *alg_bytes = bytes;
     return 1000.0 * ms / reps;
 }
 
 void Batch::read_states()
 {
-    check(hipMemcpyAsync(host_state_, d_.state, sizeof(InstState) * models_.size(),
-                         hipMemcpyDeviceToHost, stream_),
-          "hipMemcpyAsync state");
+    if (kind_ == SolverKind::RefPCG64)
+        check(hipMemcpyAsync(host_state_, d_.state, sizeof(InstState) * models_.size(),
+                             hipMemcpyDeviceToHost, stream_),
+              "hipMemcpyAsync state");
+    else
+        check(hipMemcpyAsync(host_cg_, d_.cg, 2 * sizeof(CgState) * models_.size(),
+                             hipMemcpyDeviceToHost, stream_),
+              "hipMemcpyAsync state");
     check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
 }
 
-bool Batch::all(bool (*pred)(const InstState &)) const
+void Batch::launch_chain(int length)
 {
-    for (size_t i = 0; i < models_.size(); i++)
-        if (!pred(host_state_[i]))
-            return false;
-    return true;
+    if (kind_ == SolverKind::RefPCG64)
+        for (int k = 0; k < length; k++)
+            launch_pcg_iter64(d_, stream_);
+    else
+        for (int k = 0; k < length; k++)
+            launch_cg_step(d_, kind_ == SolverKind::CG32, k & 1, stream_);
 }
 
-void Batch::run_chunk(bool fp32, int iters)
+void Batch::run_chunk(int length)
 {
     if (!use_graphs_)
     {
-        for (int k = 0; k < iters; k++)
-            fp32 ? launch_pcg_iter32(d_, stream_) : launch_pcg_iter64(d_, stream_);
-        check(hipGetLastError(), "PCG launch");
+        launch_chain(length);
+        check(hipGetLastError(), "solver launch");
         return;
     }
     hipGraphExec_t exec = nullptr;
     for (auto &g : graphs_)
-        if (g.fp32 == fp32 && g.iters == iters)
+        if (g.length == length)
             exec = g.exec;
     if (!exec)
     {
         hipGraph_t graph;
         check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "BeginCapture");
-        for (int k = 0; k < iters; k++)
-            fp32 ? launch_pcg_iter32(d_, stream_) : launch_pcg_iter64(d_, stream_);
+        launch_chain(length);
         check(hipStreamEndCapture(stream_, &graph), "EndCapture");
         check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "GraphInstantiate");
         (void)hipGraphDestroy(graph);
-        graphs_.push_back({fp32, iters, exec});
+        graphs_.push_back({length, exec});
     }
     check(hipGraphLaunch(exec, stream_), "GraphLaunch");
 }
 
 namespace
 {
-bool pred_done(const InstState &s) { return s.done != 0; }
-bool pred_outer_done(const InstState &s) { return s.outer_done != 0; }
-
-// chunk sizes are multiples of 16 iterations so few graphs get built
-// BW_MAX_CHUNK caps the iterations queued between host checks (default 512)
+// BW_MAX_CHUNK caps the launches queued between host checks (default 512)
 int max_chunk()
 {
     static const int v = [] {
@@ -494,6 +536,8 @@ int max_chunk()
     return v;
 }
 
+// chunk lengths are multiples of 16 (even, so that a CG chunk ends on parity
+// slot 0) and few distinct graphs get built
 int chunk_for(int estimate)
 {
     int c = std::max(16, std::min(estimate, max_chunk()));
@@ -501,76 +545,95 @@ int chunk_for(int estimate)
 }
 } // namespace
 
+void Batch::solve_ref()
+{
+    launch_reset_states(d_, stream_);
+    launch_pcg_init64(d_, stream_);
+    int chunk = chunk_for(last_launches_ > 0 ? last_launches_ : 16);
+    for (;;)
+    {
+        read_states();
+        bool all_done = true;
+        for (size_t i = 0; i < models_.size(); i++)
+            all_done &= host_state_[i].done != 0;
+        if (all_done)
+            break;
+        run_chunk(chunk);
+        chunk = chunk_for(32);
+    }
+    launch_residual64(d_, 2, stream_);
+    read_states();
+    int most = 0;
+    for (size_t i = 0; i < models_.size(); i++)
+    {
+        const InstState &s = host_state_[i];
+        StepStats &st = stats_[i];
+        st.pcg_iterations = s.total_iter;
+        st.outer_iterations = 1;
+        st.rel_residual = s.bb > 0 ? std::sqrt(s.rr64 / s.bb) : std::sqrt(s.rr64);
+        most = std::max(most, s.total_iter);
+    }
+    last_launches_ = most;
+}
+
+void Batch::solve_cg()
+{
+    launch_cg_reset(d_, stream_);
+    int chunk = chunk_for(last_launches_ > 0 ? last_launches_ : 16);
+    long long launched = 0;
+    long long cap = 16;
+    for (auto &m : models_)
+        cap = std::max<long long>(cap, 4LL * m.cfg.solver_max_iters + 64);
+    for (;;)
+    {
+        run_chunk(chunk);
+        launched += chunk;
+        read_states();
+        bool all_done = true;
+        for (size_t i = 0; i < models_.size(); i++)
+            all_done &= host_cg_[2 * i].done != 0;
+        if (all_done)
+            break;
+        if (launched > cap)
+            throw std::runtime_error("step: solver did not terminate");
+        chunk = chunk_for(64);
+    }
+    int most = 0;
+    for (size_t i = 0; i < models_.size(); i++)
+    {
+        const CgState &s = host_cg_[2 * i];
+        StepStats &st = stats_[i];
+        st.pcg_iterations = s.total;
+        st.outer_iterations = s.updates;
+        st.rel_residual = s.bb > 0 ? std::sqrt(s.rr_true / s.bb) : std::sqrt(s.rr_true);
+        // launches this cloth needed: two per iteration and per true residual
+        most = std::max(most, 2 * (s.total + s.updates) + 2);
+    }
+    last_launches_ = most;
+}
+
 void Batch::step(int nsteps)
 {
     for (int n = 0; n < nsteps; n++)
     {
         const auto t0 = std::chrono::steady_clock::now();
-        launch_reset_states(d_, stream_);
         launch_elements(d_, stream_);
         launch_assemble_rows(d_, stream_);
-        int iters_this_step = 0;
-        if (!mixed_)
-        {
-            launch_pcg_init64(d_, stream_);
-            int chunk = chunk_for(last_iters_ > 0 ? last_iters_ : 16);
-            int done_iters = 0;
-            for (;;)
-            {
-                read_states();
-                if (all(pred_done))
-                    break;
-                run_chunk(false, chunk);
-                done_iters += chunk;
-                chunk = std::min(32, max_chunk());
-            }
-            iters_this_step = done_iters;
-            launch_residual64(d_, 2, stream_);
-        }
+        if (kind_ == SolverKind::RefPCG64)
+            solve_ref();
         else
-        {
-            launch_residual64(d_, 1, stream_);
-            bool first_pass = true;
-            for (;;)
-            {
-                read_states();
-                if (all(pred_outer_done))
-                    break;
-                int chunk = chunk_for(first_pass && last_iters_ > 0 ? last_iters_ : 16);
-                for (;;)
-                {
-                    if (all(pred_done))
-                        break;
-                    run_chunk(true, chunk);
-                    iters_this_step += chunk;
-                    chunk = std::min(32, max_chunk());
-                    read_states();
-                }
-                first_pass = false;
-                launch_correct(d_, stream_);
-                launch_residual64(d_, 1, stream_);
-            }
-        }
+            solve_cg();
         launch_update_positions(d_, stream_);
         check(hipGetLastError(), "step launch");
-        read_states();
+        check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
         const double us =
             std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
-        int max_iter = 0;
         bool bad = false;
-        for (size_t i = 0; i < models_.size(); i++)
+        for (auto &st : stats_)
         {
-            const InstState &s = host_state_[i];
-            StepStats &st = stats_[i];
-            st.pcg_iterations = s.total_iter;
-            st.outer_iterations = mixed_ ? s.outer : 1;
-            st.rel_residual = s.bb > 0 ? std::sqrt(s.rr64 / s.bb) : std::sqrt(s.rr64);
             st.step_us = us;
-            max_iter = std::max(max_iter, s.total_iter);
-            bad |= !std::isfinite(s.rr64);
+            bad |= !std::isfinite(st.rel_residual);
         }
-        last_iters_ = max_iter;
-        (void)iters_this_step;
         if (bad)
             throw std::runtime_error("step: non-finite residual (NaN/Inf in the state)");
     }

@@ -14,9 +14,18 @@ namespace bw
 struct StepStats
 {
     int pcg_iterations = 0;
-    int outer_iterations = 0;
+    int outer_iterations = 0; // FP64 true-residual checks (reliable updates)
     double rel_residual = 0;
     double step_us = 0;
+};
+
+// how a batch solves: the reference's own PCG with exactly max_iters-1
+// iterations (FP64), or the tolerance-mode single-reduction CG in FP32/FP64
+enum class SolverKind
+{
+    RefPCG64,
+    CG32,
+    CG64
 };
 
 class Batch
@@ -29,6 +38,7 @@ public:
 
     int size() const { return (int)models_.size(); }
     const ClothModel &model(int i) const { return models_[i]; }
+    SolverKind kind() const { return kind_; }
     // pins changed on the host model: re-upload the pinned mask
     void refresh_pins(int i, const std::vector<int> &pins);
 
@@ -42,10 +52,10 @@ public:
     void set_pre_pos(int i, const double *x);
     const StepStats &stats(int i) const { return stats_[i]; }
     void set_graphs(bool on) { use_graphs_ = on && !graphs_disabled_; }
-    // mean microseconds per SpMV launch over reps launches; algorithmic bytes
-    double time_spmv(int reps, double *alg_bytes);
+    // mean microseconds per launch of the dominant solver kernel over reps
+    // launches, every cloth active; algorithmic bytes per launch
+    double time_solver_kernel(int reps, double *alg_bytes);
 
-    // device layout (for tests and tools)
     const DeviceBatch &device() const { return d_; }
     int row_base(int i) const { return row_base_[i]; }
 
@@ -54,24 +64,25 @@ private:
     void upload_state(int i, const double *src, double *dst_dev);
     void download(int i, const double *src_dev, double *dst) const;
     void read_states();
-    bool all(bool (*pred)(const InstState &)) const;
-    void run_chunk(bool fp32, int iters);
+    void launch_chain(int length); // one chunk of solver launches
+    void run_chunk(int length);    // the chunk, as a replayed graph when enabled
+    void solve_ref();
+    void solve_cg();
 
     std::vector<ClothModel> models_;
     std::vector<int> row_base_, slice_base_, nslices_;
     std::vector<StepStats> stats_;
     DeviceBatch d_;
     hipStream_t stream_ = nullptr;
-    InstState *host_state_ = nullptr; // pinned
-    bool mixed_ = false, any_ref_ = false;
+    SolverKind kind_ = SolverKind::CG32;
+    InstState *host_state_ = nullptr; // pinned, reference PCG state
+    CgState *host_cg_ = nullptr;      // pinned, 2 per cloth
     bool use_graphs_ = true;
     bool graphs_disabled_ = false; // BW_GRAPHS=0 in the environment
-    int last_iters_ = 0;
-    // cached chunk graphs: key (fp32, iterations)
+    int last_launches_ = 0;
     struct GraphEntry
     {
-        bool fp32;
-        int iters;
+        int length;
         hipGraphExec_t exec;
     };
     std::vector<GraphEntry> graphs_;

@@ -400,12 +400,12 @@ int bw_batch_get_stats(const bw_batch *b, int i, bw_step_stats *st)
     return BW_OK;
 }
 
-int bw_batch_time_spmv(bw_batch *b, int reps, double *avg_us, double *alg_bytes)
+int bw_batch_time_solver_kernel(bw_batch *b, int reps, double *avg_us, double *alg_bytes)
 {
     return guard([&] {
         if (!b || !avg_us)
             return fail(BW_ERR_ARG, "NULL argument");
-        *avg_us = b->batch->time_spmv(reps, alg_bytes);
+        *avg_us = b->batch->time_solver_kernel(reps, alg_bytes);
         return BW_OK;
     });
 }

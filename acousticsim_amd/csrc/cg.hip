@@ -1,0 +1,390 @@
+// Tolerance-mode solver: Jacobi-PCG (the recurrences of cBaraffCloth::Solve,
+// BaraffCloth.cpp:507-611) as a per-cloth state machine on the device, with
+// reliable updates.
+//
+// One iteration is two launches, k_cg_spmv then k_cg_update.  Every cloth's
+// state, and the FP64 partial sums (one set per workgroup), have two slots:
+// k_cg_spmv reads slot 0 and writes slot 1, k_cg_update reads slot 1 and
+// writes slot 0, so no workgroup reads what another workgroup of the same
+// launch writes, and no grid-wide synchronisation is needed.  Every workgroup
+// of a cloth sums the cloth's partials of the previous launch in the same
+// fixed order, so all of them derive identical scalars and take the same
+// branch, and a cloth's arithmetic does not depend on the rest of the batch.
+//
+//   k_cg_spmv    prologue: reduce r.(P r) and r.r of the previous update;
+//                beta = rho_new / rho; d = P r + beta d_prev, formed on the
+//                fly for every gathered column; q = W d; partial d.q.
+//                When a check is due it computes the true residual instead,
+//                r = b - W64 dx in FP64 (partials r.(P r), r.r, b.b), and the
+//                following k_cg_update idles for that cloth.
+//   k_cg_update  alpha = rho / d.q; dx += alpha d (FP64), r -= alpha q;
+//                partials r.(P r), r.r.
+//
+// The recurrences are Hestenes-Stiefel PCG, the reference's own.  Single-
+// reduction rearrangements (Chronopoulos-Gear, or r.(P r) from an alpha^2
+// q.Pq recurrence) would save a launch per iteration but stall with FP32
+// recurrences on stiff cloths; this form does not.
+//
+// "mixed": W (FP32 copy) and r, q, d are FP32; every dot product and scalar
+// is FP64 and dx accumulates in FP64.  Reliable updates (Sleijpen and van der
+// Vorst, 1996) replace the recursive residual by the true FP64 residual each
+// time r.r has dropped by rel2 since the last replacement, after max_since
+// iterations, and before convergence is accepted; the Krylov recurrence then
+// continues from the replaced residual (beta = r_new.(P r_new) / rho_old).
+// Convergence is always decided on the FP64 true residual:
+// ||b - W64 dx|| <= tol ||b||.  "fp64" runs the same machine with T = double.
+#include "device.h"
+#include "reduce.h"
+
+namespace bw
+{
+namespace
+{
+
+// field-wise state copies: whole-struct copies make the compiler keep a
+// per-thread copy of the state in LDS or scratch
+__device__ inline CgState load_state(const CgState *p)
+{
+    CgState s;
+    s.rho = p->rho;
+    s.rr_ref = p->rr_ref;
+    s.bb = p->bb;
+    s.rr_true = p->rr_true;
+    s.rr = p->rr;
+    s.start = p->start;
+    s.first = p->first;
+    s.iter = p->iter;
+    s.total = p->total;
+    s.since = p->since;
+    s.updates = p->updates;
+    s.done = p->done;
+    s.breakdown = p->breakdown;
+    s.pending = p->pending;
+    s.force_check = p->force_check;
+    s.final_check = p->final_check;
+    s.pad = 0;
+    return s;
+}
+
+__device__ inline void store_state(CgState *p, const CgState &s)
+{
+    p->rho = s.rho;
+    p->rr_ref = s.rr_ref;
+    p->bb = s.bb;
+    p->rr_true = s.rr_true;
+    p->rr = s.rr;
+    p->start = s.start;
+    p->first = s.first;
+    p->iter = s.iter;
+    p->total = s.total;
+    p->since = s.since;
+    p->updates = s.updates;
+    p->done = s.done;
+    p->breakdown = s.breakdown;
+    p->pending = s.pending;
+    p->force_check = s.force_check;
+    p->final_check = s.final_check;
+}
+
+template <typename T> __device__ inline T dir_value(T P, T r, T dold, T beta, bool first)
+{
+    return first ? P * r : P * r + beta * dold;
+}
+
+template <typename T, int SB>
+__global__ void __launch_bounds__(SB * 64)
+k_cg_spmv(DeviceBatch d, const T *__restrict__ W, const T *__restrict__ P, T *__restrict__ r,
+          T *__restrict__ q, T *__restrict__ dbuf0, T *__restrict__ dbuf1)
+{
+    __shared__ double lds[8 * SB];
+    const int blk = blockIdx.x;
+    const int inst = d.blk_inst[blk];
+    const InstParams &ip = d.params[inst];
+    const bool lead = blk == ip.blk_begin && threadIdx.x == 0;
+    CgState st = load_state(d.cg + 2 * inst);
+    CgState *st_out = d.cg + 2 * inst + 1;
+    if (st.done)
+    {
+        if (lead)
+            store_state(st_out, st);
+        return;
+    }
+    const double *part_in = d.cg_partial;
+    double *part_out = d.cg_partial + (size_t)PARTS * d.n_blocks_wg;
+    const double tol2 = ip.tol * ip.tol;
+    bool residual = false;
+    double beta = 0.0;
+    if (st.start)
+        residual = true; // r0 = b - W dx0 (BaraffCloth.cpp:515)
+    else
+    {
+        double v[3];
+        const int comp[3] = {P_RZ, P_RR, P_BB};
+        reduce_cloth<SB, 3>(part_in, ip.blk_begin, ip.blk_count, comp, v, lds);
+        const double rz = v[0], rr = v[1];
+        if (st.pending)
+        {
+            // the previous k_cg_spmv computed the true residual
+            st.pending = 0;
+            st.rr_true = rr;
+            st.bb = v[2];
+            if (rr <= tol2 * v[2] || st.final_check || st.total >= ip.max_iter || !isfinite(rr))
+            {
+                st.done = 1;
+                if (lead)
+                    store_state(st_out, st);
+                return;
+            }
+            st.rr_ref = rr;
+            st.since = 0;
+        }
+        else
+        {
+            st.rr = rr;
+            if (st.force_check || st.total >= ip.max_iter || !isfinite(rr) ||
+                (st.since > 0 && (rr <= tol2 * st.bb || rr <= ip.rel2 * st.rr_ref)) ||
+                st.since >= ip.max_since)
+                residual = true;
+        }
+        if (!residual)
+        {
+            beta = st.first ? 0.0 : rz / st.rho;
+            st.rho = rz;
+        }
+    }
+    const int lane = threadIdx.x & 63;
+    const int sl = blk * SB + (threadIdx.x >> 6);
+    const int p = sl * SLICE + lane;
+    const size_t p3 = 3 * (size_t)p;
+    const int w = d.row_w[p];
+    const size_t base = (size_t)d.slice_off[sl];
+    const int *cl = d.col + base + lane;
+    if (residual)
+    {
+        // true residual in FP64: r = b - W64 dx
+        const double *Wl = d.W64 + 9 * base + lane;
+        double y[3] = {0.0, 0.0, 0.0};
+        for (int k = 0; k < w; k++)
+        {
+            const size_t c3 = 3 * (size_t)cl[k * SLICE];
+            const double *wk = Wl + 9 * (size_t)k * SLICE;
+            const double x0 = d.dx[c3], x1 = d.dx[c3 + 1], x2 = d.dx[c3 + 2];
+            for (int a = 0; a < 3; a++)
+                y[a] += wk[(3 * a) * SLICE] * x0 + wk[(3 * a + 1) * SLICE] * x1 +
+                        wk[(3 * a + 2) * SLICE] * x2;
+        }
+        double v[3] = {0.0, 0.0, 0.0};
+        for (int a = 0; a < 3; a++)
+        {
+            const double bv = d.b[p3 + a];
+            const double rv = bv - y[a];
+            const T rt = (T)rv;
+            r[p3 + a] = rt;
+            v[0] += (double)rt * (double)(P[p3 + a] * rt);
+            v[1] += rv * rv;
+            v[2] += bv * bv;
+        }
+        block_sum<SB, 3>(v, lds);
+        const int comp[3] = {P_RZ, P_RR, P_BB};
+        write_partials<3>(part_out, blk, comp, v);
+        st.pending = 1;
+        st.force_check = 0;
+        st.updates += 1;
+        if (st.start)
+        {
+            st.start = 0;
+            st.first = 1;
+        }
+        if (lead)
+            store_state(st_out, st);
+        return;
+    }
+    const bool first = st.first != 0;
+    const T bt = (T)beta;
+    const T *dold = (st.iter & 1) ? dbuf0 : dbuf1;
+    T *dnew = (st.iter & 1) ? dbuf1 : dbuf0;
+    T dn[3];
+    for (int a = 0; a < 3; a++)
+        dn[a] = dir_value<T>(P[p3 + a], r[p3 + a], dold[p3 + a], bt, first);
+    const T *Wl = W + 9 * base + lane;
+    T y0 = 0, y1 = 0, y2 = 0;
+#pragma unroll 2
+    for (int k = 0; k < w; k++)
+    {
+        const size_t c3 = 3 * (size_t)cl[k * SLICE];
+        const T v0 = dir_value<T>(P[c3], r[c3], dold[c3], bt, first);
+        const T v1 = dir_value<T>(P[c3 + 1], r[c3 + 1], dold[c3 + 1], bt, first);
+        const T v2 = dir_value<T>(P[c3 + 2], r[c3 + 2], dold[c3 + 2], bt, first);
+        const T *wk = Wl + 9 * (size_t)k * SLICE;
+        y0 += wk[0 * SLICE] * v0 + wk[1 * SLICE] * v1 + wk[2 * SLICE] * v2;
+        y1 += wk[3 * SLICE] * v0 + wk[4 * SLICE] * v1 + wk[5 * SLICE] * v2;
+        y2 += wk[6 * SLICE] * v0 + wk[7 * SLICE] * v1 + wk[8 * SLICE] * v2;
+    }
+    dnew[p3] = dn[0];
+    dnew[p3 + 1] = dn[1];
+    dnew[p3 + 2] = dn[2];
+    q[p3] = y0;
+    q[p3 + 1] = y1;
+    q[p3 + 2] = y2;
+    double v[1] = {(double)dn[0] * (double)y0 + (double)dn[1] * (double)y1 +
+                   (double)dn[2] * (double)y2};
+    block_sum<SB, 1>(v, lds);
+    const int comp[1] = {P_DQ};
+    write_partials<1>(part_out, blk, comp, v);
+    if (lead)
+        store_state(st_out, st);
+}
+
+template <typename T, int SB>
+__global__ void __launch_bounds__(SB * 64)
+k_cg_update(DeviceBatch d, const T *__restrict__ P, T *__restrict__ r, const T *__restrict__ q,
+            const T *__restrict__ dbuf0, const T *__restrict__ dbuf1)
+{
+    __shared__ double lds[8 * SB];
+    const int blk = blockIdx.x;
+    const int inst = d.blk_inst[blk];
+    const InstParams &ip = d.params[inst];
+    const bool lead = blk == ip.blk_begin && threadIdx.x == 0;
+    CgState st = load_state(d.cg + 2 * inst + 1);
+    CgState *st_out = d.cg + 2 * inst;
+    const double *part_in = d.cg_partial + (size_t)PARTS * d.n_blocks_wg;
+    double *part_out = d.cg_partial;
+    if (st.done)
+    {
+        if (lead)
+            store_state(st_out, st);
+        return;
+    }
+    if (st.pending)
+    {
+        // idle: hand the true-residual partials on to the next k_cg_spmv
+        if (threadIdx.x < PARTS)
+            part_out[PARTS * (size_t)blk + threadIdx.x] = part_in[PARTS * (size_t)blk + threadIdx.x];
+        if (lead)
+            store_state(st_out, st);
+        return;
+    }
+    double v1[1];
+    {
+        const int comp[1] = {P_DQ};
+        reduce_cloth<SB, 1>(part_in, ip.blk_begin, ip.blk_count, comp, v1, lds);
+    }
+    const double dq = v1[0];
+    if (!(dq > 0.0) || !isfinite(dq))
+    {
+        // d.Wd == 0 (BaraffCloth.cpp:547-551: the reference stops before
+        // updating x) or a loss of definiteness: stop after a final true
+        // residual, which the next k_cg_spmv computes
+        st.final_check = 1;
+        st.force_check = 1;
+        st.breakdown = dq == 0.0 ? 0 : 1;
+        if (threadIdx.x < PARTS)
+            part_out[PARTS * (size_t)blk + threadIdx.x] = 0.0;
+        if (lead)
+            store_state(st_out, st);
+        return;
+    }
+    const double alpha = st.rho / dq;
+    const int lane = threadIdx.x & 63;
+    const int sl = blk * SB + (threadIdx.x >> 6);
+    const size_t p3 = 3 * (size_t)(sl * SLICE + lane);
+    const T *dn = (st.iter & 1) ? dbuf1 : dbuf0;
+    double v[2] = {0.0, 0.0};
+    for (int a = 0; a < 3; a++)
+    {
+        d.dx[p3 + a] = d.dx[p3 + a] + alpha * (double)dn[p3 + a];
+        const T rn = (T)((double)r[p3 + a] - alpha * (double)q[p3 + a]);
+        r[p3 + a] = rn;
+        const T z = P[p3 + a] * rn;
+        v[0] += (double)rn * (double)z;
+        v[1] += (double)rn * (double)rn;
+    }
+    block_sum<SB, 2>(v, lds);
+    const int comp[2] = {P_RZ, P_RR};
+    write_partials<2>(part_out, blk, comp, v);
+    st.iter += 1;
+    st.total += 1;
+    st.since += 1;
+    st.first = 0;
+    if (lead)
+        store_state(st_out, st);
+}
+
+// both slots to the solve start (true residual of the warm start first)
+__global__ void k_cg_reset(CgState *st, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 2 * n)
+        return;
+    CgState z = {};
+    z.start = 1;
+    z.first = 1;
+    st[i] = z;
+}
+
+// timing set-up: every cloth mid-solve, at a k_cg_spmv with residual 1
+template <typename T> __global__ void k_cg_timing_setup(DeviceBatch d, T *r)
+{
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < 3 * (size_t)d.n_rows)
+        r[i] = (T)1;
+    if (i < (size_t)d.n_blocks_wg)
+        for (int slot = 0; slot < 2; slot++)
+        {
+            double *part = d.cg_partial + (size_t)slot * PARTS * d.n_blocks_wg + PARTS * i;
+            part[P_RZ] = 1.0;
+            part[P_RR] = 1.0;
+            part[P_BB] = 1.0;
+            part[P_DQ] = 2.0;
+        }
+    if (i < 2 * (size_t)d.n_inst)
+    {
+        CgState z = {};
+        z.rho = 1.0;
+        d.cg[i] = z;
+    }
+}
+
+constexpr int SB = BW_SB;
+
+} // namespace
+
+void launch_cg_reset(const DeviceBatch &d, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_cg_reset, dim3((2 * d.n_inst + 63) / 64), dim3(64), 0, s, d.cg, d.n_inst);
+}
+
+void launch_cg_step(const DeviceBatch &d, bool fp32, int phase, hipStream_t s)
+{
+    const dim3 grid(d.n_blocks_wg), block(SB * 64);
+    if (fp32)
+    {
+        if (phase == 0)
+            hipLaunchKernelGGL((k_cg_spmv<float, SB>), grid, block, 0, s, d, d.W32, d.P32, d.r32,
+                               d.q32, d.d32[0], d.d32[1]);
+        else
+            hipLaunchKernelGGL((k_cg_update<float, SB>), grid, block, 0, s, d, d.P32, d.r32,
+                               d.q32, d.d32[0], d.d32[1]);
+    }
+    else
+    {
+        if (phase == 0)
+            hipLaunchKernelGGL((k_cg_spmv<double, SB>), grid, block, 0, s, d, d.W64, d.P64, d.r64,
+                               d.q64, d.d64[0], d.d64[1]);
+        else
+            hipLaunchKernelGGL((k_cg_update<double, SB>), grid, block, 0, s, d, d.P64, d.r64,
+                               d.q64, d.d64[0], d.d64[1]);
+    }
+}
+
+void launch_cg_timing_setup(const DeviceBatch &d, bool fp32, hipStream_t s)
+{
+    const size_t n = std::max<size_t>(3 * (size_t)d.n_rows, 2 * (size_t)d.n_inst);
+    const dim3 grid((unsigned)((n + 255) / 256));
+    if (fp32)
+        hipLaunchKernelGGL(k_cg_timing_setup<float>, grid, dim3(256), 0, s, d, d.r32);
+    else
+        hipLaunchKernelGGL(k_cg_timing_setup<double>, grid, dim3(256), 0, s, d, d.r64);
+}
+
+} // namespace bw

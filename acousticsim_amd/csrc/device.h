@@ -31,21 +31,43 @@ struct InstParams
     double tol;       // relative residual tolerance
     int max_iter;     // PCG iteration cap (reference mode: max_iters - 1)
     int ref_mode;     // 1: reference semantics (no tolerance stop)
-    int mixed;        // 1: FP32 inner PCG + FP64 refinement
-    int max_outer;    // refinement passes (mixed)
-    double eta_min;   // deepest relative reduction asked of one inner solve
+    double rel2;      // reliable update when r.r drops below rel2 * (r.r at the last one)
+    int max_since;    // ... or after this many iterations
     int slice_begin, slice_count;
     int blk_begin, blk_count; // solver workgroups of the cloth
 };
 
-// Per-cloth solver state.  Each field has one writing kernel (see pcg.hip).
+// Per-cloth state of the reference-semantics PCG (pcg.hip).  Each field has
+// one writing kernel.
 struct InstState
 {
     double rho[2];      // r.(P r) of iterations of parity 0 / 1
-    double rr, bb, rr64, inner_tol2;
+    double rr, bb, rr64;
     int it_a;           // next iteration (written by the update kernel)
     int it_b;           // current iteration (written by the SpMV kernel)
-    int done, outer, outer_done, total_iter, breakdown, pad;
+    int done, total_iter, breakdown, pad;
+};
+
+// Per-cloth state of the tolerance-mode PCG machine (cg.hip); two slots:
+// k_cg_spmv reads slot 0 and writes slot 1, k_cg_update the reverse.
+struct CgState
+{
+    double rho;     // r.(P r) of the current direction
+    double rr_ref;  // true r.r at the last residual replacement
+    double bb;      // b.b
+    double rr_true; // true r.r at the last check
+    double rr;      // recursive r.r
+    int start;      // 1: the warm start's residual is still to be formed
+    int first;      // 1: next direction is P r (no beta term)
+    int iter;       // direction index (selects the direction buffer)
+    int total;      // iterations this solve
+    int since;      // iterations since the last residual replacement
+    int updates;    // true residuals formed this solve
+    int done, breakdown;
+    int pending;     // 1: the last k_cg_spmv formed a true residual
+    int force_check; // 1: form a true residual next
+    int final_check; // 1: stop after the next true residual
+    int pad;
 };
 
 struct DeviceBatch
@@ -81,11 +103,16 @@ struct DeviceBatch
     // state and vectors (3 per row)
     double *x = nullptr, *xpre = nullptr, *dx = nullptr, *b = nullptr, *P64 = nullptr;
     double *r64 = nullptr, *d64[2] = {nullptr, nullptr}, *q64 = nullptr;
-    float *P32 = nullptr, *r32 = nullptr, *e32 = nullptr, *d32[2] = {nullptr, nullptr},
-          *q32 = nullptr;
+    float *P32 = nullptr;
     double *partial = nullptr; // PARTS per solver workgroup
     InstParams *params = nullptr;
     InstState *state = nullptr;
+    // tolerance-mode PCG: state (2 slots per cloth), partials (2 slots x
+    // PARTS per solver workgroup), FP32 vectors of the mixed form (the FP64
+    // form uses r64, q64, d64)
+    CgState *cg = nullptr;
+    double *cg_partial = nullptr;
+    float *r32 = nullptr, *q32 = nullptr, *d32[2] = {nullptr, nullptr};
 };
 
 // assembly (assemble.hip)
@@ -95,11 +122,13 @@ void launch_assemble_rows(const DeviceBatch &d, hipStream_t s);
 void launch_reset_states(const DeviceBatch &d, hipStream_t s);
 void launch_pcg_init64(const DeviceBatch &d, hipStream_t s);
 void launch_pcg_iter64(const DeviceBatch &d, hipStream_t s);
-// mode 0: FP64 solve start; 1: refinement pass; 2: counters only
+// mode 0: FP64 solve start; 2: counters only
 void launch_residual64(const DeviceBatch &d, int mode, hipStream_t s);
-void launch_pcg_iter32(const DeviceBatch &d, hipStream_t s);
-void launch_correct(const DeviceBatch &d, hipStream_t s);
 void launch_update_positions(const DeviceBatch &d, hipStream_t s);
+// tolerance-mode PCG (cg.hip); phase 0: k_cg_spmv, 1: k_cg_update
+void launch_cg_reset(const DeviceBatch &d, hipStream_t s);
+void launch_cg_step(const DeviceBatch &d, bool fp32, int phase, hipStream_t s);
+void launch_cg_timing_setup(const DeviceBatch &d, bool fp32, hipStream_t s);
 // measurement helpers: set the solver residual to 1 on every row; one SpMV
 // launch of the solver's precision
 void launch_fill_residual(const DeviceBatch &d, bool fp32, hipStream_t s);

@@ -27,79 +27,17 @@
 //                   r -= alpha q, partials r.(P r) and r.r.
 // Direction buffers alternate by iteration parity.
 //
-// Precision: "fp64" runs everything in FP64.  "mixed" runs the inner PCG on
-// the FP32 copy of W with FP32 vectors but FP64 dots and scalars, inside an
-// FP64 residual refinement: r64 = b - W64 dx, inner solve W32 e = r64 to a
-// relative reduction eta, dx += e, repeat until ||r64|| <= tol ||b||.
+// This is the solver of "reference" solver mode (FP64, exactly max_iters-1
+// iterations, the reference's own recurrences and break conditions).  The
+// tolerance modes use the state machine of cg.hip, which runs the same
+// recurrences with a tolerance stop and reliable updates.
 #include "device.h"
+#include "reduce.h"
 
 namespace bw
 {
 namespace
 {
-
-constexpr int NP = PARTS; // partial components per workgroup
-enum Part
-{
-    P_DQ = 0,   // d.Wd
-    P_RZ = 1,   // r.(P r)
-    P_RR = 2,   // r.r (solver precision)
-    P_BB = 3,   // b.b
-    P_RR64 = 4, // FP64 residual r64.r64
-};
-
-__device__ inline double wave_sum(double v)
-{
-    for (int off = 32; off > 0; off >>= 1)
-        v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-// block-wide sums of up to 5 values, fixed order; result in every thread
-template <int SB, int N>
-__device__ inline void block_sum(double (&v)[N], double *lds)
-{
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int j = 0; j < N; j++)
-        v[j] = wave_sum(v[j]);
-    if (SB == 1)
-        return;
-    __syncthreads();
-    if (lane == 0)
-        for (int j = 0; j < N; j++)
-            lds[wave * 8 + j] = v[j];
-    __syncthreads();
-    for (int j = 0; j < N; j++)
-    {
-        double s = lds[j];
-        for (int w = 1; w < SB; w++)
-            s += lds[w * 8 + j];
-        v[j] = s;
-    }
-    __syncthreads();
-}
-
-// write this workgroup's partials (values already block-summed)
-template <int N>
-__device__ inline void write_partials(double *part, int blk, const int (&comp)[N], const double (&v)[N])
-{
-    if (threadIdx.x == 0)
-        for (int j = 0; j < N; j++)
-            part[NP * (size_t)blk + comp[j]] = v[j];
-}
-
-// sum the cloth's partials (fixed order), result in every thread
-template <int SB, int N>
-__device__ inline void reduce_cloth(const double *part, int b0, int nb, const int (&comp)[N],
-                                    double (&out)[N], double *lds)
-{
-    for (int j = 0; j < N; j++)
-        out[j] = 0.0;
-    for (int i = threadIdx.x; i < nb; i += SB * 64)
-        for (int j = 0; j < N; j++)
-            out[j] += part[NP * (size_t)(b0 + i) + comp[j]];
-    block_sum<SB, N>(out, lds);
-}
 
 template <typename T>
 __device__ inline T dir_value(T P, T r, T dold, T beta, bool first)
@@ -129,11 +67,7 @@ k_pcg_spmv(DeviceBatch d, const T *__restrict__ W, const T *__restrict__ P,
         reduce_cloth<SB, 2>(d.partial, ip.blk_begin, ip.blk_count, comp, v, lds);
         const double rho = v[0], rr = v[1];
         beta = rho / st->rho[(k - 1) & 1];
-        bool done;
-        if (ip.ref_mode)
-            done = k >= ip.max_iter || beta == 0.0; // :529, :585-590
-        else
-            done = rr <= st->inner_tol2 || st->total_iter >= ip.max_iter || !isfinite(rr);
+        const bool done = k >= ip.max_iter || beta == 0.0; // :529, :585-590
         if (blk == ip.blk_begin && threadIdx.x == 0)
         {
             st->rho[k & 1] = rho;
@@ -243,17 +177,12 @@ k_pcg_update(DeviceBatch d, const T *__restrict__ P, T *__restrict__ x, T *__res
     write_partials<2>(d.partial, blk, comp, vals);
 }
 
-// r64 = b - W64 dx.  mode 0: FP64 solver start (r64 kept, partials rz, rr,
-// bb); mode 1: refinement pass (r32 = float(r64), e32 = 0, partials of r32
-// plus rr64 and bb); mode 2: residual for the step counters only.
+// r64 = b - W64 dx.  mode 0: solver start (r64 kept, partials rz, rr, bb);
+// mode 2: residual for the step counters only.
 template <int SB> __global__ void __launch_bounds__(SB * 64) k_residual64(DeviceBatch d, int mode)
 {
     __shared__ double lds[8 * SB];
     const int blk = blockIdx.x;
-    const int inst = d.blk_inst[blk];
-    const InstState *st = d.state + inst;
-    if (mode == 1 && st->outer_done)
-        return;
     const int lane = threadIdx.x & 63;
     const int s = blk * SB + (threadIdx.x >> 6);
     const int p = s * SLICE + lane;
@@ -285,82 +214,35 @@ template <int SB> __global__ void __launch_bounds__(SB * 64) k_residual64(Device
             v[0] += rv * (d.P64[p3 + a] * rv);
             v[1] += rv * rv;
         }
-        else if (mode == 1)
-        {
-            const float rf = (float)rv;
-            d.r32[p3 + a] = rf;
-            d.e32[p3 + a] = 0.0f;
-            const float z = d.P32[p3 + a] * rf;
-            v[0] += (double)rf * (double)z;
-            v[1] += (double)rf * (double)rf;
-        }
     }
     block_sum<SB, 4>(v, lds);
     const int comp[4] = {P_RZ, P_RR, P_BB, P_RR64};
     write_partials<4>(d.partial, blk, comp, v);
 }
 
-// One workgroup per cloth: reduce the residual partials and decide.
-//   mode 0 (FP64 start): rho_0, stop if ||r|| <= tol ||b||
-//   mode 1 (refinement): converged, or start an inner FP32 solve
-//   mode 2 (counters):   record ||r64|| and ||b||
+// One workgroup per cloth: reduce the residual partials.
+//   mode 0 (solve start): rho_0; done at once when max_iters <= 1
+//   mode 2 (counters):    record ||r64|| and ||b||
 template <int SB> __global__ void __launch_bounds__(SB * 64) k_decide(DeviceBatch d, int mode)
 {
     __shared__ double lds[8 * SB];
     const int inst = blockIdx.x;
     InstState *st = d.state + inst;
     const InstParams &ip = d.params[inst];
-    if (mode == 1 && st->outer_done)
-        return;
     double v[4];
     const int comp[4] = {P_RZ, P_RR, P_BB, P_RR64};
     reduce_cloth<SB, 4>(d.partial, ip.blk_begin, ip.blk_count, comp, v, lds);
     if (threadIdx.x != 0)
         return;
-    const double rz = v[0], rr = v[1], bb = v[2], rr64 = v[3];
-    st->bb = bb;
-    st->rr64 = rr64;
+    st->bb = v[2];
+    st->rr64 = v[3];
     if (mode == 2)
         return;
     st->it_a = 0;
     st->it_b = 0;
-    st->rho[0] = rz;
-    st->rr = rr;
-    if (mode == 0)
-    {
-        st->inner_tol2 = ip.tol * ip.tol * bb;
-        st->done = ip.ref_mode ? (ip.max_iter <= 0) : (rr <= st->inner_tol2);
-        return;
-    }
-    const double tol2 = ip.tol * ip.tol * bb;
-    if (rr64 <= tol2 || st->outer >= ip.max_outer || st->total_iter >= ip.max_iter ||
-        !isfinite(rr64))
-    {
-        st->outer_done = 1;
-        st->done = 1;
-        return;
-    }
-    st->outer += 1;
-    // inner target: the outer tolerance with a margin, but no deeper than
-    // FP32 arithmetic carries
-    double eta = 0.5 * ip.tol * sqrt(bb / rr64);
-    if (eta < ip.eta_min)
-        eta = ip.eta_min;
-    st->inner_tol2 = eta * eta * rr;
-    st->done = rr <= st->inner_tol2 ? 1 : 0;
-}
-
-// dx += e after an inner solve
-__global__ void __launch_bounds__(256) k_correct(DeviceBatch d)
-{
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= d.n_rows)
-        return;
-    if (d.state[d.blk_inst[p / (SLICE * BW_SB)]].outer_done)
-        return;
-    const size_t p3 = 3 * (size_t)p;
-    for (int a = 0; a < 3; a++)
-        d.dx[p3 + a] = d.dx[p3 + a] + (double)d.e32[p3 + a];
+    st->rho[0] = v[0];
+    st->rr = v[1];
+    st->done = ip.max_iter <= 0;
 }
 
 // x_pre <- x ; x <- x_pre + dx (BaraffCloth.cpp:150-153)
@@ -420,23 +302,10 @@ void launch_pcg_iter64(const DeviceBatch &d, hipStream_t s)
                        d.P64, d.dx, d.r64, d.d64[0], d.d64[1], d.q64);
 }
 
-void launch_pcg_iter32(const DeviceBatch &d, hipStream_t s)
-{
-    hipLaunchKernelGGL((k_pcg_spmv<float, SB>), dim3(d.n_blocks_wg), dim3(SB * 64), 0, s, d,
-                       d.W32, d.P32, d.r32, d.d32[0], d.d32[1], d.q32);
-    hipLaunchKernelGGL((k_pcg_update<float, SB>), dim3(d.n_blocks_wg), dim3(SB * 64), 0, s, d,
-                       d.P32, d.e32, d.r32, d.d32[0], d.d32[1], d.q32);
-}
-
 void launch_residual64(const DeviceBatch &d, int mode, hipStream_t s)
 {
     hipLaunchKernelGGL(k_residual64<SB>, dim3(d.n_blocks_wg), dim3(SB * 64), 0, s, d, mode);
     hipLaunchKernelGGL(k_decide<SB>, dim3(d.n_inst), dim3(SB * 64), 0, s, d, mode);
-}
-
-void launch_correct(const DeviceBatch &d, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_correct, dim3((d.n_rows + 255) / 256), dim3(256), 0, s, d);
 }
 
 void launch_update_positions(const DeviceBatch &d, hipStream_t s)
@@ -444,24 +313,17 @@ void launch_update_positions(const DeviceBatch &d, hipStream_t s)
     hipLaunchKernelGGL(k_update_positions, dim3((d.n_rows + 255) / 256), dim3(256), 0, s, d);
 }
 
-void launch_fill_residual(const DeviceBatch &d, bool fp32, hipStream_t s)
+void launch_fill_residual(const DeviceBatch &d, bool, hipStream_t s)
 {
     const size_t n = 3 * (size_t)d.n_rows;
     const dim3 grid((unsigned)((n + 255) / 256));
-    if (fp32)
-        hipLaunchKernelGGL(k_fill<float>, grid, dim3(256), 0, s, d.r32, n, 1.0f);
-    else
-        hipLaunchKernelGGL(k_fill<double>, grid, dim3(256), 0, s, d.r64, n, 1.0);
+    hipLaunchKernelGGL(k_fill<double>, grid, dim3(256), 0, s, d.r64, n, 1.0);
 }
 
-void launch_spmv_only(const DeviceBatch &d, bool fp32, hipStream_t s)
+void launch_spmv_only(const DeviceBatch &d, bool, hipStream_t s)
 {
-    if (fp32)
-        hipLaunchKernelGGL((k_pcg_spmv<float, SB>), dim3(d.n_blocks_wg), dim3(SB * 64), 0, s, d,
-                           d.W32, d.P32, d.r32, d.d32[0], d.d32[1], d.q32);
-    else
-        hipLaunchKernelGGL((k_pcg_spmv<double, SB>), dim3(d.n_blocks_wg), dim3(SB * 64), 0, s,
-                           d, d.W64, d.P64, d.r64, d.d64[0], d.d64[1], d.q64);
+    hipLaunchKernelGGL((k_pcg_spmv<double, SB>), dim3(d.n_blocks_wg), dim3(SB * 64), 0, s, d,
+                       d.W64, d.P64, d.r64, d.d64[0], d.d64[1], d.q64);
 }
 
 } // namespace bw

@@ -1,0 +1,75 @@
+// Deterministic workgroup reductions shared by the solver kernels.
+//
+// A workgroup is SB wavefronts of 64 lanes.  Sums run in a fixed order: a
+// butterfly inside each wavefront, then wavefront 0..SB-1 in order.  A
+// cloth's partials (one per workgroup) are summed by every workgroup of the
+// cloth in the same fixed order, so all of them obtain identical scalars.
+#pragma once
+#include "device.h"
+
+namespace bw
+{
+
+// partial-sum components (PARTS per workgroup)
+enum Part
+{
+    P_DQ = 0,   // d.Wd  (PCG) / w.u (single-reduction CG)
+    P_RZ = 1,   // r.(P r)
+    P_RR = 2,   // r.r in the solver's precision
+    P_BB = 3,   // b.b
+    P_RR64 = 4, // FP64 true residual r64.r64
+};
+
+__device__ inline double wave_sum(double v)
+{
+    for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// block-wide sums of N values (N <= 8), result in every thread
+template <int SB, int N> __device__ inline void block_sum(double (&v)[N], double *lds)
+{
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int j = 0; j < N; j++)
+        v[j] = wave_sum(v[j]);
+    if (SB == 1)
+        return;
+    __syncthreads();
+    if (lane == 0)
+        for (int j = 0; j < N; j++)
+            lds[wave * 8 + j] = v[j];
+    __syncthreads();
+    for (int j = 0; j < N; j++)
+    {
+        double s = lds[j];
+        for (int w = 1; w < SB; w++)
+            s += lds[w * 8 + j];
+        v[j] = s;
+    }
+    __syncthreads();
+}
+
+// store this workgroup's (block-summed) values as its partials
+template <int N>
+__device__ inline void write_partials(double *part, int blk, const int (&comp)[N], const double (&v)[N])
+{
+    if (threadIdx.x == 0)
+        for (int j = 0; j < N; j++)
+            part[PARTS * (size_t)blk + comp[j]] = v[j];
+}
+
+// sum the cloth's partials (workgroups b0 .. b0+nb-1), result in every thread
+template <int SB, int N>
+__device__ inline void reduce_cloth(const double *part, int b0, int nb, const int (&comp)[N],
+                                    double (&out)[N], double *lds)
+{
+    for (int j = 0; j < N; j++)
+        out[j] = 0.0;
+    for (int i = threadIdx.x; i < nb; i += SB * 64)
+        for (int j = 0; j < N; j++)
+            out[j] += part[PARTS * (size_t)(b0 + i) + comp[j]];
+    block_sum<SB, N>(out, lds);
+}
+
+} // namespace bw

@@ -12,8 +12,10 @@
  * Configuration is the reference's cloth JSON (keys read by cBaraffCloth::Init,
  * BaraffCloth.cpp:22-83, and cBaseCloth::Init, BaseCloth.cpp:44-55), plus
  * optional keys of this library:
- *   "precision":        "mixed" (default; FP32 ELL SpMV, FP64 reductions and
- *                        FP64 residual refinement) or "fp64" (all FP64)
+ *   "precision":        "mixed" (default: FP32 matrix and PCG recurrences,
+ *                        FP64 dot products, FP64 solution, FP64 true-residual
+ *                        replacement and convergence checks) or "fp64"
+ *                        (everything FP64)
  *   "solver_tolerance": relative residual ||b - W dx|| / ||b|| (default 1e-6)
  *   "solver_max_iters": PCG iteration cap per solve (default 20000)
  *   "solver_mode":      "tolerance" (default) or "reference" (exactly
@@ -39,8 +41,8 @@ typedef struct bw_batch bw_batch;
 
 typedef struct bw_step_stats
 {
-    int pcg_iterations;   /* PCG iterations of the last step (all passes) */
-    int outer_iterations; /* FP64 residual-refinement passes (1 in fp64) */
+    int pcg_iterations;   /* PCG iterations of the last step */
+    int outer_iterations; /* FP64 true-residual checks of the last step */
     double rel_residual;  /* final FP64 true residual / ||b|| on free dofs */
     double step_us;       /* host wall time of the last step */
 } bw_step_stats;
@@ -124,12 +126,14 @@ int bw_batch_get_stats(const bw_batch *b, int i, bw_step_stats *st);
 /* use HIP graph replay for the per-iteration launch chains (default on) */
 int bw_batch_set_graphs(bw_batch *b, int enable);
 /* Measurement (no reference counterpart): time `reps` back-to-back launches
-   of the solver's SpMV kernel (the batch's precision, every cloth active) with
-   HIP events on the batch's stream.  avg_us gets the mean time per launch,
-   alg_bytes the algorithmic bytes one launch moves: 36+4 (FP32) or 72+4
-   (FP64) bytes per true 3x3 block, plus five 3-vectors per row (r, P and the
-   previous direction read; the new direction and W d written). */
-int bw_batch_time_spmv(bw_batch *b, int reps, double *avg_us, double *alg_bytes);
+   of the batch's dominant solver kernel, the PCG SpMV (direction update
+   d = P r + beta d_prev fused into q = W d and the partial d.q), with HIP
+   events on the batch's stream, every cloth active.  avg_us gets the mean
+   time per launch, alg_bytes the algorithmic bytes of one launch: 36 + 4
+   (FP32 matrix) or 72 + 4 (FP64) bytes per true 3x3 block plus 15 vector
+   values per vertex (r, P, d_prev read, d and q written).  Call after at
+   least one step. */
+int bw_batch_time_solver_kernel(bw_batch *b, int reps, double *avg_us, double *alg_bytes);
 
 #ifdef __cplusplus
 }
