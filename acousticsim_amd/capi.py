@@ -286,7 +286,7 @@ class ClothBatch:
 
     def time_solver_kernel(self, reps=50):
         """(mean us per launch of the dominant solver kernel, algorithmic
-        bytes per launch); call after at least one step so W is assembled."""
+        bytes per launch), on the system assembled at the current state."""
         us = np.zeros(1)
         nbytes = np.zeros(1)
         _check(lib().bw_batch_time_solver_kernel(self._h, reps, _dp(us), _dp(nbytes)))

@@ -39,6 +39,39 @@ template <typename T> void upload(T *dst, const std::vector<T> &src)
 }
 } // namespace
 
+// Row order of one cloth: vertices go to "wide" or "narrow" rows by their
+// block-row width and are packed into 64-row slices in vertex order, one open
+// slice per class.  Regular grids alternate 13- and 9-wide rows in a
+// checkerboard, so vertex order would pad every slice to 13; this order pads
+// under 1% while keeping both classes' slices on the same stretch of the
+// cloth (the gathered columns stay cache-local).
+std::vector<int> slice_row_order(const ClothModel &m)
+{
+    int wmin = 1 << 30, wmax = 0;
+    for (int v = 0; v < m.nv; v++)
+    {
+        const int w = m.row_ptr[v + 1] - m.row_ptr[v];
+        wmin = std::min(wmin, w);
+        wmax = std::max(wmax, w);
+    }
+    std::vector<int> order, wide, narrow;
+    order.reserve(m.nv);
+    for (int v = 0; v < m.nv; v++)
+    {
+        const int w = m.row_ptr[v + 1] - m.row_ptr[v];
+        std::vector<int> &cls = 2 * w > wmin + wmax ? wide : narrow;
+        cls.push_back(v);
+        if ((int)cls.size() == SLICE)
+        {
+            order.insert(order.end(), cls.begin(), cls.end());
+            cls.clear();
+        }
+    }
+    order.insert(order.end(), wide.begin(), wide.end());
+    order.insert(order.end(), narrow.begin(), narrow.end());
+    return order;
+}
+
 Batch::Batch(std::vector<ClothModel> models) : models_(std::move(models))
 {
     if (models_.empty())
@@ -87,6 +120,7 @@ void Batch::build_layout()
 {
     const int n = (int)models_.size();
     row_base_.resize(n);
+    row_of_.resize(n);
     slice_base_.resize(n);
     nslices_.resize(n);
     int slices = 0;
@@ -95,6 +129,10 @@ void Batch::build_layout()
     for (int i = 0; i < n; i++)
     {
         const ClothModel &m = models_[i];
+        const std::vector<int> order = slice_row_order(m);
+        row_of_[i].assign(m.nv, 0);
+        for (int r = 0; r < m.nv; r++)
+            row_of_[i][order[r]] = r;
         slice_base_[i] = slices;
         nslices_[i] = ((m.nv + SLICE - 1) / SLICE + BW_SB - 1) / BW_SB * BW_SB;
         row_base_[i] = slices * SLICE;
@@ -111,6 +149,8 @@ void Batch::build_layout()
     if ((long long)ntri >= (1LL << 27) || (long long)nie >= (1LL << 27))
         throw std::runtime_error("batch: too many elements");
     d_.n_inst = n;
+    if (const char *e = std::getenv("BW_XCD_MAP"))
+        d_.xcd_map = std::atoi(e) != 0;
     d_.n_slices = slices;
     d_.n_rows = slices * SLICE;
     d_.n_vert = nvert;
@@ -128,7 +168,7 @@ void Batch::build_layout()
         const ClothModel &m = models_[i];
         for (int v = 0; v < m.nv; v++)
         {
-            const int p = row_base_[i] + v;
+            const int p = row_base_[i] + row_of_[i][v];
             row_vertex[p] = vert_base[i] + v;
             row_w[p] = m.row_ptr[v + 1] - m.row_ptr[v];
             row_pinned[p] = m.pinned[v];
@@ -165,18 +205,19 @@ void Batch::build_layout()
     {
         const ClothModel &m = models_[i];
         const int rb = row_base_[i];
+        const std::vector<int> &ro = row_of_[i];
         for (int v = 0; v < m.nv; v++)
         {
             const int vg = vert_base[i] + v;
-            vert_row[vg] = rb + v;
+            vert_row[vg] = rb + ro[v];
             blk_ptr[vg + 1] = blk_base[i] + m.row_ptr[v + 1];
             vt_ptr[vg + 1] = (int)vt_list.size() + m.vt_ptr[v + 1];
             ve_ptr[vg + 1] = (int)ve_list.size() + m.ve_ptr[v + 1];
             for (int k = m.row_ptr[v]; k < m.row_ptr[v + 1]; k++)
             {
-                const int p = rb + v, ks = k - m.row_ptr[v];
+                const int p = rb + ro[v], ks = k - m.row_ptr[v];
                 const int s = p / SLICE, l = p % SLICE;
-                col[slice_off[s] + (size_t)ks * SLICE + l] = rb + m.cols[k];
+                col[slice_off[s] + (size_t)ks * SLICE + l] = rb + ro[m.cols[k]];
             }
         }
         if (m.nv > 0)
@@ -193,7 +234,7 @@ void Batch::build_layout()
         for (int k = 0; k < m.nnzb(); k++)
         {
             const int gb = blk_base[i] + k;
-            blk_col[gb] = rb + m.cols[k];
+            blk_col[gb] = rb + ro[m.cols[k]];
             s_ptr[gb + 1] = sb + m.s_ptr[k + 1];
             b_ptr[gb + 1] = bb + m.b_ptr[k + 1];
         }
@@ -210,7 +251,7 @@ void Batch::build_layout()
         {
             const int tg = tri_base[i] + t;
             for (int k = 0; k < 3; k++)
-                tri_rows[3 * (size_t)tg + k] = rb + m.tri[t][k];
+                tri_rows[3 * (size_t)tg + k] = rb + ro[m.tri[t][k]];
             tri_inst[tg] = i;
             for (int k = 0; k < 12; k++)
                 tri_coef[12 * (size_t)tg + k] = m.coef[12 * (size_t)t + k];
@@ -220,7 +261,7 @@ void Batch::build_layout()
         {
             const size_t g = ie_base[i] + ie;
             for (int k = 0; k < 4; k++)
-                ie_rows[4 * g + k] = rb + m.stencil[m.interior[ie]][k];
+                ie_rows[4 * g + k] = rb + ro[m.stencil[m.interior[ie]][k]];
             ie_factor[g] = m.bend_factor[ie];
         }
         InstParams &P = params[i];
@@ -362,22 +403,33 @@ void Batch::refresh_pins(int i, const std::vector<int> &pins)
     ClothModel &m = models_[i];
     m.pins = pins;
     m.rebuild_pins_from_list();
-    check(hipMemcpy(d_.row_pinned + row_base_[i], m.pinned.data(), m.nv, hipMemcpyHostToDevice),
+    std::vector<uint8_t> rows((size_t)nslices_[i] * SLICE, 0);
+    for (int v = 0; v < m.nv; v++)
+        rows[row_of_[i][v]] = m.pinned[v];
+    check(hipMemcpy(d_.row_pinned + row_base_[i], rows.data(), rows.size(), hipMemcpyHostToDevice),
           "hipMemcpy pins");
 }
 
 void Batch::upload_state(int i, const double *src, double *dst_dev)
 {
-    check(hipMemcpy(dst_dev + 3 * (size_t)row_base_[i], src,
-                    3 * (size_t)models_[i].nv * sizeof(double), hipMemcpyHostToDevice),
+    std::vector<double> rows(3 * (size_t)nslices_[i] * SLICE, 0.0);
+    for (int v = 0; v < models_[i].nv; v++)
+        for (int a = 0; a < 3; a++)
+            rows[3 * (size_t)row_of_[i][v] + a] = src[3 * (size_t)v + a];
+    check(hipMemcpy(dst_dev + 3 * (size_t)row_base_[i], rows.data(), rows.size() * sizeof(double),
+                    hipMemcpyHostToDevice),
           "hipMemcpy state");
 }
 
 void Batch::download(int i, const double *src_dev, double *dst) const
 {
-    check(hipMemcpy(dst, src_dev + 3 * (size_t)row_base_[i],
-                    3 * (size_t)models_[i].nv * sizeof(double), hipMemcpyDeviceToHost),
+    std::vector<double> rows(3 * (size_t)nslices_[i] * SLICE);
+    check(hipMemcpy(rows.data(), src_dev + 3 * (size_t)row_base_[i], rows.size() * sizeof(double),
+                    hipMemcpyDeviceToHost),
           "hipMemcpy D2H");
+    for (int v = 0; v < models_[i].nv; v++)
+        for (int a = 0; a < 3; a++)
+            dst[3 * (size_t)v + a] = rows[3 * (size_t)row_of_[i][v] + a];
 }
 
 void Batch::get_pos(int i, double *x) const { download(i, d_.x, x); }
@@ -411,7 +463,7 @@ void Batch::assemble_system(int i, double *W, double *rhs)
           "hipMemcpy W");
     for (int v = 0; v < m.nv; v++)
     {
-        const int p = row_base_[i] + v, s = p / SLICE, l = p % SLICE;
+        const int p = row_base_[i] + row_of_[i][v], s = p / SLICE, l = p % SLICE;
         const size_t base = 9 * (size_t)(off[s - s0] - e0);
         for (int k = m.row_ptr[v]; k < m.row_ptr[v + 1]; k++)
         {
@@ -427,7 +479,9 @@ double Batch::time_solver_kernel(int reps, double *alg_bytes)
 {
     if (reps < 2)
         throw std::runtime_error("batch: reps must be >= 2");
-    reps &= ~1;
+    // the system at the current state (W and the Jacobi diagonal)
+    launch_elements(d_, stream_);
+    launch_assemble_rows(d_, stream_);
     hipEvent_t e0, e1;
     check(hipEventCreate(&e0), "hipEventCreate");
     check(hipEventCreate(&e1), "hipEventCreate");

@@ -20,7 +20,7 @@ struct StepStats
 };
 
 // how a batch solves: the reference's own PCG with exactly max_iters-1
-// iterations (FP64), or the tolerance-mode single-reduction CG in FP32/FP64
+// iterations (FP64), or the tolerance-mode PCG machine in FP32 (mixed) or FP64
 enum class SolverKind
 {
     RefPCG64,
@@ -57,7 +57,6 @@ public:
     double time_solver_kernel(int reps, double *alg_bytes);
 
     const DeviceBatch &device() const { return d_; }
-    int row_base(int i) const { return row_base_[i]; }
 
 private:
     void build_layout();
@@ -71,6 +70,7 @@ private:
 
     std::vector<ClothModel> models_;
     std::vector<int> row_base_, slice_base_, nslices_;
+    std::vector<std::vector<int>> row_of_; // per cloth: vertex -> row within the cloth
     std::vector<StepStats> stats_;
     DeviceBatch d_;
     hipStream_t stream_ = nullptr;

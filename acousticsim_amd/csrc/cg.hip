@@ -97,7 +97,7 @@ k_cg_spmv(DeviceBatch d, const T *__restrict__ W, const T *__restrict__ P, T *__
           T *__restrict__ q, T *__restrict__ dbuf0, T *__restrict__ dbuf1)
 {
     __shared__ double lds[8 * SB];
-    const int blk = blockIdx.x;
+    const int blk = solver_block(d);
     const int inst = d.blk_inst[blk];
     const InstParams &ip = d.params[inst];
     const bool lead = blk == ip.blk_begin && threadIdx.x == 0;
@@ -241,7 +241,7 @@ k_cg_update(DeviceBatch d, const T *__restrict__ P, T *__restrict__ r, const T *
             const T *__restrict__ dbuf0, const T *__restrict__ dbuf1)
 {
     __shared__ double lds[8 * SB];
-    const int blk = blockIdx.x;
+    const int blk = solver_block(d);
     const int inst = d.blk_inst[blk];
     const InstParams &ip = d.params[inst];
     const bool lead = blk == ip.blk_begin && threadIdx.x == 0;

@@ -73,6 +73,7 @@ struct CgState
 struct DeviceBatch
 {
     int n_inst = 0, n_slices = 0, n_rows = 0;
+    int xcd_map = 1; // solver workgroups remapped so each XCD owns a contiguous range
     long long n_entries = 0;
     int n_tri = 0, n_ie = 0, n_vert = 0, n_blocks = 0;
     int n_blocks_wg = 0; // solver workgroups
@@ -114,6 +115,20 @@ struct DeviceBatch
     double *cg_partial = nullptr;
     float *r32 = nullptr, *q32 = nullptr, *d32[2] = {nullptr, nullptr};
 };
+
+// Workgroup b of a G-workgroup launch runs on XCD b % 8 (the dispatcher deals
+// workgroups round-robin over the 8 XCDs).  The logical block returned here
+// gives XCD x the contiguous range [x*q + min(x, G%8), ...) of G/8 (+1)
+// blocks, so neighbouring slices of a cloth, which gather the same vector
+// entries, share one XCD's L2.
+__device__ inline int solver_block(const DeviceBatch &d)
+{
+    const int b = blockIdx.x;
+    if (!d.xcd_map)
+        return b;
+    const int G = gridDim.x, q = G / 8, rem = G % 8, x = b % 8;
+    return x * q + (x < rem ? x : rem) + b / 8;
+}
 
 // assembly (assemble.hip)
 void launch_elements(const DeviceBatch &d, hipStream_t s);

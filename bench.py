@@ -5,8 +5,9 @@ side are drawn uniformly from 17..129 (seeded), stiffness scaled
 log-uniformly in [0.1, 10] per cloth, two top corners pinned, +-1 mm seeded
 out-of-plane jitter, dt = 0.01 s, Jacobi-PCG to a 1e-6 relative residual.
 All cloths step together (mixed precision: FP32 matrix and PCG recurrences,
-FP64 dot products and solution, FP64 true-residual replacement).  A "step" is one substep of all 64 cloths; the
-headline value is cloth-steps per second (64 per step).
+FP64 dot products and solution, FP64 true-residual replacement).  A "step" is
+one substep of all 64 cloths; the headline value is cloth-steps per second
+(64 per step).
 
 Reported beside it, in the same run:
   * step latency p50 / p99 (per batch step)
@@ -20,6 +21,10 @@ Reported beside it, in the same run:
     method, one thread, on a bounded sample of the same cloths
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batch64|single257|single21]
+       python bench.py --roofline-only [--copy-calibration]
+The second form only assembles the system and times the dominant kernel (and,
+with --copy-calibration, first copies 256 MiB device-to-device three times):
+the command profiled for the kernel's rocprofv3 duration and HBM counters.
 """
 import argparse
 import json
@@ -38,6 +43,11 @@ from acousticsim_amd import workloads as wl  # noqa: E402
 
 METRIC = "cloth steps/s & p50/p99 step latency at 1e-6 PCG residual; SpMV HBM GB/s"
 HBM_PEAK_GBS = 8000.0
+KERNEL = "k_cg_spmv<float, 4> (PCG SpMV: d = P r + beta d, q = W d, d.q)"
+# HBM bytes per k_cg_spmv launch on batch64, from rocprofv3 --pmc FETCH_SIZE
+# and WRITE_SIZE over the --roofline-only command, scaled by the same
+# counters' reading for a 256 MiB device-to-device copy
+TRAFFIC = {"batch64": None}
 
 
 def members(workload, precision):
@@ -147,6 +157,25 @@ def cpu_baseline(ms, budget_s=20.0):
                       "gcc -O2, host %s" % (n_steps, len(ms), cpu)}
 
 
+def roofline_only(args, ms):
+    if args.copy_calibration:
+        import torch
+        n = 64 << 20  # 256 MiB of float32
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print("bench: copied %d bytes x3" % (4 * n), file=sys.stderr, flush=True)
+    b = capi.ClothBatch([c for c, _ in ms], [x for _, x in ms])
+    us, nbytes = b.time_solver_kernel(reps=200)
+    b.close()
+    achieved = nbytes / us / 1e3
+    print(json.dumps({"workload": args.workload, "kernel": KERNEL, "launches": 201,
+                      "launch_us": us, "alg_bytes": nbytes, "achieved_GBs": achieved,
+                      "frac": achieved / HBM_PEAK_GBS}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,6 +184,10 @@ def main():
     ap.add_argument("--workload", default="batch64", choices=["batch64", "single257", "single21"])
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="assemble once and time the dominant kernel only")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="with --roofline-only: copy 256 MiB device-to-device first")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X; multi-GPU "
@@ -163,6 +196,9 @@ def main():
         sys.exit(2)
 
     ms = members(args.workload, "mixed")
+    if args.roofline_only:
+        roofline_only(args, ms)
+        return
     print("bench: %s, %d steps after %d warm-up" % (args.workload, args.steps, args.warmup),
           file=sys.stderr, flush=True)
     b, total, lat, iters, worst = time_batch(ms, args.steps, args.warmup)
@@ -189,8 +225,8 @@ def main():
         "pcg_iterations_mean": iters,
         "max_rel_residual": worst,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_cg_spmv<float, 4> (PCG SpMV: d = P r + beta d, q = W d, d.q)",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC.get(args.workload),
+                     "kernel": KERNEL,
                      "launch_us": us, "alg_bytes": nbytes},
     }
     b.close()
