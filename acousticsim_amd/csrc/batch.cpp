@@ -112,6 +112,10 @@ Batch::~Batch()
         (void)hipHostFree(host_state_);
     if (host_cg_)
         (void)hipHostFree(host_cg_);
+    if (host_spmv_list_)
+        (void)hipHostFree(host_spmv_list_);
+    if (host_upd_list_)
+        (void)hipHostFree(host_upd_list_);
     if (stream_)
         (void)hipStreamDestroy(stream_);
 }
@@ -348,6 +352,7 @@ void Batch::build_layout()
         if (kind_ == SolverKind::CG32)
         {
             d_.r32 = dalloc<float>(nv3);
+            d_.z32 = dalloc<float>(nv3);
             d_.q32 = dalloc<float>(nv3);
             d_.d32[0] = dalloc<float>(nv3);
             d_.d32[1] = dalloc<float>(nv3);
@@ -355,12 +360,19 @@ void Batch::build_layout()
         else
         {
             d_.r64 = dalloc<double>(nv3);
+            d_.z64 = dalloc<double>(nv3);
             d_.q64 = dalloc<double>(nv3);
             d_.d64[0] = dalloc<double>(nv3);
             d_.d64[1] = dalloc<double>(nv3);
         }
         d_.cg = dalloc<CgState>(2 * (size_t)n);
-        d_.cg_partial = dalloc<double>(2 * PARTS * (size_t)d_.n_blocks_wg);
+        d_.spmv_list = dalloc<int>(slices);
+        d_.upd_list = dalloc<int>(d_.n_blocks_wg);
+        check(hipHostMalloc((void **)&host_spmv_list_, sizeof(int) * (size_t)slices),
+              "hipHostMalloc");
+        check(hipHostMalloc((void **)&host_upd_list_, sizeof(int) * (size_t)d_.n_blocks_wg),
+              "hipHostMalloc");
+        d_.cg_partial = dalloc<double>(2 * PARTS * (size_t)d_.n_slices);
     }
 
     upload(d_.slice_off, slice_off);
@@ -510,14 +522,17 @@ double Batch::time_solver_kernel(int reps, double *alg_bytes)
         // k_cg_spmv alone: it reads state slot 0 and writes slot 1, so
         // repeated launches redo the same SpMV with every cloth active
         const bool fp32 = kind_ == SolverKind::CG32;
-        launch_cg_timing_setup(d_, fp32, stream_);
-        launch_cg_step(d_, fp32, 0, stream_); // warm
+        DeviceBatch all = d_; // every slice, no lists
+        all.spmv_list = nullptr;
+        all.upd_list = nullptr;
+        launch_cg_timing_setup(all, fp32, stream_);
+        launch_cg_step(all, fp32, 0, all.n_slices, stream_); // warm
         check(hipEventRecord(e0, stream_), "hipEventRecord");
         for (int k = 0; k < reps; k++)
-            launch_cg_step(d_, fp32, 0, stream_);
+            launch_cg_step(all, fp32, 0, all.n_slices, stream_);
         check(hipEventRecord(e1, stream_), "hipEventRecord");
-        // per block: W + column; per row: r, P, d_prev read, d, W d written
-        bytes = fp32 ? 40.0 * nnzb + 60.0 * nv : 76.0 * nnzb + 120.0 * nv;
+        // per block: W + column; per row: z, d_prev read, d, W d written
+        bytes = fp32 ? 40.0 * nnzb + 48.0 * nv : 76.0 * nnzb + 96.0 * nv;
     }
     check(hipEventSynchronize(e1), "hipEventSynchronize");
     float ms = 0;
@@ -542,60 +557,98 @@ void Batch::read_states()
     check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
 }
 
-void Batch::launch_chain(int length)
+void Batch::launch_chain(int length, int slices)
 {
     if (kind_ == SolverKind::RefPCG64)
         for (int k = 0; k < length; k++)
             launch_pcg_iter64(d_, stream_);
     else
         for (int k = 0; k < length; k++)
-            launch_cg_step(d_, kind_ == SolverKind::CG32, k & 1, stream_);
+            launch_cg_step(d_, kind_ == SolverKind::CG32, k & 1, slices, stream_);
 }
 
-void Batch::run_chunk(int length)
+void Batch::run_chunk(int length, int slices)
 {
     if (!use_graphs_)
     {
-        launch_chain(length);
+        launch_chain(length, slices);
         check(hipGetLastError(), "solver launch");
         return;
     }
+    // replayed as graphs of power-of-two lengths (>= 32): chunk lengths vary
+    // from step to step, and a graph per exact length would be instantiated
+    // over and over
+    for (int piece = 1 << 30; piece >= 32; piece >>= 1)
+        if (length & piece)
+            run_graph(piece, slices);
+    if (length & 31)
+        launch_chain(length & 31, slices);
+}
+
+void Batch::run_graph(int length, int slices)
+{
+    ++graph_clock_;
     hipGraphExec_t exec = nullptr;
     for (auto &g : graphs_)
-        if (g.length == length)
+        if (g.length == length && g.slices == slices)
+        {
             exec = g.exec;
+            g.last_use = graph_clock_;
+        }
     if (!exec)
     {
+        // bounded cache: drop the least recently used graph
+        constexpr size_t max_graphs = 128;
+        if (graphs_.size() >= max_graphs)
+        {
+            auto lru = std::min_element(graphs_.begin(), graphs_.end(),
+                                        [](const GraphEntry &a, const GraphEntry &b) {
+                                            return a.last_use < b.last_use;
+                                        });
+            check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+            (void)hipGraphExecDestroy(lru->exec);
+            graphs_.erase(lru);
+        }
         hipGraph_t graph;
         check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "BeginCapture");
-        launch_chain(length);
+        launch_chain(length, slices);
         check(hipStreamEndCapture(stream_, &graph), "EndCapture");
         check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "GraphInstantiate");
         (void)hipGraphDestroy(graph);
-        graphs_.push_back({length, exec});
+        graphs_.push_back({length, slices, exec, graph_clock_});
     }
     check(hipGraphLaunch(exec, stream_), "GraphLaunch");
 }
 
 namespace
 {
-// BW_MAX_CHUNK caps the launches queued between host checks (default 512)
+// BW_MAX_CHUNK caps the launches queued between host checks (default 1024)
 int max_chunk()
 {
     static const int v = [] {
         const char *e = std::getenv("BW_MAX_CHUNK");
-        const int c = e ? std::atoi(e) : 512;
-        return std::max(16, std::min(c, 4096));
+        const int c = e ? std::atoi(e) : 1024;
+        return std::max(32, std::min(c, 4096));
     }();
     return v;
 }
 
-// chunk lengths are multiples of 16 (even, so that a CG chunk ends on parity
-// slot 0) and few distinct graphs get built
-int chunk_for(int estimate)
+// chunk lengths are multiples of 32 (even, so that a CG chunk ends with an
+// update launch) and few distinct graphs get built
+int chunk_for(long long estimate)
 {
-    int c = std::max(16, std::min(estimate, max_chunk()));
-    return ((c + 15) / 16) * 16;
+    const long long c = std::max<long long>(32, std::min<long long>(estimate, max_chunk()));
+    return (int)(((c + 31) / 32) * 32);
+}
+
+// grid sizes of list launches, rounded up to 64 * {1, 1.5, 2, 3, 4, 6, ...}
+// so that few distinct graphs get built; at most `all`
+int slice_bucket(int n, int all)
+{
+    int b = 64;
+    while (b < n)
+        b = (b & (b - 1)) == 0 ? b / 2 * 3 : b / 3 * 4;
+    return std::min(b, all);
 }
 } // namespace
 
@@ -603,7 +656,7 @@ void Batch::solve_ref()
 {
     launch_reset_states(d_, stream_);
     launch_pcg_init64(d_, stream_);
-    int chunk = chunk_for(last_launches_ > 0 ? last_launches_ : 16);
+    int chunk = chunk_for(last_launches_ > 0 ? last_launches_ : 32);
     for (;;)
     {
         read_states();
@@ -612,7 +665,7 @@ void Batch::solve_ref()
             all_done &= host_state_[i].done != 0;
         if (all_done)
             break;
-        run_chunk(chunk);
+        run_chunk(chunk, d_.n_slices);
         chunk = chunk_for(32);
     }
     launch_residual64(d_, 2, stream_);
@@ -630,30 +683,101 @@ void Batch::solve_ref()
     last_launches_ = most;
 }
 
+// Chunk boundaries (launch counts) for this solve from the launches every
+// cloth needed in the previous one: a boundary wherever the cloths expected
+// to be still solving own at most half the slices of the previous boundary.
+std::vector<long long> Batch::chunk_schedule() const
+{
+    std::vector<long long> bounds;
+    if (pred_launches_.empty())
+        return bounds;
+    std::vector<std::pair<long long, int>> v;
+    long long remaining = 0;
+    for (int i = 0; i < size(); i++)
+    {
+        v.push_back({pred_launches_[i], nslices_[i]});
+        remaining += nslices_[i];
+    }
+    std::sort(v.begin(), v.end());
+    long long level = remaining / 2;
+    for (size_t k = 0; k < v.size(); k++)
+    {
+        remaining -= v[k].second;
+        if (remaining <= level || k + 1 == v.size())
+        {
+            bounds.push_back(v[k].first);
+            while (level > 0 && remaining <= level)
+                level /= 2;
+        }
+    }
+    return bounds;
+}
+
+// Lists of the active cloths' slices and solver workgroups (padded with -1
+// to the bucketed grid), uploaded on the stream; returns the k_cg_spmv grid.
+int Batch::upload_lists(const std::vector<char> &active)
+{
+    int n = 0, nb = 0;
+    for (int i = 0; i < size(); i++)
+        if (active[i])
+        {
+            for (int s = 0; s < nslices_[i]; s++)
+                host_spmv_list_[n++] = slice_base_[i] + s;
+            for (int b = 0; b < nslices_[i] / BW_SB; b++)
+                host_upd_list_[nb++] = slice_base_[i] / BW_SB + b;
+        }
+    const int grid = slice_bucket(n, d_.n_slices);
+    const int ugrid = (grid + BW_SB - 1) / BW_SB;
+    for (int k = n; k < grid; k++)
+        host_spmv_list_[k] = -1;
+    for (int k = nb; k < ugrid; k++)
+        host_upd_list_[k] = -1;
+    check(hipMemcpyAsync(d_.spmv_list, host_spmv_list_, grid * sizeof(int), hipMemcpyHostToDevice,
+                         stream_),
+          "hipMemcpyAsync list");
+    check(hipMemcpyAsync(d_.upd_list, host_upd_list_, ugrid * sizeof(int), hipMemcpyHostToDevice,
+                         stream_),
+          "hipMemcpyAsync list");
+    return grid;
+}
+
 void Batch::solve_cg()
 {
     launch_cg_reset(d_, stream_);
-    int chunk = chunk_for(last_launches_ > 0 ? last_launches_ : 16);
-    long long launched = 0;
     long long cap = 16;
     for (auto &m : models_)
         cap = std::max<long long>(cap, 4LL * m.cfg.solver_max_iters + 64);
+    const std::vector<long long> bounds = chunk_schedule();
+    std::vector<char> active(size(), 1);
+    long long launched = 0;
+    size_t next = 0;
+    int blind = 0; // chunks run without a predicted boundary
     for (;;)
     {
-        run_chunk(chunk);
-        launched += chunk;
+        while (next < bounds.size() && bounds[next] < launched + 32)
+            next++;
+        long long want;
+        if (next < bounds.size())
+            want = bounds[next] - launched;
+        else
+            want = 64LL << std::min(blind++, 4);
+        const int length = chunk_for(want);
+        run_chunk(length, upload_lists(active));
+        launched += length;
         read_states();
-        bool all_done = true;
-        for (size_t i = 0; i < models_.size(); i++)
-            all_done &= host_cg_[2 * i].done != 0;
-        if (all_done)
+        bool any = false;
+        for (int i = 0; i < size(); i++)
+        {
+            active[i] = host_cg_[2 * i].done == 0;
+            any |= active[i] != 0;
+        }
+        if (!any)
             break;
         if (launched > cap)
             throw std::runtime_error("step: solver did not terminate");
-        chunk = chunk_for(64);
     }
-    int most = 0;
-    for (size_t i = 0; i < models_.size(); i++)
+    pred_launches_.assign(size(), 0);
+    for (int i = 0; i < size(); i++)
     {
         const CgState &s = host_cg_[2 * i];
         StepStats &st = stats_[i];
@@ -661,9 +785,8 @@ void Batch::solve_cg()
         st.outer_iterations = s.updates;
         st.rel_residual = s.bb > 0 ? std::sqrt(s.rr_true / s.bb) : std::sqrt(s.rr_true);
         // launches this cloth needed: two per iteration and per true residual
-        most = std::max(most, 2 * (s.total + s.updates) + 2);
+        pred_launches_[i] = 2LL * (s.total + s.updates) + 2;
     }
-    last_launches_ = most;
 }
 
 void Batch::step(int nsteps)

@@ -63,8 +63,12 @@ private:
     void upload_state(int i, const double *src, double *dst_dev);
     void download(int i, const double *src_dev, double *dst) const;
     void read_states();
-    void launch_chain(int length); // one chunk of solver launches
-    void run_chunk(int length);    // the chunk, as a replayed graph when enabled
+    // one chunk of solver launches; slices: the k_cg_spmv grid
+    void launch_chain(int length, int slices);
+    void run_chunk(int length, int slices); // as replayed graphs when enabled
+    void run_graph(int length, int slices);
+    std::vector<long long> chunk_schedule() const;
+    int upload_lists(const std::vector<char> &active);
     void solve_ref();
     void solve_cg();
 
@@ -80,12 +84,16 @@ private:
     bool use_graphs_ = true;
     bool graphs_disabled_ = false; // BW_GRAPHS=0 in the environment
     int last_launches_ = 0;
+    std::vector<long long> pred_launches_;      // per cloth, from the last solve
+    int *host_spmv_list_ = nullptr, *host_upd_list_ = nullptr; // pinned
     struct GraphEntry
     {
-        int length;
+        int length, slices;
         hipGraphExec_t exec;
+        unsigned long long last_use;
     };
     std::vector<GraphEntry> graphs_;
+    unsigned long long graph_clock_ = 0;
     std::vector<void *> allocations_;
     template <typename T> T *dalloc(size_t n);
 };

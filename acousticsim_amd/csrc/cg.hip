@@ -3,36 +3,47 @@
 // reliable updates.
 //
 // One iteration is two launches, k_cg_spmv then k_cg_update.  Every cloth's
-// state, and the FP64 partial sums (one set per workgroup), have two slots:
-// k_cg_spmv reads slot 0 and writes slot 1, k_cg_update reads slot 1 and
-// writes slot 0, so no workgroup reads what another workgroup of the same
+// state, and the FP64 partial sums (one set per 64-row slice), have two
+// slots: k_cg_spmv reads slot 0 and writes slot 1, k_cg_update reads slot 1
+// and writes slot 0, so no workgroup reads what another workgroup of the same
 // launch writes, and no grid-wide synchronisation is needed.  Every workgroup
 // of a cloth sums the cloth's partials of the previous launch in the same
 // fixed order, so all of them derive identical scalars and take the same
 // branch, and a cloth's arithmetic does not depend on the rest of the batch.
 //
-//   k_cg_spmv    prologue: reduce r.(P r) and r.r of the previous update;
-//                beta = rho_new / rho; d = P r + beta d_prev, formed on the
+//   k_cg_spmv    one 64-row slice per workgroup, four lanes per row (lane
+//                phase j takes ELL slots j, j+4, ...; the four partial row
+//                sums meet by lane shuffles), which cuts the serial chain of
+//                dependent loads per lane from ~13 blocks to ~4: a cloth
+//                solved alone, late in a batch, is latency-bound.
+//                Prologue: reduce r.z and r.r of the previous update;
+//                beta = rho_new / rho; d = z + beta d_prev, formed on the
 //                fly for every gathered column; q = W d; partial d.q.
 //                When a check is due it computes the true residual instead,
-//                r = b - W64 dx in FP64 (partials r.(P r), r.r, b.b), and the
-//                following k_cg_update idles for that cloth.
-//   k_cg_update  alpha = rho / d.q; dx += alpha d (FP64), r -= alpha q;
-//                partials r.(P r), r.r.
+//                r = b - W64 dx in FP64, z = P r (partials r.z, r.r, b.b),
+//                and the following k_cg_update idles for that cloth.
+//   k_cg_update  four slices per workgroup, one lane per row: alpha =
+//                rho / d.q; dx += alpha d (FP64), r -= alpha q, z = P r;
+//                partials r.z, r.r.
+// z = P r is stored so that the SpMV gathers two vectors per column (z and
+// d_prev) rather than three.  Both kernels take an optional list of the
+// slices (workgroups) to run, so that the host can shrink the grid to the
+// cloths still solving (entries < 0 are padding).
 //
 // The recurrences are Hestenes-Stiefel PCG, the reference's own.  Single-
 // reduction rearrangements (Chronopoulos-Gear, or r.(P r) from an alpha^2
 // q.Pq recurrence) would save a launch per iteration but stall with FP32
 // recurrences on stiff cloths; this form does not.
 //
-// "mixed": W (FP32 copy) and r, q, d are FP32; every dot product and scalar
-// is FP64 and dx accumulates in FP64.  Reliable updates (Sleijpen and van der
-// Vorst, 1996) replace the recursive residual by the true FP64 residual each
-// time r.r has dropped by rel2 since the last replacement, after max_since
-// iterations, and before convergence is accepted; the Krylov recurrence then
-// continues from the replaced residual (beta = r_new.(P r_new) / rho_old).
-// Convergence is always decided on the FP64 true residual:
-// ||b - W64 dx|| <= tol ||b||.  "fp64" runs the same machine with T = double.
+// "mixed": W (FP32 copy) and r, z, q, d are FP32; every dot product and
+// scalar is FP64 and dx accumulates in FP64.  Reliable updates (Sleijpen and
+// van der Vorst, 1996) replace the recursive residual by the true FP64
+// residual each time r.r has dropped by rel2 since the last replacement,
+// after max_since iterations, and before convergence is accepted; the Krylov
+// recurrence then continues from the replaced residual
+// (beta = r_new.z_new / rho_old).  Convergence is always decided on the FP64
+// true residual: ||b - W64 dx|| <= tol ||b||.  "fp64" runs the same machine
+// with T = double.
 #include "device.h"
 #include "reduce.h"
 
@@ -86,21 +97,47 @@ __device__ inline void store_state(CgState *p, const CgState &s)
     p->final_check = s.final_check;
 }
 
-template <typename T> __device__ inline T dir_value(T P, T r, T dold, T beta, bool first)
+#ifndef BW_SPMV_UNROLL
+#define BW_SPMV_UNROLL 1
+#endif
+#define BW_PRAGMA(x) _Pragma(#x)
+#define BW_UNROLL(n) BW_PRAGMA(unroll n)
+
+constexpr int LPR = 4;           // SpMV lanes per row
+constexpr int RPW = SLICE / LPR; // SpMV rows per wavefront
+constexpr int SPMV_WAVES = SLICE / RPW;
+
+// d = z + beta d_prev, z = P r
+template <typename T> __device__ inline T dir_value(T z, T dold, T beta, bool first)
 {
-    return first ? P * r : P * r + beta * dold;
+    return first ? z : z + beta * dold;
 }
 
-template <typename T, int SB>
-__global__ void __launch_bounds__(SB * 64)
-k_cg_spmv(DeviceBatch d, const T *__restrict__ W, const T *__restrict__ P, T *__restrict__ r,
-          T *__restrict__ q, T *__restrict__ dbuf0, T *__restrict__ dbuf1)
+// sum over the four lane phases of a row (lanes l, l+16, l+32, l+48)
+template <typename T> __device__ inline T phase_sum(T v)
 {
-    __shared__ double lds[8 * SB];
-    const int blk = solver_block(d);
-    const int inst = d.blk_inst[blk];
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    return v;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(SPMV_WAVES * 64)
+k_cg_spmv(DeviceBatch d, const T *__restrict__ W, const T *__restrict__ P, T *__restrict__ r,
+          T *__restrict__ z, T *__restrict__ q, T *__restrict__ dbuf0, T *__restrict__ dbuf1)
+{
+    __shared__ double lds[8 * SPMV_WAVES];
+    // one slice per workgroup, from the list of active slices when given
+    int sl = solver_block(d);
+    if (d.spmv_list)
+    {
+        sl = d.spmv_list[sl];
+        if (sl < 0)
+            return;
+    }
+    const int inst = d.slice_inst[sl];
     const InstParams &ip = d.params[inst];
-    const bool lead = blk == ip.blk_begin && threadIdx.x == 0;
+    const bool lead = sl == ip.slice_begin && threadIdx.x == 0;
     CgState st = load_state(d.cg + 2 * inst);
     CgState *st_out = d.cg + 2 * inst + 1;
     if (st.done)
@@ -110,7 +147,7 @@ k_cg_spmv(DeviceBatch d, const T *__restrict__ W, const T *__restrict__ P, T *__
         return;
     }
     const double *part_in = d.cg_partial;
-    double *part_out = d.cg_partial + (size_t)PARTS * d.n_blocks_wg;
+    double *part_out = d.cg_partial + (size_t)PARTS * d.n_slices;
     const double tol2 = ip.tol * ip.tol;
     bool residual = false;
     double beta = 0.0;
@@ -120,7 +157,7 @@ k_cg_spmv(DeviceBatch d, const T *__restrict__ W, const T *__restrict__ P, T *__
     {
         double v[3];
         const int comp[3] = {P_RZ, P_RR, P_BB};
-        reduce_cloth<SB, 3>(part_in, ip.blk_begin, ip.blk_count, comp, v, lds);
+        reduce_cloth<SPMV_WAVES, 3>(part_in, ip.slice_begin, ip.slice_count, comp, v, lds);
         const double rz = v[0], rr = v[1];
         if (st.pending)
         {
@@ -153,18 +190,19 @@ k_cg_spmv(DeviceBatch d, const T *__restrict__ W, const T *__restrict__ P, T *__
         }
     }
     const int lane = threadIdx.x & 63;
-    const int sl = blk * SB + (threadIdx.x >> 6);
-    const int p = sl * SLICE + lane;
+    const int row = (threadIdx.x >> 6) * RPW + (lane & (RPW - 1)); // row within the slice
+    const int j = lane / RPW;                                        // lane phase
+    const int p = sl * SLICE + row;
     const size_t p3 = 3 * (size_t)p;
     const int w = d.row_w[p];
     const size_t base = (size_t)d.slice_off[sl];
-    const int *cl = d.col + base + lane;
+    const int *cl = d.col + base + row;
     if (residual)
     {
         // true residual in FP64: r = b - W64 dx
-        const double *Wl = d.W64 + 9 * base + lane;
+        const double *Wl = d.W64 + 9 * base + row;
         double y[3] = {0.0, 0.0, 0.0};
-        for (int k = 0; k < w; k++)
+        for (int k = j; k < w; k += LPR)
         {
             const size_t c3 = 3 * (size_t)cl[k * SLICE];
             const double *wk = Wl + 9 * (size_t)k * SLICE;
@@ -173,20 +211,25 @@ k_cg_spmv(DeviceBatch d, const T *__restrict__ W, const T *__restrict__ P, T *__
                 y[a] += wk[(3 * a) * SLICE] * x0 + wk[(3 * a + 1) * SLICE] * x1 +
                         wk[(3 * a + 2) * SLICE] * x2;
         }
-        double v[3] = {0.0, 0.0, 0.0};
         for (int a = 0; a < 3; a++)
-        {
-            const double bv = d.b[p3 + a];
-            const double rv = bv - y[a];
-            const T rt = (T)rv;
-            r[p3 + a] = rt;
-            v[0] += (double)rt * (double)(P[p3 + a] * rt);
-            v[1] += rv * rv;
-            v[2] += bv * bv;
-        }
-        block_sum<SB, 3>(v, lds);
+            y[a] = phase_sum(y[a]);
+        double v[3] = {0.0, 0.0, 0.0};
+        if (j == 0)
+            for (int a = 0; a < 3; a++)
+            {
+                const double bv = d.b[p3 + a];
+                const double rv = bv - y[a];
+                const T rt = (T)rv;
+                const T zt = P[p3 + a] * rt;
+                r[p3 + a] = rt;
+                z[p3 + a] = zt;
+                v[0] += (double)rt * (double)zt;
+                v[1] += rv * rv;
+                v[2] += bv * bv;
+            }
+        block_sum<SPMV_WAVES, 3>(v, lds);
         const int comp[3] = {P_RZ, P_RR, P_BB};
-        write_partials<3>(part_out, blk, comp, v);
+        write_partials<3>(part_out, sl, comp, v);
         st.pending = 1;
         st.force_check = 0;
         st.updates += 1;
@@ -203,52 +246,65 @@ k_cg_spmv(DeviceBatch d, const T *__restrict__ W, const T *__restrict__ P, T *__
     const T bt = (T)beta;
     const T *dold = (st.iter & 1) ? dbuf0 : dbuf1;
     T *dnew = (st.iter & 1) ? dbuf1 : dbuf0;
-    T dn[3];
-    for (int a = 0; a < 3; a++)
-        dn[a] = dir_value<T>(P[p3 + a], r[p3 + a], dold[p3 + a], bt, first);
-    const T *Wl = W + 9 * base + lane;
+    const T *Wl = W + 9 * base + row;
     T y0 = 0, y1 = 0, y2 = 0;
-#pragma unroll 2
-    for (int k = 0; k < w; k++)
+    BW_UNROLL(BW_SPMV_UNROLL)
+    for (int k = j; k < w; k += LPR)
     {
         const size_t c3 = 3 * (size_t)cl[k * SLICE];
-        const T v0 = dir_value<T>(P[c3], r[c3], dold[c3], bt, first);
-        const T v1 = dir_value<T>(P[c3 + 1], r[c3 + 1], dold[c3 + 1], bt, first);
-        const T v2 = dir_value<T>(P[c3 + 2], r[c3 + 2], dold[c3 + 2], bt, first);
+        const T v0 = dir_value<T>(z[c3], dold[c3], bt, first);
+        const T v1 = dir_value<T>(z[c3 + 1], dold[c3 + 1], bt, first);
+        const T v2 = dir_value<T>(z[c3 + 2], dold[c3 + 2], bt, first);
         const T *wk = Wl + 9 * (size_t)k * SLICE;
         y0 += wk[0 * SLICE] * v0 + wk[1 * SLICE] * v1 + wk[2 * SLICE] * v2;
         y1 += wk[3 * SLICE] * v0 + wk[4 * SLICE] * v1 + wk[5 * SLICE] * v2;
         y2 += wk[6 * SLICE] * v0 + wk[7 * SLICE] * v1 + wk[8 * SLICE] * v2;
     }
-    dnew[p3] = dn[0];
-    dnew[p3 + 1] = dn[1];
-    dnew[p3 + 2] = dn[2];
-    q[p3] = y0;
-    q[p3 + 1] = y1;
-    q[p3 + 2] = y2;
-    double v[1] = {(double)dn[0] * (double)y0 + (double)dn[1] * (double)y1 +
-                   (double)dn[2] * (double)y2};
-    block_sum<SB, 1>(v, lds);
+    y0 = phase_sum(y0);
+    y1 = phase_sum(y1);
+    y2 = phase_sum(y2);
+    double v[1] = {0.0};
+    if (j == 0)
+    {
+        T dn[3];
+        for (int a = 0; a < 3; a++)
+            dn[a] = dir_value<T>(z[p3 + a], dold[p3 + a], bt, first);
+        dnew[p3] = dn[0];
+        dnew[p3 + 1] = dn[1];
+        dnew[p3 + 2] = dn[2];
+        q[p3] = y0;
+        q[p3 + 1] = y1;
+        q[p3 + 2] = y2;
+        v[0] = (double)dn[0] * (double)y0 + (double)dn[1] * (double)y1 + (double)dn[2] * (double)y2;
+    }
+    block_sum<SPMV_WAVES, 1>(v, lds);
     const int comp[1] = {P_DQ};
-    write_partials<1>(part_out, blk, comp, v);
+    write_partials<1>(part_out, sl, comp, v);
     if (lead)
         store_state(st_out, st);
 }
 
 template <typename T, int SB>
 __global__ void __launch_bounds__(SB * 64)
-k_cg_update(DeviceBatch d, const T *__restrict__ P, T *__restrict__ r, const T *__restrict__ q,
-            const T *__restrict__ dbuf0, const T *__restrict__ dbuf1)
+k_cg_update(DeviceBatch d, const T *__restrict__ P, T *__restrict__ r, T *__restrict__ z,
+            const T *__restrict__ q, const T *__restrict__ dbuf0, const T *__restrict__ dbuf1)
 {
     __shared__ double lds[8 * SB];
-    const int blk = solver_block(d);
+    int blk = solver_block(d);
+    if (d.upd_list)
+    {
+        blk = d.upd_list[blk];
+        if (blk < 0)
+            return;
+    }
     const int inst = d.blk_inst[blk];
     const InstParams &ip = d.params[inst];
     const bool lead = blk == ip.blk_begin && threadIdx.x == 0;
     CgState st = load_state(d.cg + 2 * inst + 1);
     CgState *st_out = d.cg + 2 * inst;
-    const double *part_in = d.cg_partial + (size_t)PARTS * d.n_blocks_wg;
+    const double *part_in = d.cg_partial + (size_t)PARTS * d.n_slices;
     double *part_out = d.cg_partial;
+    const size_t part0 = (size_t)PARTS * SB * blk; // this workgroup's slices
     if (st.done)
     {
         if (lead)
@@ -258,8 +314,8 @@ k_cg_update(DeviceBatch d, const T *__restrict__ P, T *__restrict__ r, const T *
     if (st.pending)
     {
         // idle: hand the true-residual partials on to the next k_cg_spmv
-        if (threadIdx.x < PARTS)
-            part_out[PARTS * (size_t)blk + threadIdx.x] = part_in[PARTS * (size_t)blk + threadIdx.x];
+        if (threadIdx.x < PARTS * SB)
+            part_out[part0 + threadIdx.x] = part_in[part0 + threadIdx.x];
         if (lead)
             store_state(st_out, st);
         return;
@@ -267,7 +323,7 @@ k_cg_update(DeviceBatch d, const T *__restrict__ P, T *__restrict__ r, const T *
     double v1[1];
     {
         const int comp[1] = {P_DQ};
-        reduce_cloth<SB, 1>(part_in, ip.blk_begin, ip.blk_count, comp, v1, lds);
+        reduce_cloth<SB, 1>(part_in, ip.slice_begin, ip.slice_count, comp, v1, lds);
     }
     const double dq = v1[0];
     if (!(dq > 0.0) || !isfinite(dq))
@@ -278,8 +334,8 @@ k_cg_update(DeviceBatch d, const T *__restrict__ P, T *__restrict__ r, const T *
         st.final_check = 1;
         st.force_check = 1;
         st.breakdown = dq == 0.0 ? 0 : 1;
-        if (threadIdx.x < PARTS)
-            part_out[PARTS * (size_t)blk + threadIdx.x] = 0.0;
+        if (threadIdx.x < PARTS * SB)
+            part_out[part0 + threadIdx.x] = 0.0;
         if (lead)
             store_state(st_out, st);
         return;
@@ -294,14 +350,20 @@ k_cg_update(DeviceBatch d, const T *__restrict__ P, T *__restrict__ r, const T *
     {
         d.dx[p3 + a] = d.dx[p3 + a] + alpha * (double)dn[p3 + a];
         const T rn = (T)((double)r[p3 + a] - alpha * (double)q[p3 + a]);
+        const T zn = P[p3 + a] * rn;
         r[p3 + a] = rn;
-        const T z = P[p3 + a] * rn;
-        v[0] += (double)rn * (double)z;
+        z[p3 + a] = zn;
+        v[0] += (double)rn * (double)zn;
         v[1] += (double)rn * (double)rn;
     }
-    block_sum<SB, 2>(v, lds);
-    const int comp[2] = {P_RZ, P_RR};
-    write_partials<2>(part_out, blk, comp, v);
+    // one partial set per slice (wavefront)
+    v[0] = wave_sum(v[0]);
+    v[1] = wave_sum(v[1]);
+    if (lane == 0)
+    {
+        part_out[(size_t)PARTS * sl + P_RZ] = v[0];
+        part_out[(size_t)PARTS * sl + P_RR] = v[1];
+    }
     st.iter += 1;
     st.total += 1;
     st.since += 1;
@@ -323,15 +385,18 @@ __global__ void k_cg_reset(CgState *st, int n)
 }
 
 // timing set-up: every cloth mid-solve, at a k_cg_spmv with residual 1
-template <typename T> __global__ void k_cg_timing_setup(DeviceBatch d, T *r)
+template <typename T> __global__ void k_cg_timing_setup(DeviceBatch d, T *r, T *z)
 {
     const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (i < 3 * (size_t)d.n_rows)
+    {
         r[i] = (T)1;
-    if (i < (size_t)d.n_blocks_wg)
+        z[i] = (T)1;
+    }
+    if (i < (size_t)d.n_slices)
         for (int slot = 0; slot < 2; slot++)
         {
-            double *part = d.cg_partial + (size_t)slot * PARTS * d.n_blocks_wg + PARTS * i;
+            double *part = d.cg_partial + (size_t)slot * PARTS * d.n_slices + PARTS * i;
             part[P_RZ] = 1.0;
             part[P_RR] = 1.0;
             part[P_BB] = 1.0;
@@ -339,9 +404,9 @@ template <typename T> __global__ void k_cg_timing_setup(DeviceBatch d, T *r)
         }
     if (i < 2 * (size_t)d.n_inst)
     {
-        CgState z = {};
-        z.rho = 1.0;
-        d.cg[i] = z;
+        CgState s = {};
+        s.rho = 1.0;
+        d.cg[i] = s;
     }
 }
 
@@ -354,26 +419,27 @@ void launch_cg_reset(const DeviceBatch &d, hipStream_t s)
     hipLaunchKernelGGL(k_cg_reset, dim3((2 * d.n_inst + 63) / 64), dim3(64), 0, s, d.cg, d.n_inst);
 }
 
-void launch_cg_step(const DeviceBatch &d, bool fp32, int phase, hipStream_t s)
+void launch_cg_step(const DeviceBatch &d, bool fp32, int phase, int slices, hipStream_t s)
 {
-    const dim3 grid(d.n_blocks_wg), block(SB * 64);
+    const dim3 spmv_grid(slices), spmv_block(SPMV_WAVES * 64);
+    const dim3 upd_grid((slices + SB - 1) / SB), upd_block(SB * 64);
     if (fp32)
     {
         if (phase == 0)
-            hipLaunchKernelGGL((k_cg_spmv<float, SB>), grid, block, 0, s, d, d.W32, d.P32, d.r32,
-                               d.q32, d.d32[0], d.d32[1]);
+            hipLaunchKernelGGL((k_cg_spmv<float>), spmv_grid, spmv_block, 0, s, d, d.W32, d.P32,
+                               d.r32, d.z32, d.q32, d.d32[0], d.d32[1]);
         else
-            hipLaunchKernelGGL((k_cg_update<float, SB>), grid, block, 0, s, d, d.P32, d.r32,
-                               d.q32, d.d32[0], d.d32[1]);
+            hipLaunchKernelGGL((k_cg_update<float, SB>), upd_grid, upd_block, 0, s, d, d.P32,
+                               d.r32, d.z32, d.q32, d.d32[0], d.d32[1]);
     }
     else
     {
         if (phase == 0)
-            hipLaunchKernelGGL((k_cg_spmv<double, SB>), grid, block, 0, s, d, d.W64, d.P64, d.r64,
-                               d.q64, d.d64[0], d.d64[1]);
+            hipLaunchKernelGGL((k_cg_spmv<double>), spmv_grid, spmv_block, 0, s, d, d.W64, d.P64,
+                               d.r64, d.z64, d.q64, d.d64[0], d.d64[1]);
         else
-            hipLaunchKernelGGL((k_cg_update<double, SB>), grid, block, 0, s, d, d.P64, d.r64,
-                               d.q64, d.d64[0], d.d64[1]);
+            hipLaunchKernelGGL((k_cg_update<double, SB>), upd_grid, upd_block, 0, s, d, d.P64,
+                               d.r64, d.z64, d.q64, d.d64[0], d.d64[1]);
     }
 }
 
@@ -382,9 +448,9 @@ void launch_cg_timing_setup(const DeviceBatch &d, bool fp32, hipStream_t s)
     const size_t n = std::max<size_t>(3 * (size_t)d.n_rows, 2 * (size_t)d.n_inst);
     const dim3 grid((unsigned)((n + 255) / 256));
     if (fp32)
-        hipLaunchKernelGGL(k_cg_timing_setup<float>, grid, dim3(256), 0, s, d, d.r32);
+        hipLaunchKernelGGL(k_cg_timing_setup<float>, grid, dim3(256), 0, s, d, d.r32, d.z32);
     else
-        hipLaunchKernelGGL(k_cg_timing_setup<double>, grid, dim3(256), 0, s, d, d.r64);
+        hipLaunchKernelGGL(k_cg_timing_setup<double>, grid, dim3(256), 0, s, d, d.r64, d.z64);
 }
 
 } // namespace bw

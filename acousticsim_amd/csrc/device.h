@@ -109,11 +109,15 @@ struct DeviceBatch
     InstParams *params = nullptr;
     InstState *state = nullptr;
     // tolerance-mode PCG: state (2 slots per cloth), partials (2 slots x
-    // PARTS per solver workgroup), FP32 vectors of the mixed form (the FP64
-    // form uses r64, q64, d64)
+    // PARTS per slice), FP32 vectors of the mixed form (the FP64
+    // form uses r64, z64, q64, d64); z = P r
     CgState *cg = nullptr;
     double *cg_partial = nullptr;
-    float *r32 = nullptr, *q32 = nullptr, *d32[2] = {nullptr, nullptr};
+    float *r32 = nullptr, *z32 = nullptr, *q32 = nullptr, *d32[2] = {nullptr, nullptr};
+    double *z64 = nullptr;
+    // optional lists of the slices (k_cg_spmv) and solver workgroups
+    // (k_cg_update) a launch runs; null: all of them
+    int *spmv_list = nullptr, *upd_list = nullptr;
 };
 
 // Workgroup b of a G-workgroup launch runs on XCD b % 8 (the dispatcher deals
@@ -140,9 +144,11 @@ void launch_pcg_iter64(const DeviceBatch &d, hipStream_t s);
 // mode 0: FP64 solve start; 2: counters only
 void launch_residual64(const DeviceBatch &d, int mode, hipStream_t s);
 void launch_update_positions(const DeviceBatch &d, hipStream_t s);
-// tolerance-mode PCG (cg.hip); phase 0: k_cg_spmv, 1: k_cg_update
+// tolerance-mode PCG (cg.hip); phase 0: k_cg_spmv, 1: k_cg_update.  slices:
+// the k_cg_spmv grid (list length, or n_slices without lists); k_cg_update
+// runs ceil(slices / BW_SB) workgroups.
 void launch_cg_reset(const DeviceBatch &d, hipStream_t s);
-void launch_cg_step(const DeviceBatch &d, bool fp32, int phase, hipStream_t s);
+void launch_cg_step(const DeviceBatch &d, bool fp32, int phase, int slices, hipStream_t s);
 void launch_cg_timing_setup(const DeviceBatch &d, bool fp32, hipStream_t s);
 // measurement helpers: set the solver residual to 1 on every row; one SpMV
 // launch of the solver's precision

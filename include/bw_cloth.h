@@ -130,8 +130,10 @@ int bw_batch_set_graphs(bw_batch *b, int enable);
    d = P r + beta d_prev fused into q = W d and the partial d.q), with HIP
    events on the batch's stream, every cloth active.  avg_us gets the mean
    time per launch, alg_bytes the algorithmic bytes of one launch: 36 + 4
-   (FP32 matrix) or 72 + 4 (FP64) bytes per true 3x3 block plus 15 vector
-   values per vertex (r, P, d_prev read, d and q written).  The system is
+   (FP32 matrix) or 72 + 4 (FP64) bytes per true 3x3 block plus the vector
+   values per vertex, each read or written once (tolerance modes: P r and
+   d_prev read, d and q written, 12 values; reference mode: r, P, d_prev
+   read, d and q written, 15 values).  The system is
    assembled at the current state first; the solver's vectors are
    overwritten (positions are not). */
 int bw_batch_time_solver_kernel(bw_batch *b, int reps, double *avg_us, double *alg_bytes);
