@@ -1,6 +1,6 @@
 """CPU-only checks of the product library: it builds, loads, exports every
 entry point include/bw_cloth.h declares, and its host-side workload helpers
-behave.  No device calls here."""
+behave.  No device computation here."""
 import ctypes
 import os
 import re
@@ -63,3 +63,11 @@ def test_batch64_members_in_range():
     assert min(scales) >= 0.1 and max(scales) <= 10
     total_v = sum(s * s for s in sides)
     assert 2e5 < total_v < 7e5
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present")
+def test_create_without_device_fails_loudly():
+    """No CPU fallback: without a GPU the library refuses with BW_ERR_DEVICE."""
+    with pytest.raises(capi.BwError) as e:
+        capi.BaraffClothHip(wl.grid_config(5), None)
+    assert e.value.code == -2
