@@ -31,6 +31,7 @@ SOURCES = [
     ("model.cpp", ["-ffp-contract=off"]),
     ("batch.cpp", ["-ffp-contract=off"]),
     ("capi.cpp", ["-ffp-contract=off"]),
+    ("server.cpp", ["-ffp-contract=off", "-pthread"]),
     ("assemble.hip", DEVICE + ["-ffp-contract=off"]),
     ("pcg.hip", DEVICE + ["-ffp-contract=fast"]),
     ("cg.hip", DEVICE + ["-ffp-contract=fast"]),
@@ -51,7 +52,7 @@ def build(verbose=False):
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 4)) as ex:
         objs = list(ex.map(lambda a: _compile(*a), SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs
+    cmd = [HIPCC, "-shared", "-fPIC", "-pthread", "--offload-arch=" + ARCH, "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s" % (r.stdout, r.stderr))

@@ -38,6 +38,41 @@ class StepStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class RequestInfo(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int),
+        ("done", ctypes.c_int),
+        ("num_vertices", ctypes.c_int),
+        ("steps", ctypes.c_int),
+        ("steps_done", ctypes.c_int),
+        ("pcg_iterations", ctypes.c_longlong),
+        ("max_rel_residual", ctypes.c_double),
+        ("t_submit", ctypes.c_double),
+        ("t_admit", ctypes.c_double),
+        ("t_first_step", ctypes.c_double),
+        ("t_done", ctypes.c_double),
+        ("device_bytes", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerCounters(ctypes.Structure):
+    _fields_ = [
+        ("queued", ctypes.c_int),
+        ("resident", ctypes.c_int),
+        ("completed", ctypes.c_int),
+        ("max_resident_seen", ctypes.c_int),
+        ("bytes_in_use", ctypes.c_double),
+        ("chunks", ctypes.c_longlong),
+        ("launches", ctypes.c_longlong),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _lib = None
 EXPORTS = [
     "bw_create", "bw_destroy", "bw_update_pos", "bw_reset", "bw_get_pos", "bw_get_pre_pos",
@@ -48,7 +83,9 @@ EXPORTS = [
     "bw_get_ell_map", "bw_assemble_system", "bw_get_last_stats", "bw_last_error", "bw_batch_create",
     "bw_batch_destroy", "bw_batch_size", "bw_batch_step", "bw_batch_num_vertices",
     "bw_batch_get_pos", "bw_batch_set_state", "bw_batch_get_stats", "bw_batch_set_graphs",
-    "bw_batch_time_solver_kernel",
+    "bw_batch_time_solver_kernel", "bw_server_create", "bw_server_destroy", "bw_server_submit",
+    "bw_server_wait", "bw_server_get_info", "bw_server_get_pos", "bw_server_release",
+    "bw_server_get_counters", "bw_server_now",
 ]
 
 
@@ -97,6 +134,15 @@ def lib():
         "bw_batch_get_stats": (i32, [vp, i32, P(StepStats)]),
         "bw_batch_set_graphs": (i32, [vp, i32]),
         "bw_batch_time_solver_kernel": (i32, [vp, i32, dp, dp]),
+        "bw_server_create": (i32, [ctypes.c_double, i32, cp, P(vp)]),
+        "bw_server_destroy": (None, [vp]),
+        "bw_server_submit": (i32, [vp, cp, dp, i32, i32, P(ctypes.c_longlong)]),
+        "bw_server_wait": (i32, [vp, ctypes.c_longlong, ctypes.c_double]),
+        "bw_server_get_info": (i32, [vp, ctypes.c_longlong, P(RequestInfo)]),
+        "bw_server_get_pos": (i32, [vp, ctypes.c_longlong, dp]),
+        "bw_server_release": (i32, [vp, ctypes.c_longlong]),
+        "bw_server_get_counters": (i32, [vp, P(ServerCounters)]),
+        "bw_server_now": (ctypes.c_double, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -301,3 +347,70 @@ class ClothBatch:
         st = StepStats()
         _check(lib().bw_batch_get_stats(self._h, i, ctypes.byref(st)))
         return st.as_dict()
+
+
+class Server:
+    """Serving runtime (bw_server_*): submit step requests of any cloth at
+    any time; they are admitted against a device-memory budget, solved
+    together with everything resident, and retired with their final
+    positions."""
+
+    def __init__(self, budget_bytes=200e9, max_resident=256, precision="mixed"):
+        h = ctypes.c_void_p()
+        _check(lib().bw_server_create(float(budget_bytes), int(max_resident),
+                                      precision.encode(), ctypes.byref(h)))
+        self._h = h
+
+    def submit(self, config, x0=None, nsteps=10):
+        text = json.dumps(config).encode()
+        rid = ctypes.c_longlong()
+        if x0 is None:
+            _check(lib().bw_server_submit(self._h, text, None, 0, int(nsteps), ctypes.byref(rid)))
+        else:
+            x0 = np.ascontiguousarray(x0, dtype=np.float64)
+            _check(lib().bw_server_submit(self._h, text, _dp(x0), x0.size, int(nsteps),
+                                          ctypes.byref(rid)))
+        return rid.value
+
+    def wait(self, rid, timeout=600.0):
+        """True when the request is done (or failed) within timeout seconds."""
+        rc = lib().bw_server_wait(self._h, rid, float(timeout))
+        if rc == -4:
+            return False
+        _check(rc)
+        return True
+
+    def info(self, rid):
+        st = RequestInfo()
+        _check(lib().bw_server_get_info(self._h, rid, ctypes.byref(st)))
+        d = st.as_dict()
+        d["error"] = lib().bw_last_error().decode() if st.status != 0 else ""
+        return d
+
+    def get_pos(self, rid):
+        nv = self.info(rid)["num_vertices"]
+        x = np.empty(3 * nv)
+        _check(lib().bw_server_get_pos(self._h, rid, _dp(x)))
+        return x
+
+    def release(self, rid):
+        _check(lib().bw_server_release(self._h, rid))
+
+    def counters(self):
+        c = ServerCounters()
+        _check(lib().bw_server_get_counters(self._h, ctypes.byref(c)))
+        return c.as_dict()
+
+    def now(self):
+        return lib().bw_server_now(self._h)
+
+    def close(self):
+        if self._h:
+            lib().bw_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

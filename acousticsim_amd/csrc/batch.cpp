@@ -25,6 +25,7 @@ template <typename T> T *Batch::dalloc(size_t n)
     const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
     check(hipMalloc(&p, bytes), "hipMalloc");
     allocations_.push_back(p);
+    device_bytes_ += bytes;
     check(hipMemset(p, 0, bytes), "hipMemset");
     return static_cast<T *>(p);
 }
@@ -72,7 +73,8 @@ std::vector<int> slice_row_order(const ClothModel &m)
     return order;
 }
 
-Batch::Batch(std::vector<ClothModel> models) : models_(std::move(models))
+Batch::Batch(std::vector<ClothModel> models, const BatchOptions &opt)
+    : models_(std::move(models)), opt_(opt)
 {
     if (models_.empty())
         throw std::runtime_error("batch: no cloths");
@@ -87,7 +89,13 @@ Batch::Batch(std::vector<ClothModel> models) : models_(std::move(models))
             throw std::runtime_error(
                 "batch: members must share one precision and solver_mode");
     stats_.resize(models_.size());
-    check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    if (opt_.stream)
+        stream_ = opt_.stream;
+    else
+    {
+        check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+        own_stream_ = true;
+    }
     // BW_GRAPHS=0 launches the iteration chains directly (profilers that do
     // not follow graph launches)
     if (const char *g = std::getenv("BW_GRAPHS"))
@@ -116,7 +124,7 @@ Batch::~Batch()
         (void)hipHostFree(host_spmv_list_);
     if (host_upd_list_)
         (void)hipHostFree(host_upd_list_);
-    if (stream_)
+    if (stream_ && own_stream_)
         (void)hipStreamDestroy(stream_);
 }
 
@@ -365,7 +373,7 @@ void Batch::build_layout()
             d_.d64[0] = dalloc<double>(nv3);
             d_.d64[1] = dalloc<double>(nv3);
         }
-        d_.cg = dalloc<CgState>(2 * (size_t)n);
+        d_.cg = opt_.cg_states ? opt_.cg_states : dalloc<CgState>(2 * (size_t)n);
         d_.spmv_list = dalloc<int>(slices);
         d_.upd_list = dalloc<int>(d_.n_blocks_wg);
         check(hipHostMalloc((void **)&host_spmv_list_, sizeof(int) * (size_t)slices),
@@ -787,6 +795,35 @@ void Batch::solve_cg()
         // launches this cloth needed: two per iteration and per true residual
         pred_launches_[i] = 2LL * (s.total + s.updates) + 2;
     }
+}
+
+void Batch::begin_step()
+{
+    if (kind_ == SolverKind::RefPCG64)
+        throw std::runtime_error("batch: reference-semantics batches step through step()");
+    launch_elements(d_, stream_);
+    launch_assemble_rows(d_, stream_);
+    launch_cg_reset(d_, stream_);
+    check(hipGetLastError(), "step launch");
+}
+
+void Batch::end_step(const CgState *states, double step_us)
+{
+    bool bad = false;
+    for (int i = 0; i < size(); i++)
+    {
+        const CgState &s = states[2 * i];
+        StepStats &st = stats_[i];
+        st.pcg_iterations = s.total;
+        st.outer_iterations = s.updates;
+        st.rel_residual = s.bb > 0 ? std::sqrt(s.rr_true / s.bb) : std::sqrt(s.rr_true);
+        st.step_us = step_us;
+        bad |= !std::isfinite(st.rel_residual);
+    }
+    launch_update_positions(d_, stream_);
+    check(hipGetLastError(), "step launch");
+    if (bad)
+        throw std::runtime_error("step: non-finite residual (NaN/Inf in the state)");
 }
 
 void Batch::step(int nsteps)

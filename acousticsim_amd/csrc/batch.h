@@ -28,10 +28,19 @@ enum class SolverKind
     CG64
 };
 
+// Optional resources a Batch borrows instead of creating: the stream its
+// kernels run on, and device storage for its solver states (2 CgState per
+// cloth), so that a server can keep many batches' states in one table.
+struct BatchOptions
+{
+    hipStream_t stream = nullptr;
+    CgState *cg_states = nullptr;
+};
+
 class Batch
 {
 public:
-    explicit Batch(std::vector<ClothModel> models);
+    explicit Batch(std::vector<ClothModel> models, const BatchOptions &opt = BatchOptions());
     ~Batch();
     Batch(const Batch &) = delete;
     Batch &operator=(const Batch &) = delete;
@@ -43,6 +52,14 @@ public:
     void refresh_pins(int i, const std::vector<int> &pins);
 
     void step(int nsteps);
+    // One step driven from outside (tolerance solvers): begin_step queues the
+    // element, assembly and solver-reset kernels; the caller then runs the
+    // solver launches (launch_cg_step_multi) until every cloth's state says
+    // done, and end_step takes those states (host copy, 2 per cloth), fills
+    // the step counters and queues the position update.
+    void begin_step();
+    void end_step(const CgState *states, double step_us);
+    size_t device_bytes() const { return device_bytes_; }
     // assemble W and rhs at the current state (no solve) and copy them out in
     // the model's block order
     void assemble_system(int i, double *W, double *rhs);
@@ -78,6 +95,9 @@ private:
     std::vector<StepStats> stats_;
     DeviceBatch d_;
     hipStream_t stream_ = nullptr;
+    BatchOptions opt_;
+    bool own_stream_ = false;
+    size_t device_bytes_ = 0;
     SolverKind kind_ = SolverKind::CG32;
     InstState *host_state_ = nullptr; // pinned, reference PCG state
     CgState *host_cg_ = nullptr;      // pinned, 2 per cloth

@@ -12,6 +12,8 @@
 #include "batch.h"
 #include "json_lite.h"
 #include "model.h"
+#include "server.h"
+#include "status.h"
 
 namespace
 {
@@ -29,28 +31,11 @@ template <typename F> int guard(F &&f)
     {
         return f();
     }
-    catch (const std::invalid_argument &e)
-    {
-        return fail(BW_ERR_ARG, e.what());
-    }
-    catch (const std::runtime_error &e)
-    {
-        const std::string m = e.what();
-        if (m.rfind("HIP error", 0) == 0)
-            return fail(BW_ERR_DEVICE, m);
-        if (m.rfind("step:", 0) == 0)
-            return fail(BW_ERR_NUMERIC, m);
-        if (m.rfind("config", 0) == 0 || m.rfind("batch:", 0) == 0)
-            return fail(BW_ERR_ARG, m);
-        return fail(BW_ERR_STATE, m);
-    }
-    catch (const std::bad_alloc &)
-    {
-        return fail(BW_ERR_CAPACITY, "out of host memory");
-    }
     catch (...)
     {
-        return fail(BW_ERR_STATE, "unknown error");
+        std::string msg;
+        const int code = bw::status_of_current_exception(msg);
+        return fail(code, msg);
     }
 }
 
@@ -58,17 +43,7 @@ bw::ClothModel model_from_json(const char *json)
 {
     if (!json)
         throw std::runtime_error("config: NULL JSON text");
-    bw::Json j = bw::Json::parse(json);
-    if (j.kind != bw::Json::Object)
-        throw std::runtime_error("config: top level must be an object");
-    if (j.has("cloth_type"))
-    {
-        const std::string t = j.at("cloth_type").as_string();
-        if (t != "fem" && t != "fem_hip")
-            throw std::runtime_error("config: cloth_type \"" + t +
-                                     "\" is not the plain implicit method (\"fem\")");
-    }
-    return bw::ClothModel::build(bw::ClothConfig::from_json(j));
+    return bw::ClothModel::from_json_text(json);
 }
 
 bool all_finite(const double *x, size_t n)
@@ -419,3 +394,127 @@ int bw_batch_set_graphs(bw_batch *b, int enable)
 }
 
 } // extern "C"
+
+// ---- serving ----------------------------------------------------------
+
+struct bw_server
+{
+    std::unique_ptr<bw::Server> server;
+};
+
+extern "C" int bw_server_create(double budget_bytes, int max_resident, const char *precision,
+                                bw_server **out)
+{
+    return guard([&] {
+        if (!out)
+            return fail(BW_ERR_ARG, "NULL output pointer");
+        *out = nullptr;
+        bw::ServerOptions o;
+        if (!(budget_bytes > 0))
+            return fail(BW_ERR_ARG, "budget_bytes must be positive");
+        o.budget_bytes = (size_t)budget_bytes;
+        o.max_resident = max_resident;
+        const std::string p = precision ? precision : "mixed";
+        if (p != "mixed" && p != "fp64")
+            return fail(BW_ERR_ARG, "precision must be \"mixed\" or \"fp64\"");
+        o.fp32 = p == "mixed";
+        auto s = std::make_unique<bw_server>();
+        s->server = std::make_unique<bw::Server>(o);
+        *out = s.release();
+        return BW_OK;
+    });
+}
+
+extern "C" void bw_server_destroy(bw_server *s) { delete s; }
+
+extern "C" int bw_server_submit(bw_server *s, const char *json, const double *x0, int nx0,
+                                int nsteps, long long *id)
+{
+    return guard([&] {
+        if (!s || !json || !id)
+            return fail(BW_ERR_ARG, "NULL argument");
+        if (x0 && nx0 > 0 && !all_finite(x0, (size_t)nx0))
+            return fail(BW_ERR_ARG, "x0 holds NaN/Inf");
+        *id = s->server->submit(json, x0, x0 ? nx0 : 0, nsteps);
+        return BW_OK;
+    });
+}
+
+extern "C" int bw_server_wait(bw_server *s, long long id, double timeout_s)
+{
+    return guard([&] {
+        if (!s)
+            return fail(BW_ERR_ARG, "NULL server");
+        if (!s->server->wait(id, timeout_s))
+            return fail(BW_ERR_STATE, "request not done within the timeout");
+        return BW_OK;
+    });
+}
+
+extern "C" int bw_server_get_info(bw_server *s, long long id, bw_request_info *info)
+{
+    return guard([&] {
+        if (!s || !info)
+            return fail(BW_ERR_ARG, "NULL argument");
+        const bw::RequestInfo r = s->server->info(id);
+        info->status = r.status;
+        info->done = r.done ? 1 : 0;
+        info->num_vertices = r.nv;
+        info->steps = r.steps;
+        info->steps_done = r.steps_done;
+        info->pcg_iterations = r.iterations;
+        info->max_rel_residual = r.max_residual;
+        info->t_submit = r.t_submit;
+        info->t_admit = r.t_admit;
+        info->t_first_step = r.t_first;
+        info->t_done = r.t_done;
+        info->device_bytes = (double)r.device_bytes;
+        if (r.status != BW_OK)
+            g_error = r.error;
+        return BW_OK;
+    });
+}
+
+extern "C" int bw_server_get_pos(bw_server *s, long long id, double *x)
+{
+    return guard([&] {
+        if (!s || !x)
+            return fail(BW_ERR_ARG, "NULL argument");
+        const bw::RequestInfo r = s->server->info(id);
+        if (!r.done)
+            return fail(BW_ERR_STATE, "request still running");
+        if (r.status != BW_OK)
+            return fail(r.status, r.error);
+        std::memcpy(x, r.pos.data(), r.pos.size() * sizeof(double));
+        return BW_OK;
+    });
+}
+
+extern "C" int bw_server_release(bw_server *s, long long id)
+{
+    return guard([&] {
+        if (!s)
+            return fail(BW_ERR_ARG, "NULL server");
+        s->server->release(id);
+        return BW_OK;
+    });
+}
+
+extern "C" int bw_server_get_counters(bw_server *s, bw_server_counters *c)
+{
+    return guard([&] {
+        if (!s || !c)
+            return fail(BW_ERR_ARG, "NULL argument");
+        const bw::ServerCounters k = s->server->counters();
+        c->queued = k.queued;
+        c->resident = k.resident;
+        c->completed = k.completed;
+        c->max_resident_seen = k.max_resident_seen;
+        c->bytes_in_use = (double)k.bytes_in_use;
+        c->chunks = k.chunks;
+        c->launches = k.launches;
+        return BW_OK;
+    });
+}
+
+extern "C" double bw_server_now(bw_server *s) { return s ? s->server->now() : 0.0; }

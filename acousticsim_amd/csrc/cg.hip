@@ -54,7 +54,7 @@ namespace
 
 // field-wise state copies: whole-struct copies make the compiler keep a
 // per-thread copy of the state in LDS or scratch
-__device__ inline CgState load_state(const CgState *p)
+template <typename SP> __device__ inline CgState load_state(SP p)
 {
     CgState s;
     s.rho = p->rho;
@@ -77,7 +77,7 @@ __device__ inline CgState load_state(const CgState *p)
     return s;
 }
 
-__device__ inline void store_state(CgState *p, const CgState &s)
+template <typename SP> __device__ inline void store_state(SP p, const CgState &s)
 {
     p->rho = s.rho;
     p->rr_ref = s.rr_ref;
@@ -103,7 +103,10 @@ __device__ inline void store_state(CgState *p, const CgState &s)
 #define BW_PRAGMA(x) _Pragma(#x)
 #define BW_UNROLL(n) BW_PRAGMA(unroll n)
 
-constexpr int LPR = 4;           // SpMV lanes per row
+#ifndef BW_LPR
+#define BW_LPR 4
+#endif
+constexpr int LPR = BW_LPR;      // SpMV lanes per row (1, 2 or 4)
 constexpr int RPW = SLICE / LPR; // SpMV rows per wavefront
 constexpr int SPMV_WAVES = SLICE / RPW;
 
@@ -113,41 +116,57 @@ template <typename T> __device__ inline T dir_value(T z, T dold, T beta, bool fi
     return first ? z : z + beta * dold;
 }
 
-// sum over the four lane phases of a row (lanes l, l+16, l+32, l+48)
+// sum over the lane phases of a row (lanes l, l+RPW, l+2 RPW, ...)
 template <typename T> __device__ inline T phase_sum(T v)
 {
-    v += __shfl_xor(v, 16, 64);
-    v += __shfl_xor(v, 32, 64);
+    for (int off = RPW; off < 64; off <<= 1)
+        v += __shfl_xor(v, off, 64);
     return v;
 }
 
-template <typename T>
-__global__ void __launch_bounds__(SPMV_WAVES * 64)
-k_cg_spmv(DeviceBatch d, const T *__restrict__ W, const T *__restrict__ P, T *__restrict__ r,
-          T *__restrict__ z, T *__restrict__ q, T *__restrict__ dbuf0, T *__restrict__ dbuf1)
+// the solver vectors of precision T
+template <typename T> struct Vecs;
+template <> struct Vecs<float>
 {
-    __shared__ double lds[8 * SPMV_WAVES];
-    // one slice per workgroup, from the list of active slices when given
-    int sl = solver_block(d);
-    if (d.spmv_list)
-    {
-        sl = d.spmv_list[sl];
-        if (sl < 0)
-            return;
-    }
-    const int inst = d.slice_inst[sl];
-    const InstParams &ip = d.params[inst];
+    __device__ static const float *W(const DeviceBatch &d) { return d.W32; }
+    __device__ static const float *P(const DeviceBatch &d) { return d.P32; }
+    __device__ static float *r(const DeviceBatch &d) { return d.r32; }
+    __device__ static float *z(const DeviceBatch &d) { return d.z32; }
+    __device__ static float *q(const DeviceBatch &d) { return d.q32; }
+    __device__ static float *d0(const DeviceBatch &d) { return d.d32[0]; }
+    __device__ static float *d1(const DeviceBatch &d) { return d.d32[1]; }
+};
+template <> struct Vecs<double>
+{
+    __device__ static const double *W(const DeviceBatch &d) { return d.W64; }
+    __device__ static const double *P(const DeviceBatch &d) { return d.P64; }
+    __device__ static double *r(const DeviceBatch &d) { return d.r64; }
+    __device__ static double *z(const DeviceBatch &d) { return d.z64; }
+    __device__ static double *q(const DeviceBatch &d) { return d.q64; }
+    __device__ static double *d0(const DeviceBatch &d) { return d.d64[0]; }
+    __device__ static double *d1(const DeviceBatch &d) { return d.d64[1]; }
+};
+
+// k_cg_spmv on slice sl of batch d
+template <typename T>
+__device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> *__restrict__ W,
+                                 const G<T> *__restrict__ P, G<T> *__restrict__ r,
+                                 G<T> *__restrict__ z, G<T> *__restrict__ q,
+                                 G<T> *__restrict__ dbuf0, G<T> *__restrict__ dbuf1, double *lds)
+{
+    const int inst = glob(d.slice_inst)[sl];
+    const InstParams ip = glob(d.params)[inst];
     const bool lead = sl == ip.slice_begin && threadIdx.x == 0;
-    CgState st = load_state(d.cg + 2 * inst);
-    CgState *st_out = d.cg + 2 * inst + 1;
+    CgState st = load_state(glob(d.cg) + 2 * inst);
+    G<CgState> *st_out = glob(d.cg) + 2 * inst + 1;
     if (st.done)
     {
         if (lead)
             store_state(st_out, st);
         return;
     }
-    const double *part_in = d.cg_partial;
-    double *part_out = d.cg_partial + (size_t)PARTS * d.n_slices;
+    const G<double> *part_in = glob(d.cg_partial);
+    G<double> *part_out = glob(d.cg_partial) + (size_t)PARTS * d.n_slices;
     const double tol2 = ip.tol * ip.tol;
     bool residual = false;
     double beta = 0.0;
@@ -194,19 +213,20 @@ k_cg_spmv(DeviceBatch d, const T *__restrict__ W, const T *__restrict__ P, T *__
     const int j = lane / RPW;                                        // lane phase
     const int p = sl * SLICE + row;
     const size_t p3 = 3 * (size_t)p;
-    const int w = d.row_w[p];
-    const size_t base = (size_t)d.slice_off[sl];
-    const int *cl = d.col + base + row;
+    const int w = glob(d.row_w)[p];
+    const size_t base = (size_t)glob(d.slice_off)[sl];
+    const G<int> *cl = glob(d.col) + base + row;
     if (residual)
     {
         // true residual in FP64: r = b - W64 dx
-        const double *Wl = d.W64 + 9 * base + row;
+        const G<double> *Wl = glob(d.W64) + 9 * base + row;
+        const G<double> *dx = glob(d.dx);
         double y[3] = {0.0, 0.0, 0.0};
         for (int k = j; k < w; k += LPR)
         {
             const size_t c3 = 3 * (size_t)cl[k * SLICE];
-            const double *wk = Wl + 9 * (size_t)k * SLICE;
-            const double x0 = d.dx[c3], x1 = d.dx[c3 + 1], x2 = d.dx[c3 + 2];
+            const G<double> *wk = Wl + 9 * (size_t)k * SLICE;
+            const double x0 = dx[c3], x1 = dx[c3 + 1], x2 = dx[c3 + 2];
             for (int a = 0; a < 3; a++)
                 y[a] += wk[(3 * a) * SLICE] * x0 + wk[(3 * a + 1) * SLICE] * x1 +
                         wk[(3 * a + 2) * SLICE] * x2;
@@ -217,7 +237,7 @@ k_cg_spmv(DeviceBatch d, const T *__restrict__ W, const T *__restrict__ P, T *__
         if (j == 0)
             for (int a = 0; a < 3; a++)
             {
-                const double bv = d.b[p3 + a];
+                const double bv = glob(d.b)[p3 + a];
                 const double rv = bv - y[a];
                 const T rt = (T)rv;
                 const T zt = P[p3 + a] * rt;
@@ -244,9 +264,9 @@ k_cg_spmv(DeviceBatch d, const T *__restrict__ W, const T *__restrict__ P, T *__
     }
     const bool first = st.first != 0;
     const T bt = (T)beta;
-    const T *dold = (st.iter & 1) ? dbuf0 : dbuf1;
-    T *dnew = (st.iter & 1) ? dbuf1 : dbuf0;
-    const T *Wl = W + 9 * base + row;
+    const G<T> *dold = (st.iter & 1) ? dbuf0 : dbuf1;
+    G<T> *dnew = (st.iter & 1) ? dbuf1 : dbuf0;
+    const G<T> *Wl = W + 9 * base + row;
     T y0 = 0, y1 = 0, y2 = 0;
     BW_UNROLL(BW_SPMV_UNROLL)
     for (int k = j; k < w; k += LPR)
@@ -255,7 +275,7 @@ k_cg_spmv(DeviceBatch d, const T *__restrict__ W, const T *__restrict__ P, T *__
         const T v0 = dir_value<T>(z[c3], dold[c3], bt, first);
         const T v1 = dir_value<T>(z[c3 + 1], dold[c3 + 1], bt, first);
         const T v2 = dir_value<T>(z[c3 + 2], dold[c3 + 2], bt, first);
-        const T *wk = Wl + 9 * (size_t)k * SLICE;
+        const G<T> *wk = Wl + 9 * (size_t)k * SLICE;
         y0 += wk[0 * SLICE] * v0 + wk[1 * SLICE] * v1 + wk[2 * SLICE] * v2;
         y1 += wk[3 * SLICE] * v0 + wk[4 * SLICE] * v1 + wk[5 * SLICE] * v2;
         y2 += wk[6 * SLICE] * v0 + wk[7 * SLICE] * v1 + wk[8 * SLICE] * v2;
@@ -284,26 +304,21 @@ k_cg_spmv(DeviceBatch d, const T *__restrict__ W, const T *__restrict__ P, T *__
         store_state(st_out, st);
 }
 
+// k_cg_update on solver workgroup blk (SB slices) of batch d
 template <typename T, int SB>
-__global__ void __launch_bounds__(SB * 64)
-k_cg_update(DeviceBatch d, const T *__restrict__ P, T *__restrict__ r, T *__restrict__ z,
-            const T *__restrict__ q, const T *__restrict__ dbuf0, const T *__restrict__ dbuf1)
+__device__ inline void update_body(const DeviceBatch &d, const int blk,
+                                   const G<T> *__restrict__ P, G<T> *__restrict__ r,
+                                   G<T> *__restrict__ z, const G<T> *__restrict__ q,
+                                   const G<T> *__restrict__ dbuf0, const G<T> *__restrict__ dbuf1,
+                                   double *lds)
 {
-    __shared__ double lds[8 * SB];
-    int blk = solver_block(d);
-    if (d.upd_list)
-    {
-        blk = d.upd_list[blk];
-        if (blk < 0)
-            return;
-    }
-    const int inst = d.blk_inst[blk];
-    const InstParams &ip = d.params[inst];
+    const int inst = glob(d.blk_inst)[blk];
+    const InstParams ip = glob(d.params)[inst];
     const bool lead = blk == ip.blk_begin && threadIdx.x == 0;
-    CgState st = load_state(d.cg + 2 * inst + 1);
-    CgState *st_out = d.cg + 2 * inst;
-    const double *part_in = d.cg_partial + (size_t)PARTS * d.n_slices;
-    double *part_out = d.cg_partial;
+    CgState st = load_state(glob(d.cg) + 2 * inst + 1);
+    G<CgState> *st_out = glob(d.cg) + 2 * inst;
+    const G<double> *part_in = glob(d.cg_partial) + (size_t)PARTS * d.n_slices;
+    G<double> *part_out = glob(d.cg_partial);
     const size_t part0 = (size_t)PARTS * SB * blk; // this workgroup's slices
     if (st.done)
     {
@@ -344,11 +359,12 @@ k_cg_update(DeviceBatch d, const T *__restrict__ P, T *__restrict__ r, T *__rest
     const int lane = threadIdx.x & 63;
     const int sl = blk * SB + (threadIdx.x >> 6);
     const size_t p3 = 3 * (size_t)(sl * SLICE + lane);
-    const T *dn = (st.iter & 1) ? dbuf1 : dbuf0;
+    const G<T> *dn = (st.iter & 1) ? dbuf1 : dbuf0;
+    G<double> *dx = glob(d.dx);
     double v[2] = {0.0, 0.0};
     for (int a = 0; a < 3; a++)
     {
-        d.dx[p3 + a] = d.dx[p3 + a] + alpha * (double)dn[p3 + a];
+        dx[p3 + a] = dx[p3 + a] + alpha * (double)dn[p3 + a];
         const T rn = (T)((double)r[p3 + a] - alpha * (double)q[p3 + a]);
         const T zn = P[p3 + a] * rn;
         r[p3 + a] = rn;
@@ -370,6 +386,70 @@ k_cg_update(DeviceBatch d, const T *__restrict__ P, T *__restrict__ r, T *__rest
     st.first = 0;
     if (lead)
         store_state(st_out, st);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(SPMV_WAVES * 64)
+k_cg_spmv(DeviceBatch d, const T *__restrict__ W, const T *__restrict__ P, T *__restrict__ r,
+          T *__restrict__ z, T *__restrict__ q, T *__restrict__ dbuf0, T *__restrict__ dbuf1)
+{
+    __shared__ double lds[8 * SPMV_WAVES];
+    // one slice per workgroup, from the list of active slices when given
+    int sl = solver_block(d.xcd_map);
+    if (d.spmv_list)
+    {
+        sl = d.spmv_list[sl];
+        if (sl < 0)
+            return;
+    }
+    spmv_body<T>(d, sl, glob(W), glob(P), glob(r), glob(z), glob(q), glob(dbuf0), glob(dbuf1),
+                 lds);
+}
+
+template <typename T, int SB>
+__global__ void __launch_bounds__(SB * 64)
+k_cg_update(DeviceBatch d, const T *__restrict__ P, T *__restrict__ r, T *__restrict__ z,
+            const T *__restrict__ q, const T *__restrict__ dbuf0, const T *__restrict__ dbuf1)
+{
+    __shared__ double lds[8 * SB];
+    int blk = solver_block(d.xcd_map);
+    if (d.upd_list)
+    {
+        blk = d.upd_list[blk];
+        if (blk < 0)
+            return;
+    }
+    update_body<T, SB>(d, blk, glob(P), glob(r), glob(z), glob(q), glob(dbuf0), glob(dbuf1), lds);
+}
+
+// Many batches in one launch: entry (batch, slice) of `list` per workgroup,
+// batches in the device table `table`; entries with batch < 0 are padding.
+template <typename T>
+__global__ void __launch_bounds__(SPMV_WAVES * 64)
+k_cg_spmv_multi(const DeviceBatch *__restrict__ table, const int2 *__restrict__ list, int xcd_map)
+{
+    __shared__ double lds[8 * SPMV_WAVES];
+    const int2 e = list[solver_block(xcd_map)];
+    if (e.x < 0)
+        return;
+    const DeviceBatch &d = table[e.x];
+    spmv_body<T>(d, e.y, glob(Vecs<T>::W(d)), glob(Vecs<T>::P(d)), glob(Vecs<T>::r(d)),
+                 glob(Vecs<T>::z(d)), glob(Vecs<T>::q(d)), glob(Vecs<T>::d0(d)),
+                 glob(Vecs<T>::d1(d)), lds);
+}
+
+template <typename T, int SB>
+__global__ void __launch_bounds__(SB * 64)
+k_cg_update_multi(const DeviceBatch *__restrict__ table, const int2 *__restrict__ list,
+                  int xcd_map)
+{
+    __shared__ double lds[8 * SB];
+    const int2 e = list[solver_block(xcd_map)];
+    if (e.x < 0)
+        return;
+    const DeviceBatch &d = table[e.x];
+    update_body<T, SB>(d, e.y, glob(Vecs<T>::P(d)), glob(Vecs<T>::r(d)), glob(Vecs<T>::z(d)),
+                       glob(Vecs<T>::q(d)), glob(Vecs<T>::d0(d)), glob(Vecs<T>::d1(d)), lds);
 }
 
 // both slots to the solve start (true residual of the warm start first)
@@ -440,6 +520,30 @@ void launch_cg_step(const DeviceBatch &d, bool fp32, int phase, int slices, hipS
         else
             hipLaunchKernelGGL((k_cg_update<double, SB>), upd_grid, upd_block, 0, s, d, d.P64,
                                d.r64, d.z64, d.q64, d.d64[0], d.d64[1]);
+    }
+}
+
+void launch_cg_step_multi(const DeviceBatch *table, const int2 *spmv_list, int spmv_grid,
+                          const int2 *upd_list, int upd_grid, bool fp32, int phase, int xcd_map,
+                          hipStream_t s)
+{
+    if (fp32)
+    {
+        if (phase == 0)
+            hipLaunchKernelGGL((k_cg_spmv_multi<float>), dim3(spmv_grid), dim3(SPMV_WAVES * 64), 0,
+                               s, table, spmv_list, xcd_map);
+        else
+            hipLaunchKernelGGL((k_cg_update_multi<float, SB>), dim3(upd_grid), dim3(SB * 64), 0, s,
+                               table, upd_list, xcd_map);
+    }
+    else
+    {
+        if (phase == 0)
+            hipLaunchKernelGGL((k_cg_spmv_multi<double>), dim3(spmv_grid), dim3(SPMV_WAVES * 64),
+                               0, s, table, spmv_list, xcd_map);
+        else
+            hipLaunchKernelGGL((k_cg_update_multi<double, SB>), dim3(upd_grid), dim3(SB * 64), 0,
+                               s, table, upd_list, xcd_map);
     }
 }
 

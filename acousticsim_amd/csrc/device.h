@@ -2,12 +2,12 @@
 // launchers.  One "batch" holds any number of independent cloths; every
 // array below is indexed by global (batch-wide) ids.
 //
-// Matrix layout: sliced ELL with slice height SLICE = 64 (one wavefront per
-// slice, one lane per block row).  Rows of one cloth occupy whole slices.
-// Entry (slice s, slot k, lane l) has index e = slice_off[s] + k*64 + l; its
-// column is col[e] and its 3x3 block is stored as nine planes of 64 values:
-// W[9*slice_off[s] + (9*k + q)*64 + l] for q = 3*a + b.  Every wave load is
-// one contiguous 64-value run.
+// Matrix layout: sliced ELL with slice height SLICE = 64.  Rows of one cloth
+// occupy whole slices.  Entry (slice s, slot k, row l) has index
+// e = slice_off[s] + k*64 + l and column col[e]; its 3x3 block is stored as
+// nine planes of 64 values: W[9*slice_off[s] + (9*k + q)*64 + l] for
+// q = 3*a + b, so a load of one q over 64 rows is one contiguous run.
+// (Block-major FP32 storage read with 16-byte loads measured no faster.)
 #pragma once
 #include <cstdint>
 
@@ -17,6 +17,18 @@ namespace bw
 {
 
 constexpr int SLICE = 64;
+
+// Global-address-space views of device pointers.  Pointers that kernels read
+// from device memory (a table of batches) are generic to the compiler, and
+// generic (flat) accesses cost more than global ones; every buffer here is
+// global memory, so the solver kernels access all of them through G<T>.
+// (The host compilation pass of these sources has no such address space.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T> using G = __attribute__((address_space(1))) T;
+#else
+template <typename T> using G = T;
+#endif
+template <typename T> __host__ __device__ inline G<T> *glob(T *p) { return (G<T> *)p; }
 // slices per solver workgroup (SB * 64 threads); a cloth's slices are padded
 // to a multiple of it so that no workgroup spans two cloths
 #ifndef BW_SB
@@ -125,10 +137,10 @@ struct DeviceBatch
 // gives XCD x the contiguous range [x*q + min(x, G%8), ...) of G/8 (+1)
 // blocks, so neighbouring slices of a cloth, which gather the same vector
 // entries, share one XCD's L2.
-__device__ inline int solver_block(const DeviceBatch &d)
+__device__ inline int solver_block(int xcd_map)
 {
     const int b = blockIdx.x;
-    if (!d.xcd_map)
+    if (!xcd_map)
         return b;
     const int G = gridDim.x, q = G / 8, rem = G % 8, x = b % 8;
     return x * q + (x < rem ? x : rem) + b / 8;
@@ -149,6 +161,12 @@ void launch_update_positions(const DeviceBatch &d, hipStream_t s);
 // runs ceil(slices / BW_SB) workgroups.
 void launch_cg_reset(const DeviceBatch &d, hipStream_t s);
 void launch_cg_step(const DeviceBatch &d, bool fp32, int phase, int slices, hipStream_t s);
+// the same over many batches: `table` holds the batches (device memory),
+// spmv_list one (batch, slice) entry per k_cg_spmv workgroup, upd_list one
+// (batch, solver workgroup) entry per k_cg_update workgroup; batch < 0 pads
+void launch_cg_step_multi(const DeviceBatch *table, const int2 *spmv_list, int spmv_grid,
+                          const int2 *upd_list, int upd_grid, bool fp32, int phase, int xcd_map,
+                          hipStream_t s);
 void launch_cg_timing_setup(const DeviceBatch &d, bool fp32, hipStream_t s);
 // measurement helpers: set the solver residual to 1 on every row; one SpMV
 // launch of the solver's precision

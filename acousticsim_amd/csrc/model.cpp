@@ -108,6 +108,38 @@ namespace
 // cells: hd "top" slots, then left/skew/right interleaved (left of col c+1
 // is the right of col c), then the next row's top slots are this row's
 // bottoms.
+// triangle -> edge ids (Triangulator.cpp:11-31): edge (tri[k], tri[k+1]) of
+// every triangle, found through the edges listed by their smaller vertex
+void triangle_edge_ids(ClothModel &m)
+{
+    std::vector<int> ptr(m.nv + 1, 0), hi(m.ne), id(m.ne);
+    for (const Edge &e : m.edges)
+        ptr[std::min(e.v0, e.v1) + 1]++;
+    for (int v = 0; v < m.nv; v++)
+        ptr[v + 1] += ptr[v];
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    for (int e = 0; e < m.ne; e++)
+    {
+        const int lo = std::min(m.edges[e].v0, m.edges[e].v1);
+        hi[fill[lo]] = std::max(m.edges[e].v0, m.edges[e].v1);
+        id[fill[lo]++] = e;
+    }
+    m.tri_e.resize(m.nt);
+    for (int t = 0; t < m.nt; t++)
+        for (int k = 0; k < 3; k++)
+        {
+            const int a = m.tri[t][k], b = m.tri[t][(k + 1) % 3];
+            const int lo = std::min(a, b), h = std::max(a, b);
+            int found = -1;
+            for (int q = ptr[lo]; q < ptr[lo + 1]; q++)
+                if (hi[q] == h)
+                    found = id[q];
+            if (found < 0)
+                throw std::runtime_error("config: triangle side is not an edge of the mesh");
+            m.tri_e[t][k] = found;
+        }
+}
+
 void make_grid(const ClothConfig &c, ClothModel &m)
 {
     const double height = c.cloth_size[0], width = c.cloth_size[1];
@@ -177,17 +209,7 @@ void make_grid(const ClothConfig &c, ClothModel &m)
         if (!set[e])
             throw std::runtime_error("internal: grid edge slot not filled");
 
-    // triangle -> edge ids (Triangulator.cpp:11-31)
-    std::map<std::pair<int, int>, int> key;
-    for (int e = 0; e < m.ne; e++)
-        key[{std::min(m.edges[e].v0, m.edges[e].v1), std::max(m.edges[e].v0, m.edges[e].v1)}] = e;
-    m.tri_e.resize(m.nt);
-    for (int t = 0; t < m.nt; t++)
-        for (int k = 0; k < 3; k++)
-        {
-            int a = m.tri[t][k], b = m.tri[t][(k + 1) % 3];
-            m.tri_e[t][k] = key.at({std::min(a, b), std::max(a, b)});
-        }
+    triangle_edge_ids(m);
 
     // init transform (cRotUtil::TransformMat, XYZ Euler: Rz*Ry*Rx, then
     // translation) and UV = rot(uv_rotation) * float(untransformed xy)
@@ -373,6 +395,135 @@ void ClothModel::reference_ell(std::vector<int> &rp, std::vector<int> &cl) const
     }
 }
 
+// Per-vertex element lists, the block pattern (the reference ELL rule plus
+// every triangle pair; the latter is inside the rule whenever each triangle
+// has an interior edge) and the per-block contribution lists in ascending
+// element order, built row by row with a column -> slot scatter array.
+void ClothModel::build_blocks()
+{
+    const int nie = (int)interior.size();
+    vt_ptr.assign(nv + 1, 0);
+    ve_ptr.assign(nv + 1, 0);
+    for (int t = 0; t < nt; t++)
+        for (int i = 0; i < 3; i++)
+            vt_ptr[tri[t][i] + 1]++;
+    for (int ie = 0; ie < nie; ie++)
+        for (int i = 0; i < 4; i++)
+            ve_ptr[stencil[interior[ie]][i] + 1]++;
+    for (int v = 0; v < nv; v++)
+    {
+        vt_ptr[v + 1] += vt_ptr[v];
+        ve_ptr[v + 1] += ve_ptr[v];
+    }
+    vt_list.assign(vt_ptr[nv], 0);
+    ve_list.assign(ve_ptr[nv], 0);
+    {
+        std::vector<int> tf(vt_ptr.begin(), vt_ptr.end() - 1);
+        std::vector<int> ef(ve_ptr.begin(), ve_ptr.end() - 1);
+        for (int t = 0; t < nt; t++)
+            for (int i = 0; i < 3; i++)
+                vt_list[tf[tri[t][i]]++] = (t << 2) | i;
+        for (int ie = 0; ie < nie; ie++)
+            for (int i = 0; i < 4; i++)
+                ve_list[ef[stencil[interior[ie]][i]]++] = (ie << 2) | i;
+    }
+    // rows: {v} U incident interior-edge stencils U incident triangles
+    std::vector<int> mark(nv, -1), row;
+    row_ptr.assign(nv + 1, 0);
+    cols.clear();
+    cols.reserve(14 * (size_t)nv);
+    diag_slot.resize(nv);
+    auto add = [&](int v, int c) {
+        if (mark[c] != v)
+        {
+            mark[c] = v;
+            row.push_back(c);
+        }
+    };
+    for (int v = 0; v < nv; v++)
+    {
+        row.clear();
+        add(v, v);
+        for (int q = ve_ptr[v]; q < ve_ptr[v + 1]; q++)
+            for (int c : stencil[interior[ve_list[q] >> 2]])
+                add(v, c);
+        for (int q = vt_ptr[v]; q < vt_ptr[v + 1]; q++)
+            for (int c : tri[vt_list[q] >> 2])
+                add(v, c);
+        std::sort(row.begin(), row.end());
+        diag_slot[v] = (int)(std::lower_bound(row.begin(), row.end(), v) - row.begin());
+        cols.insert(cols.end(), row.begin(), row.end());
+        row_ptr[v + 1] = (int)cols.size();
+    }
+    // contribution lists: count, then fill, row by row; slot_of[c] is the
+    // block (v, c) of the current row v (only columns of row v are read)
+    const int nb = nnzb();
+    std::vector<int> &slot_of = mark;
+    s_ptr.assign(nb + 1, 0);
+    b_ptr.assign(nb + 1, 0);
+    std::vector<int> sfill, bfill;
+    for (int pass = 0; pass < 2; pass++)
+    {
+        if (pass == 1)
+        {
+            for (int b = 0; b < nb; b++)
+            {
+                s_ptr[b + 1] += s_ptr[b];
+                b_ptr[b + 1] += b_ptr[b];
+            }
+            s_list.assign(s_ptr[nb], 0);
+            b_list.assign(b_ptr[nb], 0);
+            sfill.assign(s_ptr.begin(), s_ptr.end() - 1);
+            bfill.assign(b_ptr.begin(), b_ptr.end() - 1);
+        }
+        for (int v = 0; v < nv; v++)
+        {
+            for (int k = row_ptr[v]; k < row_ptr[v + 1]; k++)
+                slot_of[cols[k]] = k;
+            for (int q = vt_ptr[v]; q < vt_ptr[v + 1]; q++)
+            {
+                const int t = vt_list[q] >> 2, i = vt_list[q] & 3;
+                for (int j = 0; j < 3; j++)
+                {
+                    const int b = slot_of[tri[t][j]];
+                    if (pass == 0)
+                        s_ptr[b + 1]++;
+                    else
+                        s_list[sfill[b]++] = (t << 4) | (i << 2) | j;
+                }
+            }
+            for (int q = ve_ptr[v]; q < ve_ptr[v + 1]; q++)
+            {
+                const int ie = ve_list[q] >> 2, i = ve_list[q] & 3;
+                const auto &st = stencil[interior[ie]];
+                for (int j = 0; j < 4; j++)
+                {
+                    const int b = slot_of[st[j]];
+                    if (pass == 0)
+                        b_ptr[b + 1]++;
+                    else
+                        b_list[bfill[b]++] = (ie << 4) | (i << 2) | j;
+                }
+            }
+        }
+    }
+}
+
+ClothModel ClothModel::from_json_text(const std::string &text)
+{
+    Json j = Json::parse(text);
+    if (j.kind != Json::Object)
+        throw std::runtime_error("config: top level must be an object");
+    if (j.has("cloth_type"))
+    {
+        const std::string t = j.at("cloth_type").as_string();
+        if (t != "fem" && t != "fem_hip")
+            throw std::runtime_error("config: cloth_type \"" + t +
+                                     "\" is not the plain implicit method (\"fem\")");
+    }
+    return build(ClothConfig::from_json(j));
+}
+
 ClothModel ClothModel::build(const ClothConfig &cfg)
 {
     ClothModel m;
@@ -414,107 +565,7 @@ ClothModel ClothModel::build(const ClothConfig &cfg)
     }
     m.rebuild_pins_from_list();
 
-    // block structure: the reference ELL rule plus every triangle pair (the
-    // latter is already inside the rule whenever each triangle has an
-    // interior edge, which holds on every grid)
-    {
-        std::vector<int> rp, cl;
-        m.reference_ell(rp, cl);
-        std::vector<std::vector<int>> rows(m.nv);
-        for (int v = 0; v < m.nv; v++)
-            rows[v].assign(cl.begin() + rp[v], cl.begin() + rp[v + 1]);
-        for (auto &t : m.tri)
-            for (int a : t)
-                for (int b : t)
-                    rows[a].push_back(b);
-        m.row_ptr.assign(m.nv + 1, 0);
-        m.cols.clear();
-        m.diag_slot.resize(m.nv);
-        for (int v = 0; v < m.nv; v++)
-        {
-            auto &r = rows[v];
-            std::sort(r.begin(), r.end());
-            r.erase(std::unique(r.begin(), r.end()), r.end());
-            m.diag_slot[v] = (int)(std::lower_bound(r.begin(), r.end(), v) - r.begin());
-            m.cols.insert(m.cols.end(), r.begin(), r.end());
-            m.row_ptr[v + 1] = (int)m.cols.size();
-        }
-    }
-    auto slot = [&](int r, int c) {
-        auto b = m.cols.begin() + m.row_ptr[r], e = m.cols.begin() + m.row_ptr[r + 1];
-        auto it = std::lower_bound(b, e, c);
-        return (int)(it - m.cols.begin());
-    };
-    // contribution lists (ascending element id inside every block)
-    const int nb = m.nnzb();
-    std::vector<int> scount(nb + 1, 0), bcount(nb + 1, 0);
-    for (int t = 0; t < m.nt; t++)
-        for (int i = 0; i < 3; i++)
-            for (int j = 0; j < 3; j++)
-                scount[slot(m.tri[t][i], m.tri[t][j]) + 1]++;
-    for (size_t ie = 0; ie < m.interior.size(); ie++)
-    {
-        const auto &s = m.stencil[m.interior[ie]];
-        for (int i = 0; i < 4; i++)
-            for (int j = 0; j < 4; j++)
-                bcount[slot(s[i], s[j]) + 1]++;
-    }
-    for (int b = 0; b < nb; b++)
-    {
-        scount[b + 1] += scount[b];
-        bcount[b + 1] += bcount[b];
-    }
-    m.s_ptr = scount;
-    m.b_ptr = bcount;
-    m.s_list.assign(scount[nb], 0);
-    m.b_list.assign(bcount[nb], 0);
-    std::vector<int> sfill(nb, 0), bfill(nb, 0);
-    for (int t = 0; t < m.nt; t++)
-        for (int i = 0; i < 3; i++)
-            for (int j = 0; j < 3; j++)
-            {
-                const int s = slot(m.tri[t][i], m.tri[t][j]);
-                m.s_list[m.s_ptr[s] + sfill[s]++] = (t << 4) | (i << 2) | j;
-            }
-    for (size_t ie = 0; ie < m.interior.size(); ie++)
-    {
-        const auto &st = m.stencil[m.interior[ie]];
-        for (int i = 0; i < 4; i++)
-            for (int j = 0; j < 4; j++)
-            {
-                const int s = slot(st[i], st[j]);
-                m.b_list[m.b_ptr[s] + bfill[s]++] = ((int)ie << 4) | (i << 2) | j;
-            }
-    }
-    // per-vertex force lists
-    m.vt_ptr.assign(m.nv + 1, 0);
-    m.ve_ptr.assign(m.nv + 1, 0);
-    for (int t = 0; t < m.nt; t++)
-        for (int i = 0; i < 3; i++)
-            m.vt_ptr[m.tri[t][i] + 1]++;
-    for (size_t ie = 0; ie < m.interior.size(); ie++)
-        for (int i = 0; i < 4; i++)
-            m.ve_ptr[m.stencil[m.interior[ie]][i] + 1]++;
-    for (int v = 0; v < m.nv; v++)
-    {
-        m.vt_ptr[v + 1] += m.vt_ptr[v];
-        m.ve_ptr[v + 1] += m.ve_ptr[v];
-    }
-    m.vt_list.assign(m.vt_ptr[m.nv], 0);
-    m.ve_list.assign(m.ve_ptr[m.nv], 0);
-    std::vector<int> tf(m.nv, 0), ef(m.nv, 0);
-    for (int t = 0; t < m.nt; t++)
-        for (int i = 0; i < 3; i++)
-        {
-            const int v = m.tri[t][i];
-            m.vt_list[m.vt_ptr[v] + tf[v]++] = (t << 2) | i;
-        }
-    for (size_t ie = 0; ie < m.interior.size(); ie++)
-        for (int i = 0; i < 4; i++)
-        {
-            const int v = m.stencil[m.interior[ie]][i];
-            m.ve_list[m.ve_ptr[v] + ef[v]++] = ((int)ie << 2) | i;
-        }
+    m.build_blocks();
     return m;
 }
 

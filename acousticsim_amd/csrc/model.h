@@ -87,7 +87,12 @@ struct ClothModel
     std::vector<int> vt_ptr, vt_list, ve_ptr, ve_list;
 
     static ClothModel build(const ClothConfig &cfg);
+    // parse the cloth JSON (cloth_type, when present, must be "fem" or
+    // "fem_hip") and build
+    static ClothModel from_json_text(const std::string &text);
     void rebuild_pins_from_list();
+    // element lists, block pattern and contribution lists from the topology
+    void build_blocks();
     int nnzb() const { return (int)cols.size(); }
     // the reference GPU ELL rule (BaraffClothGpu.cpp:438-500)
     void reference_ell(std::vector<int> &rp, std::vector<int> &cl) const;

@@ -52,7 +52,7 @@ k_pcg_spmv(DeviceBatch d, const T *__restrict__ W, const T *__restrict__ P,
            T *__restrict__ q)
 {
     __shared__ double lds[8 * SB];
-    const int blk = solver_block(d);
+    const int blk = solver_block(d.xcd_map);
     const int inst = d.blk_inst[blk];
     InstState *st = d.state + inst;
     if (st->done)
@@ -129,7 +129,7 @@ k_pcg_update(DeviceBatch d, const T *__restrict__ P, T *__restrict__ x, T *__res
              const T *__restrict__ dbuf0, const T *__restrict__ dbuf1, const T *__restrict__ q)
 {
     __shared__ double lds[8 * SB];
-    const int blk = solver_block(d);
+    const int blk = solver_block(d.xcd_map);
     const int inst = d.blk_inst[blk];
     InstState *st = d.state + inst;
     if (st->done)
@@ -182,7 +182,7 @@ k_pcg_update(DeviceBatch d, const T *__restrict__ P, T *__restrict__ x, T *__res
 template <int SB> __global__ void __launch_bounds__(SB * 64) k_residual64(DeviceBatch d, int mode)
 {
     __shared__ double lds[8 * SB];
-    const int blk = solver_block(d);
+    const int blk = solver_block(d.xcd_map);
     const int lane = threadIdx.x & 63;
     const int s = blk * SB + (threadIdx.x >> 6);
     const int p = s * SLICE + lane;

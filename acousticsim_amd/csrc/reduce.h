@@ -51,8 +51,8 @@ template <int SB, int N> __device__ inline void block_sum(double (&v)[N], double
 }
 
 // store this workgroup's (block-summed) values as its partials
-template <int N>
-__device__ inline void write_partials(double *part, int blk, const int (&comp)[N], const double (&v)[N])
+template <int N, typename PP>
+__device__ inline void write_partials(PP part, int blk, const int (&comp)[N], const double (&v)[N])
 {
     if (threadIdx.x == 0)
         for (int j = 0; j < N; j++)
@@ -60,8 +60,8 @@ __device__ inline void write_partials(double *part, int blk, const int (&comp)[N
 }
 
 // sum the cloth's partials (workgroups b0 .. b0+nb-1), result in every thread
-template <int SB, int N>
-__device__ inline void reduce_cloth(const double *part, int b0, int nb, const int (&comp)[N],
+template <int SB, int N, typename PP>
+__device__ inline void reduce_cloth(PP part, int b0, int nb, const int (&comp)[N],
                                     double (&out)[N], double *lds)
 {
     for (int j = 0; j < N; j++)

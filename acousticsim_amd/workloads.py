@@ -123,3 +123,24 @@ def batch64_configs(**kw):
     members = batch64_members(**kw)
     return [(grid_config(s, dt=BATCH64["dt"], stiffness_scale=sc), seed)
             for (s, sc, seed) in members]
+
+
+# Serving: step requests arrive as a Poisson stream; each is a unit-square
+# cloth with a log-uniform vertex count, asking for `steps` substeps.
+SERVE = dict(rate=2000.0, v_lo=1e3, v_hi=2e5, steps=10, dt=0.01, size=1.0, seed=4)
+
+
+def serve_requests(n, seed=SERVE["seed"], rate=SERVE["rate"], v_lo=SERVE["v_lo"],
+                   v_hi=SERVE["v_hi"]):
+    """[(arrival_s, side, jitter_seed)] of n requests: exponential
+    inter-arrival times at `rate`, vertex count log-uniform in [v_lo, v_hi]
+    rounded to a square grid of side round(sqrt(V))."""
+    u = uniform01(seed, 2 * n)
+    t = 0.0
+    out = []
+    for k in range(n):
+        t += -math.log(1.0 - u[2 * k]) / rate
+        v = math.exp(math.log(v_lo) + u[2 * k + 1] * (math.log(v_hi) - math.log(v_lo)))
+        side = max(4, int(round(math.sqrt(v))))
+        out.append((t, side, seed * 100000 + k + 1))
+    return out

@@ -21,6 +21,7 @@ Reported beside it, in the same run:
     method, one thread, on a bounded sample of the same cloths
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batch64|single257|single21]
+       python bench.py --workload serve [--requests N] [--rate R]
        python bench.py --roofline-only [--copy-calibration]
 The second form only assembles the system and times the dominant kernel (and,
 with --copy-calibration, first copies 256 MiB device-to-device three times):
@@ -176,12 +177,84 @@ def roofline_only(args, ms):
                       "frac": achieved / HBM_PEAK_GBS}), flush=True)
 
 
+def serve(args):
+    """Serving workload: requests arrive as a Poisson stream (seeded), each a
+    unit-square cloth of log-uniform vertex count asking for 10 steps; the
+    server admits them against a 200 GB device budget and solves whatever is
+    resident together.  Reports throughput and request latency."""
+    reqs = wl.serve_requests(args.requests, rate=args.rate)
+    work = []
+    for t, side, seed in reqs:
+        c = wl.grid_config(side, size=wl.SERVE["size"], dt=wl.SERVE["dt"])
+        c["precision"] = "mixed"
+        work.append((t, c, wl.jittered_positions(c, seed)))
+    s = capi.Server(budget_bytes=200e9, max_resident=1024)
+    print("bench: serving %d requests at %.0f req/s" % (len(work), args.rate), file=sys.stderr,
+          flush=True)
+    ids = []
+    t0 = time.perf_counter()
+    for t, c, x0 in work:
+        delay = t0 + t - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        ids.append(s.submit(c, x0, wl.SERVE["steps"]))
+    for k, rid in enumerate(ids):
+        if not s.wait(rid, 1200.0):
+            raise RuntimeError("request %d did not finish" % k)
+        if (k + 1) % 16 == 0:
+            print("bench: %d requests done" % (k + 1), file=sys.stderr, flush=True)
+    infos = [s.info(r) for r in ids]
+    counters = s.counters()
+    s.close()
+    if args.dump_requests:
+        with open(args.dump_requests, "w") as f:
+            json.dump(infos, f)
+    bad = [i for i in infos if i["status"] != 0]
+    if bad:
+        raise RuntimeError("%d requests failed; first: %s" % (len(bad), bad[0]))
+    lat = np.array([i["t_done"] - i["t_submit"] for i in infos])
+    queue = np.array([i["t_admit"] - i["t_submit"] for i in infos])
+    t_first = min(i["t_submit"] for i in infos)
+    makespan = max(i["t_done"] for i in infos) - t_first
+    steps = sum(i["steps_done"] for i in infos)
+    vsteps = sum(i["steps_done"] * i["num_vertices"] for i in infos)
+    out = {
+        "metric": METRIC,
+        "value": steps / makespan,
+        "unit": "cloth steps/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "serving: Poisson %.0f req/s, %d requests, unit-square grid cloths "
+                               "with V log-uniform in [1e3, 2e5], 10 steps each, dt 0.01 s, tol "
+                               "1e-6, 200 GB budget" % (args.rate, len(infos))},
+        "requests_per_s": len(infos) / makespan,
+        "vertex_steps_per_s": vsteps / makespan,
+        "latency_p50_s": float(np.percentile(lat, 50)),
+        "latency_p99_s": float(np.percentile(lat, 99)),
+        "queue_p50_s": float(np.percentile(queue, 50)),
+        "makespan_s": makespan,
+        "max_resident": counters["max_resident_seen"],
+        "pcg_iterations_mean": float(np.mean([i["pcg_iterations"] / i["steps_done"] for i in infos])),
+        "max_rel_residual": max(i["max_rel_residual"] for i in infos),
+        "solver_chunks": counters["chunks"],
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="batch64", choices=["batch64", "single257", "single21"])
+    ap.add_argument("--workload", default="batch64",
+                    choices=["batch64", "single257", "single21", "serve"])
+    ap.add_argument("--requests", type=int, default=64, help="serve: number of requests")
+    ap.add_argument("--rate", type=float, default=wl.SERVE["rate"], help="serve: arrivals per s")
+    ap.add_argument("--dump-requests", default=None, help="serve: write per-request records here")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--roofline-only", action="store_true",
@@ -195,6 +268,9 @@ def main():
               "of it", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "serve":
+        serve(args)
+        return
     ms = members(args.workload, "mixed")
     if args.roofline_only:
         roofline_only(args, ms)

@@ -138,6 +138,56 @@ int bw_batch_set_graphs(bw_batch *b, int enable);
    overwritten (positions are not). */
 int bw_batch_time_solver_kernel(bw_batch *b, int reps, double *avg_us, double *alg_bytes);
 
+/* ---- serving (no reference counterpart): step requests of different
+   cloths arrive at any time, are admitted against a device-memory budget,
+   stay resident on the device for their steps, and are solved together with
+   everything else resident (continuous batching at the granularity of solver
+   chunks; a request's results are the same alone and in any mix).  All
+   requests of one server use the same precision and the tolerance solver. -- */
+
+typedef struct bw_server bw_server;
+
+typedef struct bw_request_info
+{
+    int status;              /* BW_OK, or the failed request's error code */
+    int done;                /* all steps taken, or failed */
+    int num_vertices;
+    int steps, steps_done;
+    long long pcg_iterations; /* over all steps */
+    double max_rel_residual;  /* worst final residual over the steps */
+    double t_submit, t_admit, t_first_step, t_done; /* seconds, server clock */
+    double device_bytes;      /* device memory the request held */
+} bw_request_info;
+
+typedef struct bw_server_counters
+{
+    int queued, resident, completed, max_resident_seen;
+    double bytes_in_use;
+    long long chunks, launches; /* multi-request solver chunks and launches */
+} bw_server_counters;
+
+/* budget_bytes: device memory for resident requests (e.g. 200e9);
+   max_resident: state slots (<= 65536); precision: "mixed" or "fp64" */
+int bw_server_create(double budget_bytes, int max_resident, const char *precision,
+                     bw_server **out);
+/* stops the server; requests still running are abandoned */
+void bw_server_destroy(bw_server *s);
+/* enqueue nsteps substeps of the cloth in json (the cloth JSON of
+   bw_create); x0 (nx0 = 3V doubles) may be NULL.  Returns at once; errors
+   in the configuration are reported by the request's status. */
+int bw_server_submit(bw_server *s, const char *json, const double *x0, int nx0, int nsteps,
+                     long long *id);
+/* BW_OK once the request is done (or failed), BW_ERR_STATE on timeout */
+int bw_server_wait(bw_server *s, long long id, double timeout_s);
+int bw_server_get_info(bw_server *s, long long id, bw_request_info *info);
+/* final positions (3V doubles) of a request that completed with BW_OK */
+int bw_server_get_pos(bw_server *s, long long id, double *x);
+/* forget a finished request */
+int bw_server_release(bw_server *s, long long id);
+int bw_server_get_counters(bw_server *s, bw_server_counters *c);
+/* seconds on the server's clock */
+double bw_server_now(bw_server *s);
+
 #ifdef __cplusplus
 }
 #endif
