@@ -83,7 +83,7 @@ EXPORTS = [
     "bw_get_ell_map", "bw_assemble_system", "bw_get_last_stats", "bw_last_error", "bw_batch_create",
     "bw_batch_destroy", "bw_batch_size", "bw_batch_step", "bw_batch_num_vertices",
     "bw_batch_get_pos", "bw_batch_set_state", "bw_batch_get_stats", "bw_batch_set_graphs",
-    "bw_batch_time_solver_kernel", "bw_server_create", "bw_server_destroy", "bw_server_submit",
+    "bw_batch_time_solver_kernel", "bw_batch_autotune", "bw_server_create", "bw_server_destroy", "bw_server_submit",
     "bw_server_wait", "bw_server_get_info", "bw_server_get_pos", "bw_server_release",
     "bw_server_get_counters", "bw_server_now",
 ]
@@ -134,6 +134,7 @@ def lib():
         "bw_batch_get_stats": (i32, [vp, i32, P(StepStats)]),
         "bw_batch_set_graphs": (i32, [vp, i32]),
         "bw_batch_time_solver_kernel": (i32, [vp, i32, dp, dp]),
+        "bw_batch_autotune": (i32, [vp, ip, dp]),
         "bw_server_create": (i32, [ctypes.c_double, i32, cp, P(vp)]),
         "bw_server_destroy": (None, [vp]),
         "bw_server_submit": (i32, [vp, cp, dp, i32, i32, P(ctypes.c_longlong)]),
@@ -337,6 +338,13 @@ class ClothBatch:
         nbytes = np.zeros(1)
         _check(lib().bw_batch_time_solver_kernel(self._h, reps, _dp(us), _dp(nbytes)))
         return float(us[0]), float(nbytes[0])
+
+    def autotune(self):
+        """(lanes per row chosen, {lanes: mean SpMV launch us})"""
+        lanes = ctypes.c_int()
+        us = np.zeros(3)
+        _check(lib().bw_batch_autotune(self._h, ctypes.byref(lanes), _dp(us)))
+        return lanes.value, {1: us[0], 2: us[1], 4: us[2]}
 
     def set_state(self, i, x, xpre):
         x = np.ascontiguousarray(x, dtype=np.float64)

@@ -826,8 +826,47 @@ void Batch::end_step(const CgState *states, double step_us)
         throw std::runtime_error("step: non-finite residual (NaN/Inf in the state)");
 }
 
+// Picks the SpMV variant (lanes per row, see cg.hip) that runs this batch's
+// matrix fastest, timing each on the system assembled at the current state.
+// BW_SPMV_LPR=1|2|4 in the environment fixes the choice instead.
+int Batch::autotune()
+{
+    if (kind_ == SolverKind::RefPCG64)
+        return d_.spmv_lpr;
+    for (auto &g : graphs_) // graphs hold the previous choice
+        (void)hipGraphExecDestroy(g.exec);
+    graphs_.clear();
+    tuned_ = true;
+    if (const char *e = std::getenv("BW_SPMV_LPR"))
+    {
+        const int v = std::atoi(e);
+        if (v == 1 || v == 2 || v == 4)
+        {
+            d_.spmv_lpr = v;
+            return v;
+        }
+    }
+    double best = 1e300;
+    int pick = 4;
+    const int choices[3] = {1, 2, 4};
+    for (int k = 0; k < 3; k++)
+    {
+        d_.spmv_lpr = choices[k];
+        tune_us_[k] = time_solver_kernel(20, nullptr);
+        if (tune_us_[k] < best)
+        {
+            best = tune_us_[k];
+            pick = choices[k];
+        }
+    }
+    d_.spmv_lpr = pick;
+    return pick;
+}
+
 void Batch::step(int nsteps)
 {
+    if (!tuned_)
+        autotune();
     for (int n = 0; n < nsteps; n++)
     {
         const auto t0 = std::chrono::steady_clock::now();

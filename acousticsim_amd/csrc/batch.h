@@ -60,6 +60,11 @@ public:
     void begin_step();
     void end_step(const CgState *states, double step_us);
     size_t device_bytes() const { return device_bytes_; }
+    // choose the SpMV variant by timing (done by the first step()); returns
+    // the lanes per row picked; tune_us() has the times of 1, 2, 4
+    int autotune();
+    const double *tune_us() const { return tune_us_; }
+    int spmv_lpr() const { return d_.spmv_lpr; }
     // assemble W and rhs at the current state (no solve) and copy them out in
     // the model's block order
     void assemble_system(int i, double *W, double *rhs);
@@ -97,6 +102,8 @@ private:
     hipStream_t stream_ = nullptr;
     BatchOptions opt_;
     bool own_stream_ = false;
+    bool tuned_ = false;
+    double tune_us_[3] = {0, 0, 0};
     size_t device_bytes_ = 0;
     SolverKind kind_ = SolverKind::CG32;
     InstState *host_state_ = nullptr; // pinned, reference PCG state

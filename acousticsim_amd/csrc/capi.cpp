@@ -385,6 +385,19 @@ int bw_batch_time_solver_kernel(bw_batch *b, int reps, double *avg_us, double *a
     });
 }
 
+int bw_batch_autotune(bw_batch *b, int *lanes, double *us)
+{
+    return guard([&] {
+        if (!b || !lanes)
+            return fail(BW_ERR_ARG, "NULL argument");
+        *lanes = b->batch->autotune();
+        if (us)
+            for (int k = 0; k < 3; k++)
+                us[k] = b->batch->tune_us()[k];
+        return BW_OK;
+    });
+}
+
 int bw_batch_set_graphs(bw_batch *b, int enable)
 {
     if (!b)

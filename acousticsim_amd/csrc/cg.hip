@@ -97,18 +97,12 @@ template <typename SP> __device__ inline void store_state(SP p, const CgState &s
     p->final_check = s.final_check;
 }
 
-#ifndef BW_SPMV_UNROLL
-#define BW_SPMV_UNROLL 1
-#endif
-#define BW_PRAGMA(x) _Pragma(#x)
-#define BW_UNROLL(n) BW_PRAGMA(unroll n)
-
-#ifndef BW_LPR
-#define BW_LPR 4
-#endif
-constexpr int LPR = BW_LPR;      // SpMV lanes per row (1, 2 or 4)
-constexpr int RPW = SLICE / LPR; // SpMV rows per wavefront
-constexpr int SPMV_WAVES = SLICE / RPW;
+// SpMV variants: L lanes per row (1, 2 or 4; a workgroup of L wavefronts
+// covers one 64-row slice, lane phase j takes ELL slots j, j+L, ...), slot
+// loop unrolled 4/L times.  Fewer lanes per row keep more independent loads
+// per lane in flight; more lanes shorten each lane's chain of dependent
+// loads, which is what bounds a cloth solved alone.  The batch picks the
+// fastest on its own matrix (Batch::autotune).
 
 // d = z + beta d_prev, z = P r
 template <typename T> __device__ inline T dir_value(T z, T dold, T beta, bool first)
@@ -116,10 +110,10 @@ template <typename T> __device__ inline T dir_value(T z, T dold, T beta, bool fi
     return first ? z : z + beta * dold;
 }
 
-// sum over the lane phases of a row (lanes l, l+RPW, l+2 RPW, ...)
-template <typename T> __device__ inline T phase_sum(T v)
+// sum over the L lane phases of a row (lanes l, l+64/L, ...)
+template <int L, typename T> __device__ inline T phase_sum(T v)
 {
-    for (int off = RPW; off < 64; off <<= 1)
+    for (int off = SLICE / L; off < 64; off <<= 1)
         v += __shfl_xor(v, off, 64);
     return v;
 }
@@ -148,7 +142,7 @@ template <> struct Vecs<double>
 };
 
 // k_cg_spmv on slice sl of batch d
-template <typename T>
+template <typename T, int L>
 __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> *__restrict__ W,
                                  const G<T> *__restrict__ P, G<T> *__restrict__ r,
                                  G<T> *__restrict__ z, G<T> *__restrict__ q,
@@ -176,7 +170,7 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
     {
         double v[3];
         const int comp[3] = {P_RZ, P_RR, P_BB};
-        reduce_cloth<SPMV_WAVES, 3>(part_in, ip.slice_begin, ip.slice_count, comp, v, lds);
+        reduce_cloth<L, 3>(part_in, ip.slice_begin, ip.slice_count, comp, v, lds);
         const double rz = v[0], rr = v[1];
         if (st.pending)
         {
@@ -209,6 +203,7 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
         }
     }
     const int lane = threadIdx.x & 63;
+    constexpr int RPW = SLICE / L;                                  // rows per wavefront
     const int row = (threadIdx.x >> 6) * RPW + (lane & (RPW - 1)); // row within the slice
     const int j = lane / RPW;                                        // lane phase
     const int p = sl * SLICE + row;
@@ -222,7 +217,7 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
         const G<double> *Wl = glob(d.W64) + 9 * base + row;
         const G<double> *dx = glob(d.dx);
         double y[3] = {0.0, 0.0, 0.0};
-        for (int k = j; k < w; k += LPR)
+        for (int k = j; k < w; k += L)
         {
             const size_t c3 = 3 * (size_t)cl[k * SLICE];
             const G<double> *wk = Wl + 9 * (size_t)k * SLICE;
@@ -232,7 +227,7 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
                         wk[(3 * a + 2) * SLICE] * x2;
         }
         for (int a = 0; a < 3; a++)
-            y[a] = phase_sum(y[a]);
+            y[a] = phase_sum<L>(y[a]);
         double v[3] = {0.0, 0.0, 0.0};
         if (j == 0)
             for (int a = 0; a < 3; a++)
@@ -247,7 +242,7 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
                 v[1] += rv * rv;
                 v[2] += bv * bv;
             }
-        block_sum<SPMV_WAVES, 3>(v, lds);
+        block_sum<L, 3>(v, lds);
         const int comp[3] = {P_RZ, P_RR, P_BB};
         write_partials<3>(part_out, sl, comp, v);
         st.pending = 1;
@@ -268,8 +263,8 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
     G<T> *dnew = (st.iter & 1) ? dbuf1 : dbuf0;
     const G<T> *Wl = W + 9 * base + row;
     T y0 = 0, y1 = 0, y2 = 0;
-    BW_UNROLL(BW_SPMV_UNROLL)
-    for (int k = j; k < w; k += LPR)
+#pragma unroll(4 / L)
+    for (int k = j; k < w; k += L)
     {
         const size_t c3 = 3 * (size_t)cl[k * SLICE];
         const T v0 = dir_value<T>(z[c3], dold[c3], bt, first);
@@ -280,9 +275,9 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
         y1 += wk[3 * SLICE] * v0 + wk[4 * SLICE] * v1 + wk[5 * SLICE] * v2;
         y2 += wk[6 * SLICE] * v0 + wk[7 * SLICE] * v1 + wk[8 * SLICE] * v2;
     }
-    y0 = phase_sum(y0);
-    y1 = phase_sum(y1);
-    y2 = phase_sum(y2);
+    y0 = phase_sum<L>(y0);
+    y1 = phase_sum<L>(y1);
+    y2 = phase_sum<L>(y2);
     double v[1] = {0.0};
     if (j == 0)
     {
@@ -297,7 +292,7 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
         q[p3 + 2] = y2;
         v[0] = (double)dn[0] * (double)y0 + (double)dn[1] * (double)y1 + (double)dn[2] * (double)y2;
     }
-    block_sum<SPMV_WAVES, 1>(v, lds);
+    block_sum<L, 1>(v, lds);
     const int comp[1] = {P_DQ};
     write_partials<1>(part_out, sl, comp, v);
     if (lead)
@@ -388,12 +383,12 @@ __device__ inline void update_body(const DeviceBatch &d, const int blk,
         store_state(st_out, st);
 }
 
-template <typename T>
-__global__ void __launch_bounds__(SPMV_WAVES * 64)
+template <typename T, int L>
+__global__ void __launch_bounds__(L * 64)
 k_cg_spmv(DeviceBatch d, const T *__restrict__ W, const T *__restrict__ P, T *__restrict__ r,
           T *__restrict__ z, T *__restrict__ q, T *__restrict__ dbuf0, T *__restrict__ dbuf1)
 {
-    __shared__ double lds[8 * SPMV_WAVES];
+    __shared__ double lds[8 * L];
     // one slice per workgroup, from the list of active slices when given
     int sl = solver_block(d.xcd_map);
     if (d.spmv_list)
@@ -402,8 +397,8 @@ k_cg_spmv(DeviceBatch d, const T *__restrict__ W, const T *__restrict__ P, T *__
         if (sl < 0)
             return;
     }
-    spmv_body<T>(d, sl, glob(W), glob(P), glob(r), glob(z), glob(q), glob(dbuf0), glob(dbuf1),
-                 lds);
+    spmv_body<T, L>(d, sl, glob(W), glob(P), glob(r), glob(z), glob(q), glob(dbuf0),
+                    glob(dbuf1), lds);
 }
 
 template <typename T, int SB>
@@ -424,16 +419,16 @@ k_cg_update(DeviceBatch d, const T *__restrict__ P, T *__restrict__ r, T *__rest
 
 // Many batches in one launch: entry (batch, slice) of `list` per workgroup,
 // batches in the device table `table`; entries with batch < 0 are padding.
-template <typename T>
-__global__ void __launch_bounds__(SPMV_WAVES * 64)
+template <typename T, int L>
+__global__ void __launch_bounds__(L * 64)
 k_cg_spmv_multi(const DeviceBatch *__restrict__ table, const int2 *__restrict__ list, int xcd_map)
 {
-    __shared__ double lds[8 * SPMV_WAVES];
+    __shared__ double lds[8 * L];
     const int2 e = list[solver_block(xcd_map)];
     if (e.x < 0)
         return;
     const DeviceBatch &d = table[e.x];
-    spmv_body<T>(d, e.y, glob(Vecs<T>::W(d)), glob(Vecs<T>::P(d)), glob(Vecs<T>::r(d)),
+    spmv_body<T, L>(d, e.y, glob(Vecs<T>::W(d)), glob(Vecs<T>::P(d)), glob(Vecs<T>::r(d)),
                  glob(Vecs<T>::z(d)), glob(Vecs<T>::q(d)), glob(Vecs<T>::d0(d)),
                  glob(Vecs<T>::d1(d)), lds);
 }
@@ -499,15 +494,49 @@ void launch_cg_reset(const DeviceBatch &d, hipStream_t s)
     hipLaunchKernelGGL(k_cg_reset, dim3((2 * d.n_inst + 63) / 64), dim3(64), 0, s, d.cg, d.n_inst);
 }
 
+template <typename T, int L>
+void launch_spmv(const DeviceBatch &d, int slices, const T *W, const T *P, T *r, T *z, T *q,
+                 T *d0, T *d1, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_cg_spmv<T, L>), dim3(slices), dim3(L * 64), 0, s, d, W, P, r, z, q, d0,
+                       d1);
+}
+
+template <typename T>
+void launch_spmv_lpr(const DeviceBatch &d, int slices, const T *W, const T *P, T *r, T *z, T *q,
+                     T *d0, T *d1, hipStream_t s)
+{
+    if (d.spmv_lpr == 1)
+        launch_spmv<T, 1>(d, slices, W, P, r, z, q, d0, d1, s);
+    else if (d.spmv_lpr == 2)
+        launch_spmv<T, 2>(d, slices, W, P, r, z, q, d0, d1, s);
+    else
+        launch_spmv<T, 4>(d, slices, W, P, r, z, q, d0, d1, s);
+}
+
+template <typename T>
+void launch_spmv_multi(const DeviceBatch *table, const int2 *list, int grid, int lpr, int xcd_map,
+                       hipStream_t s)
+{
+    if (lpr == 1)
+        hipLaunchKernelGGL((k_cg_spmv_multi<T, 1>), dim3(grid), dim3(64), 0, s, table, list,
+                           xcd_map);
+    else if (lpr == 2)
+        hipLaunchKernelGGL((k_cg_spmv_multi<T, 2>), dim3(grid), dim3(128), 0, s, table, list,
+                           xcd_map);
+    else
+        hipLaunchKernelGGL((k_cg_spmv_multi<T, 4>), dim3(grid), dim3(256), 0, s, table, list,
+                           xcd_map);
+}
+
 void launch_cg_step(const DeviceBatch &d, bool fp32, int phase, int slices, hipStream_t s)
 {
-    const dim3 spmv_grid(slices), spmv_block(SPMV_WAVES * 64);
     const dim3 upd_grid((slices + SB - 1) / SB), upd_block(SB * 64);
     if (fp32)
     {
         if (phase == 0)
-            hipLaunchKernelGGL((k_cg_spmv<float>), spmv_grid, spmv_block, 0, s, d, d.W32, d.P32,
-                               d.r32, d.z32, d.q32, d.d32[0], d.d32[1]);
+            launch_spmv_lpr<float>(d, slices, d.W32, d.P32, d.r32, d.z32, d.q32, d.d32[0], d.d32[1],
+                                   s);
         else
             hipLaunchKernelGGL((k_cg_update<float, SB>), upd_grid, upd_block, 0, s, d, d.P32,
                                d.r32, d.z32, d.q32, d.d32[0], d.d32[1]);
@@ -515,8 +544,8 @@ void launch_cg_step(const DeviceBatch &d, bool fp32, int phase, int slices, hipS
     else
     {
         if (phase == 0)
-            hipLaunchKernelGGL((k_cg_spmv<double>), spmv_grid, spmv_block, 0, s, d, d.W64, d.P64,
-                               d.r64, d.z64, d.q64, d.d64[0], d.d64[1]);
+            launch_spmv_lpr<double>(d, slices, d.W64, d.P64, d.r64, d.z64, d.q64, d.d64[0],
+                                    d.d64[1], s);
         else
             hipLaunchKernelGGL((k_cg_update<double, SB>), upd_grid, upd_block, 0, s, d, d.P64,
                                d.r64, d.z64, d.q64, d.d64[0], d.d64[1]);
@@ -524,14 +553,13 @@ void launch_cg_step(const DeviceBatch &d, bool fp32, int phase, int slices, hipS
 }
 
 void launch_cg_step_multi(const DeviceBatch *table, const int2 *spmv_list, int spmv_grid,
-                          const int2 *upd_list, int upd_grid, bool fp32, int phase, int xcd_map,
-                          hipStream_t s)
+                          const int2 *upd_list, int upd_grid, bool fp32, int phase, int lpr,
+                          int xcd_map, hipStream_t s)
 {
     if (fp32)
     {
         if (phase == 0)
-            hipLaunchKernelGGL((k_cg_spmv_multi<float>), dim3(spmv_grid), dim3(SPMV_WAVES * 64), 0,
-                               s, table, spmv_list, xcd_map);
+            launch_spmv_multi<float>(table, spmv_list, spmv_grid, lpr, xcd_map, s);
         else
             hipLaunchKernelGGL((k_cg_update_multi<float, SB>), dim3(upd_grid), dim3(SB * 64), 0, s,
                                table, upd_list, xcd_map);
@@ -539,8 +567,7 @@ void launch_cg_step_multi(const DeviceBatch *table, const int2 *spmv_list, int s
     else
     {
         if (phase == 0)
-            hipLaunchKernelGGL((k_cg_spmv_multi<double>), dim3(spmv_grid), dim3(SPMV_WAVES * 64),
-                               0, s, table, spmv_list, xcd_map);
+            launch_spmv_multi<double>(table, spmv_list, spmv_grid, lpr, xcd_map, s);
         else
             hipLaunchKernelGGL((k_cg_update_multi<double, SB>), dim3(upd_grid), dim3(SB * 64), 0,
                                s, table, upd_list, xcd_map);

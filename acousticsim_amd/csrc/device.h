@@ -85,7 +85,8 @@ struct CgState
 struct DeviceBatch
 {
     int n_inst = 0, n_slices = 0, n_rows = 0;
-    int xcd_map = 1; // solver workgroups remapped so each XCD owns a contiguous range
+    int xcd_map = 1;   // solver workgroups remapped so each XCD owns a contiguous range
+    int spmv_lpr = 4;  // k_cg_spmv variant: lanes per row (1, 2 or 4)
     long long n_entries = 0;
     int n_tri = 0, n_ie = 0, n_vert = 0, n_blocks = 0;
     int n_blocks_wg = 0; // solver workgroups
@@ -165,8 +166,8 @@ void launch_cg_step(const DeviceBatch &d, bool fp32, int phase, int slices, hipS
 // spmv_list one (batch, slice) entry per k_cg_spmv workgroup, upd_list one
 // (batch, solver workgroup) entry per k_cg_update workgroup; batch < 0 pads
 void launch_cg_step_multi(const DeviceBatch *table, const int2 *spmv_list, int spmv_grid,
-                          const int2 *upd_list, int upd_grid, bool fp32, int phase, int xcd_map,
-                          hipStream_t s);
+                          const int2 *upd_list, int upd_grid, bool fp32, int phase, int lpr,
+                          int xcd_map, hipStream_t s);
 void launch_cg_timing_setup(const DeviceBatch &d, bool fp32, hipStream_t s);
 // measurement helpers: set the solver residual to 1 on every row; one SpMV
 // launch of the solver's precision

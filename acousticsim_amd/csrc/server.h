@@ -36,6 +36,7 @@ struct ServerOptions
     int max_resident = 256;                  // state slots
     bool fp32 = true;                        // "mixed" (true) or "fp64" requests
     int builders = 4;                        // host threads building requests
+    int spmv_lpr = 1;                        // SpMV lanes per row (fastest on serving mixes)
 };
 
 struct RequestInfo

@@ -44,7 +44,7 @@ from acousticsim_amd import workloads as wl  # noqa: E402
 
 METRIC = "cloth steps/s & p50/p99 step latency at 1e-6 PCG residual; SpMV HBM GB/s"
 HBM_PEAK_GBS = 8000.0
-KERNEL = "k_cg_spmv<float> (PCG SpMV: d = P r + beta d_prev, q = W d, partial d.q)"
+KERNEL = "k_cg_spmv<float, %d> (PCG SpMV: d = P r + beta d_prev, q = W d, partial d.q)"
 # HBM bytes per k_cg_spmv launch on batch64, from rocprofv3 --pmc FETCH_SIZE
 # and WRITE_SIZE over the --roofline-only command, scaled by the same
 # counters' reading for a 256 MiB device-to-device copy
@@ -87,6 +87,7 @@ def describe(workload, ms):
 def time_batch(ms, steps, warmup, graphs=True):
     b = capi.ClothBatch([c for c, _ in ms], [x for _, x in ms])
     b.set_graphs(graphs)
+    b.tuning = b.autotune()
     for _ in range(warmup):
         b.step(1)
     lat = []
@@ -169,10 +170,11 @@ def roofline_only(args, ms):
         torch.cuda.synchronize()
         print("bench: copied %d bytes x3" % (4 * n), file=sys.stderr, flush=True)
     b = capi.ClothBatch([c for c, _ in ms], [x for _, x in ms])
+    lanes, tune = b.autotune()
     us, nbytes = b.time_solver_kernel(reps=200)
     b.close()
     achieved = nbytes / us / 1e3
-    print(json.dumps({"workload": args.workload, "kernel": KERNEL, "launches": 201,
+    print(json.dumps({"workload": args.workload, "kernel": KERNEL % lanes, "launches": 221,
                       "launch_us": us, "alg_bytes": nbytes, "achieved_GBs": achieved,
                       "frac": achieved / HBM_PEAK_GBS}), flush=True)
 
@@ -299,10 +301,12 @@ def main():
         "p50_ms": 1000 * float(np.percentile(lat, 50)),
         "p99_ms": 1000 * float(np.percentile(lat, 99)),
         "pcg_iterations_mean": iters,
+        "autotune": {"spmv_lanes_per_row": b.tuning[0],
+                     "spmv_us_by_lanes": {str(k): v for k, v in b.tuning[1].items()}},
         "max_rel_residual": worst,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC.get(args.workload),
-                     "kernel": KERNEL,
+                     "kernel": KERNEL % b.tuning[0],
                      "launch_us": us, "alg_bytes": nbytes},
     }
     b.close()

@@ -137,6 +137,12 @@ int bw_batch_set_graphs(bw_batch *b, int enable);
    assembled at the current state first; the solver's vectors are
    overwritten (positions are not). */
 int bw_batch_time_solver_kernel(bw_batch *b, int reps, double *avg_us, double *alg_bytes);
+/* Autotuner (no reference counterpart): time the SpMV variants (1, 2 or 4
+   lanes per matrix row) on the batch's own system and keep the fastest; the
+   first bw_batch_step does this by itself.  lanes gets the choice, us (3
+   doubles, may be NULL) the mean launch times of the 1-, 2- and 4-lane
+   variants.  BW_SPMV_LPR=1|2|4 in the environment fixes the choice. */
+int bw_batch_autotune(bw_batch *b, int *lanes, double *us);
 
 /* ---- serving (no reference counterpart): step requests of different
    cloths arrive at any time, are admitted against a device-memory budget,

@@ -31,9 +31,13 @@ def _confs(precision):
 
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
 @pytest.mark.parametrize("graphs", [True, False])
-def test_alone_equals_in_mix(precision, graphs):
+def test_alone_equals_in_mix(precision, graphs, monkeypatch):
+    """Bit-identical for one SpMV variant (the autotuner may pick different
+    variants for different batches, and variants sum a row's blocks in
+    different orders), so the variant is fixed here."""
     from acousticsim_amd import capi
 
+    monkeypatch.setenv("BW_SPMV_LPR", "2")
     cs = _confs(precision)
     say("case batch", precision, "graphs", graphs)
     b = capi.ClothBatch([c for c, _ in cs], [x for _, x in cs])
@@ -74,3 +78,26 @@ def test_batched_step_matches_oracle(precision):
                                                                  st["rel_residual"]))
         assert st["rel_residual"] <= 1e-6
         assert err <= 1e-4 * d
+
+
+def test_autotuner_variants_agree(monkeypatch):
+    """The autotuner times the 1-, 2- and 4-lane SpMV variants and keeps the
+    fastest; every variant gives the same step within the solve tolerance."""
+    from acousticsim_amd import capi
+
+    cs = _confs("mixed")
+    b = capi.ClothBatch([c for c, _ in cs], [x for _, x in cs])
+    lanes, us = b.autotune()
+    say("case autotune: picked %d, us %s" % (lanes, us))
+    assert lanes in (1, 2, 4) and us[lanes] == min(us.values())
+    out = {}
+    for lpr in ("1", "2", "4"):
+        monkeypatch.setenv("BW_SPMV_LPR", lpr)
+        b = capi.ClothBatch([c for c, _ in cs], [x for _, x in cs])
+        b.step(1)
+        out[lpr] = [b.get_pos(i) for i in range(len(cs))]
+        assert max(b.stats(i)["rel_residual"] for i in range(len(cs))) <= 1e-6
+    for i, (c, x0) in enumerate(cs):
+        d = np.abs(out["4"][i] - x0).max()
+        for lpr in ("1", "2"):
+            assert np.abs(out[lpr][i] - out["4"][i]).max() <= 1e-4 * d

@@ -49,9 +49,11 @@ def _served(reqs, stagger=0.0, **kw):
     return out, counters
 
 
-def test_served_alone_equals_batch_alone():
+def test_served_alone_equals_batch_alone(monkeypatch):
+    """The server runs the 1-lane SpMV variant; the batch is held to it."""
     from acousticsim_amd import capi
 
+    monkeypatch.setenv("BW_SPMV_LPR", "1")
     c, x0 = REQS[0]
     say("case served alone vs batch alone")
     info, x = _served([(c, x0)])[0][0]
