@@ -45,10 +45,11 @@ from acousticsim_amd import workloads as wl  # noqa: E402
 METRIC = "cloth steps/s & p50/p99 step latency at 1e-6 PCG residual; SpMV HBM GB/s"
 HBM_PEAK_GBS = 8000.0
 KERNEL = "k_cg_spmv<float, %d> (PCG SpMV: d = P r + beta d_prev, q = W d, partial d.q)"
-# HBM bytes per k_cg_spmv launch on batch64, from rocprofv3 --pmc FETCH_SIZE
-# and WRITE_SIZE over the --roofline-only command, scaled by the same
-# counters' reading for a 256 MiB device-to-device copy
-TRAFFIC = {"batch64": 208187820.0, "single257": 33050965.0}
+# HBM bytes per launch of the autotuned k_cg_spmv variant (2 lanes per row on
+# both workloads), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE over the
+# --roofline-only command, scaled by the same counters' reading for a 256 MiB
+# device-to-device copy (profiles/k_cg_spmv_<workload>.md)
+TRAFFIC = {"batch64": 209616316.0, "single257": 33058071.0}
 
 
 def members(workload, precision):
