@@ -29,6 +29,7 @@ DEVICE = ["-x", "hip", "--offload-arch=" + ARCH,
 SOURCES = [
     ("json_lite.cpp", ["-ffp-contract=off"]),
     ("model.cpp", ["-ffp-contract=off"]),
+    ("mesh_gen.cpp", ["-ffp-contract=off"]),
     ("batch.cpp", ["-ffp-contract=off"]),
     ("capi.cpp", ["-ffp-contract=off"]),
     ("server.cpp", ["-ffp-contract=off", "-pthread"]),

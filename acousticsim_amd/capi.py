@@ -75,7 +75,7 @@ class ServerCounters(ctypes.Structure):
 
 _lib = None
 EXPORTS = [
-    "bw_create", "bw_destroy", "bw_update_pos", "bw_reset", "bw_get_pos", "bw_get_pre_pos",
+    "bw_create", "bw_create_mesh", "bw_delaunay_mesh", "bw_destroy", "bw_update_pos", "bw_reset", "bw_get_pos", "bw_get_pre_pos",
     "bw_get_velocity", "bw_set_pos", "bw_set_pre_pos", "bw_get_init_pos",
     "bw_get_default_timestep", "bw_num_vertices", "bw_num_triangles", "bw_num_edges",
     "bw_get_constraint_static_vertices", "bw_add_constraint_static_vertices",
@@ -102,6 +102,9 @@ def lib():
     i32 = ctypes.c_int
     sig = {
         "bw_create": (i32, [cp, dp, P(vp)]),
+        "bw_create_mesh": (i32, [cp, dp, i32, ip, i32, dp, P(vp)]),
+        "bw_delaunay_mesh": (i32, [ctypes.c_double, ctypes.c_double, i32, ctypes.c_ulonglong,
+                                   ctypes.c_double, dp, ip, ip]),
         "bw_destroy": (None, [vp]),
         "bw_update_pos": (i32, [vp, ctypes.c_double]),
         "bw_reset": (i32, [vp]),
@@ -171,19 +174,42 @@ def _json(conf):
     return (conf if isinstance(conf, str) else json.dumps(conf)).encode()
 
 
+def delaunay_mesh(cloth_size, n, seed=0, min_spacing=0.0):
+    """The mesh of geometry_type "delaunay" (cloth_size = [height, width],
+    n points, delaunay_seed, delaunay_min_spacing), before the init
+    transform: rest positions (n, 3) and triangles (T, 3) in the cloth's
+    winding.  Host only."""
+    pos = np.zeros(3 * n, dtype=np.float64)
+    tri = np.zeros(6 * n, dtype=np.int32)
+    nt = ctypes.c_int()
+    _check(lib().bw_delaunay_mesh(float(cloth_size[0]), float(cloth_size[1]), int(n), int(seed),
+                                  float(min_spacing), _dp(pos), _ip(tri), ctypes.byref(nt)))
+    return pos.reshape(-1, 3), tri[:3 * nt.value].reshape(-1, 3).copy()
+
+
 class BaraffClothHip:
     """The plain implicit cloth step on the GPU, cBaseCloth-style interface."""
 
-    def __init__(self, conf, x0=None):
+    def __init__(self, conf, x0=None, mesh=None):
+        """conf: the cloth JSON (dict); x0: initial positions (3V) applied as
+        SetPos + SetPrePos; mesh: optional (rest_pos (V x 3), triangles
+        (T x 3)) for geometry_type "mesh"."""
         self._h = ctypes.c_void_p()
-        x = None
-        if x0 is not None:
-            x = np.ascontiguousarray(x0, dtype=np.float64)
-        _check(lib().bw_create(_json(conf), _dp(x) if x is not None else None,
-                               ctypes.byref(self._h)))
+        if mesh is None:
+            _check(lib().bw_create(_json(conf), None, ctypes.byref(self._h)))
+        else:
+            pos = np.ascontiguousarray(mesh[0], dtype=np.float64).reshape(-1)
+            tri = np.ascontiguousarray(mesh[1], dtype=np.int32).reshape(-1)
+            _check(lib().bw_create_mesh(_json(conf), _dp(pos), pos.size // 3, _ip(tri),
+                                        tri.size // 3, None, ctypes.byref(self._h)))
         self.nv = lib().bw_num_vertices(self._h)
-        if x is not None and x.size != 3 * self.nv:
-            raise ValueError("x0 has %d values, expected %d" % (x.size, 3 * self.nv))
+        if x0 is not None:
+            x = np.ascontiguousarray(x0, dtype=np.float64).reshape(-1)
+            if x.size != 3 * self.nv:
+                self.close()
+                raise ValueError("x0 has %d values, expected %d" % (x.size, 3 * self.nv))
+            self.SetPos(x)
+            self.SetPrePos(x)
 
     def close(self):
         if self._h:
@@ -294,20 +320,27 @@ class ClothBatch:
     """Many independent cloths stepped together (one launch chain per phase)."""
 
     def __init__(self, confs, x0s=None):
+        """confs: cloth JSON dicts; x0s: per cloth, initial positions (3V)
+        applied as SetPos + SetPrePos, or None for the rest state."""
         n = len(confs)
+        if x0s is not None and len(x0s) != n:
+            raise ValueError("%d configs but %d initial states" % (n, len(x0s)))
         texts = [_json(c) for c in confs]
         arr = (ctypes.c_char_p * n)(*texts)
-        self._x0 = [None] * n
         ptrs = (ctypes.POINTER(ctypes.c_double) * n)()
-        if x0s is not None:
-            for i, x in enumerate(x0s):
-                if x is not None:
-                    self._x0[i] = np.ascontiguousarray(x, dtype=np.float64)
-                    ptrs[i] = _dp(self._x0[i])
         self._h = ctypes.c_void_p()
         _check(lib().bw_batch_create(arr, ptrs, n, ctypes.byref(self._h)))
         self.n = n
         self.nv = [lib().bw_batch_num_vertices(self._h, i) for i in range(n)]
+        for i, x in enumerate(x0s or []):
+            if x is None:
+                continue
+            x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+            if x.size != 3 * self.nv[i]:
+                self.close()
+                raise ValueError("x0 of cloth %d has %d values, expected %d"
+                                 % (i, x.size, 3 * self.nv[i]))
+            _check(lib().bw_batch_set_state(self._h, i, _dp(x), _dp(x)))
 
     def close(self):
         if self._h:

@@ -11,6 +11,7 @@
 
 #include "batch.h"
 #include "json_lite.h"
+#include "mesh_gen.h"
 #include "model.h"
 #include "server.h"
 #include "status.h"
@@ -69,6 +70,70 @@ struct bw_batch
 extern "C" {
 
 const char *bw_last_error(void) { return g_error.c_str(); }
+
+namespace
+{
+int create_from_model(bw::ClothModel model, const double *x0, bw_cloth **out)
+{
+    std::vector<bw::ClothModel> ms;
+    ms.push_back(std::move(model));
+    auto c = std::make_unique<bw_cloth>();
+    c->init_pos = ms[0].xrest;
+    const size_t n3 = 3 * (size_t)ms[0].nv;
+    if (x0 && !all_finite(x0, n3))
+        return fail(BW_ERR_ARG, "x0 has NaN/Inf");
+    c->batch = std::make_unique<bw::Batch>(std::move(ms));
+    if (x0)
+    {
+        c->batch->set_pos(0, x0);
+        c->batch->set_pre_pos(0, x0);
+    }
+    *out = c.release();
+    return BW_OK;
+}
+} // namespace
+
+int bw_create_mesh(const char *json, const double *rest_pos, int nv, const int *tri, int nt,
+                   const double *x0, bw_cloth **out)
+{
+    return guard([&] {
+        if (!out || !json || !rest_pos || !tri)
+            return fail(BW_ERR_ARG, "NULL argument");
+        *out = nullptr;
+        if (nv < 3 || nt < 1)
+            return fail(BW_ERR_ARG, "a mesh needs >= 3 vertices and >= 1 triangle");
+        if (!all_finite(rest_pos, 3 * (size_t)nv))
+            return fail(BW_ERR_ARG, "rest positions hold NaN/Inf");
+        bw::Json j = bw::Json::parse(json);
+        if (j.kind != bw::Json::Object)
+            return fail(BW_ERR_ARG, "config: top level must be an object");
+        bw::ClothConfig cfg = bw::ClothConfig::from_json(j);
+        if (cfg.geometry_type != "mesh")
+            return fail(BW_ERR_ARG, "config: bw_create_mesh needs geometry_type \"mesh\"");
+        return create_from_model(bw::ClothModel::build_mesh(cfg, rest_pos, nv, tri, nt), x0, out);
+    });
+}
+
+int bw_delaunay_mesh(double height, double width, int n, unsigned long long seed,
+                     double min_spacing, double *rest_pos, int *tri, int *nt)
+{
+    return guard([&] {
+        if (!rest_pos || !tri || !nt)
+            return fail(BW_ERR_ARG, "NULL argument");
+        if (n < 4 || !(height > 0) || !(width > 0) || !(min_spacing >= 0))
+            return fail(BW_ERR_ARG, "a delaunay cloth needs n >= 4, a positive size and "
+                                    "min_spacing >= 0");
+        std::vector<double> xyz;
+        std::vector<std::array<int, 3>> t;
+        bw::delaunay_cloth((float)width, (float)height, n, seed, (float)min_spacing, xyz, t);
+        if ((int)xyz.size() != 3 * n || t.size() > 2 * (size_t)n)
+            return fail(BW_ERR_STATE, "delaunay mesh has unexpected size");
+        std::memcpy(rest_pos, xyz.data(), xyz.size() * sizeof(double));
+        std::memcpy(tri, t.data(), t.size() * sizeof(t[0]));
+        *nt = (int)t.size();
+        return BW_OK;
+    });
+}
 
 int bw_create(const char *json, const double *x0, bw_cloth **out)
 {
@@ -357,6 +422,9 @@ int bw_batch_set_state(bw_batch *b, int i, const double *x, const double *xpre)
     return guard([&] {
         if (!b || !x || !xpre || i < 0 || i >= b->batch->size())
             return fail(BW_ERR_ARG, "bad arguments");
+        const size_t n3 = 3 * (size_t)b->batch->model(i).nv;
+        if (!all_finite(x, n3) || !all_finite(xpre, n3))
+            return fail(BW_ERR_ARG, "state has NaN/Inf");
         b->batch->set_pos(i, x);
         b->batch->set_pre_pos(i, xpre);
         return BW_OK;

@@ -1,4 +1,5 @@
 #include "model.h"
+#include "mesh_gen.h"
 
 #include <algorithm>
 #include <cmath>
@@ -15,13 +16,22 @@ ClothConfig ClothConfig::from_json(const Json &j)
 {
     ClothConfig c;
     c.geometry_type = j.at("geometry_type").as_string();
-    if (c.geometry_type != "regular_triangle")
+    if (c.geometry_type != "regular_triangle" && c.geometry_type != "delaunay" &&
+        c.geometry_type != "mesh")
         throw std::runtime_error("config: geometry_type \"" + c.geometry_type +
-                                 "\" is not supported (regular_triangle only)");
+                                 "\" is not supported (regular_triangle, delaunay, mesh)");
+    c.delaunay_seed = (uint64_t)j.get_double("delaunay_seed", 0);
+    c.delaunay_min_spacing = j.get_double("delaunay_min_spacing", 0);
+    if (!(c.delaunay_min_spacing >= 0))
+        throw std::runtime_error("config: delaunay_min_spacing must be >= 0");
+    c.pin_top_edge = j.has("pin_top_edge") && j.at("pin_top_edge").as_bool();
     if (j.has("enable_cloth_from_obj") && j.at("enable_cloth_from_obj").as_bool())
         throw std::runtime_error("config: enable_cloth_from_obj is not supported");
-    auto size = j.at("cloth_size").as_vector();
-    auto sub = j.at("subdivision").as_vector();
+    const bool mesh = c.geometry_type == "mesh"; // sizes come with the mesh arrays
+    auto size = mesh && !j.has("cloth_size") ? std::vector<double>{1.0, 1.0}
+                                             : j.at("cloth_size").as_vector();
+    auto sub = mesh && !j.has("subdivision") ? std::vector<double>{1.0, 1.0}
+                                             : j.at("subdivision").as_vector();
     if (size.size() < 2 || sub.size() < 2)
         throw std::runtime_error("config: cloth_size and subdivision need 2 values");
     c.cloth_size[0] = size[0];
@@ -140,6 +150,77 @@ void triangle_edge_ids(ClothModel &m)
         }
 }
 
+void apply_init_transform(const ClothConfig &c, ClothModel &m);
+
+// Edges of a triangle list as cTriangulator::DelaunayTriangulation files
+// them (add_edge_into_edge_array, TriangulatorDelaunay.cpp:9-43): sides
+// (v0,v1), (v1,v2), (v2,v0) of each triangle in order, an edge numbered at
+// its first appearance with {min, max} vertices, tri0 = that triangle, tri1
+// = the second one holding it; boundary when there is none.
+void edges_from_triangles(ClothModel &m)
+{
+    const size_t n3 = 3 * (size_t)m.nt;
+    std::vector<std::pair<uint64_t, uint32_t>> rec(n3); // (vertex pair, side index 3t+k)
+    for (int t = 0; t < m.nt; t++)
+        for (int k = 0; k < 3; k++)
+        {
+            const int a = m.tri[t][k], b = m.tri[t][(k + 1) % 3];
+            if (a == b)
+                throw std::runtime_error("config: degenerate triangle in the mesh");
+            const uint64_t key = ((uint64_t)std::min(a, b) << 32) | (uint32_t)std::max(a, b);
+            rec[3 * (size_t)t + k] = {key, (uint32_t)(3 * t + k)};
+        }
+    std::sort(rec.begin(), rec.end());
+    // groups: first appearance -> edge; order edges by first appearance
+    std::vector<std::pair<uint32_t, size_t>> first; // (side index, group start)
+    for (size_t i = 0; i < n3;)
+    {
+        size_t j = i + 1;
+        while (j < n3 && rec[j].first == rec[i].first)
+            j++;
+        if (j - i > 2)
+            throw std::runtime_error("config: mesh edge shared by more than two triangles");
+        first.push_back({rec[i].second, i});
+        i = j;
+    }
+    std::sort(first.begin(), first.end());
+    m.ne = (int)first.size();
+    m.edges.resize(m.ne);
+    for (int e = 0; e < m.ne; e++)
+    {
+        const size_t i = first[e].second;
+        const uint64_t key = rec[i].first;
+        Edge ed;
+        ed.v0 = (int)(key >> 32);
+        ed.v1 = (int)(key & 0xffffffffu);
+        ed.t0 = (int)(rec[i].second / 3);
+        const bool two = i + 1 < n3 && rec[i + 1].first == key;
+        ed.t1 = two ? (int)(rec[i + 1].second / 3) : -1;
+        ed.boundary = !two;
+        m.edges[e] = ed;
+    }
+    triangle_edge_ids(m);
+}
+
+// geometry_type "delaunay" (cTriangulator::DelaunayTriangulation,
+// TriangulatorDelaunay.cpp:45-160): subdivision[0] * subdivision[1] seeded
+// points on the cloth rectangle, Delaunay-triangulated.  Like the reference,
+// every triangle's first and last vertex are swapped when its normal is not
+// +y (always, for the planar start), and UV = the point.
+void make_delaunay(const ClothConfig &c, ClothModel &m)
+{
+    const float height = (float)c.cloth_size[0], width = (float)c.cloth_size[1];
+    const long long want = (long long)c.subdivision[0] * c.subdivision[1];
+    if (want > (1LL << 28))
+        throw std::runtime_error("config: delaunay cloth too large");
+    delaunay_cloth(width, height, (int)want, c.delaunay_seed, (float)c.delaunay_min_spacing,
+                   m.xrest, m.tri);
+    m.nv = (int)(m.xrest.size() / 3);
+    m.nt = (int)m.tri.size();
+    edges_from_triangles(m);
+    apply_init_transform(c, m);
+}
+
 void make_grid(const ClothConfig &c, ClothModel &m)
 {
     const double height = c.cloth_size[0], width = c.cloth_size[1];
@@ -210,10 +291,20 @@ void make_grid(const ClothConfig &c, ClothModel &m)
             throw std::runtime_error("internal: grid edge slot not filled");
 
     triangle_edge_ids(m);
+    apply_init_transform(c, m);
+}
 
-    // init transform (cRotUtil::TransformMat, XYZ Euler: Rz*Ry*Rx, then
-    // translation) and UV = rot(uv_rotation) * float(untransformed xy)
-    // (Triangulator.cpp:116-131, 621-658)
+// init transform (cRotUtil::TransformMat, XYZ Euler: Rz*Ry*Rx, then
+// translation) and UV = rot(uv_rotation) * float(untransformed xy)
+// (Triangulator.cpp:116-131, 621-658); also keeps the untransformed xy
+void apply_init_transform(const ClothConfig &c, ClothModel &m)
+{
+    m.flat_xy.resize(2 * (size_t)m.nv);
+    for (int v = 0; v < m.nv; v++)
+    {
+        m.flat_xy[2 * v] = (float)m.xrest[3 * (size_t)v];
+        m.flat_xy[2 * v + 1] = (float)m.xrest[3 * (size_t)v + 1];
+    }
     const bool identity = c.init_pos[0] == 0 && c.init_pos[1] == 0 && c.init_pos[2] == 0 &&
                           c.init_euler[0] == 0 && c.init_euler[1] == 0 && c.init_euler[2] == 0;
     double R[3][3];
@@ -528,7 +619,45 @@ ClothModel ClothModel::build(const ClothConfig &cfg)
 {
     ClothModel m;
     m.cfg = cfg;
-    make_grid(cfg, m);
+    if (cfg.geometry_type == "delaunay")
+        make_delaunay(cfg, m);
+    else if (cfg.geometry_type == "regular_triangle")
+        make_grid(cfg, m);
+    else
+        throw std::runtime_error("config: geometry_type \"mesh\" needs the mesh arrays (bw_create_mesh)");
+    m.finish();
+    return m;
+}
+
+ClothModel ClothModel::build_mesh(const ClothConfig &cfg, const double *pos3, int nv,
+                                  const int *tri3, int nt)
+{
+    if (nv < 3 || nt < 1 || !pos3 || !tri3)
+        throw std::runtime_error("config: a mesh needs >= 3 vertices and >= 1 triangle");
+    ClothModel m;
+    m.cfg = cfg;
+    m.nv = nv;
+    m.nt = nt;
+    m.xrest.assign(pos3, pos3 + 3 * (size_t)nv);
+    m.tri.resize(nt);
+    for (int t = 0; t < nt; t++)
+        for (int k = 0; k < 3; k++)
+        {
+            const int v = tri3[3 * (size_t)t + k];
+            if (v < 0 || v >= nv)
+                throw std::runtime_error("config: mesh triangle vertex out of range");
+            m.tri[t][k] = v;
+        }
+    edges_from_triangles(m);
+    apply_init_transform(cfg, m);
+    m.finish();
+    return m;
+}
+
+void ClothModel::finish()
+{
+    ClothModel &m = *this;
+    const ClothConfig &cfg = m.cfg;
     // bending stencils (BaseCloth.cpp:383-404): v2, v3 = the third vertices
     // of tri0 and tri1
     m.stencil.resize(m.ne);
@@ -563,10 +692,19 @@ ClothModel ClothModel::build(const ClothConfig &cfg)
         }
         m.pins.push_back(id);
     }
+    // pin_top_edge: every vertex on the top side (largest untransformed y)
+    if (cfg.pin_top_edge)
+    {
+        float top = -3.4e38f;
+        for (int v = 0; v < m.nv; v++)
+            top = std::max(top, m.flat_xy[2 * v + 1]);
+        for (int v = 0; v < m.nv; v++)
+            if (m.flat_xy[2 * v + 1] == top)
+                m.pins.push_back(v);
+    }
     m.rebuild_pins_from_list();
 
     m.build_blocks();
-    return m;
 }
 
 } // namespace bw

@@ -41,6 +41,9 @@ struct ClothConfig
     bool gravity = true;
     int max_iters = 500;
     std::vector<std::array<float, 2>> pin_uv; // constraints.static_point
+    uint64_t delaunay_seed = 0; // "delaunay_seed" (this library's key)
+    double delaunay_min_spacing = 0; // "delaunay_min_spacing"; 0: the reference's rule
+    bool pin_top_edge = false;  // "pin_top_edge" (this library's key)
     Precision precision = Precision::Mixed;
     SolveMode solve_mode = SolveMode::Tolerance;
     double tolerance = 1e-6;
@@ -61,6 +64,7 @@ struct ClothModel
     int nv = 0, nt = 0, ne = 0;
     std::vector<double> xrest;              // 3 nv
     std::vector<float> uv;                  // 2 nv
+    std::vector<float> flat_xy;             // 2 nv, untransformed rest xy
     std::vector<std::array<int, 3>> tri;    // vertex ids
     std::vector<std::array<int, 3>> tri_e;  // edge ids (id0,id1),(id1,id2),(id2,id0)
     std::vector<Edge> edges;
@@ -87,6 +91,12 @@ struct ClothModel
     std::vector<int> vt_ptr, vt_list, ve_ptr, ve_list;
 
     static ClothModel build(const ClothConfig &cfg);
+    // a cloth on a given triangle mesh (rest positions 3 nv, triangles
+    // 3 nt); edges, UV (= rest xy) and rest data as for geometry "delaunay"
+    static ClothModel build_mesh(const ClothConfig &cfg, const double *pos3, int nv,
+                                 const int *tri3, int nt);
+    // stencils, rest data, pins and blocks, once the topology is in place
+    void finish();
     // parse the cloth JSON (cloth_type, when present, must be "fem" or
     // "fem_hip") and build
     static ClothModel from_json_text(const std::string &text);

@@ -92,11 +92,48 @@ def grid_rest_positions(conf):
     return x.reshape(-1)
 
 
-def jittered_positions(conf, seed, amplitude=1e-3):
-    x = grid_rest_positions(conf).reshape(-1, 3).copy()
+def jitter(rest, seed, amplitude=1e-3):
+    """rest positions (3V) with the seeded z jitter added"""
+    x = np.asarray(rest, dtype=np.float64).reshape(-1, 3).copy()
     u = uniform01(seed, x.shape[0])
     x[:, 2] += (2.0 * u - 1.0) * amplitude
     return x.reshape(-1)
+
+
+def jittered_positions(conf, seed, amplitude=1e-3):
+    return jitter(grid_rest_positions(conf), seed, amplitude)
+
+
+def delaunay_config(points, size=1.0, seed=0, dt=0.01, min_spacing=0.0, top_edge=True, **over):
+    """Square cloth of `points` seeded random points, Delaunay-triangulated
+    (geometry_type "delaunay"), its whole top side pinned (or, with
+    top_edge=False, its two top corners)."""
+    c = copy.deepcopy(BASE_CLOTH)
+    c["geometry_type"] = "delaunay"
+    c["cloth_size"] = [size, size]
+    c["subdivision"] = [1, int(points)]
+    c["delaunay_seed"] = int(seed)
+    if min_spacing:
+        c["delaunay_min_spacing"] = float(min_spacing)
+    c["default_timestep"] = dt
+    h = size / 2
+    if top_edge:
+        c["pin_top_edge"] = True
+        c["constraints"] = {"static_point": []}
+    else:
+        c["constraints"] = {"static_point": [[-h, h], [h, h]]}
+    c.update(over)
+    return c
+
+
+def delaunay_rest_positions(conf):
+    """Untransformed rest positions and triangles of a "delaunay" config,
+    from the library's host-side mesh builder."""
+    from acousticsim_amd import capi
+
+    n = int(conf["subdivision"][0]) * int(conf["subdivision"][1])
+    return capi.delaunay_mesh(conf["cloth_size"], n, conf.get("delaunay_seed", 0),
+                              conf.get("delaunay_min_spacing", 0.0))
 
 
 # --- benchmark configurations -------------------------------------------
@@ -123,6 +160,20 @@ def batch64_configs(**kw):
     members = batch64_members(**kw)
     return [(grid_config(s, dt=BATCH64["dt"], stiffness_scale=sc), seed)
             for (s, sc, seed) in members]
+
+
+# Large irregular mesh: 1.5M seeded points with 1 mm minimum spacing in a
+# 2 m x 2 m square, Delaunay-triangulated (~3M triangles), top side pinned,
+# dt 0.002 s, mixed precision to a 1e-7 residual.
+IRREGULAR = dict(points=1_500_000, size=2.0, min_spacing=1e-3, dt=0.002, steps=50, tol=1e-7,
+                 seed=5)
+
+
+def irregular_config(points=IRREGULAR["points"]):
+    c = delaunay_config(points, size=IRREGULAR["size"], seed=IRREGULAR["seed"],
+                        dt=IRREGULAR["dt"], min_spacing=IRREGULAR["min_spacing"])
+    c["solver_tolerance"] = IRREGULAR["tol"]
+    return c
 
 
 # Serving: step requests arrive as a Poisson stream; each is a unit-square

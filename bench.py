@@ -20,7 +20,14 @@ Reported beside it, in the same run:
   * the CPU baseline: the C restatement (oracle) of the reference's plain
     method, one thread, on a bounded sample of the same cloths
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batch64|single257|single21]
+Workload "irregular": one cloth of 1.5M seeded points (1 mm minimum
+spacing) in a 2 m x 2 m square, Delaunay-triangulated, top side pinned,
+dt = 0.002 s, mixed precision to a 1e-7 residual; its CPU baseline is
+extrapolated from the oracle's assembly and per-iteration times on the same
+mesh (a full CPU step takes minutes).
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+                       [--workload batch64|single257|single21|irregular]
        python bench.py --workload serve [--requests N] [--rate R]
        python bench.py --roofline-only [--copy-calibration]
 The second form only assembles the system and times the dominant kernel (and,
@@ -68,20 +75,25 @@ def members(workload, precision):
         c = wl.grid_config(21, dt=wl.SINGLE_SMALL["dt"])
         c["precision"] = precision
         return [(c, wl.jittered_positions(c, wl.SINGLE_SMALL["seed"]))]
+    if workload == "irregular":
+        c = wl.irregular_config()
+        c["precision"] = precision
+        pos, _ = wl.delaunay_rest_positions(c)
+        return [(c, wl.jitter(pos, wl.IRREGULAR["seed"]))]
     raise ValueError(workload)
 
 
 def describe(workload, ms):
-    nv = sum(c["subdivision"][0] + 1 for c, _ in ms)
-    verts = sum((c["subdivision"][0] + 1) * (c["subdivision"][1] + 1) for c, _ in ms)
+    verts = sum(len(x) // 3 for _, x in ms)
     names = {
         "batch64": "64 cloths, 17..129 vertices/side (seeded), stiffness x[0.1,10], dt 0.01 s",
         "single257": "one 257x257-vertex cloth, dt 0.005 s",
         "single21": "one 21x21-vertex cloth, dt 0.01 s",
+        "irregular": "one Delaunay cloth of 1.5M seeded points (1 mm min spacing), 2 m x 2 m, "
+                     "top side pinned, dt 0.002 s",
     }
-    del nv
     return {"workload": names[workload], "cloths": len(ms), "vertices": verts,
-            "tolerance": 1e-6,
+            "tolerance": ms[0][0].get("solver_tolerance", 1e-6),
             "precision": "mixed (FP32 matrix and recurrences; FP64 dots, solution, reliable updates)"}
 
 
@@ -158,6 +170,33 @@ def cpu_baseline(ms, budget_s=20.0):
     return {"value": n_steps / t_steps, "unit": "cloth steps/s", "cores": 1, "kind": "port",
             "sample": "%d of %d cloths (every 4th), 1 warm-up + 1 timed step each, tol 1e-6, "
                       "gcc -O2, host %s" % (n_steps, len(ms), cpu)}
+
+
+def cpu_baseline_extrapolated(ms, gpu_iters):
+    """For a cloth too large to step on the CPU within the budget: the
+    oracle's time for a step with k PCG iterations is assembly + k * per
+    iteration; both are measured on the same mesh (steps stopped after 6 and
+    after 26 PCG iterations) and combined with the GPU's mean iteration count
+    per step."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_ffi as of
+
+    (c, x), = ms
+    pos, tri = wl.delaunay_rest_positions(c)
+    oc = of.OracleCloth(c, mesh=(pos, tri))
+    oc.set_state(x, x)
+    times = {}
+    for k in (6, 26):
+        t0 = time.perf_counter()
+        oc.step(of.SOLVE_TOLERANCE, 0.0, k)
+        times[k] = time.perf_counter() - t0
+    per_iter = (times[26] - times[6]) / 20
+    base = times[6] - 6 * per_iter
+    step_s = base + gpu_iters * per_iter
+    return {"value": 1.0 / step_s, "unit": "cloth steps/s", "cores": 1, "kind": "port",
+            "sample": "extrapolated: oracle assembly %.2f s + %.1f PCG iterations (the GPU's "
+                      "mean per step) x %.3f s per iteration, both timed on the same 1.5M-point "
+                      "mesh, gcc -O2, 1 thread" % (base, gpu_iters, per_iter)}
 
 
 def roofline_only(args, ms):
@@ -254,7 +293,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="batch64",
-                    choices=["batch64", "single257", "single21", "serve"])
+                    choices=["batch64", "single257", "single21", "irregular", "serve"])
     ap.add_argument("--requests", type=int, default=64, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=wl.SERVE["rate"], help="serve: arrivals per s")
     ap.add_argument("--dump-requests", default=None, help="serve: write per-request records here")
@@ -319,7 +358,10 @@ def main():
                              "max_rel_residual": worst_p, "speedup": value / plain}
     if not args.no_cpu:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(ms)
+        if args.workload == "irregular":
+            out["cpu_baseline"] = cpu_baseline_extrapolated(ms, iters)
+        else:
+            out["cpu_baseline"] = cpu_baseline(ms)
     print(json.dumps(out), flush=True)
 
 

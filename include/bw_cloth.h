@@ -20,6 +20,12 @@
  *   "solver_max_iters": PCG iteration cap per solve (default 20000)
  *   "solver_mode":      "tolerance" (default) or "reference" (exactly
  *                        max_iters-1 FP64 PCG iterations, BaraffCloth.cpp:529)
+ *   "delaunay_seed":    seed of geometry_type "delaunay" (the reference draws
+ *                        its points unseeded, TriangulatorDelaunay.cpp:66-69)
+ *   "delaunay_min_spacing": minimum distance between random points of
+ *                        geometry_type "delaunay" (default: the reference's
+ *                        sqrt(w h / (n pi)), TriangulatorDelaunay.cpp:60-61)
+ *   "pin_top_edge":     true pins every vertex of the cloth's top side
  * Positions are 3*V doubles, xyz interleaved, in the reference's vertex order.
  */
 #ifndef BW_CLOTH_H
@@ -53,6 +59,26 @@ typedef struct bw_step_stats
    factory key "fem" (ClothBuilder.cpp:17-54).  x0 (3V, may be NULL) is then
    applied as SetPos(x0) + SetPrePos(x0) (BaseCloth.cpp:97-109). */
 int bw_create(const char *json_config, const double *x0, bw_cloth **out);
+/* A cloth on a given triangle mesh: rest positions (3V doubles), triangles
+   (3T vertex ids, the winding the reference's mesh builders produce) and the
+   cloth JSON with "geometry_type": "mesh" (cloth_size / subdivision may be
+   omitted).  Edges are numbered at their first appearance in the triangle
+   list and UV is the rest xy, as cTriangulator::DelaunayTriangulation
+   builds them (TriangulatorDelaunay.cpp:9-160); this replaces the
+   reference's mesh ingest for this path (enable_cloth_from_obj,
+   BaseCloth.cpp:301-357, reads the mesh from an OBJ file instead).  x0 may
+   be NULL. */
+int bw_create_mesh(const char *json_config, const double *rest_pos, int nv, const int *tri,
+                   int nt, const double *x0, bw_cloth **out);
+/* The mesh of "geometry_type": "delaunay" for cloth_size {height, width},
+   n = subdivision[0] * subdivision[1] points, delaunay_seed and
+   delaunay_min_spacing (0: the reference's spacing rule), before the
+   init transform: rest_pos (3n doubles, z = 0) and tri (room for 2n
+   triangles; *nt receives the count), in the winding the cloth uses.  This
+   is cTriangulator::DelaunayTriangulation (TriangulatorDelaunay.cpp:45-160)
+   with a seeded point stream.  Host only: needs no device. */
+int bw_delaunay_mesh(double height, double width, int n, unsigned long long seed,
+                     double min_spacing, double *rest_pos, int *tri, int *nt);
 /* destructor of the object (shared_ptr release in the reference) */
 void bw_destroy(bw_cloth *c);
 

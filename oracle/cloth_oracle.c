@@ -135,6 +135,7 @@ struct oc_cloth
     int *stencil;  /* 4ne, -1 for boundary edges */
     int *pins;
     unsigned char *pinned;
+    float *flat_y; /* nv, untransformed rest y as float */
     /* rest data */
     double *mass;    /* nv */
     double *gravity; /* 3nv */
@@ -206,6 +207,8 @@ static void euler_xyz(const double *e, double R[3][3])
         for (int j = 0; j < 3; j++)
             R[i][j] = (T[i][0] * Rx[0][j] + T[i][1] * Rx[1][j]) + T[i][2] * Rx[2][j];
 }
+
+static void finish_geometry(oc_cloth *c, const oc_params *p);
 
 static int build_grid(oc_cloth *c, const oc_params *p)
 {
@@ -310,7 +313,15 @@ static int build_grid(oc_cloth *c, const oc_params *p)
     for (int k = 0; k < c->ne; k++)
         if (c->edge[5 * k] == -2)
             return -2;
+    finish_geometry(c, p);
+    return 0;
+}
 
+/* triangle -> edge ids, init transform and UV, for any triangle mesh whose
+   untransformed rest positions, triangles and edges are set */
+static void finish_geometry(oc_cloth *c, const oc_params *p)
+{
+    c->flat_y = xcalloc(c->nv, sizeof(float));
     /* BuildTriangleEdgeId: (id0,id1), (id1,id2), (id2,id0) */
     epair *ep = xcalloc(c->ne, sizeof(epair));
     for (int k = 0; k < c->ne; k++)
@@ -359,8 +370,8 @@ static int build_grid(oc_cloth *c, const oc_params *p)
         float m01 = -sr;
         c->uv[2 * v + 0] = cr * px + m01 * py;
         c->uv[2 * v + 1] = sr * px + cr * py;
+        c->flat_y[v] = py;
     }
-    return 0;
 }
 
 /* SelectAnotherVertex (BaseBendingMaterial.cpp:65-74): the vertex of tri
@@ -666,7 +677,26 @@ static void find_pins(oc_cloth *c, const oc_params *p)
         if (id >= 0)
             c->pinned[id] = 1;
     }
+    if (p->pin_top_edge)
+    {
+        /* every vertex on the top side (largest untransformed y) */
+        float top = -3.4e38f;
+        int extra = 0;
+        for (int v = 0; v < c->nv; v++)
+            top = c->flat_y[v] > top ? c->flat_y[v] : top;
+        for (int v = 0; v < c->nv; v++)
+            extra += c->flat_y[v] == top;
+        c->pins = realloc(c->pins, (c->npin + extra) * sizeof(int));
+        for (int v = 0; v < c->nv; v++)
+            if (c->flat_y[v] == top)
+            {
+                c->pins[c->npin++] = v;
+                c->pinned[v] = 1;
+            }
+    }
 }
+
+static oc_cloth *complete(oc_cloth *c, const oc_params *p);
 
 oc_cloth *oc_create_grid(const oc_params *p)
 {
@@ -676,6 +706,106 @@ oc_cloth *oc_create_grid(const oc_params *p)
         oc_destroy(c);
         return NULL;
     }
+    return complete(c, p);
+}
+
+/* Edge list of a triangle list as cTriangulator::DelaunayTriangulation
+   files it (add_edge_into_edge_array, TriangulatorDelaunay.cpp:9-43,
+   118-142): each triangle's sides (id0,id1), (id1,id2), (id2,id0) in order;
+   an unseen side becomes edge {min, max} with the next id and tri0 = this
+   triangle, a seen one gets tri1 = this triangle; boundary when no tri1.
+   Restated with a sort over the sides (the reference searches its edge list
+   linearly); the first-appearance numbering is the same. */
+typedef struct
+{
+    int a, b, side;
+} oc_side;
+static int side_cmp(const void *p, const void *q)
+{
+    const oc_side *x = p, *y = q;
+    if (x->a != y->a)
+        return x->a < y->a ? -1 : 1;
+    if (x->b != y->b)
+        return x->b < y->b ? -1 : 1;
+    return x->side < y->side ? -1 : (x->side > y->side);
+}
+static int first_cmp(const void *p, const void *q)
+{
+    const int *x = p, *y = q;
+    return x[0] < y[0] ? -1 : (x[0] > y[0]);
+}
+
+oc_cloth *oc_create_mesh(const oc_params *p, const double *xyz, int nv, const int *tri, int nt)
+{
+    if (nv < 3 || nt < 1)
+        return NULL;
+    for (int k = 0; k < 3 * nt; k++)
+        if (tri[k] < 0 || tri[k] >= nv)
+            return NULL;
+    oc_cloth *c = xcalloc(1, sizeof(oc_cloth));
+    c->nv = nv;
+    c->nt = nt;
+    c->xrest = xcalloc(3 * (size_t)nv, sizeof(double));
+    memcpy(c->xrest, xyz, 3 * (size_t)nv * sizeof(double));
+    c->uv = xcalloc(2 * (size_t)nv, sizeof(float));
+    c->tri = xcalloc(3 * (size_t)nt, sizeof(int));
+    memcpy(c->tri, tri, 3 * (size_t)nt * sizeof(int));
+    c->tri_e = xcalloc(3 * (size_t)nt, sizeof(int));
+    int ns = 3 * nt;
+    oc_side *sd = xcalloc(ns, sizeof(oc_side));
+    for (int t = 0; t < nt; t++)
+        for (int k = 0; k < 3; k++)
+        {
+            int a = tri[3 * t + k], b = tri[3 * t + (k + 1) % 3];
+            sd[3 * t + k].a = a < b ? a : b;
+            sd[3 * t + k].b = a < b ? b : a;
+            sd[3 * t + k].side = 3 * t + k;
+        }
+    qsort(sd, ns, sizeof(oc_side), side_cmp);
+    /* groups of equal {a, b} as (first side, group start), then ordered by
+       first side = first appearance */
+    int *grp = xcalloc(2 * (size_t)ns, sizeof(int));
+    int ng = 0, bad = 0;
+    for (int i = 0; i < ns;)
+    {
+        int j = i + 1;
+        while (j < ns && sd[j].a == sd[i].a && sd[j].b == sd[i].b)
+            j++;
+        bad |= j - i > 2;
+        grp[2 * ng] = sd[i].side;
+        grp[2 * ng + 1] = i;
+        ng++;
+        i = j;
+    }
+    if (bad)
+    {
+        free(sd);
+        free(grp);
+        oc_destroy(c);
+        return NULL;
+    }
+    qsort(grp, ng, 2 * sizeof(int), first_cmp);
+    c->ne = ng;
+    c->edge = xcalloc(5 * (size_t)ng, sizeof(int));
+    for (int e = 0; e < ng; e++)
+    {
+        int i = grp[2 * e + 1];
+        int two = i + 1 < ns && sd[i + 1].a == sd[i].a && sd[i + 1].b == sd[i].b;
+        int *ed = c->edge + 5 * e;
+        ed[0] = sd[i].a;
+        ed[1] = sd[i].b;
+        ed[2] = sd[i].side / 3;
+        ed[3] = two ? sd[i + 1].side / 3 : -1;
+        ed[4] = !two;
+    }
+    free(sd);
+    free(grp);
+    finish_geometry(c, p);
+    return complete(c, p);
+}
+
+static oc_cloth *complete(oc_cloth *c, const oc_params *p)
+{
     /* stencils (BaseCloth.cpp:383-404) */
     c->stencil = xcalloc(4 * c->ne, sizeof(int));
     for (int e = 0; e < c->ne; e++)
@@ -718,6 +848,7 @@ void oc_destroy(oc_cloth *c)
         return;
     free(c->xrest); free(c->x); free(c->xpre); free(c->uv); free(c->tri);
     free(c->tri_e); free(c->edge); free(c->stencil); free(c->pins); free(c->pinned);
+    free(c->flat_y);
     free(c->mass); free(c->gravity); free(c->coef); free(c->area);
     free(c->edge_len); free(c->heron); free(c->height);
     free(c->row_ptr); free(c->cols); free(c->diag);

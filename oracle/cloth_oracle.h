@@ -33,6 +33,7 @@ typedef struct oc_params
     int max_iters;              /* "max_iters" */
     int num_pins;
     const double *pin_uv;       /* num_pins x 2, "constraints.static_point" */
+    int pin_top_edge;           /* also pin every vertex of the top side */
 } oc_params;
 
 typedef struct oc_cloth oc_cloth;
@@ -49,6 +50,12 @@ typedef struct oc_stats
 } oc_stats;
 
 oc_cloth *oc_create_grid(const oc_params *p);
+/* a cloth on a given triangle mesh: untransformed rest positions (3 nv) and
+   triangles (3 nt) in their final winding; edges numbered at first
+   appearance (TriangulatorDelaunay.cpp:9-43, 118-142), UV = rest xy.
+   cloth_size and subdivision are not read.  NULL for an index out of range
+   or an edge with more than two triangles. */
+oc_cloth *oc_create_mesh(const oc_params *p, const double *xyz, int nv, const int *tri, int nt);
 void oc_destroy(oc_cloth *c);
 
 int oc_num_vertices(const oc_cloth *c);

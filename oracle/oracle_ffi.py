@@ -34,6 +34,7 @@ class OcParams(ctypes.Structure):
         ("max_iters", ctypes.c_int),
         ("num_pins", ctypes.c_int),
         ("pin_uv", ctypes.POINTER(ctypes.c_double)),
+        ("pin_top_edge", ctypes.c_int),
     ]
 
 
@@ -65,6 +66,8 @@ def lib():
         ip = P(ctypes.c_int)
         L.oc_create_grid.restype = vp
         L.oc_create_grid.argtypes = [P(OcParams)]
+        L.oc_create_mesh.restype = vp
+        L.oc_create_mesh.argtypes = [P(OcParams), dp, ctypes.c_int, ip, ctypes.c_int]
         L.oc_destroy.argtypes = [vp]
         for n in ("oc_num_vertices", "oc_num_triangles", "oc_num_edges",
                   "oc_num_pins", "oc_ell_nnzb"):
@@ -105,8 +108,10 @@ def _ip(a):
 def params_from_config(conf):
     """OcParams from a reference cloth JSON dict (keys of config/his_config.json)."""
     p = OcParams()
-    p.cloth_size[:] = [float(v) for v in conf["cloth_size"][:2]]
-    p.subdivision[:] = [int(v) for v in conf["subdivision"][:2]]
+    if "cloth_size" in conf:
+        p.cloth_size[:] = [float(v) for v in conf["cloth_size"][:2]]
+    if "subdivision" in conf:
+        p.subdivision[:] = [int(v) for v in conf["subdivision"][:2]]
     ip = list(conf.get("cloth_init_pos", [0, 0, 0])) + [0, 0, 0]
     p.init_pos[:] = [float(v) for v in ip[:3]]
     io = list(conf.get("cloth_init_orientation", [0, 0, 0])) + [0, 0, 0]
@@ -125,18 +130,27 @@ def params_from_config(conf):
     p.num_pins = arr.shape[0]
     p._pin_keepalive = arr
     p.pin_uv = _dp(arr)
+    p.pin_top_edge = int(bool(conf.get("pin_top_edge", False)))
     return p
 
 
 class OracleCloth:
-    """One cloth held by the C oracle."""
+    """One cloth held by the C oracle: the regular grid of `conf`, or, with
+    `mesh` = (rest positions (nv, 3), triangles (nt, 3)), a cloth on that
+    triangle mesh (untransformed rest positions, final winding)."""
 
-    def __init__(self, conf):
+    def __init__(self, conf, mesh=None):
         L = lib()
         self._params = params_from_config(conf)
-        self.h = L.oc_create_grid(ctypes.byref(self._params))
+        if mesh is None:
+            self.h = L.oc_create_grid(ctypes.byref(self._params))
+        else:
+            xyz = np.ascontiguousarray(np.asarray(mesh[0], dtype=np.float64).reshape(-1))
+            tri = np.ascontiguousarray(np.asarray(mesh[1], dtype=np.int32).reshape(-1))
+            self.h = L.oc_create_mesh(ctypes.byref(self._params), _dp(xyz), xyz.size // 3,
+                                      _ip(tri), tri.size // 3)
         if not self.h:
-            raise ValueError("oracle: invalid grid configuration")
+            raise ValueError("oracle: invalid cloth geometry")
         self.nv = L.oc_num_vertices(self.h)
         self.nt = L.oc_num_triangles(self.h)
         self.ne = L.oc_num_edges(self.h)

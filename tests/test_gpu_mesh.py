@@ -1,0 +1,129 @@
+"""Irregular meshes on the GPU, through the C ABI: Delaunay cloths
+(geometry_type "delaunay") and caller-given meshes (bw_create_mesh) against
+the oracle's mesh constructor on the same triangles.
+
+Tolerances (stated per test):
+  * topology, edges (first-appearance numbering), pins, ELL structure,
+    rest positions: identical
+  * reference-semantics solve (max_iters-1 FP64 PCG iterations): first step
+    within 1e-9 of the step displacement
+  * 1e-6 tolerance solve, 20 steps, top side pinned: within 1e-4 relative to
+    the positions, every step's FP64 true residual <= 1e-6
+  * the same mesh through bw_create and bw_create_mesh: bit-identical
+"""
+import copy
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_ffi as of
+from acousticsim_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def capi():
+    from acousticsim_amd import capi as c
+    c.lib()
+    return c
+
+
+def _mode(conf, precision, mode="tolerance", tol=1e-6):
+    c = copy.deepcopy(conf)
+    c["precision"] = precision
+    c["solver_mode"] = mode
+    c["solver_tolerance"] = tol
+    return c
+
+
+def _case(points=300, seed=4, **kw):
+    conf = wl.delaunay_config(points, seed=seed, **kw)
+    pos, tri = wl.delaunay_rest_positions(conf)
+    return conf, pos, tri, wl.jitter(pos, seed + 100)
+
+
+@pytest.mark.parametrize("points,seed,rot", [(300, 4, 0), (1200, 5, 30)])
+def test_delaunay_topology_matches_oracle(capi, points, seed, rot):
+    say("case delaunay topology", points, seed)
+    conf, pos, tri, x0 = _case(points, seed, cloth_uv_rotation=rot,
+                               cloth_init_orientation=[0.3, 0.0, 0.1])
+    cl = capi.BaraffClothHip(conf)
+    oc = of.OracleCloth(conf, mesh=(pos, tri))
+    assert np.array_equal(cl.triangles(), tri)
+    assert np.array_equal(cl.edges(), oc.edges())
+    assert np.array_equal(np.sort(cl.GetConstraintStaticVertices()), np.sort(oc.pins()))
+    rp, cols = cl.ell_map()
+    orp, ocols = oc.ref_ell()
+    assert np.array_equal(rp, orp) and np.array_equal(cols, ocols)
+    assert np.array_equal(cl.GetInitPos(), oc.rest_pos())
+    width = np.diff(rp)
+    say("  ELL row widths %d..%d" % (width.min(), width.max()))
+
+
+def test_delaunay_reference_semantics_first_steps(capi):
+    conf, pos, tri, x0 = _case(300, 4, top_edge=False)
+    say("case delaunay reference-mode")
+    cl = capi.BaraffClothHip(_mode(conf, "fp64", "reference"), x0)
+    oc = of.OracleCloth(conf, mesh=(pos, tri))
+    oc.set_state(x0, x0)
+    for s in range(3):
+        xs = cl.GetPos()
+        cl.UpdatePos()
+        oc.step(of.SOLVE_REFERENCE)
+        d = np.abs(oc.pos() - xs).max()
+        err = np.abs(cl.GetPos() - oc.pos()).max()
+        say("  step %d err/|dx| %.3e" % (s + 1, err / d))
+        assert err <= (1e-9 if s == 0 else 1e-6) * d
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_delaunay_twenty_tolerance_steps(capi, precision):
+    conf, pos, tri, x0 = _case(1200, 5)
+    say("case delaunay 20 steps", precision)
+    cl = capi.BaraffClothHip(_mode(conf, precision), x0)
+    oc = of.OracleCloth(conf, mesh=(pos, tri))
+    oc.set_state(x0, x0)
+    worst = 0.0
+    for _ in range(20):
+        cl.UpdatePos()
+        oc.step(of.SOLVE_TOLERANCE, 1e-12)
+        worst = max(worst, cl.last_stats()["rel_residual"])
+    x, xr = cl.GetPos(), oc.pos()
+    err = np.abs(x - xr).max()
+    say("  err/|x| %.3e err/disp %.3e worst residual %.2e" % (
+        err / np.abs(xr).max(), err / np.abs(xr - x0).max(), worst))
+    assert worst <= 1e-6
+    assert err <= 1e-4 * np.abs(xr).max()
+
+
+def test_mesh_entry_equals_delaunay_geometry(capi):
+    conf, pos, tri, x0 = _case(500, 6)
+    say("case bw_create_mesh vs delaunay")
+    a = capi.BaraffClothHip(conf, x0)
+    mconf = dict(conf, geometry_type="mesh")
+    del mconf["cloth_size"], mconf["subdivision"]
+    b = capi.BaraffClothHip(mconf, x0, mesh=(pos, tri))
+    assert np.array_equal(a.edges(), b.edges())
+    for _ in range(3):
+        a.UpdatePos()
+        b.UpdatePos()
+    assert np.array_equal(a.GetPos(), b.GetPos())
+
+
+def test_delaunay_in_batch_equals_alone(capi):
+    conf, pos, tri, x0 = _case(800, 7)
+    g = wl.grid_config(33)
+    gx = wl.jittered_positions(g, 8)
+    say("case delaunay in batch")
+    alone = capi.ClothBatch([conf], [x0])
+    alone.step(3)
+    mix = capi.ClothBatch([g, conf], [gx, x0])
+    mix.step(3)
+    assert np.array_equal(alone.get_pos(0), mix.get_pos(1))
