@@ -20,6 +20,14 @@ void check(hipError_t e, const char *what)
 }
 } // namespace
 
+// slices above which a cloth's solver sums go through group sums
+// (BW_GROUP_MIN overrides the default for tuning runs)
+static int group_min()
+{
+    const char *e = std::getenv("BW_GROUP_MIN");
+    return e ? std::max(0, std::atoi(e)) : 512;
+}
+
 template <typename T> T *Batch::dalloc(size_t n)
 {
     void *p = nullptr;
@@ -232,6 +240,7 @@ void Batch::build_layout()
     std::vector<float> tri_coef(12 * (size_t)ntri), tri_area(ntri);
     std::vector<double> ie_factor(nie);
     std::vector<InstParams> params(n);
+    std::vector<int> group_inst;
     for (int i = 0; i < n; i++)
     {
         const ClothModel &m = models_[i];
@@ -321,7 +330,13 @@ void Batch::build_layout()
         P.blk_count = nslices_[i] / BW_SB;
         P.slice_begin = slice_base_[i];
         P.slice_count = nslices_[i];
+        // group sums for cloths whose per-workgroup pass over all slice
+        // partials would cost more than one extra small launch
+        P.group_begin = (int)group_inst.size();
+        P.group_count = nslices_[i] > group_min() ? (nslices_[i] + GROUP - 1) / GROUP : 0;
+        group_inst.insert(group_inst.end(), P.group_count, i);
     }
+    d_.n_groups = (int)group_inst.size();
 
     d_.slice_off = dalloc<int>(slices);
     d_.slice_w = dalloc<int>(slices);
@@ -400,6 +415,9 @@ void Batch::build_layout()
         check(hipHostMalloc((void **)&host_upd_list_, sizeof(int) * (size_t)d_.n_blocks_wg),
               "hipHostMalloc");
         d_.cg_partial = dalloc<double>(2 * PARTS * (size_t)d_.n_slices);
+        d_.cg_group = dalloc<double>(2 * PARTS * (size_t)d_.n_groups);
+        d_.group_inst = dalloc<int>(d_.n_groups);
+        upload(d_.group_inst, group_inst);
     }
 
     upload(d_.slice_off, slice_off);
@@ -553,10 +571,10 @@ double Batch::time_solver_kernel(int reps, double *alg_bytes)
         all.spmv_list = nullptr;
         all.upd_list = nullptr;
         launch_cg_timing_setup(all, fp32, stream_);
-        launch_cg_step(all, fp32, 0, all.n_slices, stream_); // warm
+        launch_cg_spmv_only(all, fp32, stream_); // warm
         check(hipEventRecord(e0, stream_), "hipEventRecord");
         for (int k = 0; k < reps; k++)
-            launch_cg_step(all, fp32, 0, all.n_slices, stream_);
+            launch_cg_spmv_only(all, fp32, stream_);
         check(hipEventRecord(e1, stream_), "hipEventRecord");
         // per block: W + column; per row: z, d_prev read, d, W d written
         bytes = fp32 ? 40.0 * nnzb + 48.0 * nv : 76.0 * nnzb + 96.0 * nv;

@@ -6,10 +6,16 @@
 // state, and the FP64 partial sums (one set per 64-row slice), have two
 // slots: k_cg_spmv reads slot 0 and writes slot 1, k_cg_update reads slot 1
 // and writes slot 0, so no workgroup reads what another workgroup of the same
-// launch writes, and no grid-wide synchronisation is needed.  Every workgroup
-// of a cloth sums the cloth's partials of the previous launch in the same
-// fixed order, so all of them derive identical scalars and take the same
-// branch, and a cloth's arithmetic does not depend on the rest of the batch.
+// launch writes, and no grid-wide synchronisation is needed.  Every
+// wavefront of a cloth sums the cloth's partials of the previous launch in
+// the same fixed order, so all of them derive identical scalars and take the
+// same branch, and a cloth's arithmetic does not depend on the rest of the
+// batch.  That redundant sum costs every workgroup a pass over all of its
+// cloth's partials, which grows with the square of the cloth's size; cloths
+// of more than group_min slices therefore get a third, tiny launch after each
+// of the two, k_cg_group, which sums each group of 64 slice partials once,
+// and their workgroups read the group sums instead (a 1.5M-vertex cloth
+// reads 366 sums rather than 23k partials per workgroup).
 //
 //   k_cg_spmv    one 64-row slice per workgroup, four lanes per row (lane
 //                phase j takes ELL slots j, j+4, ...; the four partial row
@@ -141,6 +147,22 @@ template <> struct Vecs<double>
     __device__ static double *d1(const DeviceBatch &d) { return d.d64[1]; }
 };
 
+// Sums of components comp[] of the previous launch's partials of a cloth
+// (slot 0: written by k_cg_update, 1: by k_cg_spmv), in every wavefront.
+// Large cloths read their group sums, which k_cg_group formed after that
+// launch; the others read their per-slice partials.
+template <int N>
+__device__ inline void solver_sums(const DeviceBatch &d, const InstParams &ip, int slot,
+                                   const int (&comp)[N], double (&out)[N])
+{
+    if (ip.group_count > 0)
+        wave_sets_sum<N>(glob(d.cg_group) + (size_t)slot * PARTS * d.n_groups, ip.group_begin,
+                         ip.group_count, comp, out);
+    else
+        wave_sets_sum<N>(glob(d.cg_partial) + (size_t)slot * PARTS * d.n_slices, ip.slice_begin,
+                         ip.slice_count, comp, out);
+}
+
 // k_cg_spmv on slice sl of batch d
 template <typename T, int L>
 __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> *__restrict__ W,
@@ -159,7 +181,6 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
             store_state(st_out, st);
         return;
     }
-    const G<double> *part_in = glob(d.cg_partial);
     G<double> *part_out = glob(d.cg_partial) + (size_t)PARTS * d.n_slices;
     const double tol2 = ip.tol * ip.tol;
     bool residual = false;
@@ -170,7 +191,7 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
     {
         double v[3];
         const int comp[3] = {P_RZ, P_RR, P_BB};
-        reduce_cloth<L, 3>(part_in, ip.slice_begin, ip.slice_count, comp, v, lds);
+        solver_sums<3>(d, ip, 0, comp, v);
         const double rz = v[0], rr = v[1];
         if (st.pending)
         {
@@ -304,8 +325,7 @@ template <typename T, int SB>
 __device__ inline void update_body(const DeviceBatch &d, const int blk,
                                    const G<T> *__restrict__ P, G<T> *__restrict__ r,
                                    G<T> *__restrict__ z, const G<T> *__restrict__ q,
-                                   const G<T> *__restrict__ dbuf0, const G<T> *__restrict__ dbuf1,
-                                   double *lds)
+                                   const G<T> *__restrict__ dbuf0, const G<T> *__restrict__ dbuf1)
 {
     const int inst = glob(d.blk_inst)[blk];
     const InstParams ip = glob(d.params)[inst];
@@ -333,7 +353,7 @@ __device__ inline void update_body(const DeviceBatch &d, const int blk,
     double v1[1];
     {
         const int comp[1] = {P_DQ};
-        reduce_cloth<SB, 1>(part_in, ip.slice_begin, ip.slice_count, comp, v1, lds);
+        solver_sums<1>(d, ip, 1, comp, v1);
     }
     const double dq = v1[0];
     if (!(dq > 0.0) || !isfinite(dq))
@@ -406,7 +426,6 @@ __global__ void __launch_bounds__(SB * 64)
 k_cg_update(DeviceBatch d, const T *__restrict__ P, T *__restrict__ r, T *__restrict__ z,
             const T *__restrict__ q, const T *__restrict__ dbuf0, const T *__restrict__ dbuf1)
 {
-    __shared__ double lds[8 * SB];
     int blk = solver_block(d.xcd_map);
     if (d.upd_list)
     {
@@ -414,7 +433,35 @@ k_cg_update(DeviceBatch d, const T *__restrict__ P, T *__restrict__ r, T *__rest
         if (blk < 0)
             return;
     }
-    update_body<T, SB>(d, blk, glob(P), glob(r), glob(z), glob(q), glob(dbuf0), glob(dbuf1), lds);
+    update_body<T, SB>(d, blk, glob(P), glob(r), glob(z), glob(q), glob(dbuf0), glob(dbuf1));
+}
+
+// group sums of the partials in `slot` (just written by the launch before):
+// one wavefront per group of GROUP consecutive slices of a large cloth,
+// lane l taking slice l of the group, then the wavefront butterfly
+__device__ inline void group_body(const DeviceBatch &d, int g, int slot)
+{
+    const int inst = glob(d.group_inst)[g];
+    const InstParams ip = glob(d.params)[inst];
+    const int k = g - ip.group_begin;
+    const int s0 = ip.slice_begin + k * GROUP;
+    const int n = min(GROUP, ip.slice_count - k * GROUP);
+    const int lane = threadIdx.x & 63;
+    const G<double> *part = glob(d.cg_partial) + (size_t)slot * PARTS * d.n_slices;
+    double v[PARTS];
+    for (int j = 0; j < PARTS; j++)
+        v[j] = lane < n ? part[PARTS * (size_t)(s0 + lane) + j] : 0.0;
+    for (int j = 0; j < PARTS; j++)
+        v[j] = wave_sum(v[j]);
+    G<double> *out = glob(d.cg_group) + (size_t)slot * PARTS * d.n_groups + PARTS * (size_t)g;
+    if (lane == 0)
+        for (int j = 0; j < PARTS; j++)
+            out[j] = v[j];
+}
+
+__global__ void __launch_bounds__(64) k_cg_group(DeviceBatch d, int slot)
+{
+    group_body(d, blockIdx.x, slot);
 }
 
 // Many batches in one launch: entry (batch, slice) of `list` per workgroup,
@@ -438,13 +485,21 @@ __global__ void __launch_bounds__(SB * 64)
 k_cg_update_multi(const DeviceBatch *__restrict__ table, const int2 *__restrict__ list,
                   int xcd_map)
 {
-    __shared__ double lds[8 * SB];
     const int2 e = list[solver_block(xcd_map)];
     if (e.x < 0)
         return;
     const DeviceBatch &d = table[e.x];
     update_body<T, SB>(d, e.y, glob(Vecs<T>::P(d)), glob(Vecs<T>::r(d)), glob(Vecs<T>::z(d)),
-                       glob(Vecs<T>::q(d)), glob(Vecs<T>::d0(d)), glob(Vecs<T>::d1(d)), lds);
+                       glob(Vecs<T>::q(d)), glob(Vecs<T>::d0(d)), glob(Vecs<T>::d1(d)));
+}
+
+__global__ void __launch_bounds__(64)
+k_cg_group_multi(const DeviceBatch *__restrict__ table, const int2 *__restrict__ list, int slot)
+{
+    const int2 e = list[blockIdx.x];
+    if (e.x < 0)
+        return;
+    group_body(table[e.x], e.y, slot);
 }
 
 // both slots to the solve start (true residual of the warm start first)
@@ -476,6 +531,15 @@ template <typename T> __global__ void k_cg_timing_setup(DeviceBatch d, T *r, T *
             part[P_RR] = 1.0;
             part[P_BB] = 1.0;
             part[P_DQ] = 2.0;
+        }
+    if (i < (size_t)d.n_groups)
+        for (int slot = 0; slot < 2; slot++)
+        {
+            double *grp = d.cg_group + (size_t)slot * PARTS * d.n_groups + PARTS * i;
+            grp[P_RZ] = 1.0;
+            grp[P_RR] = 1.0;
+            grp[P_BB] = 1.0;
+            grp[P_DQ] = 2.0;
         }
     if (i < 2 * (size_t)d.n_inst)
     {
@@ -529,6 +593,16 @@ void launch_spmv_multi(const DeviceBatch *table, const int2 *list, int grid, int
                            xcd_map);
 }
 
+void launch_cg_spmv_only(const DeviceBatch &d, bool fp32, hipStream_t s)
+{
+    if (fp32)
+        launch_spmv_lpr<float>(d, d.n_slices, d.W32, d.P32, d.r32, d.z32, d.q32, d.d32[0],
+                               d.d32[1], s);
+    else
+        launch_spmv_lpr<double>(d, d.n_slices, d.W64, d.P64, d.r64, d.z64, d.q64, d.d64[0],
+                                d.d64[1], s);
+}
+
 void launch_cg_step(const DeviceBatch &d, bool fp32, int phase, int slices, hipStream_t s)
 {
     const dim3 upd_grid((slices + SB - 1) / SB), upd_block(SB * 64);
@@ -550,11 +624,13 @@ void launch_cg_step(const DeviceBatch &d, bool fp32, int phase, int slices, hipS
             hipLaunchKernelGGL((k_cg_update<double, SB>), upd_grid, upd_block, 0, s, d, d.P64,
                                d.r64, d.z64, d.q64, d.d64[0], d.d64[1]);
     }
+    if (d.n_groups > 0)
+        hipLaunchKernelGGL(k_cg_group, dim3(d.n_groups), dim3(64), 0, s, d, phase == 0 ? 1 : 0);
 }
 
 void launch_cg_step_multi(const DeviceBatch *table, const int2 *spmv_list, int spmv_grid,
-                          const int2 *upd_list, int upd_grid, bool fp32, int phase, int lpr,
-                          int xcd_map, hipStream_t s)
+                          const int2 *upd_list, int upd_grid, const int2 *grp_list, int grp_grid,
+                          bool fp32, int phase, int lpr, int xcd_map, hipStream_t s)
 {
     if (fp32)
     {
@@ -572,6 +648,9 @@ void launch_cg_step_multi(const DeviceBatch *table, const int2 *spmv_list, int s
             hipLaunchKernelGGL((k_cg_update_multi<double, SB>), dim3(upd_grid), dim3(SB * 64), 0,
                                s, table, upd_list, xcd_map);
     }
+    if (grp_grid > 0)
+        hipLaunchKernelGGL(k_cg_group_multi, dim3(grp_grid), dim3(64), 0, s, table, grp_list,
+                           phase == 0 ? 1 : 0);
 }
 
 void launch_cg_timing_setup(const DeviceBatch &d, bool fp32, hipStream_t s)

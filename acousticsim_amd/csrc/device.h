@@ -36,6 +36,9 @@ template <typename T> __host__ __device__ inline G<T> *glob(T *p) { return (G<T>
 #endif
 // FP64 partial sums per workgroup
 constexpr int PARTS = 5;
+// slices per group sum of the tolerance-mode solver (cloths of more than
+// group_min slices are reduced through group sums)
+constexpr int GROUP = 64;
 
 struct InstParams
 {
@@ -47,6 +50,7 @@ struct InstParams
     int max_since;    // ... or after this many iterations
     int slice_begin, slice_count;
     int blk_begin, blk_count; // solver workgroups of the cloth
+    int group_begin, group_count; // group sums of the cloth (0: none)
 };
 
 // Per-cloth state of the reference-semantics PCG (pcg.hip).  Each field has
@@ -126,6 +130,11 @@ struct DeviceBatch
     // form uses r64, z64, q64, d64); z = P r
     CgState *cg = nullptr;
     double *cg_partial = nullptr;
+    // group sums (2 slots x PARTS per group of GROUP slices) of large cloths,
+    // and the cloth of each group
+    double *cg_group = nullptr;
+    int *group_inst = nullptr;
+    int n_groups = 0;
     float *r32 = nullptr, *z32 = nullptr, *q32 = nullptr, *d32[2] = {nullptr, nullptr};
     double *z64 = nullptr;
     // optional lists of the slices (k_cg_spmv) and solver workgroups
@@ -157,17 +166,22 @@ void launch_pcg_iter64(const DeviceBatch &d, hipStream_t s);
 // mode 0: FP64 solve start; 2: counters only
 void launch_residual64(const DeviceBatch &d, int mode, hipStream_t s);
 void launch_update_positions(const DeviceBatch &d, hipStream_t s);
-// tolerance-mode PCG (cg.hip); phase 0: k_cg_spmv, 1: k_cg_update.  slices:
-// the k_cg_spmv grid (list length, or n_slices without lists); k_cg_update
-// runs ceil(slices / BW_SB) workgroups.
+// tolerance-mode PCG (cg.hip); phase 0: k_cg_spmv, 1: k_cg_update, each
+// followed by k_cg_group when the batch has group sums.  slices: the
+// k_cg_spmv grid (list length, or n_slices without lists); k_cg_update runs
+// ceil(slices / BW_SB) workgroups.
 void launch_cg_reset(const DeviceBatch &d, hipStream_t s);
 void launch_cg_step(const DeviceBatch &d, bool fp32, int phase, int slices, hipStream_t s);
+// k_cg_spmv alone (kernel timing)
+void launch_cg_spmv_only(const DeviceBatch &d, bool fp32, hipStream_t s);
 // the same over many batches: `table` holds the batches (device memory),
 // spmv_list one (batch, slice) entry per k_cg_spmv workgroup, upd_list one
-// (batch, solver workgroup) entry per k_cg_update workgroup; batch < 0 pads
+// (batch, solver workgroup) entry per k_cg_update workgroup, grp_list one
+// (batch, group) entry per group-sum wavefront (grp_grid 0: none); batch < 0
+// pads
 void launch_cg_step_multi(const DeviceBatch *table, const int2 *spmv_list, int spmv_grid,
-                          const int2 *upd_list, int upd_grid, bool fp32, int phase, int lpr,
-                          int xcd_map, hipStream_t s);
+                          const int2 *upd_list, int upd_grid, const int2 *grp_list, int grp_grid,
+                          bool fp32, int phase, int lpr, int xcd_map, hipStream_t s);
 void launch_cg_timing_setup(const DeviceBatch &d, bool fp32, hipStream_t s);
 // measurement helpers: set the solver residual to 1 on every row; one SpMV
 // launch of the solver's precision

@@ -59,6 +59,34 @@ __device__ inline void write_partials(PP part, int blk, const int (&comp)[N], co
             part[PARTS * (size_t)blk + comp[j]] = v[j];
 }
 
+// Sum of components comp[] over partial sets b0 .. b0+nb-1, computed by
+// every wavefront on its own (identical in all of them, whatever the
+// workgroup size): lane l adds sets l, l+64, l+128, ... in that order, four
+// loads in flight, then the wavefront butterfly.
+template <int N, typename PP>
+__device__ inline void wave_sets_sum(PP sets, int b0, int nb, const int (&comp)[N], double (&out)[N])
+{
+    const int lane = threadIdx.x & 63;
+    for (int j = 0; j < N; j++)
+        out[j] = 0.0;
+    int i = lane;
+    for (; i + 192 < nb; i += 256)
+    {
+        double a[4][N];
+        for (int u = 0; u < 4; u++)
+            for (int j = 0; j < N; j++)
+                a[u][j] = sets[PARTS * (size_t)(b0 + i + 64 * u) + comp[j]];
+        for (int u = 0; u < 4; u++)
+            for (int j = 0; j < N; j++)
+                out[j] += a[u][j];
+    }
+    for (; i < nb; i += 64)
+        for (int j = 0; j < N; j++)
+            out[j] += sets[PARTS * (size_t)(b0 + i) + comp[j]];
+    for (int j = 0; j < N; j++)
+        out[j] = wave_sum(out[j]);
+}
+
 // sum the cloth's partials (workgroups b0 .. b0+nb-1), result in every thread
 template <int SB, int N, typename PP>
 __device__ inline void reduce_cloth(PP part, int b0, int nb, const int (&comp)[N],

@@ -95,6 +95,8 @@ Server::~Server()
     (void)hipHostFree(table_host_);
     (void)hipHostFree(spmv_list_host_);
     (void)hipHostFree(upd_list_host_);
+    (void)hipFree(grp_list_dev_);
+    (void)hipHostFree(grp_list_host_);
     (void)hipStreamDestroy(stream_);
 }
 
@@ -279,15 +281,15 @@ void Server::upload_table(const std::vector<std::unique_ptr<Resident>> &res)
               "hipMemcpyAsync table");
 }
 
-void Server::launch_chain(int length, int spmv_grid)
+void Server::launch_chain(int length, int spmv_grid, int grp_grid)
 {
     const int ugrid = (spmv_grid + BW_SB - 1) / BW_SB;
     for (int k = 0; k < length; k++)
-        launch_cg_step_multi(table_dev_, spmv_list_dev_, spmv_grid, upd_list_dev_, ugrid, opt_.fp32,
-                             k & 1, opt_.spmv_lpr, 1, stream_);
+        launch_cg_step_multi(table_dev_, spmv_list_dev_, spmv_grid, upd_list_dev_, ugrid,
+                             grp_list_dev_, grp_grid, opt_.fp32, k & 1, opt_.spmv_lpr, 1, stream_);
 }
 
-void Server::run_chunk(int length, int spmv_grid)
+void Server::run_chunk(int length, int spmv_grid, int grp_grid)
 {
     // BW_GRAPHS=0: direct launches (profilers that do not follow graphs)
     static const bool graphs = [] {
@@ -296,11 +298,11 @@ void Server::run_chunk(int length, int spmv_grid)
     }();
     if (!graphs)
     {
-        launch_chain(length, spmv_grid);
+        launch_chain(length, spmv_grid, grp_grid);
         check(hipGetLastError(), "solver launch");
         return;
     }
-    // graphs of power-of-two lengths, cached by (length, grid)
+    // graphs of power-of-two lengths, cached by (length, grids)
     for (int piece = 1 << 30; piece >= 32; piece >>= 1)
     {
         if (!(length & piece))
@@ -308,7 +310,7 @@ void Server::run_chunk(int length, int spmv_grid)
         ++graph_clock_;
         hipGraphExec_t exec = nullptr;
         for (auto &g : graphs_)
-            if (g.length == piece && g.spmv_grid == spmv_grid)
+            if (g.length == piece && g.spmv_grid == spmv_grid && g.grp_grid == grp_grid)
             {
                 exec = g.exec;
                 g.last_use = graph_clock_;
@@ -330,17 +332,17 @@ void Server::run_chunk(int length, int spmv_grid)
                 std::lock_guard<std::mutex> cap(capture_mu_);
                 check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal),
                       "BeginCapture");
-                launch_chain(piece, spmv_grid);
+                launch_chain(piece, spmv_grid, grp_grid);
                 check(hipStreamEndCapture(stream_, &graph), "EndCapture");
             }
             check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "GraphInstantiate");
             (void)hipGraphDestroy(graph);
-            graphs_.push_back({piece, spmv_grid, exec, graph_clock_});
+            graphs_.push_back({piece, spmv_grid, grp_grid, exec, graph_clock_});
         }
         check(hipGraphLaunch(exec, stream_), "GraphLaunch");
     }
     if (length & 31)
-        launch_chain(length & 31, spmv_grid);
+        launch_chain(length & 31, spmv_grid, grp_grid);
 }
 
 void Server::retire(Resident &r, int error_code, const std::string &error)
@@ -394,13 +396,27 @@ int Server::run_round(std::vector<std::unique_ptr<Resident>> &res)
             }
         }
     // lists of every solving request's slices and solver workgroups
-    int n = 0, nb = 0;
+    int n = 0, nb = 0, ng = 0;
     for (auto &r : res)
     {
         n += r->batch->device().n_slices;
         nb += r->batch->device().n_blocks_wg;
+        ng += r->batch->device().n_groups;
     }
     const int grid = bucket(n), ugrid = (grid + BW_SB - 1) / BW_SB;
+    const int ggrid = ng > 0 ? bucket(ng) : 0;
+    if (ggrid > grp_cap_)
+    {
+        check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+        for (auto &g : graphs_)
+            (void)hipGraphExecDestroy(g.exec);
+        graphs_.clear();
+        (void)hipFree(grp_list_dev_);
+        (void)hipHostFree(grp_list_host_);
+        grp_cap_ = std::max(ggrid, 2 * grp_cap_);
+        check(hipMalloc((void **)&grp_list_dev_, grp_cap_ * sizeof(int2)), "hipMalloc");
+        check(hipHostMalloc((void **)&grp_list_host_, grp_cap_ * sizeof(int2)), "hipHostMalloc");
+    }
     if (grid > list_cap_)
     {
         // grow the lists; graphs hold the old addresses
@@ -431,6 +447,16 @@ int Server::run_round(std::vector<std::unique_ptr<Resident>> &res)
         spmv_list_host_[k] = make_int2(-1, 0);
     for (; kb < ugrid; kb++)
         upd_list_host_[kb] = make_int2(-1, 0);
+    int kg = 0;
+    for (auto &r : res)
+        for (int g = 0; g < r->batch->device().n_groups; g++)
+            grp_list_host_[kg++] = make_int2(r->slot, g);
+    for (; kg < ggrid; kg++)
+        grp_list_host_[kg] = make_int2(-1, 0);
+    if (ggrid > 0)
+        check(hipMemcpyAsync(grp_list_dev_, grp_list_host_, ggrid * sizeof(int2),
+                             hipMemcpyHostToDevice, stream_),
+              "hipMemcpyAsync list");
     check(hipMemcpyAsync(spmv_list_dev_, spmv_list_host_, grid * sizeof(int2),
                          hipMemcpyHostToDevice, stream_),
           "hipMemcpyAsync list");
@@ -442,7 +468,7 @@ int Server::run_round(std::vector<std::unique_ptr<Resident>> &res)
     for (auto &r : res)
         want = std::min(want, r->predicted > r->launched ? r->predicted - r->launched : 64LL);
     const int length = (int)std::min<long long>(1024, std::max<long long>(32, (want + 31) / 32 * 32));
-    run_chunk(length, grid);
+    run_chunk(length, grid, ggrid);
     for (auto &r : res)
         r->launched += length;
     // every slot's states in one copy

@@ -122,6 +122,8 @@ private:
     int2 *spmv_list_dev_ = nullptr, *spmv_list_host_ = nullptr;
     int2 *upd_list_dev_ = nullptr, *upd_list_host_ = nullptr;
     int list_cap_ = 0; // entries of each list
+    int2 *grp_list_dev_ = nullptr, *grp_list_host_ = nullptr; // (slot, group) of large cloths
+    int grp_cap_ = 0;
 
     std::mutex mu_;
     std::mutex capture_mu_; // graph capture vs. device allocations on the builder thread
@@ -137,14 +139,14 @@ private:
 
     struct GraphEntry
     {
-        int length, spmv_grid;
+        int length, spmv_grid, grp_grid;
         hipGraphExec_t exec;
         unsigned long long last_use;
     };
     std::vector<GraphEntry> graphs_;
     unsigned long long graph_clock_ = 0;
-    void run_chunk(int length, int spmv_grid);
-    void launch_chain(int length, int spmv_grid);
+    void run_chunk(int length, int spmv_grid, int grp_grid);
+    void launch_chain(int length, int spmv_grid, int grp_grid);
 
     std::vector<std::thread> builders_;
     std::thread device_;

@@ -101,3 +101,36 @@ def test_autotuner_variants_agree(monkeypatch):
         d = np.abs(out["4"][i] - x0).max()
         for lpr in ("1", "2"):
             assert np.abs(out[lpr][i] - out["4"][i]).max() <= 1e-4 * d
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_group_sums_alone_equals_mix_and_oracle(graphs, monkeypatch):
+    """Cloths above 512 slices reduce their solver sums through group sums
+    (one extra small launch per solver launch).  Forced on for every cloth
+    here: each result is still the same alone and in the mix (bit for bit),
+    and one step agrees with the oracle within 1e-4 of the displacement."""
+    from acousticsim_amd import capi
+
+    monkeypatch.setenv("BW_SPMV_LPR", "2")
+    monkeypatch.setenv("BW_GROUP_MIN", "0")
+    cs = _confs("mixed")
+    say("case group sums, graphs", graphs)
+    b = capi.ClothBatch([c for c, _ in cs], [x for _, x in cs])
+    b.set_graphs(graphs)
+    b.step(3)
+    for i, (c, x0) in enumerate(cs):
+        a = capi.ClothBatch([c], [x0])
+        a.set_graphs(graphs)
+        a.step(3)
+        assert np.array_equal(a.get_pos(0), b.get_pos(i)), i
+    one = capi.ClothBatch([c for c, _ in cs], [x for _, x in cs])
+    one.step(1)
+    for i, (c, x0) in enumerate(cs):
+        oc = of.OracleCloth(c)
+        oc.set_state(x0, x0)
+        oc.step(of.SOLVE_TOLERANCE, 1e-12)
+        d = np.abs(oc.pos() - x0).max()
+        err = np.abs(one.get_pos(i) - oc.pos()).max()
+        say("  member %d err/|dx| %.3e" % (i, err / d))
+        assert one.stats(i)["rel_residual"] <= 1e-6
+        assert err <= 1e-4 * d

@@ -133,3 +133,21 @@ def test_bad_requests_report_errors():
     with pytest.raises(capi.BwError):
         s.submit(c, None, 0)
     s.close()
+
+
+def test_served_with_group_sums_equals_batch(monkeypatch):
+    """Requests whose cloths reduce through group sums (forced for every
+    cloth here) give, in a mix, the same result as a batch stepping each one
+    alone."""
+    from acousticsim_amd import capi
+
+    monkeypatch.setenv("BW_SPMV_LPR", "1")
+    monkeypatch.setenv("BW_GROUP_MIN", "0")
+    say("case served group sums")
+    mix, counters = _served(REQS, stagger=0.01)
+    for i, (c, x0) in enumerate(REQS):
+        info, x = mix[i]
+        assert info["status"] == 0
+        b = capi.ClothBatch([c], [x0])
+        b.step(STEPS)
+        assert np.array_equal(b.get_pos(0), x), i
