@@ -32,10 +32,12 @@ class StepStats(ctypes.Structure):
         ("outer_iterations", ctypes.c_int),
         ("rel_residual", ctypes.c_double),
         ("step_us", ctypes.c_double),
+        ("matrix_fp64", ctypes.c_int),
+        ("reserved", ctypes.c_int),
     ]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
 class RequestInfo(ctypes.Structure):

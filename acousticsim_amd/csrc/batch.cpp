@@ -326,6 +326,13 @@ void Batch::build_layout()
             P.rel2 = std::atof(e);
         if (const char *e = std::getenv("BW_MAX_SINCE"))
             P.max_since = std::atoi(e);
+        // a forced check that finds r.r above 1e-4 of its value at the last
+        // replacement (less than 100x down in 2000 iterations) moves the
+        // cloth to the FP64 matrix for the rest of the solve (BW_STALL
+        // overrides the ratio; 0 keeps the FP32 matrix)
+        P.stall = mixed ? 1e-4 : 0.0;
+        if (const char *e = std::getenv("BW_STALL"))
+            P.stall = mixed ? std::atof(e) : 0.0;
         P.blk_begin = slice_base_[i] / BW_SB;
         P.blk_count = nslices_[i] / BW_SB;
         P.slice_begin = slice_base_[i];
@@ -828,6 +835,7 @@ void Batch::solve_cg()
         StepStats &st = stats_[i];
         st.pcg_iterations = s.total;
         st.outer_iterations = s.updates;
+        st.matrix_fp64 = s.w64;
         st.rel_residual = s.bb > 0 ? std::sqrt(s.rr_true / s.bb) : std::sqrt(s.rr_true);
         // launches this cloth needed: two per iteration and per true residual
         pred_launches_[i] = 2LL * (s.total + s.updates) + 2;
@@ -853,6 +861,7 @@ void Batch::end_step(const CgState *states, double step_us)
         StepStats &st = stats_[i];
         st.pcg_iterations = s.total;
         st.outer_iterations = s.updates;
+        st.matrix_fp64 = s.w64;
         st.rel_residual = s.bb > 0 ? std::sqrt(s.rr_true / s.bb) : std::sqrt(s.rr_true);
         st.step_us = step_us;
         bad |= !std::isfinite(st.rel_residual);

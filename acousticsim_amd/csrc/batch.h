@@ -17,6 +17,7 @@ struct StepStats
     int outer_iterations = 0; // FP64 true-residual checks (reliable updates)
     double rel_residual = 0;
     double step_us = 0;
+    int matrix_fp64 = 0; // mixed form: the last solve moved to the FP64 matrix
 };
 
 // how a batch solves: the reference's own PCG with exactly max_iters-1

@@ -355,6 +355,8 @@ int bw_get_last_stats(const bw_cloth *c, bw_step_stats *st)
     const auto &s = c->batch->stats(0);
     st->pcg_iterations = s.pcg_iterations;
     st->outer_iterations = s.outer_iterations;
+    st->matrix_fp64 = s.matrix_fp64;
+    st->reserved = 0;
     st->rel_residual = s.rel_residual;
     st->step_us = s.step_us;
     return BW_OK;
@@ -438,6 +440,8 @@ int bw_batch_get_stats(const bw_batch *b, int i, bw_step_stats *st)
     const auto &s = b->batch->stats(i);
     st->pcg_iterations = s.pcg_iterations;
     st->outer_iterations = s.outer_iterations;
+    st->matrix_fp64 = s.matrix_fp64;
+    st->reserved = 0;
     st->rel_residual = s.rel_residual;
     st->step_us = s.step_us;
     return BW_OK;

@@ -79,7 +79,7 @@ template <typename SP> __device__ inline CgState load_state(SP p)
     s.pending = p->pending;
     s.force_check = p->force_check;
     s.final_check = p->final_check;
-    s.pad = 0;
+    s.w64 = p->w64;
     return s;
 }
 
@@ -101,6 +101,7 @@ template <typename SP> __device__ inline void store_state(SP p, const CgState &s
     p->pending = s.pending;
     p->force_check = s.force_check;
     p->final_check = s.final_check;
+    p->w64 = s.w64;
 }
 
 // SpMV variants: L lanes per row (1, 2 or 4; a workgroup of L wavefronts
@@ -163,6 +164,31 @@ __device__ inline void solver_sums(const DeviceBatch &d, const InstParams &ip, i
                          ip.slice_count, comp, out);
 }
 
+// q = W d over the rows of one slice for lane phase j: W of type TW (the
+// solver's matrix, or W64 in the mixed form after a stall), d = z + beta
+// d_prev formed on the fly in T, sums in TW
+template <typename TW, typename T, int L>
+__device__ inline void slice_product(const G<TW> *__restrict__ Wl, const G<int> *__restrict__ cl,
+                                     int w, int j, const G<T> *__restrict__ z,
+                                     const G<T> *__restrict__ dold, T bt, bool first, TW (&y)[3])
+{
+    y[0] = y[1] = y[2] = 0;
+#pragma unroll(sizeof(TW) == sizeof(T) ? 4 / L : 1)
+    for (int k = j; k < w; k += L)
+    {
+        const size_t c3 = 3 * (size_t)cl[k * SLICE];
+        const TW v0 = (TW)dir_value<T>(z[c3], dold[c3], bt, first);
+        const TW v1 = (TW)dir_value<T>(z[c3 + 1], dold[c3 + 1], bt, first);
+        const TW v2 = (TW)dir_value<T>(z[c3 + 2], dold[c3 + 2], bt, first);
+        const G<TW> *wk = Wl + 9 * (size_t)k * SLICE;
+        y[0] += wk[0 * SLICE] * v0 + wk[1 * SLICE] * v1 + wk[2 * SLICE] * v2;
+        y[1] += wk[3 * SLICE] * v0 + wk[4 * SLICE] * v1 + wk[5 * SLICE] * v2;
+        y[2] += wk[6 * SLICE] * v0 + wk[7 * SLICE] * v1 + wk[8 * SLICE] * v2;
+    }
+    for (int a = 0; a < 3; a++)
+        y[a] = phase_sum<L>(y[a]);
+}
+
 // k_cg_spmv on slice sl of batch d
 template <typename T, int L>
 __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> *__restrict__ W,
@@ -205,6 +231,17 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
                 if (lead)
                     store_state(st_out, st);
                 return;
+            }
+            // FP32 matrix: cond(W) times the FP32 rounding of W bounds
+            // what the FP32-matrix Krylov process can resolve; when a check
+            // forced by max_since finds the true residual stalled, the rest
+            // of the solve reads W64 (direction restarted), which converges
+            // like FP64
+            if (ip.stall > 0.0 && !st.w64 && st.since >= ip.max_since &&
+                rr > ip.stall * st.rr_ref)
+            {
+                st.w64 = 1;
+                st.first = 1;
             }
             st.rr_ref = rr;
             st.since = 0;
@@ -282,23 +319,26 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
     const T bt = (T)beta;
     const G<T> *dold = (st.iter & 1) ? dbuf0 : dbuf1;
     G<T> *dnew = (st.iter & 1) ? dbuf1 : dbuf0;
-    const G<T> *Wl = W + 9 * base + row;
-    T y0 = 0, y1 = 0, y2 = 0;
-#pragma unroll(4 / L)
-    for (int k = j; k < w; k += L)
+    T y0, y1, y2;
+    bool wide = false; // the FP64 matrix under FP32 vectors
+    if constexpr (sizeof(T) == sizeof(float))
+        wide = st.w64 != 0;
+    if (wide)
     {
-        const size_t c3 = 3 * (size_t)cl[k * SLICE];
-        const T v0 = dir_value<T>(z[c3], dold[c3], bt, first);
-        const T v1 = dir_value<T>(z[c3 + 1], dold[c3 + 1], bt, first);
-        const T v2 = dir_value<T>(z[c3 + 2], dold[c3 + 2], bt, first);
-        const G<T> *wk = Wl + 9 * (size_t)k * SLICE;
-        y0 += wk[0 * SLICE] * v0 + wk[1 * SLICE] * v1 + wk[2 * SLICE] * v2;
-        y1 += wk[3 * SLICE] * v0 + wk[4 * SLICE] * v1 + wk[5 * SLICE] * v2;
-        y2 += wk[6 * SLICE] * v0 + wk[7 * SLICE] * v1 + wk[8 * SLICE] * v2;
+        double y[3];
+        slice_product<double, T, L>(glob(d.W64) + 9 * base + row, cl, w, j, z, dold, bt, first, y);
+        y0 = (T)y[0];
+        y1 = (T)y[1];
+        y2 = (T)y[2];
     }
-    y0 = phase_sum<L>(y0);
-    y1 = phase_sum<L>(y1);
-    y2 = phase_sum<L>(y2);
+    else
+    {
+        T y[3];
+        slice_product<T, T, L>(W + 9 * base + row, cl, w, j, z, dold, bt, first, y);
+        y0 = y[0];
+        y1 = y[1];
+        y2 = y[2];
+    }
     double v[1] = {0.0};
     if (j == 0)
     {

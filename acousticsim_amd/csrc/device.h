@@ -51,6 +51,10 @@ struct InstParams
     int slice_begin, slice_count;
     int blk_begin, blk_count; // solver workgroups of the cloth
     int group_begin, group_count; // group sums of the cloth (0: none)
+    // mixed form: a true residual forced by max_since whose r.r is above
+    // stall * (r.r at the last replacement) switches the cloth's SpMV to the
+    // FP64 matrix for the rest of the solve (0: never)
+    double stall;
 };
 
 // Per-cloth state of the reference-semantics PCG (pcg.hip).  Each field has
@@ -83,7 +87,7 @@ struct CgState
     int pending;     // 1: the last k_cg_spmv formed a true residual
     int force_check; // 1: form a true residual next
     int final_check; // 1: stop after the next true residual
-    int pad;
+    int w64;         // 1: the SpMV reads the FP64 matrix (mixed form, after a stall)
 };
 
 struct DeviceBatch

@@ -177,21 +177,31 @@ def irregular_config(points=IRREGULAR["points"]):
 
 
 # Serving: step requests arrive as a Poisson stream; each is a unit-square
-# cloth with a log-uniform vertex count, asking for `steps` substeps.
+# cloth with a log-uniform vertex count, a regular grid or seeded Delaunay
+# points (even odds), asking for `steps` substeps.
 SERVE = dict(rate=2000.0, v_lo=1e3, v_hi=2e5, steps=10, dt=0.01, size=1.0, seed=4)
 
 
 def serve_requests(n, seed=SERVE["seed"], rate=SERVE["rate"], v_lo=SERVE["v_lo"],
                    v_hi=SERVE["v_hi"]):
-    """[(arrival_s, side, jitter_seed)] of n requests: exponential
-    inter-arrival times at `rate`, vertex count log-uniform in [v_lo, v_hi]
-    rounded to a square grid of side round(sqrt(V))."""
-    u = uniform01(seed, 2 * n)
+    """[(arrival_s, conf, x0)] of n requests: exponential inter-arrival
+    times at `rate`, vertex count V log-uniform in [v_lo, v_hi]; a regular
+    grid of side round(sqrt(V)) or V Delaunay points (seeded), with even
+    odds; two top corners pinned; mixed precision."""
+    u = uniform01(seed, 3 * n)
     t = 0.0
     out = []
     for k in range(n):
-        t += -math.log(1.0 - u[2 * k]) / rate
-        v = math.exp(math.log(v_lo) + u[2 * k + 1] * (math.log(v_hi) - math.log(v_lo)))
-        side = max(4, int(round(math.sqrt(v))))
-        out.append((t, side, seed * 100000 + k + 1))
+        t += -math.log(1.0 - u[3 * k]) / rate
+        v = math.exp(math.log(v_lo) + u[3 * k + 1] * (math.log(v_hi) - math.log(v_lo)))
+        jseed = seed * 100000 + k + 1
+        if u[3 * k + 2] < 0.5:
+            c = grid_config(max(4, int(round(math.sqrt(v)))), size=SERVE["size"], dt=SERVE["dt"])
+            x0 = jittered_positions(c, jseed)
+        else:
+            c = delaunay_config(int(round(v)), size=SERVE["size"], seed=jseed, dt=SERVE["dt"],
+                                top_edge=False)
+            x0 = jitter(delaunay_rest_positions(c)[0], jseed)
+        c["precision"] = "mixed"
+        out.append((t, c, x0))
     return out

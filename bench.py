@@ -221,15 +221,11 @@ def roofline_only(args, ms):
 
 def serve(args):
     """Serving workload: requests arrive as a Poisson stream (seeded), each a
-    unit-square cloth of log-uniform vertex count asking for 10 steps; the
+    unit-square cloth (regular grid or seeded Delaunay points) of log-uniform
+    vertex count asking for 10 steps; the
     server admits them against a 200 GB device budget and solves whatever is
     resident together.  Reports throughput and request latency."""
-    reqs = wl.serve_requests(args.requests, rate=args.rate)
-    work = []
-    for t, side, seed in reqs:
-        c = wl.grid_config(side, size=wl.SERVE["size"], dt=wl.SERVE["dt"])
-        c["precision"] = "mixed"
-        work.append((t, c, wl.jittered_positions(c, seed)))
+    work = wl.serve_requests(args.requests, rate=args.rate)
     s = capi.Server(budget_bytes=200e9, max_resident=1024)
     print("bench: serving %d requests at %.0f req/s" % (len(work), args.rate), file=sys.stderr,
           flush=True)
@@ -270,9 +266,11 @@ def serve(args):
         "vs_baseline": None,
         "dtype": "f32+f64",
         "data": "synthetic",
-        "config": {"workload": "serving: Poisson %.0f req/s, %d requests, unit-square grid cloths "
-                               "with V log-uniform in [1e3, 2e5], 10 steps each, dt 0.01 s, tol "
-                               "1e-6, 200 GB budget" % (args.rate, len(infos))},
+        "config": {"workload": "serving: Poisson %.0f req/s, %d requests, unit-square cloths "
+                               "(regular grid or seeded Delaunay, even odds) with V log-uniform "
+                               "in [1e3, 2e5], 10 steps each, dt 0.01 s, tol 1e-6, 200 GB budget"
+                               % (args.rate, len(infos))},
+        "delaunay_requests": sum(1 for _, c, _ in work if c["geometry_type"] == "delaunay"),
         "requests_per_s": len(infos) / makespan,
         "vertex_steps_per_s": vsteps / makespan,
         "latency_p50_s": float(np.percentile(lat, 50)),

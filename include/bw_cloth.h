@@ -51,6 +51,9 @@ typedef struct bw_step_stats
     int outer_iterations; /* FP64 true-residual checks of the last step */
     double rel_residual;  /* final FP64 true residual / ||b|| on free dofs */
     double step_us;       /* host wall time of the last step */
+    int matrix_fp64;      /* "mixed": 1 when the last solve stalled on the FP32
+                             matrix and finished on the FP64 one */
+    int reserved;
 } bw_step_stats;
 
 /* ---- single cloth: the cBaseCloth interface --------------------------- */

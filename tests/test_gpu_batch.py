@@ -134,3 +134,38 @@ def test_group_sums_alone_equals_mix_and_oracle(graphs, monkeypatch):
         say("  member %d err/|dx| %.3e" % (i, err / d))
         assert one.stats(i)["rel_residual"] <= 1e-6
         assert err <= 1e-4 * d
+
+
+def test_stalled_solves_move_to_fp64_matrix(monkeypatch):
+    """Mixed form: a true-residual check forced by the iteration count that
+    finds the residual stalled moves the cloth's SpMV to the FP64 matrix for
+    the rest of its solve.  Forced here (a check every 20 iterations, any
+    progress counted as a stall): every cloth switches, still converges to
+    1e-6, matches the oracle within 1e-4 of the step, and is the same alone
+    and in the mix."""
+    from acousticsim_amd import capi
+
+    monkeypatch.setenv("BW_SPMV_LPR", "2")
+    monkeypatch.setenv("BW_MAX_SINCE", "20")
+    monkeypatch.setenv("BW_STALL", "1e-30")
+    cs = _confs("mixed")
+    say("case forced FP64-matrix switch")
+    b = capi.ClothBatch([c for c, _ in cs], [x for _, x in cs])
+    b.step(1)
+    for i, (c, x0) in enumerate(cs):
+        st = b.stats(i)
+        assert st["matrix_fp64"] == 1 and st["rel_residual"] <= 1e-6
+        oc = of.OracleCloth(c)
+        oc.set_state(x0, x0)
+        oc.step(of.SOLVE_TOLERANCE, 1e-12)
+        d = np.abs(oc.pos() - x0).max()
+        err = np.abs(b.get_pos(i) - oc.pos()).max()
+        say("  member %d err/|dx| %.3e iters %d" % (i, err / d, st["pcg_iterations"]))
+        assert err <= 1e-4 * d
+        a = capi.ClothBatch([c], [x0])
+        a.step(1)
+        assert np.array_equal(a.get_pos(0), b.get_pos(i))
+    monkeypatch.setenv("BW_STALL", "0")
+    n = capi.ClothBatch([c for c, _ in cs], [x for _, x in cs])
+    n.step(1)
+    assert all(n.stats(i)["matrix_fp64"] == 0 for i in range(len(cs)))
