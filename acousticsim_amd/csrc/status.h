@@ -41,6 +41,11 @@ inline int status_of_current_exception(std::string &msg)
         msg = "out of host memory";
         return BW_ERR_CAPACITY;
     }
+    catch (const std::exception &e)
+    {
+        msg = std::string("unexpected exception: ") + e.what();
+        return BW_ERR_STATE;
+    }
     catch (...)
     {
         msg = "unknown error";
