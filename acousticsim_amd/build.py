@@ -36,6 +36,7 @@ SOURCES = [
     ("assemble.hip", DEVICE + ["-ffp-contract=off"]),
     ("pcg.hip", DEVICE + ["-ffp-contract=fast"]),
     ("cg.hip", DEVICE + ["-ffp-contract=fast"]),
+    ("normals.hip", DEVICE + ["-ffp-contract=off"]),
 ]
 
 

@@ -82,7 +82,7 @@ EXPORTS = [
     "bw_get_default_timestep", "bw_num_vertices", "bw_num_triangles", "bw_num_edges",
     "bw_get_constraint_static_vertices", "bw_add_constraint_static_vertices",
     "bw_clear_constraint_static_vertices", "bw_get_triangles", "bw_get_edges",
-    "bw_get_ell_map", "bw_assemble_system", "bw_get_last_stats", "bw_last_error", "bw_batch_create",
+    "bw_get_ell_map", "bw_get_normals", "bw_batch_get_normals", "bw_assemble_system", "bw_get_last_stats", "bw_last_error", "bw_batch_create",
     "bw_batch_destroy", "bw_batch_size", "bw_batch_step", "bw_batch_num_vertices",
     "bw_batch_get_pos", "bw_batch_set_state", "bw_batch_get_stats", "bw_batch_set_graphs",
     "bw_batch_time_solver_kernel", "bw_batch_autotune", "bw_server_create", "bw_server_destroy", "bw_server_submit",
@@ -126,6 +126,8 @@ def lib():
         "bw_get_triangles": (i32, [vp, ip]),
         "bw_get_edges": (i32, [vp, ip]),
         "bw_get_ell_map": (i32, [vp, ip, ip]),
+        "bw_get_normals": (i32, [vp, dp, dp]),
+        "bw_batch_get_normals": (i32, [vp, i32, dp, dp]),
         "bw_assemble_system": (i32, [vp, dp, dp]),
         "bw_get_last_stats": (i32, [vp, P(StepStats)]),
         "bw_last_error": (cp, []),
@@ -261,6 +263,15 @@ class BaraffClothHip:
         _check(lib().bw_get_init_pos(self._h, _dp(x)))
         return x
 
+    def normals(self):
+        """(vertex normals (V, 3), triangle normals (T, 3)) at the current
+        positions, as cBaseCloth::Update refreshes them."""
+        nt = lib().bw_num_triangles(self._h)
+        vn = np.empty(3 * self.nv)
+        tn = np.empty(3 * nt)
+        _check(lib().bw_get_normals(self._h, _dp(vn), _dp(tn)))
+        return vn.reshape(-1, 3), tn.reshape(-1, 3)
+
     def GetDefaultTimestep(self):
         return lib().bw_get_default_timestep(self._h)
 
@@ -385,6 +396,13 @@ class ClothBatch:
         x = np.ascontiguousarray(x, dtype=np.float64)
         xpre = np.ascontiguousarray(xpre, dtype=np.float64)
         _check(lib().bw_batch_set_state(self._h, i, _dp(x), _dp(xpre)))
+
+    def normals(self, i, num_triangles):
+        """(vertex normals (V, 3), triangle normals (T, 3)) of member i"""
+        vn = np.empty(3 * self.nv[i])
+        tn = np.empty(3 * num_triangles)
+        _check(lib().bw_batch_get_normals(self._h, i, _dp(vn), _dp(tn)))
+        return vn.reshape(-1, 3), tn.reshape(-1, 3)
 
     def stats(self, i):
         st = StepStats()

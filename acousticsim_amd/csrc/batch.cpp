@@ -187,6 +187,8 @@ void Batch::build_layout()
     }
     if ((long long)ntri >= (1LL << 27) || (long long)nie >= (1LL << 27))
         throw std::runtime_error("batch: too many elements");
+    vert_base_ = vert_base;
+    tri_base_ = tri_base;
     d_.n_inst = n;
     if (const char *e = std::getenv("BW_XCD_MAP"))
         d_.xcd_map = std::atoi(e) != 0;
@@ -494,6 +496,25 @@ void Batch::download(int i, const double *src_dev, double *dst) const
     for (int v = 0; v < models_[i].nv; v++)
         for (int a = 0; a < 3; a++)
             dst[3 * (size_t)v + a] = rows[3 * (size_t)row_of_[i][v] + a];
+}
+
+void Batch::normals(int i, double *vn, double *tn)
+{
+    if (!normals_dev_)
+        normals_dev_ = dalloc<double>(3 * ((size_t)d_.n_tri + (size_t)d_.n_vert));
+    double *tn_dev = normals_dev_, *vn_dev = normals_dev_ + 3 * (size_t)d_.n_tri;
+    launch_normals(d_, tn_dev, vn_dev, stream_);
+    check(hipGetLastError(), "normals launch");
+    check(hipStreamSynchronize(stream_), "normals sync");
+    const ClothModel &m = models_[i];
+    if (vn)
+        check(hipMemcpy(vn, vn_dev + 3 * (size_t)vert_base_[i], 3 * (size_t)m.nv * sizeof(double),
+                        hipMemcpyDeviceToHost),
+              "hipMemcpy normals");
+    if (tn)
+        check(hipMemcpy(tn, tn_dev + 3 * (size_t)tri_base_[i], 3 * (size_t)m.nt * sizeof(double),
+                        hipMemcpyDeviceToHost),
+              "hipMemcpy normals");
 }
 
 void Batch::get_pos(int i, double *x) const { download(i, d_.x, x); }

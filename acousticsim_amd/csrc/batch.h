@@ -73,6 +73,10 @@ public:
     void get_pre_pos(int i, double *x) const;
     void set_pos(int i, const double *x);
     void set_pre_pos(int i, const double *x);
+    // normals at the current positions (cBaseObject::UpdateTriangleNormal /
+    // UpdateVertexNormalFromTriangleNormal): vertex normals 3 nv, triangle
+    // normals 3 nt, in the model's order; either may be null
+    void normals(int i, double *vertex_normals, double *triangle_normals);
     const StepStats &stats(int i) const { return stats_[i]; }
     void set_graphs(bool on) { use_graphs_ = on && !graphs_disabled_; }
     // mean microseconds per launch of the dominant solver kernel over reps
@@ -97,6 +101,8 @@ private:
 
     std::vector<ClothModel> models_;
     std::vector<int> row_base_, slice_base_, nslices_;
+    std::vector<int> vert_base_, tri_base_; // first global vertex / triangle per cloth
+    double *normals_dev_ = nullptr;         // 3 per triangle, then 3 per vertex (lazy)
     std::vector<std::vector<int>> row_of_; // per cloth: vertex -> row within the cloth
     std::vector<StepStats> stats_;
     DeviceBatch d_;

@@ -242,6 +242,16 @@ int bw_set_pre_pos(bw_cloth *c, const double *x)
     });
 }
 
+int bw_get_normals(const bw_cloth *c, double *vn, double *tn)
+{
+    return guard([&] {
+        if (!c)
+            return fail(BW_ERR_ARG, "NULL cloth");
+        c->batch->normals(0, vn, tn);
+        return BW_OK;
+    });
+}
+
 int bw_get_init_pos(const bw_cloth *c, double *x)
 {
     return guard([&] {
@@ -429,6 +439,16 @@ int bw_batch_set_state(bw_batch *b, int i, const double *x, const double *xpre)
             return fail(BW_ERR_ARG, "state has NaN/Inf");
         b->batch->set_pos(i, x);
         b->batch->set_pre_pos(i, xpre);
+        return BW_OK;
+    });
+}
+
+int bw_batch_get_normals(bw_batch *b, int i, double *vn, double *tn)
+{
+    return guard([&] {
+        if (!b || i < 0 || i >= b->batch->size())
+            return fail(BW_ERR_ARG, "bad arguments");
+        b->batch->normals(i, vn, tn);
         return BW_OK;
     });
 }

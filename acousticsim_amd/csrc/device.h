@@ -187,6 +187,9 @@ void launch_cg_step_multi(const DeviceBatch *table, const int2 *spmv_list, int s
                           const int2 *upd_list, int upd_grid, const int2 *grp_list, int grp_grid,
                           bool fp32, int phase, int lpr, int xcd_map, hipStream_t s);
 void launch_cg_timing_setup(const DeviceBatch &d, bool fp32, hipStream_t s);
+// render readback (normals.hip): triangle normals (3 per global triangle)
+// and vertex normals (3 per global vertex) at the current positions
+void launch_normals(const DeviceBatch &d, double *tn, double *vn, hipStream_t s);
 // measurement helpers: set the solver residual to 1 on every row; one SpMV
 // launch of the solver's precision
 void launch_fill_residual(const DeviceBatch &d, bool fp32, hipStream_t s);

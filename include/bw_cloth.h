@@ -116,6 +116,14 @@ int bw_clear_constraint_static_vertices(bw_cloth *c);
    boundary} (E x 5) as built by cTriangulator (Triangulator.cpp:141-390) */
 int bw_get_triangles(const bw_cloth *c, int *tri3);
 int bw_get_edges(const bw_cloth *c, int *edge5);
+/* Render-side readback: cBaseCloth::Update's normal refresh after a substep
+   (BaseCloth.cpp:585-590: cBaseObject::UpdateTriangleNormal and
+   UpdateVertexNormalFromTriangleNormal, BaseObject.cpp:119-160) at the
+   current positions, computed on the device.  vertex_normals (3V doubles):
+   normalised sum of the incident triangles' normals weighted by their rest
+   areas; triangle_normals (3T doubles): normalised (x1-x0) x (x2-x1).
+   Either may be NULL. */
+int bw_get_normals(const bw_cloth *c, double *vertex_normals, double *triangle_normals);
 /* ELL column structure in reference numbering: row v lists, ascending,
    {v} U every interior-edge stencil containing v, as
    cBaraffClothGpu::InitELLMatrix builds it (BaraffClothGpu.cpp:438-500), with
@@ -152,6 +160,8 @@ int bw_batch_num_vertices(const bw_batch *b, int i);
 int bw_batch_get_pos(const bw_batch *b, int i, double *x);
 int bw_batch_set_state(bw_batch *b, int i, const double *x, const double *xpre);
 int bw_batch_get_stats(const bw_batch *b, int i, bw_step_stats *st);
+/* bw_get_normals for member i */
+int bw_batch_get_normals(bw_batch *b, int i, double *vertex_normals, double *triangle_normals);
 /* use HIP graph replay for the per-iteration launch chains (default on) */
 int bw_batch_set_graphs(bw_batch *b, int enable);
 /* Measurement (no reference counterpart): time `reps` back-to-back launches

@@ -1309,3 +1309,55 @@ void oc_get_force(const oc_cloth *c, double *f3, int which)
 }
 void oc_get_rhs(const oc_cloth *c, double *b3) { memcpy(b3, c->b, 3 * c->nv * sizeof(double)); }
 void oc_get_dx(const oc_cloth *c, double *dx3) { memcpy(dx3, c->dx, 3 * c->nv * sizeof(double)); }
+
+/* cBaseObject::UpdateTriangleNormal (BaseObject.cpp:119-131): n_t =
+   normalized((x1 - x0).cross3(x2 - x1)); UpdateVertexNormalFromTriangleNormal
+   (:138-160): n_v = normalized(sum over t in ascending order of n_t * A_t),
+   A_t the float rest area (CalcTriangleInitArea, :291-302).  Eigen's
+   normalized() on the 4-wide vector (w = 0) divides by sqrt of the squared
+   norm summed pairwise, (x^2 + z^2) + (y^2 + w^2), and leaves a zero vector
+   as it is. */
+static void oc_normalize3(double *u)
+{
+    double n2 = (u[0] * u[0] + u[2] * u[2]) + u[1] * u[1];
+    if (n2 > 0)
+    {
+        double n = sqrt(n2);
+        u[0] = u[0] / n;
+        u[1] = u[1] / n;
+        u[2] = u[2] / n;
+    }
+}
+
+void oc_normals(const oc_cloth *c, double *vn3, double *tn3)
+{
+    double *tn = xcalloc(3 * (size_t)c->nt, sizeof(double));
+    for (int t = 0; t < c->nt; t++)
+    {
+        const double *p0 = c->x + 3 * c->tri[3 * t], *p1 = c->x + 3 * c->tri[3 * t + 1];
+        const double *p2 = c->x + 3 * c->tri[3 * t + 2];
+        double a[3], b[3], *n = tn + 3 * t;
+        v3sub(p1, p0, a);
+        v3sub(p2, p1, b);
+        n[0] = a[1] * b[2] - a[2] * b[1];
+        n[1] = a[2] * b[0] - a[0] * b[2];
+        n[2] = a[0] * b[1] - a[1] * b[0];
+        oc_normalize3(n);
+    }
+    if (vn3)
+    {
+        memset(vn3, 0, 3 * (size_t)c->nv * sizeof(double));
+        for (int t = 0; t < c->nt; t++)
+            for (int k = 0; k < 3; k++)
+            {
+                double *v = vn3 + 3 * c->tri[3 * t + k];
+                for (int a = 0; a < 3; a++)
+                    v[a] = v[a] + tn[3 * t + a] * (double)c->area[t];
+            }
+        for (int v = 0; v < c->nv; v++)
+            oc_normalize3(vn3 + 3 * v);
+    }
+    if (tn3)
+        memcpy(tn3, tn, 3 * (size_t)c->nt * sizeof(double));
+    free(tn);
+}

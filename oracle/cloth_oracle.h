@@ -99,6 +99,9 @@ void oc_get_W(const oc_cloth *c, double *w9);
 void oc_get_force(const oc_cloth *c, double *f3, int which); /* 0 total,1 stretch,2 bend */
 void oc_get_rhs(const oc_cloth *c, double *b3);
 void oc_get_dx(const oc_cloth *c, double *dx3); /* last solve's dx */
+/* normals at the current positions (BaseObject.cpp:119-160): triangle
+   normals (3T) and area-weighted vertex normals (3V); either may be NULL */
+void oc_normals(const oc_cloth *c, double *vn3, double *tn3);
 
 /* element-level pieces for finite-difference and closed-form checks */
 void oc_stretch_element(const double *x0, const double *x1, const double *x2,

@@ -79,6 +79,7 @@ def lib():
                   "oc_get_pos", "oc_get_pre_pos", "oc_get_W", "oc_get_rhs", "oc_get_dx"):
             getattr(L, n).argtypes = [vp, dp]
         L.oc_get_stretch_coef.argtypes = [vp, dp, dp]
+        L.oc_normals.argtypes = [vp, dp, dp]
         L.oc_get_bending_rest.argtypes = [vp, dp, dp, dp]
         L.oc_get_pattern.argtypes = [vp, ip, ip]
         L.oc_get_ref_ell.argtypes = [vp, ip, ip]
@@ -239,6 +240,14 @@ class OracleCloth:
         lib().oc_step(self.h, mode, tol, max_iters, ctypes.byref(st))
         return {"iterations": st.iterations, "rel_residual": st.rel_residual,
                 "true_rel_residual": st.true_rel_residual}
+
+    def normals(self):
+        """(vertex normals (V, 3), triangle normals (T, 3)) at the current
+        positions"""
+        vn = np.zeros(3 * self.nv)
+        tn = np.zeros(3 * self.nt)
+        lib().oc_normals(self.h, _dp(vn), _dp(tn))
+        return vn.reshape(-1, 3), tn.reshape(-1, 3)
 
     def assemble(self):
         lib().oc_assemble(self.h)

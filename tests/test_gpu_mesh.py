@@ -127,3 +127,31 @@ def test_delaunay_in_batch_equals_alone(capi):
     mix = capi.ClothBatch([g, conf], [gx, x0])
     mix.step(3)
     assert np.array_equal(alone.get_pos(0), mix.get_pos(1))
+
+
+@pytest.mark.parametrize("kind", ["grid", "delaunay"])
+def test_normals_match_oracle(capi, kind):
+    """Normals read back after 4 steps (cBaseCloth::Update's refresh) equal
+    the oracle's at the same positions to 1e-14."""
+    say("case normals", kind)
+    if kind == "grid":
+        conf = wl.grid_config(21)
+        x0 = wl.jittered_positions(conf, 3)
+        oc = of.OracleCloth(conf)
+    else:
+        conf, pos, tri, x0 = _case(700, 9)
+        oc = of.OracleCloth(conf, mesh=(pos, tri))
+    cl = capi.BaraffClothHip(conf, x0)
+    for _ in range(4):
+        cl.UpdatePos()
+    x = cl.GetPos()
+    oc.set_state(x, x)
+    vn, tn = cl.normals()
+    ovn, otn = oc.normals()
+    say("  max diff vertex %.2e triangle %.2e" % (np.abs(vn - ovn).max(), np.abs(tn - otn).max()))
+    assert np.abs(tn - otn).max() <= 1e-14
+    assert np.abs(vn - ovn).max() <= 1e-14
+    b = capi.ClothBatch([conf], [x0])
+    b.step(4)
+    bvn, btn = b.normals(0, len(tri) if kind != "grid" else oc.nt)
+    assert np.array_equal(bvn, vn) and np.array_equal(btn, tn)
