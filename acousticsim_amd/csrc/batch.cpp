@@ -36,6 +36,9 @@ template <typename T> T *Batch::dalloc(size_t n)
     allocations_.push_back(p);
     device_bytes_ += bytes;
     check(hipMemset(p, 0, bytes), "hipMemset");
+    // the fill runs on the null stream, which does not order work on this
+    // batch's non-blocking stream: finish it before a kernel can write here
+    check(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
     return static_cast<T *>(p);
 }
 

@@ -2,6 +2,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <fstream>
 #include <stdexcept>
 
 namespace bw
@@ -369,6 +371,144 @@ void delaunay_cloth(float width, float height, int n, uint64_t seed, float min_s
     tri.resize(tris.size());
     for (size_t t = 0; t < tris.size(); t++)
         tri[t] = {tris[t][2], tris[t][1], tris[t][0]};
+}
+
+namespace
+{
+// next whitespace-separated token of line from position i
+bool next_token(const std::string &line, size_t &i, std::string &tok)
+{
+    while (i < line.size() && (line[i] == ' ' || line[i] == '\t' || line[i] == '\r'))
+        i++;
+    const size_t b = i;
+    while (i < line.size() && line[i] != ' ' && line[i] != '\t' && line[i] != '\r')
+        i++;
+    tok = line.substr(b, i - b);
+    return !tok.empty();
+}
+
+float obj_real(const std::string &tok, const std::string &path)
+{
+    char *end = nullptr;
+    const double v = std::strtod(tok.c_str(), &end);
+    if (end == tok.c_str())
+        throw std::runtime_error("config: bad number \"" + tok + "\" in " + path);
+    return (float)v;
+}
+
+// tinyobjloader's fixIndex: 1-based, negative = relative to the count so far
+int obj_index(const std::string &tok, int count, const std::string &path)
+{
+    char *end = nullptr;
+    const long v = std::strtol(tok.c_str(), &end, 10);
+    if (end == tok.c_str() || v == 0)
+        throw std::runtime_error("config: bad face index \"" + tok + "\" in " + path);
+    const long r = v > 0 ? v - 1 : count + v;
+    if (r < 0 || r >= count)
+        throw std::runtime_error("config: face index out of range in " + path);
+    return (int)r;
+}
+} // namespace
+
+ObjTriangles read_obj_triangles(const std::string &path)
+{
+    std::ifstream in(path);
+    if (!in)
+        throw std::runtime_error("config: cannot open cloth_obj_path " + path);
+    std::vector<float> v, vt;
+    struct Corner
+    {
+        int v, t;
+    };
+    std::vector<std::array<Corner, 3>> faces;
+    int shapes = 0;
+    size_t faces_at_group = 0;
+    std::string line, key, tok;
+    while (std::getline(in, line))
+    {
+        size_t i = 0;
+        if (!next_token(line, i, key) || key[0] == '#')
+            continue;
+        if (key == "v")
+        {
+            for (int a = 0; a < 3; a++)
+            {
+                if (!next_token(line, i, tok))
+                    throw std::runtime_error("config: short vertex line in " + path);
+                v.push_back(obj_real(tok, path));
+            }
+        }
+        else if (key == "vt")
+        {
+            for (int a = 0; a < 2; a++)
+                vt.push_back(next_token(line, i, tok) ? obj_real(tok, path) : 0.0f);
+        }
+        else if (key == "o" || key == "g")
+        {
+            if (faces.size() > faces_at_group)
+                shapes++;
+            faces_at_group = faces.size();
+        }
+        else if (key == "f")
+        {
+            std::vector<Corner> cs;
+            while (next_token(line, i, tok))
+            {
+                const size_t s1 = tok.find('/');
+                Corner c{obj_index(tok.substr(0, s1), (int)(v.size() / 3), path), -1};
+                if (s1 != std::string::npos)
+                {
+                    const size_t s2 = tok.find('/', s1 + 1);
+                    const std::string t = tok.substr(s1 + 1, s2 == std::string::npos
+                                                                 ? std::string::npos
+                                                                 : s2 - s1 - 1);
+                    if (!t.empty())
+                        c.t = obj_index(t, (int)(vt.size() / 2), path);
+                }
+                cs.push_back(c);
+            }
+            if (cs.size() < 3)
+                continue; // degenerate face, skipped as tinyobjloader does
+            if (cs.size() > 3)
+                throw std::runtime_error("config: " + path +
+                                         " has a face with more than 3 vertices (triangles only)");
+            faces.push_back({cs[0], cs[1], cs[2]});
+        }
+    }
+    if (faces.size() > faces_at_group)
+        shapes++;
+    if (shapes != 1)
+        throw std::runtime_error("config: " + path + " must hold exactly one shape of faces");
+    ObjTriangles out;
+    int nv = 0;
+    for (auto &f : faces)
+        for (auto &c : f)
+            nv = std::max(nv, c.v + 1);
+    std::vector<char> used(nv, 0);
+    out.pos.assign(3 * (size_t)nv, 0.0);
+    out.uv.assign(2 * (size_t)nv, 0.0f);
+    out.tri.reserve(faces.size());
+    for (auto &f : faces)
+    {
+        for (auto &c : f)
+        {
+            if (!used[c.v])
+                for (int a = 0; a < 3; a++)
+                    out.pos[3 * (size_t)c.v + a] = v[3 * (size_t)c.v + a];
+            used[c.v] = 1;
+            if (c.t >= 0)
+            {
+                out.uv[2 * (size_t)c.v] = vt[2 * (size_t)c.t];
+                out.uv[2 * (size_t)c.v + 1] = vt[2 * (size_t)c.t + 1];
+            }
+        }
+        out.tri.push_back({f[0].v, f[1].v, f[2].v});
+    }
+    for (int k = 0; k < nv; k++)
+        if (!used[k])
+            throw std::runtime_error("config: vertex " + std::to_string(k) + " of " + path +
+                                     " is in no face");
+    return out;
 }
 
 } // namespace bw

@@ -5,6 +5,7 @@
 #pragma once
 #include <array>
 #include <cstdint>
+#include <string>
 #include <vector>
 
 namespace bw
@@ -34,6 +35,24 @@ std::vector<std::array<int, 3>> delaunay(const std::vector<std::array<float, 2>>
 // normal -z after the swap).
 void delaunay_cloth(float width, float height, int n, uint64_t seed, float min_spacing,
                     std::vector<double> &xyz, std::vector<std::array<int, 3>> &tri);
+
+// Triangles of a Wavefront OBJ file as cObjUtil::LoadObj reads them through
+// tinyobjloader 2.0.0 (ObjUtil.cpp:10-130): positions "v x y z" and texture
+// coordinates "vt u v" stored as float values, faces "f" with 1-based or
+// negative (relative) indices in the forms i, i/t, i//n, i/t/n.  Vertex ids
+// are the OBJ position indices of the faces; every id up to the largest one
+// used must appear in some face, and a vertex's texture coordinate is the
+// one of its last face corner that names one (0, 0 if none).  Throws on
+// faces with more than three corners (the reference accepts triangles
+// only), on more than one shape ("o"/"g" groups holding faces), and on bad
+// indices or files.
+struct ObjTriangles
+{
+    std::vector<double> pos;               // 3 nv, float values
+    std::vector<float> uv;                 // 2 nv
+    std::vector<std::array<int, 3>> tri;   // nt
+};
+ObjTriangles read_obj_triangles(const std::string &path);
 
 // Vertex order along a Hilbert curve over the points' bounding box
 // (xy2: 2 floats per point); returns the permutation (order[k] = point).

@@ -25,9 +25,20 @@ ClothConfig ClothConfig::from_json(const Json &j)
     if (!(c.delaunay_min_spacing >= 0))
         throw std::runtime_error("config: delaunay_min_spacing must be >= 0");
     c.pin_top_edge = j.has("pin_top_edge") && j.at("pin_top_edge").as_bool();
-    if (j.has("enable_cloth_from_obj") && j.at("enable_cloth_from_obj").as_bool())
-        throw std::runtime_error("config: enable_cloth_from_obj is not supported");
-    const bool mesh = c.geometry_type == "mesh"; // sizes come with the mesh arrays
+    c.from_obj = j.has("enable_cloth_from_obj") && j.at("enable_cloth_from_obj").as_bool();
+    if (c.from_obj)
+    {
+        c.obj_path = j.at("cloth_obj_path").as_string();
+        c.relocate_obj = j.has("enable_relocate_obj") && j.at("enable_relocate_obj").as_bool();
+        if (j.has("cloth_obj_translation"))
+        {
+            auto t = j.at("cloth_obj_translation").as_vector();
+            for (size_t k = 0; k < 3 && k < t.size(); k++)
+                c.obj_translation[k] = t[k];
+        }
+    }
+    // sizes come with the mesh arrays or the OBJ file
+    const bool mesh = c.geometry_type == "mesh" || c.from_obj;
     auto size = mesh && !j.has("cloth_size") ? std::vector<double>{1.0, 1.0}
                                              : j.at("cloth_size").as_vector();
     auto sub = mesh && !j.has("subdivision") ? std::vector<double>{1.0, 1.0}
@@ -219,6 +230,106 @@ void make_delaunay(const ClothConfig &c, ClothModel &m)
     m.nt = (int)m.tri.size();
     edges_from_triangles(m);
     apply_init_transform(c, m);
+}
+
+// Edges as cObjUtil::BuildEdge files them (ObjUtil.cpp:141-224): every
+// side {min, max} of every triangle, tri0 = the first triangle holding it,
+// tri1 = the second; edges numbered in ascending (min, max) order (the order
+// of the std::map the reference keys them by).
+void edges_sorted_pairs(ClothModel &m)
+{
+    const size_t n3 = 3 * (size_t)m.nt;
+    std::vector<std::pair<uint64_t, uint32_t>> rec(n3);
+    for (int t = 0; t < m.nt; t++)
+        for (int k = 0; k < 3; k++)
+        {
+            const int a = m.tri[t][k], b = m.tri[t][(k + 1) % 3];
+            if (a == b)
+                throw std::runtime_error("config: degenerate triangle in the mesh");
+            const uint64_t key = ((uint64_t)std::min(a, b) << 32) | (uint32_t)std::max(a, b);
+            rec[3 * (size_t)t + k] = {key, (uint32_t)(3 * t + k)};
+        }
+    std::sort(rec.begin(), rec.end());
+    m.edges.clear();
+    for (size_t i = 0; i < n3;)
+    {
+        size_t j = i + 1;
+        while (j < n3 && rec[j].first == rec[i].first)
+            j++;
+        if (j - i > 2)
+            throw std::runtime_error("config: mesh edge shared by more than two triangles");
+        Edge ed;
+        ed.v0 = (int)(rec[i].first >> 32);
+        ed.v1 = (int)(rec[i].first & 0xffffffffu);
+        ed.t0 = (int)(rec[i].second / 3);
+        ed.t1 = j - i == 2 ? (int)(rec[i + 1].second / 3) : -1;
+        ed.boundary = j - i == 1;
+        m.edges.push_back(ed);
+        i = j;
+    }
+    m.ne = (int)m.edges.size();
+    triangle_edge_ids(m);
+}
+
+// enable_cloth_from_obj (BaseCloth.cpp:299-357): the OBJ's triangles; per
+// vertex y += 0.2, uv *= 10/3 (in float), position += cloth_obj_translation;
+// with enable_relocate_obj the top moves to y = 0.27 and x, z are centred
+// on the running mean of the vertices within 1 cm of the top (MoveObjPos,
+// BaseCloth.cpp:237-279).  No init transform or UV rotation, as in the
+// reference's OBJ branch.
+void make_obj(const ClothConfig &c, ClothModel &m)
+{
+    ObjTriangles obj = read_obj_triangles(c.obj_path);
+    m.nv = (int)(obj.pos.size() / 3);
+    m.nt = (int)obj.tri.size();
+    m.tri = obj.tri;
+    m.xrest = obj.pos;
+    m.uv = obj.uv;
+    const float uv_scale = (float)(10 / 3.0);
+    for (int v = 0; v < m.nv; v++)
+    {
+        double *p = &m.xrest[3 * (size_t)v];
+        p[1] += 0.2;
+        m.uv[2 * (size_t)v] *= uv_scale;
+        m.uv[2 * (size_t)v + 1] *= uv_scale;
+        for (int a = 0; a < 3; a++)
+            p[a] += c.obj_translation[a];
+    }
+    if (c.relocate_obj)
+    {
+        float highest = -1e3f;
+        for (int v = 0; v < m.nv; v++)
+            if (m.xrest[3 * (size_t)v + 1] > highest)
+                highest = (float)m.xrest[3 * (size_t)v + 1];
+        const double given = 0.27, shift = given - highest;
+        for (int v = 0; v < m.nv; v++)
+            m.xrest[3 * (size_t)v + 1] += shift;
+        double o[4] = {0, 0, 0, 0};
+        int count = 0;
+        for (int v = 0; v < m.nv; v++)
+        {
+            const double *p = &m.xrest[3 * (size_t)v];
+            if ((given - p[1]) < 0.01)
+            {
+                const double q[4] = {p[0], p[1], p[2], 1.0};
+                for (int a = 0; a < 4; a++)
+                    o[a] = (o[a] * count + q[a]) / (count + 1);
+                count += 1;
+            }
+        }
+        for (int v = 0; v < m.nv; v++)
+        {
+            m.xrest[3 * (size_t)v] -= o[0];
+            m.xrest[3 * (size_t)v + 2] -= o[2];
+        }
+    }
+    m.flat_xy.resize(2 * (size_t)m.nv);
+    for (int v = 0; v < m.nv; v++)
+    {
+        m.flat_xy[2 * v] = (float)m.xrest[3 * (size_t)v];
+        m.flat_xy[2 * v + 1] = (float)m.xrest[3 * (size_t)v + 1];
+    }
+    edges_sorted_pairs(m);
 }
 
 void make_grid(const ClothConfig &c, ClothModel &m)
@@ -619,7 +730,9 @@ ClothModel ClothModel::build(const ClothConfig &cfg)
 {
     ClothModel m;
     m.cfg = cfg;
-    if (cfg.geometry_type == "delaunay")
+    if (cfg.from_obj)
+        make_obj(cfg, m);
+    else if (cfg.geometry_type == "delaunay")
         make_delaunay(cfg, m);
     else if (cfg.geometry_type == "regular_triangle")
         make_grid(cfg, m);

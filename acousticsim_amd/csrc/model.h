@@ -44,6 +44,12 @@ struct ClothConfig
     uint64_t delaunay_seed = 0; // "delaunay_seed" (this library's key)
     double delaunay_min_spacing = 0; // "delaunay_min_spacing"; 0: the reference's rule
     bool pin_top_edge = false;  // "pin_top_edge" (this library's key)
+    // OBJ cloth (BaseCloth.cpp:299-357): "enable_cloth_from_obj",
+    // "cloth_obj_path", "enable_relocate_obj", "cloth_obj_translation"
+    bool from_obj = false;
+    std::string obj_path;
+    bool relocate_obj = false;
+    double obj_translation[3] = {0, 0, 0};
     Precision precision = Precision::Mixed;
     SolveMode solve_mode = SolveMode::Tolerance;
     double tolerance = 1e-6;

@@ -61,6 +61,7 @@ Server::Server(const ServerOptions &opt) : opt_(opt), t0_(std::chrono::steady_cl
     check(hipHostMalloc((void **)&table_host_, opt_.max_resident * sizeof(DeviceBatch)),
           "hipHostMalloc");
     check(hipMemset(states_dev_, 0, ns * sizeof(CgState)), "hipMemset");
+    check(hipStreamSynchronize(nullptr), "hipStreamSynchronize"); // before the server stream
     for (int s = opt_.max_resident - 1; s >= 0; s--)
         free_slots_.push_back(s);
     for (int k = 0; k < std::max(1, opt_.builders); k++)

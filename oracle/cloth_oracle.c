@@ -208,7 +208,7 @@ static void euler_xyz(const double *e, double R[3][3])
             R[i][j] = (T[i][0] * Rx[0][j] + T[i][1] * Rx[1][j]) + T[i][2] * Rx[2][j];
 }
 
-static void finish_geometry(oc_cloth *c, const oc_params *p);
+static void finish_geometry(oc_cloth *c, const oc_params *p, const double *uv_given);
 
 static int build_grid(oc_cloth *c, const oc_params *p)
 {
@@ -313,13 +313,13 @@ static int build_grid(oc_cloth *c, const oc_params *p)
     for (int k = 0; k < c->ne; k++)
         if (c->edge[5 * k] == -2)
             return -2;
-    finish_geometry(c, p);
+    finish_geometry(c, p, NULL);
     return 0;
 }
 
 /* triangle -> edge ids, init transform and UV, for any triangle mesh whose
    untransformed rest positions, triangles and edges are set */
-static void finish_geometry(oc_cloth *c, const oc_params *p)
+static void finish_geometry(oc_cloth *c, const oc_params *p, const double *uv_given)
 {
     c->flat_y = xcalloc(c->nv, sizeof(float));
     /* BuildTriangleEdgeId: (id0,id1), (id1,id2), (id2,id0) */
@@ -340,6 +340,17 @@ static void finish_geometry(oc_cloth *c, const oc_params *p)
         c->tri_e[3 * t + 2] = find_edge(ep, c->ne, v[2], v[0]);
     }
     free(ep);
+    if (uv_given)
+    {
+        /* OBJ cloth: no init transform, UV from the file */
+        for (int v = 0; v < c->nv; v++)
+        {
+            c->uv[2 * v] = (float)uv_given[2 * v];
+            c->uv[2 * v + 1] = (float)uv_given[2 * v + 1];
+            c->flat_y[v] = (float)c->xrest[3 * v + 1];
+        }
+        return;
+    }
 
     /* init transform: mPos = TransformMat(pos, euler) * mPos */
     int identity = p->init_pos[0] == 0 && p->init_pos[1] == 0 && p->init_pos[2] == 0 &&
@@ -737,6 +748,15 @@ static int first_cmp(const void *p, const void *q)
 
 oc_cloth *oc_create_mesh(const oc_params *p, const double *xyz, int nv, const int *tri, int nt)
 {
+    return oc_create_mesh2(p, xyz, nv, tri, nt, NULL, 0);
+}
+
+/* ObjUtil.cpp:141-224 (cObjUtil::BuildEdge): edges keyed by {min, max} in a
+   std::map, so numbered in ascending key order; tri0/tri1 = the first and
+   second triangle (in triangle order) holding the edge */
+oc_cloth *oc_create_mesh2(const oc_params *p, const double *xyz, int nv, const int *tri, int nt,
+                          const double *uv, int sorted_edges)
+{
     if (nv < 3 || nt < 1)
         return NULL;
     for (int k = 0; k < 3 * nt; k++)
@@ -784,7 +804,8 @@ oc_cloth *oc_create_mesh(const oc_params *p, const double *xyz, int nv, const in
         oc_destroy(c);
         return NULL;
     }
-    qsort(grp, ng, 2 * sizeof(int), first_cmp);
+    if (!sorted_edges) /* groups are in key order already */
+        qsort(grp, ng, 2 * sizeof(int), first_cmp);
     c->ne = ng;
     c->edge = xcalloc(5 * (size_t)ng, sizeof(int));
     for (int e = 0; e < ng; e++)
@@ -800,7 +821,7 @@ oc_cloth *oc_create_mesh(const oc_params *p, const double *xyz, int nv, const in
     }
     free(sd);
     free(grp);
-    finish_geometry(c, p);
+    finish_geometry(c, p, uv);
     return complete(c, p);
 }
 

@@ -56,6 +56,12 @@ oc_cloth *oc_create_grid(const oc_params *p);
    cloth_size and subdivision are not read.  NULL for an index out of range
    or an edge with more than two triangles. */
 oc_cloth *oc_create_mesh(const oc_params *p, const double *xyz, int nv, const int *tri, int nt);
+/* the same with, when uv is given (2 nv float values), no init transform
+   and that UV (the OBJ cloth path, BaseCloth.cpp:299-357), and with
+   sorted_edges, edges numbered in ascending {min, max} order
+   (cObjUtil::BuildEdge, ObjUtil.cpp:141-224) */
+oc_cloth *oc_create_mesh2(const oc_params *p, const double *xyz, int nv, const int *tri, int nt,
+                          const double *uv, int sorted_edges);
 void oc_destroy(oc_cloth *c);
 
 int oc_num_vertices(const oc_cloth *c);

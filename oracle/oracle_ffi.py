@@ -68,6 +68,9 @@ def lib():
         L.oc_create_grid.argtypes = [P(OcParams)]
         L.oc_create_mesh.restype = vp
         L.oc_create_mesh.argtypes = [P(OcParams), dp, ctypes.c_int, ip, ctypes.c_int]
+        L.oc_create_mesh2.restype = vp
+        L.oc_create_mesh2.argtypes = [P(OcParams), dp, ctypes.c_int, ip, ctypes.c_int, dp,
+                                      ctypes.c_int]
         L.oc_destroy.argtypes = [vp]
         for n in ("oc_num_vertices", "oc_num_triangles", "oc_num_edges",
                   "oc_num_pins", "oc_ell_nnzb"):
@@ -135,6 +138,81 @@ def params_from_config(conf):
     return p
 
 
+def read_obj_triangles(path):
+    """Positions (nv, 3; float values), UV (nv, 2; float32) and triangles of
+    an OBJ file as cObjUtil::LoadObj takes them from tinyobjloader 2.0.0
+    (ObjUtil.cpp:10-130): "v" and "vt" stored as float, face corners i,
+    i/t, i//n or i/t/n with 1-based or negative (relative) indices; vertex
+    ids are the position indices used by faces, all of 0..max used; a
+    vertex's UV is that of its last corner naming one.  Triangles only."""
+    v, vt, faces = [], [], []
+    for line in open(path):
+        tok = line.split()
+        if not tok or tok[0].startswith("#"):
+            continue
+        if tok[0] == "v":
+            v.append([np.float32(float(x)) for x in tok[1:4]])
+        elif tok[0] == "vt":
+            vt.append([np.float32(float(x)) for x in (tok[1:3] + ["0", "0"])[:2]])
+        elif tok[0] == "f":
+            corners = []
+            for c in tok[1:]:
+                parts = c.split("/")
+                iv = int(parts[0])
+                iv = iv - 1 if iv > 0 else len(v) + iv
+                it = -1
+                if len(parts) > 1 and parts[1]:
+                    it = int(parts[1])
+                    it = it - 1 if it > 0 else len(vt) + it
+                corners.append((iv, it))
+            if len(corners) == 3:
+                faces.append(corners)
+            elif len(corners) > 3:
+                raise ValueError("triangles only")
+    nv = 1 + max(iv for f in faces for iv, _ in f)
+    pos = np.zeros((nv, 3))
+    uv = np.zeros((nv, 2), dtype=np.float32)
+    used = np.zeros(nv, bool)
+    for f in faces:
+        for iv, it in f:
+            pos[iv] = np.array(v[iv], dtype=np.float64)
+            used[iv] = True
+            if it >= 0:
+                uv[iv] = vt[it]
+    if not used.all():
+        raise ValueError("vertex in no face")
+    tri = np.array([[iv for iv, _ in f] for f in faces], dtype=np.int32)
+    return pos, uv, tri
+
+
+def obj_cloth_geometry(conf):
+    """The OBJ cloth of cBaseCloth::InitGeometry (BaseCloth.cpp:299-357):
+    y += 0.2, uv *= float(10/3), position += cloth_obj_translation, then with
+    enable_relocate_obj MoveObjPos (BaseCloth.cpp:237-279): top moved to
+    y = 0.27 (top found in float), x and z centred on the running mean of
+    the vertices within 1 cm of the top.  Returns (xyz, tri, uv)."""
+    pos, uv, tri = read_obj_triangles(conf["cloth_obj_path"])
+    tr = np.array((list(conf.get("cloth_obj_translation", [0, 0, 0])) + [0, 0, 0])[:3], float)
+    pos[:, 1] += 0.2
+    uv = uv * np.float32(10 / 3.0)
+    pos = pos + tr
+    if conf.get("enable_relocate_obj"):
+        top = np.float32(-1e3)
+        for y in pos[:, 1]:
+            if y > top:
+                top = np.float32(y)
+        pos[:, 1] += 0.27 - float(top)
+        o = np.zeros(4)
+        count = 0
+        for p in pos:
+            if (0.27 - p[1]) < 0.01:
+                o = (o * count + np.array([p[0], p[1], p[2], 1.0])) / (count + 1)
+                count += 1
+        pos[:, 0] -= o[0]
+        pos[:, 2] -= o[2]
+    return pos, tri, uv
+
+
 class OracleCloth:
     """One cloth held by the C oracle: the regular grid of `conf`, or, with
     `mesh` = (rest positions (nv, 3), triangles (nt, 3)), a cloth on that
@@ -143,7 +221,14 @@ class OracleCloth:
     def __init__(self, conf, mesh=None):
         L = lib()
         self._params = params_from_config(conf)
-        if mesh is None:
+        if mesh is None and conf.get("enable_cloth_from_obj"):
+            xyz, tri, uv = obj_cloth_geometry(conf)
+            uv = np.ascontiguousarray(uv, dtype=np.float64).reshape(-1)
+            xyz = np.ascontiguousarray(xyz).reshape(-1)
+            tri = np.ascontiguousarray(tri, dtype=np.int32).reshape(-1)
+            self.h = L.oc_create_mesh2(ctypes.byref(self._params), _dp(xyz), xyz.size // 3,
+                                       _ip(tri), tri.size // 3, _dp(uv), 1)
+        elif mesh is None:
             self.h = L.oc_create_grid(ctypes.byref(self._params))
         else:
             xyz = np.ascontiguousarray(np.asarray(mesh[0], dtype=np.float64).reshape(-1))

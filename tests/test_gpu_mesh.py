@@ -159,3 +159,47 @@ def test_normals_match_oracle(capi, kind, monkeypatch):
     b.step(4)
     bvn, btn = b.normals(0, len(tri) if kind != "grid" else oc.nt)
     assert np.array_equal(bvn, vn) and np.array_equal(btn, tn)
+
+
+@pytest.mark.parametrize("relocate", [False, True])
+def test_obj_cloth_matches_oracle(capi, tmp_path, relocate):
+    """enable_cloth_from_obj: the library's OBJ reader and the reference's
+    OBJ-cloth transforms and BuildEdge numbering against the oracle's
+    restatement; 5 tolerance steps within 1e-4 of the displacement."""
+    import obj_fixture as fx
+
+    say("case OBJ cloth, relocate", relocate)
+    path = tmp_path / "sheet.obj"
+    fx.write_obj(path)
+    conf = fx.obj_config(path, relocate=relocate)
+    oc = of.OracleCloth(conf)
+    cl = capi.BaraffClothHip(conf)
+    assert np.array_equal(cl.triangles(), oc.triangles())
+    assert np.array_equal(cl.edges(), oc.edges())
+    assert np.array_equal(np.sort(cl.GetConstraintStaticVertices()), np.sort(oc.pins()))
+    rp, cols = cl.ell_map()
+    orp, ocols = oc.ref_ell()
+    assert np.array_equal(rp, orp) and np.array_equal(cols, ocols)
+    x0 = cl.GetInitPos()
+    assert np.array_equal(x0, oc.rest_pos())
+    oc.set_state(x0, x0)
+    for _ in range(5):
+        cl.UpdatePos()
+        oc.step(of.SOLVE_TOLERANCE, 1e-12)
+        assert cl.last_stats()["rel_residual"] <= 1e-6
+    d = np.abs(oc.pos() - x0).max()
+    err = np.abs(cl.GetPos() - oc.pos()).max()
+    say("  err/disp %.3e" % (err / d))
+    assert err <= 1e-4 * d
+
+
+def test_obj_errors_are_reported(capi, tmp_path):
+    import obj_fixture as fx
+
+    say("case OBJ errors")
+    quad = tmp_path / "quad.obj"
+    quad.write_text("v 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\nf 1 2 3 4\n")
+    for path in (quad, tmp_path / "missing.obj"):
+        with pytest.raises(capi.BwError) as e:
+            capi.BaraffClothHip(fx.obj_config(path))
+        assert e.value.code == -1
