@@ -109,6 +109,8 @@ Batch::Batch(std::vector<ClothModel> models, const BatchOptions &opt)
     if (models_.empty())
         throw std::runtime_error("batch: no cloths");
     auto kind_of = [](const ClothModel &m) {
+        if (m.cfg.gpu_form)
+            return SolverKind::GpuForm;
         if (m.cfg.solve_mode == SolveMode::Reference)
             return SolverKind::RefPCG64;
         return m.cfg.precision == Precision::Mixed ? SolverKind::CG32 : SolverKind::CG64;
@@ -135,6 +137,10 @@ Batch::Batch(std::vector<ClothModel> models, const BatchOptions &opt)
           "hipHostMalloc");
     check(hipHostMalloc((void **)&host_cg_, 2 * sizeof(CgState) * models_.size()),
           "hipHostMalloc");
+    check(hipHostMalloc((void **)&host_gf_, 2 * sizeof(GfState) * models_.size()),
+          "hipHostMalloc");
+    if (kind_ == SolverKind::GpuForm)
+        tuned_ = true; // one SpMV form
     build_layout();
 }
 
@@ -150,6 +156,8 @@ Batch::~Batch()
         (void)hipHostFree(host_state_);
     if (host_cg_)
         (void)hipHostFree(host_cg_);
+    if (host_gf_)
+        (void)hipHostFree(host_gf_);
     if (host_spmv_list_)
         (void)hipHostFree(host_spmv_list_);
     if (host_upd_list_)
@@ -338,6 +346,8 @@ void Batch::build_layout()
         P.stall = mixed ? 1e-4 : 0.0;
         if (const char *e = std::getenv("BW_STALL"))
             P.stall = mixed ? std::atof(e) : 0.0;
+        P.gf_a = m.cfg.rayleigh_a;
+        P.gf_b = m.cfg.rayleigh_b;
         P.blk_begin = slice_base_[i] / BW_SB;
         P.blk_count = nslices_[i] / BW_SB;
         P.slice_begin = slice_base_[i];
@@ -400,6 +410,45 @@ void Batch::build_layout()
         d_.q64 = dalloc<double>(nv3);
         d_.partial = dalloc<double>(PARTS * (size_t)d_.n_blocks_wg);
         d_.state = dalloc<InstState>(n);
+    }
+    else if (kind_ == SolverKind::GpuForm)
+    {
+        d_.r32 = dalloc<float>(nv3);
+        d_.q32 = dalloc<float>(nv3);
+        d_.d32[0] = dalloc<float>(nv3);
+        d_.d32[1] = dalloc<float>(nv3);
+        d_.cg_partial = dalloc<double>(2 * PARTS * (size_t)d_.n_slices);
+        d_.cg_group = dalloc<double>(2 * PARTS * (size_t)d_.n_groups);
+        d_.group_inst = dalloc<int>(d_.n_groups);
+        upload(d_.group_inst, group_inst);
+        d_.gf_qb = dalloc<float>(entries);
+        d_.gf_tri = dalloc<float>(90 * (size_t)ntri);
+        d_.gf_pinv = dalloc<float>(9 * (size_t)d_.n_rows);
+        d_.gf_dx = dalloc<float>(nv3);
+        d_.gf_target = dalloc<float>(nv3);
+        d_.gf_mass = dalloc<float>(d_.n_rows);
+        d_.gf_grav = dalloc<float>(d_.n_rows);
+        d_.gf_state = dalloc<GfState>(2 * (size_t)n);
+        std::vector<float> qb(entries, 0.0f), target(nv3, 0.0f), mass(d_.n_rows, 0.0f),
+            grav(d_.n_rows, 0.0f);
+        for (int i = 0; i < n; i++)
+        {
+            const ClothModel &m = models_[i];
+            for (int v = 0; v < m.nv; v++)
+            {
+                const int p = row_base_[i] + row_of_[i][v], s = p / SLICE, l = p % SLICE;
+                for (int k = m.row_ptr[v]; k < m.row_ptr[v + 1]; k++)
+                    qb[slice_off[s] + (size_t)(k - m.row_ptr[v]) * SLICE + l] = m.qb[k];
+                for (int a = 0; a < 3; a++)
+                    target[3 * (size_t)p + a] = (float)m.xrest[3 * (size_t)v + a];
+                mass[p] = m.mass32[v];
+                grav[p] = m.grav32[v];
+            }
+        }
+        upload(d_.gf_qb, qb);
+        upload(d_.gf_target, target);
+        upload(d_.gf_mass, mass);
+        upload(d_.gf_grav, grav);
     }
     else
     {
@@ -567,6 +616,8 @@ double Batch::time_solver_kernel(int reps, double *alg_bytes)
 {
     if (reps < 2)
         throw std::runtime_error("batch: reps must be >= 2");
+    if (kind_ == SolverKind::GpuForm)
+        throw std::runtime_error("batch: kernel timing covers the plain method only");
     // the system at the current state (W and the Jacobi diagonal)
     launch_elements(d_, stream_);
     launch_assemble_rows(d_, stream_);
@@ -728,6 +779,38 @@ int slice_bucket(int n, int all)
 }
 } // namespace
 
+void Batch::solve_gpu_form()
+{
+    launch_gf_assemble(d_, stream_);
+    launch_gf_solve(d_, stream_);
+    check(hipGetLastError(), "gpu-form launch");
+    check(hipMemcpyAsync(host_gf_, d_.gf_state, 2 * sizeof(GfState) * models_.size(),
+                         hipMemcpyDeviceToHost, stream_),
+          "hipMemcpyAsync state");
+    check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    for (int i = 0; i < size(); i++)
+    {
+        const GfState &s = host_gf_[2 * i + 1];
+        StepStats &st = stats_[i];
+        st.pcg_iterations = s.it;
+        st.outer_iterations = 0;
+        // the recursive residual (the reference checks it, absolute, every
+        // 10 iterations)
+        st.rel_residual = s.bb > 0 ? std::sqrt(s.rr / s.bb) : std::sqrt(s.rr);
+        st.matrix_fp64 = 0;
+    }
+}
+
+void Batch::clear_warm_start(int i)
+{
+    if (kind_ != SolverKind::GpuForm)
+        return;
+    check(hipMemsetAsync(d_.gf_dx + 3 * (size_t)row_base_[i], 0,
+                         3 * (size_t)nslices_[i] * SLICE * sizeof(float), stream_),
+          "hipMemsetAsync");
+    check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+}
+
 void Batch::solve_ref()
 {
     launch_reset_states(d_, stream_);
@@ -868,8 +951,8 @@ void Batch::solve_cg()
 
 void Batch::begin_step()
 {
-    if (kind_ == SolverKind::RefPCG64)
-        throw std::runtime_error("batch: reference-semantics batches step through step()");
+    if (kind_ == SolverKind::RefPCG64 || kind_ == SolverKind::GpuForm)
+        throw std::runtime_error("batch: reference-semantics and fem_gpu batches step through step()");
     launch_elements(d_, stream_);
     launch_assemble_rows(d_, stream_);
     launch_cg_reset(d_, stream_);
@@ -901,7 +984,7 @@ void Batch::end_step(const CgState *states, double step_us)
 // BW_SPMV_LPR=1|2|4 in the environment fixes the choice instead.
 int Batch::autotune()
 {
-    if (kind_ == SolverKind::RefPCG64)
+    if (kind_ == SolverKind::RefPCG64 || kind_ == SolverKind::GpuForm)
         return d_.spmv_lpr;
     for (auto &g : graphs_) // graphs hold the previous choice
         (void)hipGraphExecDestroy(g.exec);
@@ -940,13 +1023,18 @@ void Batch::step(int nsteps)
     for (int n = 0; n < nsteps; n++)
     {
         const auto t0 = std::chrono::steady_clock::now();
-        launch_elements(d_, stream_);
-        launch_assemble_rows(d_, stream_);
-        if (kind_ == SolverKind::RefPCG64)
-            solve_ref();
+        if (kind_ == SolverKind::GpuForm)
+            solve_gpu_form(); // assembly, solve and position update
         else
-            solve_cg();
-        launch_update_positions(d_, stream_);
+        {
+            launch_elements(d_, stream_);
+            launch_assemble_rows(d_, stream_);
+            if (kind_ == SolverKind::RefPCG64)
+                solve_ref();
+            else
+                solve_cg();
+            launch_update_positions(d_, stream_);
+        }
         check(hipGetLastError(), "step launch");
         check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
         const double us =

@@ -26,7 +26,8 @@ enum class SolverKind
 {
     RefPCG64,
     CG32,
-    CG64
+    CG64,
+    GpuForm // cloth_type "fem_gpu": the reference's GPU form (gpuform.hip)
 };
 
 // Optional resources a Batch borrows instead of creating: the stream its
@@ -73,6 +74,9 @@ public:
     void get_pre_pos(int i, double *x) const;
     void set_pos(int i, const double *x);
     void set_pre_pos(int i, const double *x);
+    // GPU form: forget the warm start (the reference zeroes its solution on
+    // Reset, BaraffClothGpu.cpp:293-299)
+    void clear_warm_start(int i);
     // normals at the current positions (cBaseObject::UpdateTriangleNormal /
     // UpdateVertexNormalFromTriangleNormal): vertex normals 3 nv, triangle
     // normals 3 nt, in the model's order; either may be null
@@ -115,6 +119,8 @@ private:
     SolverKind kind_ = SolverKind::CG32;
     InstState *host_state_ = nullptr; // pinned, reference PCG state
     CgState *host_cg_ = nullptr;      // pinned, 2 per cloth
+    GfState *host_gf_ = nullptr;      // pinned, 2 per cloth
+    void solve_gpu_form();
     bool use_graphs_ = true;
     bool graphs_disabled_ = false; // BW_GRAPHS=0 in the environment
     int last_launches_ = 0;

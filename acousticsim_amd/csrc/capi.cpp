@@ -178,6 +178,7 @@ int bw_reset(bw_cloth *c)
             return fail(BW_ERR_ARG, "NULL cloth");
         c->batch->set_pos(0, c->init_pos.data());
         c->batch->set_pre_pos(0, c->init_pos.data());
+        c->batch->clear_warm_start(0);
         return BW_OK;
     });
 }

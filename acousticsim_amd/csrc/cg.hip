@@ -633,6 +633,12 @@ void launch_spmv_multi(const DeviceBatch *table, const int2 *list, int grid, int
                            xcd_map);
 }
 
+void launch_group_sums(const DeviceBatch &d, int slot, hipStream_t s)
+{
+    if (d.n_groups > 0)
+        hipLaunchKernelGGL(k_cg_group, dim3(d.n_groups), dim3(64), 0, s, d, slot);
+}
+
 void launch_cg_spmv_only(const DeviceBatch &d, bool fp32, hipStream_t s)
 {
     if (fp32)

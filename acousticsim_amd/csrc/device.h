@@ -55,6 +55,7 @@ struct InstParams
     // stall * (r.r at the last replacement) switches the cloth's SpMV to the
     // FP64 matrix for the rest of the solve (0: never)
     double stall;
+    float gf_a, gf_b; // GPU form: Rayleigh a, b as the reference's float members
 };
 
 // Per-cloth state of the reference-semantics PCG (pcg.hip).  Each field has
@@ -88,6 +89,16 @@ struct CgState
     int force_check; // 1: form a true residual next
     int final_check; // 1: stop after the next true residual
     int w64;         // 1: the SpMV reads the FP64 matrix (mixed form, after a stall)
+};
+
+// Per-cloth state of the GPU-form PCG (gpuform.hip); two slots like CgState
+struct GfState
+{
+    double rmr; // r.(M^-1 r) of the current direction
+    double rr;  // r.r after the last update
+    double bb;  // b.b over unpinned rows
+    int it;     // iterations done
+    int done;
 };
 
 struct DeviceBatch
@@ -141,6 +152,13 @@ struct DeviceBatch
     int n_groups = 0;
     float *r32 = nullptr, *z32 = nullptr, *q32 = nullptr, *d32[2] = {nullptr, nullptr};
     double *z64 = nullptr;
+    // GPU form (gpuform.hip): bending scalar per ELL entry, per-triangle
+    // FP32 force and Hessian (90), block-Jacobi inverse (9 per row), the
+    // persistent FP32 solution, pinned targets, FP32 mass and gravity per
+    // row, two state slots per cloth
+    float *gf_qb = nullptr, *gf_tri = nullptr, *gf_pinv = nullptr, *gf_dx = nullptr;
+    float *gf_target = nullptr, *gf_mass = nullptr, *gf_grav = nullptr;
+    GfState *gf_state = nullptr;
     // optional lists of the slices (k_cg_spmv) and solver workgroups
     // (k_cg_update) a launch runs; null: all of them
     int *spmv_list = nullptr, *upd_list = nullptr;
@@ -187,6 +205,12 @@ void launch_cg_step_multi(const DeviceBatch *table, const int2 *spmv_list, int s
                           const int2 *upd_list, int upd_grid, const int2 *grp_list, int grp_grid,
                           bool fp32, int phase, int lpr, int xcd_map, hipStream_t s);
 void launch_cg_timing_setup(const DeviceBatch &d, bool fp32, hipStream_t s);
+// group sums of the solver partials in `slot` for cloths that have them
+void launch_group_sums(const DeviceBatch &d, int slot, hipStream_t s);
+// GPU form (gpuform.hip): pins, FP32 elements and rows; then the 100-
+// iteration block-Jacobi solve and the position update
+void launch_gf_assemble(const DeviceBatch &d, hipStream_t s);
+void launch_gf_solve(const DeviceBatch &d, hipStream_t s);
 // render readback (normals.hip): triangle normals (3 per global triangle)
 // and vertex normals (3 per global vertex) at the current positions
 void launch_normals(const DeviceBatch &d, double *tn, double *vn, hipStream_t s);

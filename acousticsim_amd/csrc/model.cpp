@@ -77,6 +77,8 @@ ClothConfig ClothConfig::from_json(const Json &j)
     if (bk.empty())
         throw std::runtime_error("config: cloth_bending_stiffness is empty");
     c.bending_k = (float)bk[0];
+    c.bending_uv[0] = (float)bk[0];
+    c.bending_uv[1] = bk.size() > 1 ? (float)bk[1] : (float)bk[0];
     c.rayleigh_a = (float)j.at("rayleigh_damping_a").as_double();
     c.rayleigh_b = (float)j.at("rayleigh_damping_b").as_double();
     c.dt = j.at("default_timestep").as_double();
@@ -719,9 +721,15 @@ ClothModel ClothModel::from_json_text(const std::string &text)
     if (j.has("cloth_type"))
     {
         const std::string t = j.at("cloth_type").as_string();
-        if (t != "fem" && t != "fem_hip")
+        if (t != "fem" && t != "fem_hip" && t != "fem_gpu")
             throw std::runtime_error("config: cloth_type \"" + t +
-                                     "\" is not the plain implicit method (\"fem\")");
+                                     "\" is not the implicit method (\"fem\", or \"fem_gpu\")");
+        if (t == "fem_gpu")
+        {
+            ClothConfig c = ClothConfig::from_json(j);
+            c.gpu_form = true;
+            return build(c);
+        }
     }
     return build(ClothConfig::from_json(j));
 }
@@ -818,6 +826,104 @@ void ClothModel::finish()
     m.rebuild_pins_from_list();
 
     m.build_blocks();
+    if (cfg.gpu_form)
+        m.build_gpu_form();
+}
+
+namespace
+{
+// CalcCotangent (QBendingMaterial.cpp:11-19): dot products rounded to float,
+// cot = AB.AC / sqrt(|AB|^2 |AC|^2 - (AB.AC)^2) in float
+double cotangent(const double *A, const double *B, const double *C)
+{
+    double ab[3], ac[3];
+    for (int k = 0; k < 3; k++)
+    {
+        ab[k] = B[k] - A[k];
+        ac[k] = C[k] - A[k];
+    }
+    const float abac = (float)(ab[0] * ac[0] + ab[1] * ac[1] + ab[2] * ac[2]);
+    const float abab = (float)(ab[0] * ab[0] + ab[1] * ab[1] + ab[2] * ab[2]);
+    const float acac = (float)(ac[0] * ac[0] + ac[1] * ac[1] + ac[2] * ac[2]);
+    return abac / std::sqrt(abab * acac - abac * abac);
+}
+} // namespace
+
+// The GPU form's rest data (cBaraffClothGpu::Init, BaraffClothGpu.cpp:22-45,
+// InitBendingMatrix :566-650, UpdateBendingMatrix :652-679; cBaseCloth::
+// InitMass, BaseCloth.cpp:418-447).  Quadratic bending: per interior edge
+// with stencil (v0, v1, v2, v3), coef = (c03 + c04, c01 + c02, -c01 - c03,
+// -c02 - c04) from the clamped (1 degree) cotangents of the rest positions,
+// base_ij = float(coef_i coef_j * float(3 / (A0 + A1))), edge stiffness
+// K_e = Ku cos^2(theta) + Kv sin^2(theta) with theta in [0, pi/2] the edge's
+// angle to the u axis, and block (v_i, v_j) += base_ij * K_e * I3, summed
+// here in edge order (the reference adds with float atomics).
+void ClothModel::build_gpu_form()
+{
+    const ClothConfig &c = cfg;
+    std::vector<double> shared(nv, 0.0);
+    for (int t = 0; t < nt; t++)
+        for (int k = 0; k < 3; k++)
+            shared[tri[t][k]] += (double)area[t] / 3;
+    mass32.resize(nv);
+    grav32.resize(nv);
+    for (int v = 0; v < nv; v++)
+    {
+        const double mv = shared[v] * (double)c.density; // float mClothDensity
+        mass32[v] = (float)mv;
+        grav32[v] = (float)(-9.8 * mv); // InitGravity, BaraffClothGpu.cpp:556-567
+    }
+    qb.assign(cols.size(), 0.0f);
+    const float threshold = (float)(1.0 / std::sin(1.0 / 180 * M_PI));
+    const float Ku = c.bending_uv[0], Kv = c.bending_uv[1];
+    for (int e = 0; e < ne; e++)
+    {
+        const Edge &ed = edges[e];
+        if (ed.boundary)
+            continue;
+        const auto &sv = stencil[e];
+        const double *x[4];
+        for (int k = 0; k < 4; k++)
+            x[k] = &xrest[3 * (size_t)sv[k]];
+        double c01 = cotangent(x[0], x[1], x[2]), c02 = cotangent(x[0], x[1], x[3]);
+        double c03 = cotangent(x[1], x[0], x[2]), c04 = cotangent(x[1], x[0], x[3]);
+        for (double *cc : {&c01, &c02, &c03, &c04})
+            *cc = std::min<double>(std::max<double>(*cc, -threshold), threshold);
+        const double coef[4] = {c03 + c04, c01 + c02, -c01 - c03, -c02 - c04};
+        if (!(std::isfinite(coef[0]) && std::isfinite(coef[1]) && std::isfinite(coef[2]) &&
+              std::isfinite(coef[3])))
+            throw std::runtime_error("config: degenerate bending stencil (fem_gpu)");
+        const float prefix = (float)(3.0 / (area[ed.t0] + area[ed.t1]));
+        // edge direction in UV (tVector2f::normalized), angle to the u axis
+        float du = uv[2 * (size_t)ed.v1] - uv[2 * (size_t)ed.v0];
+        float dv = uv[2 * (size_t)ed.v1 + 1] - uv[2 * (size_t)ed.v0 + 1];
+        const float n2 = du * du + dv * dv;
+        if (n2 > 0)
+        {
+            const float n = std::sqrt(n2);
+            du = du / n;
+            dv = dv / n;
+        }
+        float theta = std::acos(du * 1.0f + dv * 0.0f);
+        if (theta > (float)(M_PI / 2.0f))
+            theta = (float)(M_PI - theta);
+        const float ct = std::cos(theta), st = std::sin(theta);
+        const float Ke = Ku * ct * ct + Kv * st * st;
+        for (int i = 0; i < 4; i++)
+        {
+            const int r = sv[i];
+            for (int j = 0; j < 4; j++)
+            {
+                const float base = (float)(coef[i] * coef[j] * prefix);
+                const int col = sv[j];
+                const int *b = cols.data() + row_ptr[r], *e2 = cols.data() + row_ptr[r + 1];
+                const int *it = std::lower_bound(b, e2, col);
+                if (it == e2 || *it != col)
+                    throw std::runtime_error("config: bending block outside the ELL pattern");
+                qb[it - cols.data()] += base * Ke;
+            }
+        }
+    }
 }
 
 } // namespace bw

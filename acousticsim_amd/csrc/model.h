@@ -36,6 +36,10 @@ struct ClothConfig
     float density = 0.2f;    // float member (BaseCloth.h:53)
     float stretch_k[3] = {}; // tVector3f (BaraffCloth.h:25)
     float bending_k = 0;     // [0] of the bending vector (BaraffCloth.cpp:38-41)
+    float bending_uv[2] = {0, 0}; // warp, weft bending of the GPU form (BaraffClothGpu.cpp:25-28)
+    // cloth_type "fem_gpu": the reference's GPU form (cBaraffClothGpu) as an
+    // A/B comparison mode
+    bool gpu_form = false;
     float rayleigh_a = 0, rayleigh_b = 0; // float members (BaraffCloth.h:27)
     double dt = 0.01;
     bool gravity = true;
@@ -87,6 +91,14 @@ struct ClothModel
 
     std::vector<int> pins; // static vertices (may repeat)
     std::vector<uint8_t> pinned;
+
+    // GPU form (cloth_type "fem_gpu"): constant quadratic-bending matrix as
+    // one scalar per block (the block is scalar * I3), FP32 masses and the
+    // pinned vertices' FP32 target positions
+    std::vector<float> qb;     // nnzb, block order of cols
+    std::vector<float> mass32; // nv
+    std::vector<float> grav32; // nv: y gravity force float(-9.8 m)
+    void build_gpu_form();
 
     // block structure, reference numbering
     std::vector<int> row_ptr, cols, diag_slot;

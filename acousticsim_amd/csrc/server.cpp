@@ -186,8 +186,9 @@ void Server::builder_loop()
         try
         {
             ClothModel m = ClothModel::from_json_text(job.json);
-            if (m.cfg.solve_mode == SolveMode::Reference)
-                throw std::runtime_error("config: served requests use the tolerance solver");
+            if (m.cfg.solve_mode == SolveMode::Reference || m.cfg.gpu_form)
+                throw std::runtime_error("config: served requests use the tolerance solver "
+                                         "(not \"reference\" mode or \"fem_gpu\")");
             if ((m.cfg.precision == Precision::Mixed) != opt_.fp32)
                 throw std::runtime_error(std::string("config: this server runs \"") +
                                          (opt_.fp32 ? "mixed" : "fp64") + "\" requests");

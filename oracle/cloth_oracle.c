@@ -136,6 +136,9 @@ struct oc_cloth
     int *pins;
     unsigned char *pinned;
     float *flat_y; /* nv, untransformed rest y as float */
+    /* GPU form (built on first use) */
+    float *gf_qb, *gf_mass, *gf_grav, *gf_dx;
+    float Ku, Kv, gf_a, gf_b, density_f;
     /* rest data */
     double *mass;    /* nv */
     double *gravity; /* 3nv */
@@ -658,7 +661,12 @@ static void rest_precompute(oc_cloth *c, const oc_params *p)
     c->edge_k = ((kb + kb) + kb) / 3;
 
     c->ray_a = (float)p->rayleigh_a;
+    c->Ku = (float)p->bending_k;
+    c->Kv = (float)(p->bending_k2 != 0 ? p->bending_k2 : p->bending_k);
     c->ray_b = (float)p->rayleigh_b;
+    c->gf_a = c->ray_a;
+    c->gf_b = c->ray_b;
+    c->density_f = (float)p->density;
     c->dt = p->dt;
     c->max_iters = p->max_iters;
 }
@@ -870,6 +878,7 @@ void oc_destroy(oc_cloth *c)
     free(c->xrest); free(c->x); free(c->xpre); free(c->uv); free(c->tri);
     free(c->tri_e); free(c->edge); free(c->stencil); free(c->pins); free(c->pinned);
     free(c->flat_y);
+    free(c->gf_qb); free(c->gf_mass); free(c->gf_grav); free(c->gf_dx);
     free(c->mass); free(c->gravity); free(c->coef); free(c->area);
     free(c->edge_len); free(c->heron); free(c->height);
     free(c->row_ptr); free(c->cols); free(c->diag);
@@ -1381,4 +1390,318 @@ void oc_normals(const oc_cloth *c, double *vn3, double *tn3)
     if (tn3)
         memcpy(tn3, tn, 3 * (size_t)c->nt * sizeof(double));
     free(tn);
+}
+
+/* ------------------------------------------------------------------ */
+/* the reference's GPU form                                            */
+/* ------------------------------------------------------------------ */
+
+/* CalcCotangent (QBendingMaterial.cpp:11-19) */
+static double gf_cot(const double *A, const double *B, const double *C)
+{
+    double ab[3], ac[3];
+    v3sub(B, A, ab);
+    v3sub(C, A, ac);
+    float abac = (float)(ab[0] * ac[0] + ab[1] * ac[1] + ab[2] * ac[2]);
+    float abab = (float)(ab[0] * ab[0] + ab[1] * ab[1] + ab[2] * ab[2]);
+    float acac = (float)(ac[0] * ac[0] + ac[1] * ac[1] + ac[2] * ac[2]);
+    return abac / sqrtf(abab * acac - abac * abac);
+}
+
+/* cBaraffClothGpu::InitBendingMatrix / UpdateBendingMatrix
+   (BaraffClothGpu.cpp:566-679) and cBaseCloth::InitMass (BaseCloth.cpp:418-447),
+   InitGravity (BaraffClothGpu.cpp:556-567) */
+static void gf_init(oc_cloth *c)
+{
+    if (c->gf_qb)
+        return;
+    c->gf_qb = xcalloc(c->nnzb, sizeof(float));
+    c->gf_mass = xcalloc(c->nv, sizeof(float));
+    c->gf_grav = xcalloc(c->nv, sizeof(float));
+    c->gf_dx = xcalloc(3 * (size_t)c->nv, sizeof(float));
+    double *shared = xcalloc(c->nv, sizeof(double));
+    for (int t = 0; t < c->nt; t++)
+        for (int k = 0; k < 3; k++)
+            shared[c->tri[3 * t + k]] += c->area[t] / 3;
+    for (int v = 0; v < c->nv; v++)
+    {
+        double m = shared[v] * (double)c->density_f;
+        c->gf_mass[v] = (float)m;
+        c->gf_grav[v] = (float)(-9.8 * m);
+    }
+    free(shared);
+    float threshold = (float)(1.0 / sin(1.0 / 180 * M_PI));
+    for (int e = 0; e < c->ne; e++)
+    {
+        const int *ed = c->edge + 5 * e;
+        if (ed[4])
+            continue;
+        const int *sv = c->stencil + 4 * e;
+        const double *x[4];
+        for (int k = 0; k < 4; k++)
+            x[k] = c->xrest + 3 * sv[k];
+        double cc[4] = {gf_cot(x[0], x[1], x[2]), gf_cot(x[0], x[1], x[3]),
+                        gf_cot(x[1], x[0], x[2]), gf_cot(x[1], x[0], x[3])};
+        for (int k = 0; k < 4; k++)
+            cc[k] = cc[k] < -threshold ? -threshold : (cc[k] > threshold ? threshold : cc[k]);
+        double coef[4] = {cc[2] + cc[3], cc[0] + cc[1], -cc[0] - cc[2], -cc[1] - cc[3]};
+        float a0 = (float)c->area[ed[2]], a1 = (float)c->area[ed[3]];
+        float prefix = (float)(3.0 / (a0 + a1));
+        float du = c->uv[2 * ed[1]] - c->uv[2 * ed[0]];
+        float dv = c->uv[2 * ed[1] + 1] - c->uv[2 * ed[0] + 1];
+        float n2 = du * du + dv * dv;
+        if (n2 > 0)
+        {
+            float n = sqrtf(n2);
+            du = du / n;
+            dv = dv / n;
+        }
+        float theta = acosf(du * 1.0f + dv * 0.0f);
+        if (theta > (float)(M_PI / 2.0f))
+            theta = (float)(M_PI - theta);
+        float ct = cosf(theta), st = sinf(theta);
+        float Ke = c->Ku * ct * ct + c->Kv * st * st;
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++)
+            {
+                float base = (float)(coef[i] * coef[j] * prefix);
+                c->gf_qb[slot_of(c, sv[i], sv[j])] += base * Ke;
+            }
+    }
+}
+
+void oc_gpu_form_bending(oc_cloth *c, double *qb)
+{
+    gf_init(c);
+    for (int k = 0; k < c->nnzb; k++)
+        qb[k] = c->gf_qb[k];
+}
+
+/* one stretch part in FP32 (BaraffClothGpu.cu:7-105; Dv scaled by |Fu|) */
+static void gf_part(float Ku, float Kv, const float *cu, const float *cv, float x[3][3],
+                    float area, float f[9], float H[81])
+{
+    float Fu[3], Fv[3], Un[3], Vn[3], Du[9], Dv[9];
+    for (int a = 0; a < 3; a++)
+    {
+        Fu[a] = cu[0] * x[0][a] + cu[1] * x[1][a] + cu[2] * x[2][a];
+        Fv[a] = cv[0] * x[0][a] + cv[1] * x[1][a] + cv[2] * x[2][a];
+    }
+    float nu = sqrtf(Fu[0] * Fu[0] + Fu[1] * Fu[1] + Fu[2] * Fu[2]);
+    float nv = sqrtf(Fv[0] * Fv[0] + Fv[1] * Fv[1] + Fv[2] * Fv[2]);
+    for (int a = 0; a < 3; a++)
+    {
+        Un[a] = nu > 0 ? Fu[a] / nu : Fu[a];
+        Vn[a] = nv > 0 ? Fv[a] / nv : Fv[a];
+    }
+    float Cu = nu - 1, Cv = nv - 1;
+    for (int a = 0; a < 3; a++)
+    {
+        float ub = -area * Ku * Cu * Un[a], vb = -area * Kv * Cv * Vn[a];
+        for (int i = 0; i < 3; i++)
+            f[3 * i + a] += ub * cu[i] + vb * cv[i];
+    }
+    float iu = 1.0f / nu;
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++)
+        {
+            Du[3 * a + b] = iu * Un[a] * Un[b] + ((a == b && nu > 1) ? 1 - iu : 0.0f);
+            Dv[3 * a + b] = iu * Vn[a] * Vn[b] + ((a == b && nu > 1) ? 1 - iu : 0.0f);
+        }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+        {
+            float su = Ku * cu[i] * cu[j], sv = Kv * cv[i] * cv[j];
+            for (int q = 0; q < 9; q++)
+                H[9 * (3 * i + j) + q] += -area * (su * Du[q] + sv * Dv[q]);
+        }
+}
+
+static void gf_mv(const float *M, const float *v, float *o)
+{
+    for (int a = 0; a < 3; a++)
+        o[a] = M[3 * a] * v[0] + M[3 * a + 1] * v[1] + M[3 * a + 2] * v[2];
+}
+
+void oc_gpu_form_step(oc_cloth *c, oc_stats *st)
+{
+    gf_init(c);
+    const int nv = c->nv, n3 = 3 * nv;
+    const float dt = (float)c->dt;
+    /* pins back to their targets (UpdateConstraintVertexPos) */
+    for (int j = 0; j < c->npin; j++)
+        for (int a = 0; a < 3; a++)
+        {
+            int q = 3 * c->pins[j] + a;
+            c->x[q] = c->xpre[q] = (double)(float)c->xrest[q];
+        }
+    /* stretch per triangle */
+    float *tri = xcalloc(90 * (size_t)c->nt, sizeof(float));
+    for (int t = 0; t < c->nt; t++)
+    {
+        float x[3][3], cf[12];
+        for (int k = 0; k < 3; k++)
+            for (int a = 0; a < 3; a++)
+                x[k][a] = (float)c->x[3 * c->tri[3 * t + k] + a];
+        for (int k = 0; k < 12; k++)
+            cf[k] = (float)c->coef[12 * t + k];
+        float f[9] = {0}, H[81] = {0};
+        float area = (float)c->area[t];
+        gf_part((float)c->k_ww[0], (float)c->k_ww[1], cf, cf + 3, x, area, f, H);
+        gf_part((float)c->k_bias, (float)c->k_bias, cf + 6, cf + 9, x, area, f, H);
+        memcpy(tri + 90 * t, f, 9 * sizeof(float));
+        memcpy(tri + 90 * t + 9, H, 81 * sizeof(float));
+    }
+    /* K blocks (triangle order), W, rhs, block-Jacobi inverse */
+    float *K = xcalloc(9 * (size_t)c->nnzb, sizeof(float));
+    float *fs = xcalloc(n3, sizeof(float));
+    for (int t = 0; t < c->nt; t++)
+        for (int i = 0; i < 3; i++)
+        {
+            int vi = c->tri[3 * t + i];
+            for (int a = 0; a < 3; a++)
+                fs[3 * vi + a] += tri[90 * t + 3 * i + a];
+            for (int j = 0; j < 3; j++)
+            {
+                float *blk = K + 9 * (size_t)slot_of(c, vi, c->tri[3 * t + j]);
+                for (int q = 0; q < 9; q++)
+                    blk[q] += tri[90 * t + 9 + 9 * (3 * i + j) + q];
+            }
+        }
+    float *W = xcalloc(9 * (size_t)c->nnzb, sizeof(float));
+    float *b = xcalloc(n3, sizeof(float)), *Pinv = xcalloc(9 * (size_t)nv, sizeof(float));
+    const float ks = dt * c->gf_b - dt * dt;
+    for (int v = 0; v < nv; v++)
+    {
+        float fb[3] = {0, 0, 0};
+        for (int k = c->row_ptr[v]; k < c->row_ptr[v + 1]; k++)
+        {
+            float *Kb = K + 9 * (size_t)k, qb = c->gf_qb[k];
+            Kb[0] -= qb;
+            Kb[4] -= qb;
+            Kb[8] -= qb;
+            for (int a = 0; a < 3; a++)
+                fb[a] += -1.0f * qb * (float)c->x[3 * c->cols[k] + a];
+            float *Wb = W + 9 * (size_t)k;
+            for (int q = 0; q < 9; q++)
+                Wb[q] = ks * Kb[q];
+            if (c->cols[k] == v)
+            {
+                float md = c->gf_mass[v] * (1 + dt * c->gf_a);
+                Wb[0] += md;
+                Wb[4] += md;
+                Wb[8] += md;
+                float A = Wb[0], B = Wb[1], C = Wb[2], D = Wb[3], E = Wb[4], F = Wb[5],
+                      G2 = Wb[6], H2 = Wb[7], I = Wb[8];
+                float inv[9] = {E * I - F * H2, C * H2 - B * I, B * F - C * E,
+                                F * G2 - D * I, A * I - C * G2, C * D - A * F,
+                                D * H2 - E * G2, B * G2 - A * H2, A * E - B * D};
+                float pre = 1.0f / (A * (E * I - F * H2) - B * (D * I - F * G2) + C * (D * H2 - E * G2));
+                for (int q = 0; q < 9; q++)
+                    Pinv[9 * v + q] = inv[q] * pre;
+            }
+        }
+        float g[3] = {0.0f, c->gf_grav[v], 0.0f};
+        for (int a = 0; a < 3; a++)
+        {
+            float f = fs[3 * v + a] + fb[a];
+            b[3 * v + a] = dt * dt * (g[a] + f) +
+                           ((float)c->x[3 * v + a] - (float)c->xpre[3 * v + a]) * c->gf_mass[v];
+        }
+    }
+    /* PCG (PCGSolver.cu:356-481) */
+    float *r = xcalloc(n3, sizeof(float)), *d = xcalloc(n3, sizeof(float));
+    float *z = xcalloc(n3, sizeof(float)), *dx = c->gf_dx;
+    double rmr = 0, rr = 0, bb = 0;
+    for (int v = 0; v < nv; v++)
+    {
+        float y[3] = {0, 0, 0};
+        for (int k = c->row_ptr[v]; k < c->row_ptr[v + 1]; k++)
+        {
+            float t[3];
+            gf_mv(W + 9 * (size_t)k, dx + 3 * c->cols[k], t);
+            for (int a = 0; a < 3; a++)
+                y[a] += t[a];
+        }
+        float mr[3];
+        for (int a = 0; a < 3; a++)
+            r[3 * v + a] = c->pinned[v] ? 0.0f : b[3 * v + a] - y[a];
+        gf_mv(Pinv + 9 * v, r + 3 * v, mr);
+        for (int a = 0; a < 3; a++)
+        {
+            d[3 * v + a] = mr[a];
+            rmr += (double)r[3 * v + a] * mr[a];
+            rr += (double)r[3 * v + a] * r[3 * v + a];
+            if (!c->pinned[v])
+                bb += (double)b[3 * v + a] * b[3 * v + a];
+        }
+    }
+    int it = 0;
+    for (; it < 100; it++)
+    {
+        double dz = 0;
+        for (int v = 0; v < nv; v++)
+        {
+            float y[3] = {0, 0, 0};
+            if (!c->pinned[v])
+                for (int k = c->row_ptr[v]; k < c->row_ptr[v + 1]; k++)
+                {
+                    if (c->pinned[c->cols[k]])
+                        continue;
+                    float t[3];
+                    gf_mv(W + 9 * (size_t)k, d + 3 * c->cols[k], t);
+                    for (int a = 0; a < 3; a++)
+                        y[a] += t[a];
+                }
+            for (int a = 0; a < 3; a++)
+            {
+                z[3 * v + a] = y[a];
+                dz += (double)d[3 * v + a] * y[a];
+            }
+        }
+        float alpha = dz == 0.0 ? 0.0f : (float)(rmr / dz);
+        double rmr_new = 0;
+        rr = 0;
+        for (int q = 0; q < n3; q++)
+        {
+            r[q] = r[q] - alpha * z[q];
+            dx[q] = dx[q] + alpha * d[q];
+        }
+        for (int v = 0; v < nv; v++)
+        {
+            float mr[3];
+            gf_mv(Pinv + 9 * v, r + 3 * v, mr);
+            for (int a = 0; a < 3; a++)
+            {
+                rmr_new += (double)r[3 * v + a] * mr[a];
+                rr += (double)r[3 * v + a] * r[3 * v + a];
+            }
+        }
+        float beta = rmr == 0.0 ? 0.0f : (float)(rmr_new / rmr);
+        for (int v = 0; v < nv; v++)
+        {
+            float mr[3];
+            gf_mv(Pinv + 9 * v, r + 3 * v, mr);
+            for (int a = 0; a < 3; a++)
+                d[3 * v + a] = mr[a] + beta * d[3 * v + a];
+        }
+        rmr = rmr_new;
+        if ((it % 10 == 0 || it == 99) && sqrt(rr) < 1e-10)
+        {
+            it++;
+            break;
+        }
+    }
+    for (int q = 0; q < n3; q++)
+    {
+        c->xpre[q] = c->x[q];
+        c->x[q] = c->x[q] + (double)dx[q];
+    }
+    if (st)
+    {
+        st->iterations = it;
+        st->rel_residual = bb > 0 ? sqrt(rr / bb) : sqrt(rr);
+        st->true_rel_residual = st->rel_residual;
+    }
+    free(tri); free(K); free(fs); free(W); free(b); free(Pinv); free(r); free(d); free(z);
 }

@@ -34,6 +34,7 @@ typedef struct oc_params
     int num_pins;
     const double *pin_uv;       /* num_pins x 2, "constraints.static_point" */
     int pin_top_edge;           /* also pin every vertex of the top side */
+    double bending_k2;          /* "cloth_bending_stiffness"[1] (GPU form, float) */
 } oc_params;
 
 typedef struct oc_cloth oc_cloth;
@@ -105,6 +106,17 @@ void oc_get_W(const oc_cloth *c, double *w9);
 void oc_get_force(const oc_cloth *c, double *f3, int which); /* 0 total,1 stretch,2 bend */
 void oc_get_rhs(const oc_cloth *c, double *b3);
 void oc_get_dx(const oc_cloth *c, double *dx3); /* last solve's dx */
+/* The reference's GPU form (cBaraffClothGpu, cloth_type "fem_gpu",
+   BaraffClothGpu.cpp/.cu, PCGSolver.cu): one substep with FP32 stretch,
+   the constant quadratic-bending matrix, block-Jacobi PCG of 100
+   iterations (absolute 1e-10 residual checked every 10), warm-started from
+   the previous step's solution, pinned vertices reset to their targets.
+   Dot products are FP64 sums in row order (the reference adds FP32
+   atomics).  st gets the iterations and the recursive residual / ||b||. */
+void oc_gpu_form_step(oc_cloth *c, oc_stats *st);
+/* the GPU form's constant bending matrix: one scalar per block of the
+   pattern (the block is scalar * I3) */
+void oc_gpu_form_bending(oc_cloth *c, double *qb);
 /* normals at the current positions (BaseObject.cpp:119-160): triangle
    normals (3T) and area-weighted vertex normals (3V); either may be NULL */
 void oc_normals(const oc_cloth *c, double *vn3, double *tn3);

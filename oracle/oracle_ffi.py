@@ -35,6 +35,7 @@ class OcParams(ctypes.Structure):
         ("num_pins", ctypes.c_int),
         ("pin_uv", ctypes.POINTER(ctypes.c_double)),
         ("pin_top_edge", ctypes.c_int),
+        ("bending_k2", ctypes.c_double),
     ]
 
 
@@ -83,6 +84,8 @@ def lib():
             getattr(L, n).argtypes = [vp, dp]
         L.oc_get_stretch_coef.argtypes = [vp, dp, dp]
         L.oc_normals.argtypes = [vp, dp, dp]
+        L.oc_gpu_form_step.argtypes = [vp, P(OcStats)]
+        L.oc_gpu_form_bending.argtypes = [vp, dp]
         L.oc_get_bending_rest.argtypes = [vp, dp, dp, dp]
         L.oc_get_pattern.argtypes = [vp, ip, ip]
         L.oc_get_ref_ell.argtypes = [vp, ip, ip]
@@ -123,7 +126,9 @@ def params_from_config(conf):
     p.uv_rotation_deg = float(conf.get("cloth_uv_rotation", 0))
     p.density = float(conf["cloth_density"])
     p.stretch_k[:] = [float(v) for v in conf["cloth_stretch_stiffness"][:3]]
-    p.bending_k = float(conf["cloth_bending_stiffness"][0])
+    bk = conf["cloth_bending_stiffness"]
+    p.bending_k = float(bk[0])
+    p.bending_k2 = float(bk[1] if len(bk) > 1 else bk[0])
     p.rayleigh_a = float(conf["rayleigh_damping_a"])
     p.rayleigh_b = float(conf["rayleigh_damping_b"])
     p.dt = float(conf["default_timestep"])
@@ -325,6 +330,16 @@ class OracleCloth:
         lib().oc_step(self.h, mode, tol, max_iters, ctypes.byref(st))
         return {"iterations": st.iterations, "rel_residual": st.rel_residual,
                 "true_rel_residual": st.true_rel_residual}
+
+    def gpu_form_step(self):
+        """one substep of the reference's GPU form (cloth_type "fem_gpu")"""
+        st = OcStats()
+        lib().oc_gpu_form_step(self.h, ctypes.byref(st))
+        return {"iterations": st.iterations, "rel_residual": st.rel_residual}
+
+    def gpu_form_bending(self):
+        """the GPU form's constant bending matrix, one scalar per pattern block"""
+        return self._f64("oc_gpu_form_bending", self.nnzb)
 
     def normals(self):
         """(vertex normals (V, 3), triangle normals (T, 3)) at the current
