@@ -106,7 +106,7 @@ def lib():
         "bw_create": (i32, [cp, dp, P(vp)]),
         "bw_create_mesh": (i32, [cp, dp, i32, ip, i32, dp, P(vp)]),
         "bw_delaunay_mesh": (i32, [ctypes.c_double, ctypes.c_double, i32, ctypes.c_ulonglong,
-                                   ctypes.c_double, dp, ip, ip]),
+                                   ctypes.c_double, dp, i32, ip, i32, ip, ip]),
         "bw_destroy": (None, [vp]),
         "bw_update_pos": (i32, [vp, ctypes.c_double]),
         "bw_reset": (i32, [vp]),
@@ -183,12 +183,21 @@ def delaunay_mesh(cloth_size, n, seed=0, min_spacing=0.0):
     n points, delaunay_seed, delaunay_min_spacing), before the init
     transform: rest positions (n, 3) and triangles (T, 3) in the cloth's
     winding.  Host only."""
-    pos = np.zeros(3 * n, dtype=np.float64)
-    tri = np.zeros(6 * n, dtype=np.int32)
-    nt = ctypes.c_int()
-    _check(lib().bw_delaunay_mesh(float(cloth_size[0]), float(cloth_size[1]), int(n), int(seed),
-                                  float(min_spacing), _dp(pos), _ip(tri), ctypes.byref(nt)))
-    return pos.reshape(-1, 3), tri[:3 * nt.value].reshape(-1, 3).copy()
+    nv, nt = ctypes.c_int(), ctypes.c_int()
+    args = (float(cloth_size[0]), float(cloth_size[1]), int(n), int(seed), float(min_spacing))
+    cap = max(int(n), 4)
+    pos = np.zeros(3 * cap, dtype=np.float64)
+    tri = np.zeros(6 * cap, dtype=np.int32)
+    rc = lib().bw_delaunay_mesh(*args, _dp(pos), cap, _ip(tri), 2 * cap, ctypes.byref(nv),
+                                ctypes.byref(nt))
+    if rc == -5:  # more boundary points than n: ask again with room for them
+        pos = np.zeros(3 * nv.value, dtype=np.float64)
+        tri = np.zeros(3 * nt.value, dtype=np.int32)
+        rc = lib().bw_delaunay_mesh(*args, _dp(pos), nv.value, _ip(tri), nt.value,
+                                    ctypes.byref(nv), ctypes.byref(nt))
+    _check(rc)
+    return (pos[:3 * nv.value].reshape(-1, 3).copy(),
+            tri[:3 * nt.value].reshape(-1, 3).copy())
 
 
 class BaraffClothHip:

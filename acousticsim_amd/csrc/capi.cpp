@@ -115,10 +115,11 @@ int bw_create_mesh(const char *json, const double *rest_pos, int nv, const int *
 }
 
 int bw_delaunay_mesh(double height, double width, int n, unsigned long long seed,
-                     double min_spacing, double *rest_pos, int *tri, int *nt)
+                     double min_spacing, double *rest_pos, int cap_vertices, int *tri,
+                     int cap_triangles, int *nv, int *nt)
 {
     return guard([&] {
-        if (!rest_pos || !tri || !nt)
+        if (!nv || !nt)
             return fail(BW_ERR_ARG, "NULL argument");
         if (n < 4 || !(height > 0) || !(width > 0) || !(min_spacing >= 0))
             return fail(BW_ERR_ARG, "a delaunay cloth needs n >= 4, a positive size and "
@@ -126,11 +127,14 @@ int bw_delaunay_mesh(double height, double width, int n, unsigned long long seed
         std::vector<double> xyz;
         std::vector<std::array<int, 3>> t;
         bw::delaunay_cloth((float)width, (float)height, n, seed, (float)min_spacing, xyz, t);
-        if ((int)xyz.size() != 3 * n || t.size() > 2 * (size_t)n)
-            return fail(BW_ERR_STATE, "delaunay mesh has unexpected size");
-        std::memcpy(rest_pos, xyz.data(), xyz.size() * sizeof(double));
-        std::memcpy(tri, t.data(), t.size() * sizeof(t[0]));
+        *nv = (int)(xyz.size() / 3);
         *nt = (int)t.size();
+        if ((rest_pos && cap_vertices < *nv) || (tri && cap_triangles < *nt))
+            return fail(BW_ERR_CAPACITY, "delaunay mesh larger than the buffers");
+        if (rest_pos)
+            std::memcpy(rest_pos, xyz.data(), xyz.size() * sizeof(double));
+        if (tri)
+            std::memcpy(tri, t.data(), t.size() * sizeof(t[0]));
         return BW_OK;
     });
 }

@@ -17,7 +17,8 @@ namespace bw
 // sqrt(w h / n) along each side), then uniform random points (splitmix64
 // stream of `seed`) kept when farther than sqrt(w h / (n pi)) from every
 // point kept so far (TriangulatorDelaunay.cpp:62-80), or than min_spacing
-// when that is positive.  Coordinates are
+// when that is positive; when the boundary alone holds n points or more,
+// only the boundary (as the reference's loop adds nothing).  Coordinates are
 // float values, as the reference stores them; side points lie exactly on
 // the sides.  Throws if the rule cannot place n points.
 std::vector<std::array<float, 2>> rectangle_points(float width, float height, int n,

@@ -76,12 +76,17 @@ int bw_create_mesh(const char *json_config, const double *rest_pos, int nv, cons
 /* The mesh of "geometry_type": "delaunay" for cloth_size {height, width},
    n = subdivision[0] * subdivision[1] points, delaunay_seed and
    delaunay_min_spacing (0: the reference's spacing rule), before the
-   init transform: rest_pos (3n doubles, z = 0) and tri (room for 2n
-   triangles; *nt receives the count), in the winding the cloth uses.  This
-   is cTriangulator::DelaunayTriangulation (TriangulatorDelaunay.cpp:45-160)
-   with a seeded point stream.  Host only: needs no device. */
+   init transform, in the winding the cloth uses.  This is
+   cTriangulator::DelaunayTriangulation (TriangulatorDelaunay.cpp:45-160)
+   with a seeded point stream; it has n points, or more when the rectangle's
+   boundary points alone exceed n.  *nv and *nt receive the counts;
+   rest_pos (3 nv doubles, z = 0, room for cap_vertices vertices) and tri
+   (3 nt ids, room for cap_triangles) are filled when not NULL, and
+   BW_ERR_CAPACITY is returned when they are too small.  Host only: needs
+   no device. */
 int bw_delaunay_mesh(double height, double width, int n, unsigned long long seed,
-                     double min_spacing, double *rest_pos, int *tri, int *nt);
+                     double min_spacing, double *rest_pos, int cap_vertices, int *tri,
+                     int cap_triangles, int *nv, int *nt);
 /* destructor of the object (shared_ptr release in the reference) */
 void bw_destroy(bw_cloth *c);
 

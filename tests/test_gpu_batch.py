@@ -169,3 +169,45 @@ def test_stalled_solves_move_to_fp64_matrix(monkeypatch):
     n = capi.ClothBatch([c for c, _ in cs], [x for _, x in cs])
     n.step(1)
     assert all(n.stats(i)["matrix_fp64"] == 0 for i in range(len(cs)))
+
+
+def test_tiny_and_ragged_cloths_in_one_batch():
+    """The smallest cloths (one grid cell: 4 vertices, one interior edge; a
+    1 x 3 strip; a 7-point Delaunay cloth) next to a large one: each agrees
+    with the oracle, and the batch with the alone runs."""
+    from acousticsim_amd import capi
+
+    say("case tiny and ragged")
+    confs = []
+    for hd, wd in ((1, 1), (1, 3), (40, 3)):
+        c = wl.grid_config(2)
+        c["subdivision"] = [hd, wd]
+        c["cloth_size"] = [0.1 * hd / max(hd, wd), 0.1 * wd / max(hd, wd)]
+        c["constraints"] = {"static_point": [[-0.05 * wd / max(hd, wd), 0.05 * hd / max(hd, wd)]]}
+        confs.append(c)
+    d = wl.delaunay_config(7, size=0.1, seed=2, top_edge=False)
+    confs.append(d)
+    confs.append(wl.grid_config(65))
+    x0s = []
+    ocs = []
+    for c in confs:
+        if c.get("geometry_type") == "delaunay":
+            pos, tri = wl.delaunay_rest_positions(c)
+            oc = of.OracleCloth(c, mesh=(pos, tri))
+        else:
+            oc = of.OracleCloth(c)
+        x = oc.rest_pos().copy()
+        x[2::3] += 1e-3 * np.cos(np.arange(oc.nv))
+        x0s.append(x)
+        ocs.append(oc)
+    b = capi.ClothBatch(confs, x0s)
+    b.step(2)
+    for i, (oc, x0) in enumerate(zip(ocs, x0s)):
+        oc.set_state(x0, x0)
+        for _ in range(2):
+            oc.step(of.SOLVE_TOLERANCE, 1e-12)
+        dd = np.abs(oc.pos() - x0).max()
+        err = np.abs(b.get_pos(i) - oc.pos()).max()
+        say("  member %d nv %d err/disp %.3e" % (i, oc.nv, err / dd))
+        assert b.stats(i)["rel_residual"] <= 1e-6
+        assert err <= 1e-4 * dd
