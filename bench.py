@@ -52,12 +52,15 @@ from acousticsim_amd import workloads as wl  # noqa: E402
 METRIC = "cloth steps/s & p50/p99 step latency at 1e-6 PCG residual; SpMV HBM GB/s"
 HBM_PEAK_GBS = 8000.0
 KERNEL = "k_cg_spmv<float, %d> (PCG SpMV: d = P r + beta d_prev, q = W d, partial d.q)"
-# HBM bytes per launch of the autotuned k_cg_spmv variant, from rocprofv3
-# --pmc FETCH_SIZE and WRITE_SIZE over the --roofline-only command, scaled by
-# the same counters' reading for a 256 MiB device-to-device copy
-# (profiles/k_cg_spmv_<workload>.md; the counter passes abort inside the
-# profiler on the irregular workload, so it has none)
-TRAFFIC = {"batch64": 209697107.0, "single257": 32754087.0}
+# HBM bytes per launch of the k_cg_spmv variant (lanes per row) the autotuner
+# picks, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE over the
+# --roofline-only command, scaled by the same counters' reading for a 256 MiB
+# device-to-device copy (profiles/k_cg_spmv_<workload>.md; the 1- and 2-lane
+# variants time within 1% of each other on both workloads, so either may be
+# picked; the counter passes abort inside the profiler on the irregular
+# workload, so it has none)
+TRAFFIC = {"batch64": {1: 213810572.0, 2: 209697107.0},
+           "single257": {1: 33077660.0, 2: 32754087.0}}
 
 
 def members(workload, precision):
@@ -344,7 +347,8 @@ def main():
                      "spmv_us_by_lanes": {str(k): v for k, v in b.tuning[1].items()}},
         "max_rel_residual": worst,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC.get(args.workload),
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": TRAFFIC.get(args.workload, {}).get(b.tuning[0]),
                      "kernel": KERNEL % b.tuning[0],
                      "launch_us": us, "alg_bytes": nbytes},
     }
