@@ -67,23 +67,30 @@ std::vector<int> slice_row_order(const ClothModel &m)
     {
         // Irregular meshes: vertex ids carry no locality, so rows follow a
         // Hilbert curve over the rest positions; inside every window of
-        // 16384 rows they are sorted by width (widest first), so that a
-        // 64-row slice holds rows of similar width (padding under 1% on
-        // Delaunay meshes with widths 4..25, against 39% in vertex order).
-        // The window keeps a slice's gathered columns within a stretch of
-        // the curve whose vector entries fit one XCD's L2: on a 1.5M-point
-        // cloth the SpMV takes 237 us with 16384-row windows, 256 us with
-        // 1024, 270 us with 256 (more padding) and 355 us sorted globally.
+        // 16384 rows they are sorted by width class (widths in classes of
+        // 4 slots, widest first, curve order kept within a class), so that
+        // a 64-row slice holds rows of similar width (padding ~3% on
+        // Delaunay meshes with widths 4..25, against 39% in vertex order)
+        // whose columns stay close along the curve.  The window keeps a
+        // slice's gathered entries within a stretch of the curve that fits
+        // one XCD's L2.  SpMV on a 1.5M-point cloth: 226 us (classes of 4,
+        // 16384-row windows), 237 us (exact widths), 256 us (exact widths,
+        // 1024-row windows), 255 us (curve order only), 355 us (global
+        // width sort).
         std::vector<int> order = hilbert_order(m.flat_xy);
         int window = 16384; // BW_SORT_WINDOW overrides it for tuning runs
         if (const char *e = std::getenv("BW_SORT_WINDOW"))
             window = std::max(64, std::atoi(e));
+        int bucket = 4; // widths compared in classes of this many slots
+        if (const char *e = std::getenv("BW_WIDTH_BUCKET"))
+            bucket = std::max(1, std::atoi(e));
         for (size_t s = 0; s < order.size(); s += window)
         {
             auto b = order.begin() + s,
                  e = order.begin() + std::min(order.size(), s + (size_t)window);
             std::stable_sort(b, e, [&](int x, int y) {
-                return m.row_ptr[x + 1] - m.row_ptr[x] > m.row_ptr[y + 1] - m.row_ptr[y];
+                return (m.row_ptr[x + 1] - m.row_ptr[x]) / bucket >
+                       (m.row_ptr[y + 1] - m.row_ptr[y]) / bucket;
             });
         }
         return order;
