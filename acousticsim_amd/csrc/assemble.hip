@@ -382,6 +382,8 @@ k_assemble_rows(DeviceBatch d)
         const double t2 = ((dt * mass) * (x - xp)) / dt;
         d.b[r3 + a] = pin_p ? 0.0 : t1 + t2;
         d.dx[r3 + a] = pin_p ? 0.0 : x - xp;
+        if (d.dx32)
+            d.dx32[r3 + a] = 0.0f;
         const double pinv = pin_p ? 0.0 : 1.0 / diag[a];
         d.P64[r3 + a] = pinv;
         d.P32[r3 + a] = (float)pinv;

@@ -472,6 +472,7 @@ void Batch::build_layout()
     {
         if (kind_ == SolverKind::CG32)
         {
+            d_.dx32 = dalloc<float>(nv3);
             d_.r32 = dalloc<float>(nv3);
             d_.z32 = dalloc<float>(nv3);
             d_.q32 = dalloc<float>(nv3);

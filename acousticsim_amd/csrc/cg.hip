@@ -29,7 +29,7 @@
 //                r = b - W64 dx in FP64, z = P r (partials r.z, r.r, b.b),
 //                and the following k_cg_update idles for that cloth.
 //   k_cg_update  four slices per workgroup, one lane per row: alpha =
-//                rho / d.q; dx += alpha d (FP64), r -= alpha q, z = P r;
+//                rho / d.q; dx += alpha d, r -= alpha q, z = P r;
 //                partials r.z, r.r.
 // z = P r is stored so that the SpMV gathers two vectors per column (z and
 // d_prev) rather than three.  Both kernels take an optional list of the
@@ -42,7 +42,11 @@
 // recurrences on stiff cloths; this form does not.
 //
 // "mixed": W (FP32 copy) and r, z, q, d are FP32; every dot product and
-// scalar is FP64 and dx accumulates in FP64.  Reliable updates (Sleijpen and
+// scalar is FP64; the solution is dx (FP64) + dx32, where dx32 accumulates
+// the iterations' increments in FP32 and is folded into dx after each true
+// residual (which reads both), so an iteration moves 24 rather than 48
+// bytes of solution per row and the FP64 residual check still sees the
+// exact iterate it judges.  Reliable updates (Sleijpen and
 // van der Vorst, 1996) replace the recursive residual by the true FP64
 // residual each time r.r has dropped by rel2 since the last replacement,
 // after max_since iterations, and before convergence is accepted; the Krylov
@@ -271,15 +275,22 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
     const G<int> *cl = glob(d.col) + base + row;
     if (residual)
     {
-        // true residual in FP64: r = b - W64 dx
+        // true residual in FP64: r = b - W64 dx (dx + dx32 in the mixed form)
         const G<double> *Wl = glob(d.W64) + 9 * base + row;
         const G<double> *dx = glob(d.dx);
+        const G<float> *dx32 = glob(d.dx32);
         double y[3] = {0.0, 0.0, 0.0};
         for (int k = j; k < w; k += L)
         {
             const size_t c3 = 3 * (size_t)cl[k * SLICE];
             const G<double> *wk = Wl + 9 * (size_t)k * SLICE;
-            const double x0 = dx[c3], x1 = dx[c3 + 1], x2 = dx[c3 + 2];
+            double x0 = dx[c3], x1 = dx[c3 + 1], x2 = dx[c3 + 2];
+            if constexpr (sizeof(T) == sizeof(float))
+            {
+                x0 += (double)dx32[c3];
+                x1 += (double)dx32[c3 + 1];
+                x2 += (double)dx32[c3 + 2];
+            }
             for (int a = 0; a < 3; a++)
                 y[a] += wk[(3 * a) * SLICE] * x0 + wk[(3 * a + 1) * SLICE] * x1 +
                         wk[(3 * a + 2) * SLICE] * x2;
@@ -383,9 +394,21 @@ __device__ inline void update_body(const DeviceBatch &d, const int blk,
     }
     if (st.pending)
     {
-        // idle: hand the true-residual partials on to the next k_cg_spmv
+        // idle: hand the true-residual partials on to the next k_cg_spmv,
+        // and fold the FP32 increments into dx (no kernel reads them now)
         if (threadIdx.x < PARTS * SB)
             part_out[part0 + threadIdx.x] = part_in[part0 + threadIdx.x];
+        if constexpr (sizeof(T) == sizeof(float))
+        {
+            const size_t q3 = 3 * (size_t)((blk * SB + (threadIdx.x >> 6)) * SLICE + (threadIdx.x & 63));
+            G<double> *dx = glob(d.dx);
+            G<float> *dx32 = glob(d.dx32);
+            for (int a = 0; a < 3; a++)
+            {
+                dx[q3 + a] = dx[q3 + a] + (double)dx32[q3 + a];
+                dx32[q3 + a] = 0.0f;
+            }
+        }
         if (lead)
             store_state(st_out, st);
         return;
@@ -416,10 +439,14 @@ __device__ inline void update_body(const DeviceBatch &d, const int blk,
     const size_t p3 = 3 * (size_t)(sl * SLICE + lane);
     const G<T> *dn = (st.iter & 1) ? dbuf1 : dbuf0;
     G<double> *dx = glob(d.dx);
+    G<float> *dx32 = glob(d.dx32);
     double v[2] = {0.0, 0.0};
     for (int a = 0; a < 3; a++)
     {
-        dx[p3 + a] = dx[p3 + a] + alpha * (double)dn[p3 + a];
+        if constexpr (sizeof(T) == sizeof(float))
+            dx32[p3 + a] = dx32[p3 + a] + (float)(alpha * (double)dn[p3 + a]);
+        else
+            dx[p3 + a] = dx[p3 + a] + alpha * (double)dn[p3 + a];
         const T rn = (T)((double)r[p3 + a] - alpha * (double)q[p3 + a]);
         const T zn = P[p3 + a] * rn;
         r[p3 + a] = rn;

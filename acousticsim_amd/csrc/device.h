@@ -151,6 +151,10 @@ struct DeviceBatch
     int *group_inst = nullptr;
     int n_groups = 0;
     float *r32 = nullptr, *z32 = nullptr, *q32 = nullptr, *d32[2] = {nullptr, nullptr};
+    // mixed form: the solution's increments since the last true residual,
+    // accumulated in FP32 and folded into the FP64 dx by the k_cg_update
+    // that follows each true residual (the solution is dx + dx32)
+    float *dx32 = nullptr;
     double *z64 = nullptr;
     // GPU form (gpuform.hip): bending scalar per ELL entry, per-triangle
     // FP32 force and Hessian (90), block-Jacobi inverse (9 per row), the
