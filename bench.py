@@ -255,6 +255,8 @@ def serve(args):
     if bad:
         raise RuntimeError("%d requests failed; first: %s" % (len(bad), bad[0]))
     lat = np.array([i["t_done"] - i["t_submit"] for i in infos])
+    # per step of a request once it runs: (done - first step) / steps
+    step_lat = np.array([(i["t_done"] - i["t_first_step"]) / max(1, i["steps_done"]) for i in infos])
     queue = np.array([i["t_admit"] - i["t_submit"] for i in infos])
     t_first = min(i["t_submit"] for i in infos)
     makespan = max(i["t_done"] for i in infos) - t_first
@@ -279,6 +281,8 @@ def serve(args):
         "vertex_steps_per_s": vsteps / makespan,
         "latency_p50_s": float(np.percentile(lat, 50)),
         "latency_p99_s": float(np.percentile(lat, 99)),
+        "step_latency_p50_ms": 1000 * float(np.percentile(step_lat, 50)),
+        "step_latency_p99_ms": 1000 * float(np.percentile(step_lat, 99)),
         "queue_p50_s": float(np.percentile(queue, 50)),
         "makespan_s": makespan,
         "max_resident": counters["max_resident_seen"],
