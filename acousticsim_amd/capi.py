@@ -84,6 +84,7 @@ EXPORTS = [
     "bw_clear_constraint_static_vertices", "bw_get_triangles", "bw_get_edges",
     "bw_get_ell_map", "bw_get_normals", "bw_batch_get_normals", "bw_assemble_system", "bw_get_last_stats", "bw_last_error", "bw_batch_create",
     "bw_batch_destroy", "bw_batch_size", "bw_batch_step", "bw_batch_num_vertices",
+    "bw_batch_num_triangles",
     "bw_batch_get_pos", "bw_batch_set_state", "bw_batch_get_stats", "bw_batch_set_graphs",
     "bw_batch_time_solver_kernel", "bw_batch_autotune", "bw_server_create", "bw_server_destroy", "bw_server_submit",
     "bw_server_wait", "bw_server_get_info", "bw_server_get_pos", "bw_server_release",
@@ -136,6 +137,7 @@ def lib():
         "bw_batch_size": (i32, [vp]),
         "bw_batch_step": (i32, [vp, i32]),
         "bw_batch_num_vertices": (i32, [vp, i32]),
+        "bw_batch_num_triangles": (i32, [vp, i32]),
         "bw_batch_get_pos": (i32, [vp, i32, dp]),
         "bw_batch_set_state": (i32, [vp, i32, dp, dp]),
         "bw_batch_get_stats": (i32, [vp, i32, P(StepStats)]),
@@ -406,10 +408,10 @@ class ClothBatch:
         xpre = np.ascontiguousarray(xpre, dtype=np.float64)
         _check(lib().bw_batch_set_state(self._h, i, _dp(x), _dp(xpre)))
 
-    def normals(self, i, num_triangles):
+    def normals(self, i):
         """(vertex normals (V, 3), triangle normals (T, 3)) of member i"""
         vn = np.empty(3 * self.nv[i])
-        tn = np.empty(3 * num_triangles)
+        tn = np.empty(3 * lib().bw_batch_num_triangles(self._h, i))
         _check(lib().bw_batch_get_normals(self._h, i, _dp(vn), _dp(tn)))
         return vn.reshape(-1, 3), tn.reshape(-1, 3)
 

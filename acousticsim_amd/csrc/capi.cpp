@@ -424,6 +424,13 @@ int bw_batch_num_vertices(const bw_batch *b, int i)
     return b->batch->model(i).nv;
 }
 
+int bw_batch_num_triangles(const bw_batch *b, int i)
+{
+    if (!b || i < 0 || i >= b->batch->size())
+        return fail(BW_ERR_ARG, "bad index");
+    return b->batch->model(i).nt;
+}
+
 int bw_batch_get_pos(const bw_batch *b, int i, double *x)
 {
     return guard([&] {

@@ -162,6 +162,7 @@ int bw_batch_size(const bw_batch *b);
 /* nsteps substeps of every member */
 int bw_batch_step(bw_batch *b, int nsteps);
 int bw_batch_num_vertices(const bw_batch *b, int i);
+int bw_batch_num_triangles(const bw_batch *b, int i);
 int bw_batch_get_pos(const bw_batch *b, int i, double *x);
 int bw_batch_set_state(bw_batch *b, int i, const double *x, const double *xpre);
 int bw_batch_get_stats(const bw_batch *b, int i, bw_step_stats *st);

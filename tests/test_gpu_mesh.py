@@ -157,7 +157,7 @@ def test_normals_match_oracle(capi, kind, monkeypatch):
     assert np.abs(vn - ovn).max() <= 1e-14
     b = capi.ClothBatch([conf], [x0])
     b.step(4)
-    bvn, btn = b.normals(0, len(tri) if kind != "grid" else oc.nt)
+    bvn, btn = b.normals(0)
     assert np.array_equal(bvn, vn) and np.array_equal(btn, tn)
 
 
