@@ -205,3 +205,64 @@ def serve_requests(n, seed=SERVE["seed"], rate=SERVE["rate"], v_lo=SERVE["v_lo"]
         c["precision"] = "mixed"
         out.append((t, c, x0))
     return out
+
+
+# Contacts: the collision detector's output handed to a cloth before a step
+# (point-triangle and edge-edge records, as cBaseObject's
+# SetPointTriangleCollisionInfo / SetEdgeEdgeCollisionInfo take them).
+def random_contacts(nv, nt, ne, seed, n_point=24, n_tri=12, n_edge=12, repeat=4):
+    """Seeded synthetic contacts for a cloth of nv vertices, nt triangles and
+    ne edges: n_point contacts of a cloth vertex with a body triangle (the
+    first `repeat` vertices hit several times, so that impulses chain),
+    n_tri of a body vertex with a cloth triangle, n_edge edge-edge contacts
+    (cloth edge first or second at even odds); unit normals, float
+    barycentric coordinates (summing to about 1 on triangles), depths below
+    the 3 mm contact distance.  Returns a dict of numpy arrays
+    (pt_ids, pt_normal, pt_bary, pt_depth, ee_ids, ee_normal, ee_bary,
+    ee_depth)."""
+    rng = np.random.default_rng(seed)
+
+    def normals(n):
+        v = rng.normal(size=(n, 3))
+        return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+    verts = rng.integers(0, nv, n_point)
+    if repeat and n_point >= 2 * repeat:
+        verts[repeat:2 * repeat] = verts[:repeat]
+        verts[-repeat:] = verts[:repeat]
+    pt_ids = np.zeros((n_point + n_tri, 3), dtype=np.int32)
+    pt_ids[:n_point, 0] = 1
+    pt_ids[:n_point, 1] = verts
+    pt_ids[:n_point, 2] = rng.integers(0, 1000, n_point)
+    pt_ids[n_point:, 1] = rng.integers(0, 1000, n_tri)
+    pt_ids[n_point:, 2] = rng.integers(0, nt, n_tri)
+    bary = rng.dirichlet((1.0, 1.0, 1.0), n_point + n_tri).astype(np.float32).astype(np.float64)
+    ee_ids = np.zeros((n_edge, 3), dtype=np.int32)
+    ee_ids[:, 0] = rng.integers(0, 2, n_edge)
+    cloth_edges = rng.integers(0, ne, n_edge)
+    body_edges = rng.integers(0, 1000, n_edge)
+    ee_ids[:, 1] = np.where(ee_ids[:, 0] == 1, cloth_edges, body_edges)
+    ee_ids[:, 2] = np.where(ee_ids[:, 0] == 1, body_edges, cloth_edges)
+    return {
+        "pt_ids": pt_ids,
+        "pt_normal": normals(n_point + n_tri),
+        "pt_bary": bary,
+        "pt_depth": rng.uniform(-2e-3, 2.9e-3, n_point + n_tri).astype(np.float32),
+        "ee_ids": ee_ids,
+        "ee_normal": normals(n_edge),
+        "ee_bary": rng.uniform(0.0, 1.0, (n_edge, 2)).astype(np.float32),
+        "ee_depth": rng.uniform(-2e-3, 2.9e-3, n_edge).astype(np.float32),
+    }
+
+
+def contacts_file_records(ct):
+    """the contacts as one float64 array: n_pt, n_ee, then per point-triangle
+    contact {cloth_is_point, vertex, triangle, depth, normal[3], bary[3]}
+    and per edge-edge contact {cloth_is_edge0, edge0, edge1, depth, bary0,
+    bary1, normal[3]}"""
+    n_pt, n_ee = len(ct["pt_ids"]), len(ct["ee_ids"])
+    pt = np.concatenate([ct["pt_ids"].astype(np.float64), ct["pt_depth"][:, None].astype(np.float64),
+                         ct["pt_normal"], ct["pt_bary"]], axis=1) if n_pt else np.zeros((0, 10))
+    ee = np.concatenate([ct["ee_ids"].astype(np.float64), ct["ee_depth"][:, None].astype(np.float64),
+                         ct["ee_bary"].astype(np.float64), ct["ee_normal"]], axis=1) if n_ee else np.zeros((0, 9))
+    return np.concatenate([[float(n_pt), float(n_ee)], pt.reshape(-1), ee.reshape(-1)])

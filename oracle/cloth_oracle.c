@@ -157,6 +157,14 @@ struct oc_cloth
     /* per-step */
     double *Ks, *Kb, *K, *W; /* 9 nnzb each */
     double *fs, *fb, *f, *b, *dx;
+    /* contacts for the next step only (cleared by oc_step) */
+    int n_pt, n_ee;
+    int *pt_ids;              /* 3 n_pt: cloth_is_point, vertex, triangle */
+    double *pt_normal, *pt_bary; /* 3 n_pt each */
+    float *pt_depth;
+    int *ee_ids;              /* 3 n_ee: cloth_is_edge0, edge0, edge1 */
+    double *ee_normal;        /* 3 n_ee */
+    float *ee_bary, *ee_depth; /* 2 n_ee, n_ee */
 };
 
 static void *xcalloc(size_t n, size_t s)
@@ -884,6 +892,7 @@ void oc_destroy(oc_cloth *c)
     free(c->row_ptr); free(c->cols); free(c->diag);
     free(c->Ks); free(c->Kb); free(c->K); free(c->W);
     free(c->fs); free(c->fb); free(c->f); free(c->b); free(c->dx);
+    oc_set_contacts(c, 0, NULL, NULL, NULL, NULL, 0, NULL, NULL, NULL, NULL);
     free(c);
 }
 
@@ -1046,6 +1055,218 @@ void oc_dbeta_dx(const double *v0, const double *v1, const double *v2,
 /* assembly                                                            */
 /* ------------------------------------------------------------------ */
 
+/* ------------------------------------------------------------------ */
+/* collision response                                                  */
+/* ------------------------------------------------------------------ */
+
+int oc_set_contacts(oc_cloth *c, int n_pt, const int *pt_ids, const double *pt_normal,
+                    const double *pt_bary, const float *pt_depth, int n_ee, const int *ee_ids,
+                    const double *ee_normal, const float *ee_bary, const float *ee_depth)
+{
+    free(c->pt_ids); free(c->pt_normal); free(c->pt_bary); free(c->pt_depth);
+    free(c->ee_ids); free(c->ee_normal); free(c->ee_bary); free(c->ee_depth);
+    c->pt_ids = NULL; c->pt_normal = NULL; c->pt_bary = NULL; c->pt_depth = NULL;
+    c->ee_ids = NULL; c->ee_normal = NULL; c->ee_bary = NULL; c->ee_depth = NULL;
+    c->n_pt = c->n_ee = 0;
+    for (int k = 0; k < n_pt; k++)
+    {
+        int v = pt_ids[3 * k + 1], t = pt_ids[3 * k + 2];
+        if (pt_ids[3 * k] ? (v < 0 || v >= c->nv) : (t < 0 || t >= c->nt))
+            return -1;
+    }
+    for (int k = 0; k < n_ee; k++)
+    {
+        int e = ee_ids[3 * k] ? ee_ids[3 * k + 1] : ee_ids[3 * k + 2];
+        if (e < 0 || e >= c->ne)
+            return -1;
+    }
+    if (n_pt > 0)
+    {
+        c->pt_ids = xcalloc(3 * (size_t)n_pt, sizeof(int));
+        c->pt_normal = xcalloc(3 * (size_t)n_pt, sizeof(double));
+        c->pt_bary = xcalloc(3 * (size_t)n_pt, sizeof(double));
+        c->pt_depth = xcalloc(n_pt, sizeof(float));
+        memcpy(c->pt_ids, pt_ids, 3 * (size_t)n_pt * sizeof(int));
+        memcpy(c->pt_normal, pt_normal, 3 * (size_t)n_pt * sizeof(double));
+        for (int k = 0; k < 3 * n_pt; k++)
+            c->pt_bary[k] = (double)(float)pt_bary[k]; /* tVector3f cast to double */
+        memcpy(c->pt_depth, pt_depth, (size_t)n_pt * sizeof(float));
+        c->n_pt = n_pt;
+    }
+    if (n_ee > 0)
+    {
+        c->ee_ids = xcalloc(3 * (size_t)n_ee, sizeof(int));
+        c->ee_normal = xcalloc(3 * (size_t)n_ee, sizeof(double));
+        c->ee_bary = xcalloc(2 * (size_t)n_ee, sizeof(float));
+        c->ee_depth = xcalloc(n_ee, sizeof(float));
+        memcpy(c->ee_ids, ee_ids, 3 * (size_t)n_ee * sizeof(int));
+        memcpy(c->ee_normal, ee_normal, 3 * (size_t)n_ee * sizeof(double));
+        memcpy(c->ee_bary, ee_bary, 2 * (size_t)n_ee * sizeof(float));
+        memcpy(c->ee_depth, ee_depth, (size_t)n_ee * sizeof(float));
+        c->n_ee = n_ee;
+    }
+    return 0;
+}
+
+/* cBaraffCloth::PrepareCollisionHessian (BaraffCloth.cpp:928-990) added to
+   W (SolveForDx :293).  Per point-triangle contact: dC/dx = N on the cloth
+   vertex, or -bary_i N on the cloth triangle's vertex i; blocks
+   1e-2 * dC_i dC_j^T, triplets in contact order (i, j, a, b), summed per
+   entry in that order by setFromTriplets, then W + H. */
+static void add_contact_hessian(oc_cloth *c)
+{
+    if (c->n_pt == 0)
+        return;
+    size_t nb = 9 * (size_t)c->nnzb;
+    double *H = xcalloc(nb, sizeof(double));
+    unsigned char *touched = xcalloc(nb, 1);
+    for (int k = 0; k < c->n_pt; k++)
+    {
+        const double *N = c->pt_normal + 3 * k;
+        double dc[4][3];
+        int id[4] = {-1, -1, -1, -1};
+        if (c->pt_ids[3 * k])
+        {
+            for (int a = 0; a < 3; a++)
+                dc[0][a] = N[a];
+            id[0] = c->pt_ids[3 * k + 1];
+        }
+        else
+        {
+            const int *t = c->tri + 3 * c->pt_ids[3 * k + 2];
+            for (int i = 0; i < 3; i++)
+            {
+                for (int a = 0; a < 3; a++)
+                    dc[1 + i][a] = -c->pt_bary[3 * k + i] * N[a];
+                id[1 + i] = t[i];
+            }
+        }
+        for (int i = 0; i < 4; i++)
+        {
+            if (id[i] < 0)
+                continue;
+            for (int j = 0; j < 4; j++)
+            {
+                if (id[j] < 0)
+                    continue;
+                int s = slot_of(c, id[i], id[j]);
+                for (int a = 0; a < 3; a++)
+                    for (int b = 0; b < 3; b++)
+                    {
+                        double h = 1e-2 * (dc[i][a] * dc[j][b]);
+                        size_t q = 9 * (size_t)s + 3 * a + b;
+                        H[q] = touched[q] ? H[q] + h : h;
+                        touched[q] = 1;
+                    }
+            }
+        }
+    }
+    for (size_t q = 0; q < nb; q++)
+        if (touched[q])
+            c->W[q] = c->W[q] + H[q];
+    free(H);
+    free(touched);
+}
+
+/* cBaraffCloth::Repulsion (BaraffCloth.cpp:626-926), one pass: impulses from
+   edge-edge contacts, then point-triangle contacts, each applied to dx at
+   once (later contacts see earlier impulses).  K = mStretchK.mean() in float
+   (Eigen's unrolled sum k0 + (k1 + k2), then / 3). */
+static void repulsion(oc_cloth *c)
+{
+    const double dt = c->dt;
+    const float scale = 0.1f;
+    const double v_thre = 0.01;
+    const float kf = ((float)c->k_ww[0] + ((float)c->k_ww[1] + (float)c->k_bias)) / 3.0f;
+    const double K = (double)(kf * 1);
+    double *dx = c->dx;
+    for (int k = 0; k < c->n_ee; k++)
+    {
+        const double *N = c->ee_normal + 3 * k;
+        const int cloth0 = c->ee_ids[3 * k] != 0;
+        const int e = cloth0 ? c->ee_ids[3 * k + 1] : c->ee_ids[3 * k + 2];
+        const float b0 = c->ee_bary[2 * k], b1 = c->ee_bary[2 * k + 1];
+        const float bc = cloth0 ? b0 : b1;
+        const int v0 = c->edge[5 * e], v1 = c->edge[5 * e + 1];
+        const double w0 = (double)(1 - bc), w1 = (double)bc;
+        double vrel[3];
+        for (int a = 0; a < 3; a++)
+            vrel[a] = (w0 * dx[3 * v0 + a] + w1 * dx[3 * v1 + a]) / dt;
+        const double mass = c->mass[v0] + c->mass[v1];
+        const double vn = (vrel[0] * N[0] + vrel[1] * N[1]) + vrel[2] * N[2];
+        const double Ic = vn > v_thre ? 0 : mass * fabs(vn) / 2;
+        const double d = (3e-3 - (double)c->ee_depth[k]);
+        const float inhibition_coef = 0.1f;
+        double Ir = 0;
+        if (!(vn > inhibition_coef * d / dt))
+        {
+            const double Ir0 = dt * K * d;
+            const double Ir1 = mass * (inhibition_coef * d / dt - vn);
+            Ir = Ir0 < Ir1 ? Ir0 : Ir1;
+        }
+        const double I = Ic + Ir;
+        const float den = 1 + b0 * b0 + (1 - b0) * (1 - b0) + b1 * b1 + (1 - b1) * (1 - b1);
+        const double I_tilde = 2 * I / den;
+        const double s0 = scale * dt * (1 - bc) * I_tilde / mass;
+        const double s1 = scale * dt * bc * I_tilde / mass;
+        for (int a = 0; a < 3; a++)
+            dx[3 * v0 + a] = dx[3 * v0 + a] + s0 * N[a];
+        for (int a = 0; a < 3; a++)
+            dx[3 * v1 + a] = dx[3 * v1 + a] + s1 * N[a];
+    }
+    for (int k = 0; k < c->n_pt; k++)
+    {
+        const double *N = c->pt_normal + 3 * k;
+        const double *bary = c->pt_bary + 3 * k;
+        const int cloth_point = c->pt_ids[3 * k] != 0;
+        int vs[3];
+        double vrel[3], mass;
+        if (cloth_point)
+        {
+            vs[0] = c->pt_ids[3 * k + 1];
+            for (int a = 0; a < 3; a++)
+                vrel[a] = dx[3 * vs[0] + a] / dt;
+            mass = c->mass[vs[0]];
+        }
+        else
+        {
+            const int *t = c->tri + 3 * c->pt_ids[3 * k + 2];
+            for (int i = 0; i < 3; i++)
+                vs[i] = t[i];
+            for (int a = 0; a < 3; a++)
+                vrel[a] = (dx[3 * vs[0] + a] * bary[0] + dx[3 * vs[1] + a] * bary[1] +
+                           dx[3 * vs[2] + a] * bary[2]) /
+                          dt;
+            mass = c->mass[vs[0]] + c->mass[vs[1]] + c->mass[vs[2]];
+        }
+        const double vn = (vrel[0] * N[0] + vrel[1] * N[1]) + vrel[2] * N[2];
+        const double Ic = vn > v_thre ? 0 : mass * fabs(vn) / 2;
+        const double d = (3e-3 - (double)c->pt_depth[k]);
+        double Ir = 0;
+        if (!(vn > 0.1 * d / dt))
+        {
+            const double Ir0 = dt * K * d;
+            const double Ir1 = mass * (0.1 * d / dt - vn);
+            Ir = Ir0 < Ir1 ? Ir0 : Ir1;
+        }
+        const double I = Ic + Ir;
+        const double I_tilde = 2 * I / (1 + ((bary[0] * bary[0] + bary[1] * bary[1]) + bary[2] * bary[2]));
+        if (cloth_point)
+        {
+            const double f = dt * I_tilde / mass;
+            for (int a = 0; a < 3; a++)
+                dx[3 * vs[0] + a] = dx[3 * vs[0] + a] + (double)scale * (f * N[a]);
+        }
+        else
+            for (int i = 0; i < 3; i++)
+            {
+                const double f = dt * bary[i] * I_tilde / mass;
+                for (int a = 0; a < 3; a++)
+                    dx[3 * vs[i] + a] = dx[3 * vs[i] + a] + (double)scale * (f * N[a]);
+            }
+    }
+}
+
 void oc_assemble(oc_cloth *c)
 {
     int nv = c->nv;
@@ -1150,6 +1371,7 @@ void oc_assemble(oc_cloth *c)
             double t2 = ((dt * c->mass[v]) * (c->x[q] - c->xpre[q])) / dt;
             c->b[q] = t1 + t2;
         }
+    add_contact_hessian(c);
 }
 
 /* ApplyMatmul (BaraffCloth.cpp:469-506): rows and columns of pinned vertices
@@ -1268,6 +1490,8 @@ int oc_step(oc_cloth *c, int mode, double tol, int max_iters, oc_stats *st)
 {
     oc_assemble(c);
     solve(c, mode, tol, max_iters, st);
+    repulsion(c);
+    oc_set_contacts(c, 0, NULL, NULL, NULL, NULL, 0, NULL, NULL, NULL, NULL);
     int n = 3 * c->nv;
     /* mXpre = mXcur; mXcur = mXpre + dx (BaraffCloth.cpp:150-153) */
     for (int k = 0; k < n; k++)

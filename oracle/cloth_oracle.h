@@ -99,6 +99,20 @@ void oc_get_pre_pos(const oc_cloth *c, double *x);
    mode = OC_SOLVE_REFERENCE uses max_iters from params (tol ignored). */
 int oc_step(oc_cloth *c, int mode, double tol, int max_iters, oc_stats *st);
 
+/* contacts for the next oc_step only, which clears them
+   (cBaseObject::SetPointTriangleCollisionInfo / SetEdgeEdgeCollisionInfo,
+   BaseObject.cpp:199-207; cleared at BaraffCloth.cpp:156-157).
+   Point-triangle contact k: pt_ids[3k..] = {cloth_is_point, vertex,
+   triangle} (obj0_is_cloth_obj1_is_rb, mVertexId0, mTriangleId1), normal
+   (3), bary (3, float values), depth (float).  Edge-edge contact k:
+   ee_ids[3k..] = {cloth_is_edge0, edge0, edge1}, normal (3), bary (2:
+   mBary0, mBary1), depth.  The collision Hessian enters W in oc_assemble
+   (BaraffCloth.cpp:928-990, :293); the repulsion impulses are added to dx
+   after the solve (:626-926, :147).  -1 for a cloth index out of range. */
+int oc_set_contacts(oc_cloth *c, int n_pt, const int *pt_ids, const double *pt_normal,
+                    const double *pt_bary, const float *pt_depth, int n_ee, const int *ee_ids,
+                    const double *ee_normal, const float *ee_bary, const float *ee_depth);
+
 /* stage access: assemble K, f, W, b at the current state without stepping */
 void oc_assemble(oc_cloth *c);
 void oc_get_K(const oc_cloth *c, double *k9, int which); /* 0 total,1 stretch,2 bend */

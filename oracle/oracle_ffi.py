@@ -84,6 +84,9 @@ def lib():
             getattr(L, n).argtypes = [vp, dp]
         L.oc_get_stretch_coef.argtypes = [vp, dp, dp]
         L.oc_normals.argtypes = [vp, dp, dp]
+        fp = P(ctypes.c_float)
+        L.oc_set_contacts.argtypes = [vp, ctypes.c_int, ip, dp, dp, fp, ctypes.c_int, ip, dp, fp, fp]
+        L.oc_set_contacts.restype = ctypes.c_int
         L.oc_gpu_form_step.argtypes = [vp, P(OcStats)]
         L.oc_gpu_form_bending.argtypes = [vp, dp]
         L.oc_get_bending_rest.argtypes = [vp, dp, dp, dp]
@@ -330,6 +333,30 @@ class OracleCloth:
         lib().oc_step(self.h, mode, tol, max_iters, ctypes.byref(st))
         return {"iterations": st.iterations, "rel_residual": st.rel_residual,
                 "true_rel_residual": st.true_rel_residual}
+
+    def set_contacts(self, ct):
+        """contacts for the next step only (a dict as
+        acousticsim_amd.workloads.random_contacts returns)"""
+        n_pt, n_ee = len(ct["pt_ids"]), len(ct["ee_ids"])
+        a = dict(
+            pt_ids=np.ascontiguousarray(ct["pt_ids"], dtype=np.int32).reshape(-1),
+            pt_normal=np.ascontiguousarray(ct["pt_normal"], dtype=np.float64).reshape(-1),
+            pt_bary=np.ascontiguousarray(ct["pt_bary"], dtype=np.float64).reshape(-1),
+            pt_depth=np.ascontiguousarray(ct["pt_depth"], dtype=np.float32).reshape(-1),
+            ee_ids=np.ascontiguousarray(ct["ee_ids"], dtype=np.int32).reshape(-1),
+            ee_normal=np.ascontiguousarray(ct["ee_normal"], dtype=np.float64).reshape(-1),
+            ee_bary=np.ascontiguousarray(ct["ee_bary"], dtype=np.float32).reshape(-1),
+            ee_depth=np.ascontiguousarray(ct["ee_depth"], dtype=np.float32).reshape(-1))
+        self._contacts = a  # keep the buffers alive across the call
+        fp = ctypes.POINTER(ctypes.c_float)
+
+        def f(x):
+            return x.ctypes.data_as(fp)
+        rc = lib().oc_set_contacts(self.h, n_pt, _ip(a["pt_ids"]), _dp(a["pt_normal"]),
+                                   _dp(a["pt_bary"]), f(a["pt_depth"]), n_ee, _ip(a["ee_ids"]),
+                                   _dp(a["ee_normal"]), f(a["ee_bary"]), f(a["ee_depth"]))
+        if rc != 0:
+            raise ValueError("contact index out of range")
 
     def gpu_form_step(self):
         """one substep of the reference's GPU form (cloth_type "fem_gpu")"""

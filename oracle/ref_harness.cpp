@@ -16,7 +16,16 @@
 //
 // usage: ref_cloth <config.json> <x0.bin|-> <nsteps> <out.bin>
 //                  [--dump-steps a,b,c] [--stages] [--time]
+//                  [--contacts contacts.bin [--contact-steps a,b,c]]
+// contacts.bin (float64 little-endian): n_pt, n_ee, then n_pt records
+//   {cloth_is_point, vertex, triangle, depth, normal[3], bary[3]} and n_ee
+//   records {cloth_is_edge0, edge0, edge1, depth, bary0, bary1, normal[3]};
+//   the same contacts are handed to the cloth before each listed step
+//   (every step by default) through cBaseObject::SetPointTriangleCollisionInfo
+//   and SetEdgeEdgeCollisionInfo, as cSimScene::PerformCollisionDetection
+//   does (src/scenes/SimScene.cpp:174-190).
 #include "sim/cloth/BaraffCloth.h"
+#include "sim/collision/CollisionInfo.h"
 #include "sim/cloth/BaraffMaterial.h"
 #include "sim/cloth/BaseBendingMaterial.h"
 #include "geometries/Primitives.h"
@@ -114,8 +123,9 @@ int main(int argc, char **argv)
     }
     std::string conf_path = argv[1], x0_path = argv[2], out_path = argv[4];
     int nsteps = atoi(argv[3]);
-    std::set<int> dump_steps;
+    std::set<int> dump_steps, contact_steps;
     bool stages = false, timing = false;
+    std::string contacts_path;
     for (int i = 5; i < argc; i++)
     {
         if (!strcmp(argv[i], "--dump-steps") && i + 1 < argc)
@@ -127,6 +137,56 @@ int main(int argc, char **argv)
             stages = true;
         else if (!strcmp(argv[i], "--time"))
             timing = true;
+        else if (!strcmp(argv[i], "--contacts") && i + 1 < argc)
+            contacts_path = argv[++i];
+        else if (!strcmp(argv[i], "--contact-steps") && i + 1 < argc)
+        {
+            for (int s : ParseList(argv[++i]))
+                contact_steps.insert(s);
+        }
+    }
+    std::vector<tPointTriangleCollisionInfoPtr> pt_contacts;
+    std::vector<tEdgeEdgeCollisionInfoPtr> ee_contacts;
+    if (!contacts_path.empty())
+    {
+        FILE *f = fopen(contacts_path.c_str(), "rb");
+        if (!f)
+            return 7;
+        std::vector<double> buf;
+        double v;
+        while (fread(&v, sizeof(double), 1, f) == 1)
+            buf.push_back(v);
+        fclose(f);
+        if (buf.size() < 2)
+            return 8;
+        const int n_pt = (int)buf[0], n_ee = (int)buf[1];
+        if (buf.size() != size_t(2 + 10 * n_pt + 9 * n_ee))
+            return 8;
+        const double *r = buf.data() + 2;
+        for (int k = 0; k < n_pt; k++, r += 10)
+        {
+            auto info = std::make_shared<tPointTriangleCollisionInfo>();
+            info->obj0_is_cloth_obj1_is_rb = r[0] != 0;
+            info->mVertexId0 = (int)r[1];
+            info->mTriangleId1 = (int)r[2];
+            info->mDepth = (float)r[3];
+            info->mOuterNormal = tVector3d(r[4], r[5], r[6]);
+            // the detector stores float barycentric coordinates
+            info->mBary = tVector3f((float)r[7], (float)r[8], (float)r[9]).cast<double>();
+            pt_contacts.push_back(info);
+        }
+        for (int k = 0; k < n_ee; k++, r += 9)
+        {
+            auto info = std::make_shared<tEdgeEdgeCollisionInfo>();
+            info->obj0_is_cloth_obj1_is_rb = r[0] != 0;
+            info->mEdgeId0 = (int)r[1];
+            info->mEdgeId1 = (int)r[2];
+            info->mDepth = (float)r[3];
+            info->mBary0 = (float)r[4];
+            info->mBary1 = (float)r[5];
+            info->mOuterNormal = tVector3d(r[6], r[7], r[8]);
+            ee_contacts.push_back(info);
+        }
     }
     dump_steps.insert(nsteps);
 
@@ -203,6 +263,11 @@ int main(int argc, char **argv)
     for (int step = 1; step <= nsteps; step++)
     {
         tVectorXd xbefore = cloth->GetPos();
+        if (!contacts_path.empty() && (contact_steps.empty() || contact_steps.count(step)))
+        {
+            cloth->SetPointTriangleCollisionInfo(pt_contacts);
+            cloth->SetEdgeEdgeCollisionInfo(ee_contacts);
+        }
         auto t0 = std::chrono::steady_clock::now();
         cloth->UpdatePos(cloth->GetDefaultTimestep());
         auto t1 = std::chrono::steady_clock::now();

@@ -53,6 +53,19 @@ def case_defs():
     cases["grid21"] = dict(conf=c, seed=workloads.SINGLE_SMALL["seed"], steps=100,
                            dump=[1, 10, 50, 100], stages=True, full_k=False)
 
+    # contacts handed to the cloth before every step (collision Hessian in W,
+    # repulsion impulses on dx); stiffness values whose float mean depends on
+    # the summation order
+    c = workloads.grid_config(21, dt=0.01)
+    c.update({"cloth_stretch_stiffness": [440.6, 974.4, 898.7]})
+    cases["grid21_contacts"] = dict(conf=c, seed=21, steps=30, dump=[1, 10, 30], stages=True,
+                                    full_k=False, contacts=dict(seed=5))
+    c = workloads.grid_config(5, dt=0.005)
+    c.update({"rayleigh_damping_a": 0.1, "max_iters": 200})
+    cases["grid5_contacts"] = dict(conf=c, seed=22, steps=4, dump=[1, 2, 4], stages=True,
+                                   full_k=True, contacts=dict(seed=6, n_point=10, n_tri=6,
+                                                              n_edge=8, repeat=3))
+
     c = workloads.grid_config(257, dt=0.005)
     cases["grid257"] = dict(conf=c, seed=workloads.SINGLE_LARGE["seed"], steps=5,
                             dump=[1, 5], stages=False, full_k=False, stride=17)
@@ -79,6 +92,16 @@ def run_case(name, d):
                "--dump-steps", ",".join(str(s) for s in d["dump"]), "--time"]
         if d["stages"]:
             cmd.append("--stages")
+        ct = None
+        if "contacts" in d:
+            side = conf["subdivision"]
+            nv = (side[0] + 1) * (side[1] + 1)
+            nt = 2 * side[0] * side[1]
+            ne = side[0] * (side[1] + 1) + side[1] * (side[0] + 1) + side[0] * side[1]
+            ct = workloads.random_contacts(nv, nt, ne, **d["contacts"])
+            ctpath = os.path.join(td, "contacts.bin")
+            workloads.contacts_file_records(ct).astype("<f8").tofile(ctpath)
+            cmd += ["--contacts", ctpath]
         subprocess.run(cmd, check=True)
         dump = read_dump(opath)
     # the injected x0 must be a jitter of the reference's own rest positions
@@ -95,6 +118,9 @@ def run_case(name, d):
         # large meshes: x0 is regenerated from (config, seed); keep the
         # subset as a check that the regeneration matches
         out["x0_subset"] = x0.reshape(-1, 3)[sel].reshape(-1)
+    if ct is not None:
+        for k, v in ct.items():
+            out["contacts_" + k] = v
     out["pins"] = dump["pins"]
     out["mass"] = dump["mass_diag"][0::3][sel]
     for s in d["dump"]:
