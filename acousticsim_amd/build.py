@@ -37,6 +37,7 @@ SOURCES = [
     ("pcg.hip", DEVICE + ["-ffp-contract=fast"]),
     ("cg.hip", DEVICE + ["-ffp-contract=fast"]),
     ("normals.hip", DEVICE + ["-ffp-contract=off"]),
+    ("contact.hip", DEVICE + ["-ffp-contract=off"]),
     ("gpuform.hip", DEVICE + ["-ffp-contract=fast"]),
 ]
 

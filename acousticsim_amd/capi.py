@@ -84,7 +84,8 @@ EXPORTS = [
     "bw_clear_constraint_static_vertices", "bw_get_triangles", "bw_get_edges",
     "bw_get_ell_map", "bw_get_normals", "bw_batch_get_normals", "bw_assemble_system", "bw_get_last_stats", "bw_last_error", "bw_batch_create",
     "bw_batch_destroy", "bw_batch_size", "bw_batch_step", "bw_batch_num_vertices",
-    "bw_batch_num_triangles",
+    "bw_batch_num_triangles", "bw_set_point_triangle_contacts", "bw_set_edge_edge_contacts",
+    "bw_batch_set_point_triangle_contacts", "bw_batch_set_edge_edge_contacts",
     "bw_batch_get_pos", "bw_batch_set_state", "bw_batch_get_stats", "bw_batch_set_graphs",
     "bw_batch_time_solver_kernel", "bw_batch_autotune", "bw_server_create", "bw_server_destroy", "bw_server_submit",
     "bw_server_wait", "bw_server_get_info", "bw_server_get_pos", "bw_server_release",
@@ -103,6 +104,7 @@ def lib():
     P = ctypes.POINTER
     vp, dp, ip, cp = ctypes.c_void_p, P(ctypes.c_double), P(ctypes.c_int), ctypes.c_char_p
     i32 = ctypes.c_int
+    fp = P(ctypes.c_float)
     sig = {
         "bw_create": (i32, [cp, dp, P(vp)]),
         "bw_create_mesh": (i32, [cp, dp, i32, ip, i32, dp, P(vp)]),
@@ -138,6 +140,10 @@ def lib():
         "bw_batch_step": (i32, [vp, i32]),
         "bw_batch_num_vertices": (i32, [vp, i32]),
         "bw_batch_num_triangles": (i32, [vp, i32]),
+        "bw_set_point_triangle_contacts": (i32, [vp, i32, ip, dp, dp, fp]),
+        "bw_set_edge_edge_contacts": (i32, [vp, i32, ip, dp, fp, fp]),
+        "bw_batch_set_point_triangle_contacts": (i32, [vp, i32, i32, ip, dp, dp, fp]),
+        "bw_batch_set_edge_edge_contacts": (i32, [vp, i32, i32, ip, dp, fp, fp]),
         "bw_batch_get_pos": (i32, [vp, i32, dp]),
         "bw_batch_set_state": (i32, [vp, i32, dp, dp]),
         "bw_batch_get_stats": (i32, [vp, i32, P(StepStats)]),
@@ -168,6 +174,21 @@ def _dp(a):
 
 def _ip(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _contact_arrays(ct):
+    """contiguous arrays of a contact dict (keys as
+    acousticsim_amd.workloads.random_contacts returns)"""
+    def a(key, dtype):
+        return np.ascontiguousarray(ct.get(key, np.zeros(0)), dtype=dtype).reshape(-1)
+    return {"pt_ids": a("pt_ids", np.int32), "pt_normal": a("pt_normal", np.float64),
+            "pt_bary": a("pt_bary", np.float64), "pt_depth": a("pt_depth", np.float32),
+            "ee_ids": a("ee_ids", np.int32), "ee_normal": a("ee_normal", np.float64),
+            "ee_bary": a("ee_bary", np.float32), "ee_depth": a("ee_depth", np.float32)}
 
 
 def _check(rc):
@@ -282,6 +303,33 @@ class BaraffClothHip:
         tn = np.empty(3 * nt)
         _check(lib().bw_get_normals(self._h, _dp(vn), _dp(tn)))
         return vn.reshape(-1, 3), tn.reshape(-1, 3)
+
+    def SetPointTriangleCollisionInfo(self, ids, normal, bary, depth):
+        """contacts for the next substep (cBaseObject::SetPointTriangleCollisionInfo):
+        ids (n, 3) {cloth_is_point, vertex, triangle}, normal (n, 3), bary (n, 3),
+        depth (n,)"""
+        a = _contact_arrays({"pt_ids": ids, "pt_normal": normal, "pt_bary": bary,
+                             "pt_depth": depth})
+        _check(lib().bw_set_point_triangle_contacts(self._h, a["pt_depth"].size, _ip(a["pt_ids"]),
+                                                    _dp(a["pt_normal"]), _dp(a["pt_bary"]),
+                                                    _fp(a["pt_depth"])))
+
+    def SetEdgeEdgeCollisionInfo(self, ids, normal, bary, depth):
+        """contacts for the next substep (cBaseObject::SetEdgeEdgeCollisionInfo):
+        ids (n, 3) {cloth_is_edge0, edge0, edge1}, normal (n, 3), bary (n, 2),
+        depth (n,)"""
+        a = _contact_arrays({"ee_ids": ids, "ee_normal": normal, "ee_bary": bary,
+                             "ee_depth": depth})
+        _check(lib().bw_set_edge_edge_contacts(self._h, a["ee_depth"].size, _ip(a["ee_ids"]),
+                                               _dp(a["ee_normal"]), _fp(a["ee_bary"]),
+                                               _fp(a["ee_depth"])))
+
+    def set_contacts(self, ct):
+        """both kinds from one contact dict"""
+        self.SetPointTriangleCollisionInfo(ct["pt_ids"], ct["pt_normal"], ct["pt_bary"],
+                                           ct["pt_depth"])
+        self.SetEdgeEdgeCollisionInfo(ct["ee_ids"], ct["ee_normal"], ct["ee_bary"],
+                                      ct["ee_depth"])
 
     def GetDefaultTimestep(self):
         return lib().bw_get_default_timestep(self._h)
@@ -414,6 +462,18 @@ class ClothBatch:
         tn = np.empty(3 * lib().bw_batch_num_triangles(self._h, i))
         _check(lib().bw_batch_get_normals(self._h, i, _dp(vn), _dp(tn)))
         return vn.reshape(-1, 3), tn.reshape(-1, 3)
+
+    def set_contacts(self, i, ct):
+        """contacts (a dict as workloads.random_contacts returns) for member
+        i's next step"""
+        a = _contact_arrays(ct)
+        L = lib()
+        _check(L.bw_batch_set_point_triangle_contacts(
+            self._h, i, a["pt_depth"].size, _ip(a["pt_ids"]), _dp(a["pt_normal"]),
+            _dp(a["pt_bary"]), _fp(a["pt_depth"])))
+        _check(L.bw_batch_set_edge_edge_contacts(
+            self._h, i, a["ee_depth"].size, _ip(a["ee_ids"]), _dp(a["ee_normal"]),
+            _fp(a["ee_bary"]), _fp(a["ee_depth"])))
 
     def stats(self, i):
         st = StepStats()

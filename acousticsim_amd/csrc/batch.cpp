@@ -7,6 +7,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <stdexcept>
+#include <unordered_map>
 
 namespace bw
 {
@@ -180,6 +181,8 @@ Batch::~Batch()
         (void)hipHostFree(host_spmv_list_);
     if (host_upd_list_)
         (void)hipHostFree(host_upd_list_);
+    if (contact_dev_)
+        (void)hipFree(contact_dev_);
     if (stream_ && own_stream_)
         (void)hipStreamDestroy(stream_);
 }
@@ -502,6 +505,8 @@ void Batch::build_layout()
 
     upload(d_.slice_off, slice_off);
     upload(d_.slice_w, slice_w);
+    host_slice_off_ = slice_off;
+    contacts_.assign(models_.size(), Contacts());
     upload(d_.slice_inst, slice_inst);
     upload(d_.blk_inst, blk_inst);
     upload(d_.row_vertex, row_vertex);
@@ -595,8 +600,11 @@ void Batch::set_pre_pos(int i, const double *x) { upload_state(i, x, d_.xpre); }
 
 void Batch::assemble_system(int i, double *W, double *rhs)
 {
+    const bool ct = kind_ != SolverKind::GpuForm && upload_contacts();
     launch_elements(d_, stream_);
     launch_assemble_rows(d_, stream_);
+    if (ct)
+        launch_contact_hessian(d_, contact_blocks_, c_widx_, c_diag_, c_h9_, stream_);
     check(hipGetLastError(), "assemble launch");
     check(hipStreamSynchronize(stream_), "assemble sync");
     const ClothModel &m = models_[i];
@@ -1035,6 +1043,320 @@ int Batch::autotune()
     return pick;
 }
 
+// ---------------------------------------------------------------------------
+// contacts
+
+void Batch::set_point_triangle_contacts(int i, int n, const int *ids, const double *normal,
+                                        const double *bary, const float *depth)
+{
+    if (i < 0 || i >= size())
+        throw std::invalid_argument("contacts: bad cloth index");
+    if (n < 0 || (n > 0 && (!ids || !normal || !bary || !depth)))
+        throw std::invalid_argument("contacts: null array or negative count");
+    const ClothModel &m = models_[i];
+    for (int k = 0; k < n; k++)
+    {
+        const int flag = ids[3 * k], v = ids[3 * k + 1], t = ids[3 * k + 2];
+        if (flag != 0 && flag != 1)
+            throw std::invalid_argument("contacts: cloth_is_point must be 0 or 1");
+        if (flag ? (v < 0 || v >= m.nv) : (t < 0 || t >= m.nt))
+            throw std::invalid_argument("contacts: cloth vertex or triangle out of range");
+        for (int a = 0; a < 3; a++)
+            if (!std::isfinite(normal[3 * k + a]) || !std::isfinite(bary[3 * k + a]))
+                throw std::invalid_argument("contacts: non-finite normal or barycentric coordinate");
+        // the reference asserts 3e-3 - depth > 0 (BaraffCloth.cpp:832)
+        if (!std::isfinite(depth[k]) || !(3e-3 - (double)depth[k] > 0))
+            throw std::invalid_argument("contacts: point-triangle depth must be below 3e-3");
+    }
+    Contacts &c = contacts_[i];
+    contact_total_ -= c.size();
+    c.pt_ids.assign(ids, ids + 3 * (size_t)n);
+    c.pt_normal.assign(normal, normal + 3 * (size_t)n);
+    c.pt_bary.assign(bary, bary + 3 * (size_t)n);
+    c.pt_depth.assign(depth, depth + n);
+    contact_total_ += c.size();
+}
+
+void Batch::set_edge_edge_contacts(int i, int n, const int *ids, const double *normal,
+                                   const float *bary, const float *depth)
+{
+    if (i < 0 || i >= size())
+        throw std::invalid_argument("contacts: bad cloth index");
+    if (n < 0 || (n > 0 && (!ids || !normal || !bary || !depth)))
+        throw std::invalid_argument("contacts: null array or negative count");
+    const ClothModel &m = models_[i];
+    for (int k = 0; k < n; k++)
+    {
+        const int flag = ids[3 * k];
+        if (flag != 0 && flag != 1)
+            throw std::invalid_argument("contacts: cloth_is_edge0 must be 0 or 1");
+        const int e = flag ? ids[3 * k + 1] : ids[3 * k + 2];
+        if (e < 0 || e >= m.ne)
+            throw std::invalid_argument("contacts: cloth edge out of range");
+        for (int a = 0; a < 3; a++)
+            if (!std::isfinite(normal[3 * k + a]))
+                throw std::invalid_argument("contacts: non-finite normal");
+        if (!std::isfinite(bary[2 * k]) || !std::isfinite(bary[2 * k + 1]) ||
+            !std::isfinite(depth[k]))
+            throw std::invalid_argument("contacts: non-finite barycentric coordinate or depth");
+    }
+    Contacts &c = contacts_[i];
+    contact_total_ -= c.size();
+    c.ee_ids.assign(ids, ids + 3 * (size_t)n);
+    c.ee_normal.assign(normal, normal + 3 * (size_t)n);
+    c.ee_bary.assign(bary, bary + 2 * (size_t)n);
+    c.ee_depth.assign(depth, depth + n);
+    contact_total_ += c.size();
+}
+
+void Batch::clear_contacts()
+{
+    if (contact_total_ == 0)
+        return;
+    for (auto &c : contacts_)
+        c = Contacts();
+    contact_total_ = 0;
+}
+
+namespace
+{
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+} // namespace
+
+// Host side of the collision response (the arithmetic of the records is the
+// reference's, see contact.hip):
+//  - collision Hessian blocks 1e-2 dC_i dC_j^T per point-triangle contact
+//    (BaraffCloth.cpp:928-990), summed per entry in contact order as
+//    setFromTriplets does; blocks in pinned rows or columns are left out
+//    (the solve skips them, :469-506);
+//  - one repulsion record per contact, edge-edge contacts first, then
+//    point-triangle ones (:626-926), with its level for k_repulsion.
+bool Batch::upload_contacts()
+{
+    if (contact_total_ == 0)
+        return false;
+    std::vector<long long> widx;
+    std::vector<int> diag;
+    std::vector<double> h9;
+    std::vector<int> wg_level(1, 0), level_ptr(1, 0);
+    std::vector<int4> ids;
+    std::vector<double> rec;
+    std::vector<int> last_level;
+    for (int i = 0; i < size(); i++)
+    {
+        const Contacts &c = contacts_[i];
+        if (c.size() == 0)
+            continue;
+        const ClothModel &m = models_[i];
+        auto row = [&](int v) { return row_base_[i] + row_of_[i][v]; };
+        // collision Hessian
+        const int n_pt = (int)c.pt_depth.size();
+        if (n_pt > 0)
+        {
+            std::unordered_map<long long, int> at;
+            std::vector<double> hb;        // 9 per block
+            std::vector<uint8_t> touched; // 9 per block
+            std::vector<std::pair<int, int>> key;
+            for (int k = 0; k < n_pt; k++)
+            {
+                const double *N = &c.pt_normal[3 * (size_t)k];
+                double dc[3][3];
+                int vid[3], nloc;
+                if (c.pt_ids[3 * k])
+                {
+                    nloc = 1;
+                    vid[0] = c.pt_ids[3 * k + 1];
+                    for (int a = 0; a < 3; a++)
+                        dc[0][a] = N[a];
+                }
+                else
+                {
+                    nloc = 3;
+                    const auto &t = m.tri[c.pt_ids[3 * k + 2]];
+                    for (int l = 0; l < 3; l++)
+                    {
+                        vid[l] = t[l];
+                        for (int a = 0; a < 3; a++)
+                            dc[l][a] = -c.pt_bary[3 * (size_t)k + l] * N[a];
+                    }
+                }
+                for (int li = 0; li < nloc; li++)
+                    for (int lj = 0; lj < nloc; lj++)
+                    {
+                        const long long kk = (long long)vid[li] * m.nv + vid[lj];
+                        auto it = at.find(kk);
+                        int blk;
+                        if (it == at.end())
+                        {
+                            blk = (int)key.size();
+                            at.emplace(kk, blk);
+                            key.emplace_back(vid[li], vid[lj]);
+                            hb.resize(hb.size() + 9, 0.0);
+                            touched.resize(touched.size() + 9, 0);
+                        }
+                        else
+                            blk = it->second;
+                        for (int a = 0; a < 3; a++)
+                            for (int b = 0; b < 3; b++)
+                            {
+                                const double h = 1e-2 * (dc[li][a] * dc[lj][b]);
+                                const size_t q = 9 * (size_t)blk + 3 * a + b;
+                                hb[q] = touched[q] ? hb[q] + h : h;
+                                touched[q] = 1;
+                            }
+                    }
+            }
+            for (size_t blk = 0; blk < key.size(); blk++)
+            {
+                const int v = key[blk].first, col = key[blk].second;
+                if (m.pinned[v] || m.pinned[col])
+                    continue;
+                int ks = -1;
+                for (int k = m.row_ptr[v]; k < m.row_ptr[v + 1]; k++)
+                    if (m.cols[k] == col)
+                        ks = k - m.row_ptr[v];
+                if (ks < 0)
+                    throw std::runtime_error("contacts: collision Hessian block outside the pattern");
+                const int p = row(v), sl = p / SLICE, lane = p % SLICE;
+                widx.push_back(9LL * host_slice_off_[sl] + 9LL * ks * SLICE + lane);
+                diag.push_back(v == col ? p : -1);
+                h9.insert(h9.end(), hb.begin() + 9 * blk, hb.begin() + 9 * blk + 9);
+            }
+        }
+        // repulsion records and levels
+        const double dt = m.cfg.dt;
+        const float kf = (m.cfg.stretch_k[0] + (m.cfg.stretch_k[1] + m.cfg.stretch_k[2])) / 3.0f;
+        const double dtK = dt * (double)(kf * 1);
+        std::vector<int> lvl;
+        std::vector<int4> cid;
+        std::vector<double> crec;
+        last_level.assign(m.nv, -1);
+        auto add = [&](int4 id, const int *verts, int nvert, const double *r) {
+            int L = 0;
+            for (int l = 0; l < nvert; l++)
+                L = std::max(L, last_level[verts[l]] + 1);
+            for (int l = 0; l < nvert; l++)
+                last_level[verts[l]] = L;
+            lvl.push_back(L);
+            cid.push_back(id);
+            crec.insert(crec.end(), r, r + CONTACT_REC);
+        };
+        const int n_ee = (int)c.ee_depth.size();
+        for (int k = 0; k < n_ee; k++)
+        {
+            const bool cloth0 = c.ee_ids[3 * k] != 0;
+            const int e = cloth0 ? c.ee_ids[3 * k + 1] : c.ee_ids[3 * k + 2];
+            const float b0 = c.ee_bary[2 * k], b1 = c.ee_bary[2 * k + 1];
+            const float bc = cloth0 ? b0 : b1;
+            const int v[2] = {m.edges[e].v0, m.edges[e].v1};
+            const float den = 1 + b0 * b0 + (1 - b0) * (1 - b0) + b1 * b1 + (1 - b1) * (1 - b1);
+            const double r[CONTACT_REC] = {
+                c.ee_normal[3 * k], c.ee_normal[3 * k + 1], c.ee_normal[3 * k + 2],
+                (double)(1 - bc), (double)bc, 0.0,
+                m.mass[v[0]] + m.mass[v[1]], (double)den, 3e-3 - (double)c.ee_depth[k],
+                (double)0.1f, dtK, dt};
+            add(make_int4(0, row(v[0]), row(v[1]), 0), v, 2, r);
+        }
+        for (int k = 0; k < n_pt; k++)
+        {
+            const double *bary = &c.pt_bary[3 * (size_t)k];
+            const double den = 1 + ((bary[0] * bary[0] + bary[1] * bary[1]) + bary[2] * bary[2]);
+            const double d = 3e-3 - (double)c.pt_depth[k];
+            if (c.pt_ids[3 * k])
+            {
+                const int v = c.pt_ids[3 * k + 1];
+                const double r[CONTACT_REC] = {c.pt_normal[3 * k], c.pt_normal[3 * k + 1],
+                                               c.pt_normal[3 * k + 2], 0.0, 0.0, 0.0,
+                                               m.mass[v], den, d, 0.1, dtK, dt};
+                add(make_int4(1, row(v), 0, 0), &v, 1, r);
+            }
+            else
+            {
+                const auto &t = m.tri[c.pt_ids[3 * k + 2]];
+                const int v[3] = {t[0], t[1], t[2]};
+                const double r[CONTACT_REC] = {c.pt_normal[3 * k], c.pt_normal[3 * k + 1],
+                                               c.pt_normal[3 * k + 2], bary[0], bary[1], bary[2],
+                                               m.mass[v[0]] + m.mass[v[1]] + m.mass[v[2]], den, d,
+                                               0.1, dtK, dt};
+                add(make_int4(2, row(v[0]), row(v[1]), row(v[2])), v, 3, r);
+            }
+        }
+        // group by level, contact order kept within a level
+        const int nl = lvl.empty() ? 0 : *std::max_element(lvl.begin(), lvl.end()) + 1;
+        std::vector<int> count(nl + 1, 0);
+        for (int L : lvl)
+            count[L + 1]++;
+        for (int L = 0; L < nl; L++)
+            count[L + 1] += count[L];
+        const int base = (int)ids.size();
+        ids.resize(base + cid.size());
+        rec.resize(rec.size() + crec.size());
+        std::vector<int> fill(count.begin(), count.end() - 1);
+        for (size_t k = 0; k < cid.size(); k++)
+        {
+            const int at = base + fill[lvl[k]]++;
+            ids[at] = cid[k];
+            std::copy(crec.begin() + CONTACT_REC * k, crec.begin() + CONTACT_REC * (k + 1),
+                      rec.begin() + CONTACT_REC * (size_t)at);
+        }
+        for (int L = 0; L < nl; L++)
+            level_ptr.push_back(base + count[L + 1]);
+        wg_level.push_back((int)level_ptr.size() - 1);
+    }
+    contact_blocks_ = (int)widx.size();
+    contact_groups_ = (int)wg_level.size() - 1;
+    // one staging buffer, one device buffer, one copy
+    size_t off = 0;
+    const size_t o_widx = off;
+    off = align256(off + widx.size() * sizeof(long long));
+    const size_t o_diag = off;
+    off = align256(off + diag.size() * sizeof(int));
+    const size_t o_h9 = off;
+    off = align256(off + h9.size() * sizeof(double));
+    const size_t o_wgl = off;
+    off = align256(off + wg_level.size() * sizeof(int));
+    const size_t o_lvl = off;
+    off = align256(off + level_ptr.size() * sizeof(int));
+    const size_t o_ids = off;
+    off = align256(off + ids.size() * sizeof(int4));
+    const size_t o_rec = off;
+    off = align256(off + rec.size() * sizeof(double));
+    contact_host_.assign(off, 0);
+    auto put = [&](size_t o, const void *src, size_t bytes) {
+        if (bytes)
+            std::memcpy(contact_host_.data() + o, src, bytes);
+    };
+    put(o_widx, widx.data(), widx.size() * sizeof(long long));
+    put(o_diag, diag.data(), diag.size() * sizeof(int));
+    put(o_h9, h9.data(), h9.size() * sizeof(double));
+    put(o_wgl, wg_level.data(), wg_level.size() * sizeof(int));
+    put(o_lvl, level_ptr.data(), level_ptr.size() * sizeof(int));
+    put(o_ids, ids.data(), ids.size() * sizeof(int4));
+    put(o_rec, rec.data(), rec.size() * sizeof(double));
+    if (off > contact_cap_)
+    {
+        // the previous step's kernels have finished (step() synchronizes)
+        check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+        if (contact_dev_)
+            check(hipFree(contact_dev_), "hipFree");
+        contact_dev_ = nullptr;
+        contact_cap_ = 0;
+        check(hipMalloc(&contact_dev_, off), "hipMalloc contacts");
+        contact_cap_ = off;
+    }
+    check(hipMemcpyAsync(contact_dev_, contact_host_.data(), off, hipMemcpyHostToDevice, stream_),
+          "hipMemcpyAsync contacts");
+    char *b = static_cast<char *>(contact_dev_);
+    c_widx_ = reinterpret_cast<const long long *>(b + o_widx);
+    c_diag_ = reinterpret_cast<const int *>(b + o_diag);
+    c_h9_ = reinterpret_cast<const double *>(b + o_h9);
+    c_wgl_ = reinterpret_cast<const int *>(b + o_wgl);
+    c_lvl_ = reinterpret_cast<const int *>(b + o_lvl);
+    c_ids_ = reinterpret_cast<const int4 *>(b + o_ids);
+    c_rec_ = reinterpret_cast<const double *>(b + o_rec);
+    return true;
+}
+
 void Batch::step(int nsteps)
 {
     if (!tuned_)
@@ -1046,16 +1368,22 @@ void Batch::step(int nsteps)
             solve_gpu_form(); // assembly, solve and position update
         else
         {
+            const bool ct = upload_contacts();
             launch_elements(d_, stream_);
             launch_assemble_rows(d_, stream_);
+            if (ct) // W += collision Hessian (BaraffCloth.cpp:293)
+                launch_contact_hessian(d_, contact_blocks_, c_widx_, c_diag_, c_h9_, stream_);
             if (kind_ == SolverKind::RefPCG64)
                 solve_ref();
             else
                 solve_cg();
+            if (ct) // repulsion on dx (BaraffCloth.cpp:147)
+                launch_repulsion(d_, contact_groups_, c_wgl_, c_lvl_, c_ids_, c_rec_, stream_);
             launch_update_positions(d_, stream_);
         }
         check(hipGetLastError(), "step launch");
         check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+        clear_contacts(); // contacts last one substep (BaraffCloth.cpp:156-157)
         const double us =
             std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
         bool bad = false;

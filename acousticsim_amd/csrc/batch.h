@@ -30,6 +30,24 @@ enum class SolverKind
     GpuForm // cloth_type "fem_gpu": the reference's GPU form (gpuform.hip)
 };
 
+// Contacts handed to one cloth for its next substep, as the scene's
+// collision detection hands them to each object
+// (cBaseObject::SetPointTriangleCollisionInfo / SetEdgeEdgeCollisionInfo,
+// BaseObject.cpp:199-207; records tPointTriangleCollisionInfo and
+// tEdgeEdgeCollisionInfo, sim/collision/CollisionInfo.h:45-66)
+struct Contacts
+{
+    std::vector<int> pt_ids;       // 3 per contact: cloth_is_point, vertex, triangle
+    std::vector<double> pt_normal; // 3
+    std::vector<double> pt_bary;   // 3
+    std::vector<float> pt_depth;
+    std::vector<int> ee_ids;       // 3 per contact: cloth_is_edge0, edge0, edge1
+    std::vector<double> ee_normal; // 3
+    std::vector<float> ee_bary;    // 2: mBary0, mBary1
+    std::vector<float> ee_depth;
+    int size() const { return (int)(pt_depth.size() + ee_depth.size()); }
+};
+
 // Optional resources a Batch borrows instead of creating: the stream its
 // kernels run on, and device storage for its solver states (2 CgState per
 // cloth), so that a server can keep many batches' states in one table.
@@ -81,6 +99,14 @@ public:
     // UpdateVertexNormalFromTriangleNormal): vertex normals 3 nv, triangle
     // normals 3 nt, in the model's order; either may be null
     void normals(int i, double *vertex_normals, double *triangle_normals);
+    // contacts for cloth i's next substep, which clears them (the collision
+    // Hessian enters W, the repulsion impulses the solution; the GPU form
+    // ignores them, as the reference's does); n = 0 clears one kind
+    void set_point_triangle_contacts(int i, int n, const int *ids, const double *normal,
+                                     const double *bary, const float *depth);
+    void set_edge_edge_contacts(int i, int n, const int *ids, const double *normal,
+                                const float *bary, const float *depth);
+    int num_contacts(int i) const { return contacts_[i].size(); }
     const StepStats &stats(int i) const { return stats_[i]; }
     void set_graphs(bool on) { use_graphs_ = on && !graphs_disabled_; }
     // mean microseconds per launch of the dominant solver kernel over reps
@@ -102,6 +128,9 @@ private:
     int upload_lists(const std::vector<char> &active);
     void solve_ref();
     void solve_cg();
+    // contacts: device lists for the next step (false if there are none)
+    bool upload_contacts();
+    void clear_contacts();
 
     std::vector<ClothModel> models_;
     std::vector<int> row_base_, slice_base_, nslices_;
@@ -136,6 +165,17 @@ private:
     unsigned long long graph_clock_ = 0;
     std::vector<void *> allocations_;
     template <typename T> T *dalloc(size_t n);
+    std::vector<int> host_slice_off_;
+    std::vector<Contacts> contacts_;
+    int contact_total_ = 0;          // contacts over all cloths
+    void *contact_dev_ = nullptr;    // device lists of the next step
+    size_t contact_cap_ = 0;
+    std::vector<char> contact_host_; // their host staging (kept until the step ends)
+    int contact_blocks_ = 0, contact_groups_ = 0;
+    const long long *c_widx_ = nullptr;
+    const int *c_diag_ = nullptr, *c_wgl_ = nullptr, *c_lvl_ = nullptr;
+    const double *c_h9_ = nullptr, *c_rec_ = nullptr;
+    const int4 *c_ids_ = nullptr;
 };
 
 } // namespace bw

@@ -257,6 +257,28 @@ int bw_get_normals(const bw_cloth *c, double *vn, double *tn)
     });
 }
 
+int bw_set_point_triangle_contacts(bw_cloth *c, int n, const int *ids, const double *normal,
+                                   const double *bary, const float *depth)
+{
+    return guard([&] {
+        if (!c)
+            return fail(BW_ERR_ARG, "NULL cloth");
+        c->batch->set_point_triangle_contacts(0, n, ids, normal, bary, depth);
+        return BW_OK;
+    });
+}
+
+int bw_set_edge_edge_contacts(bw_cloth *c, int n, const int *ids, const double *normal,
+                              const float *bary, const float *depth)
+{
+    return guard([&] {
+        if (!c)
+            return fail(BW_ERR_ARG, "NULL cloth");
+        c->batch->set_edge_edge_contacts(0, n, ids, normal, bary, depth);
+        return BW_OK;
+    });
+}
+
 int bw_get_init_pos(const bw_cloth *c, double *x)
 {
     return guard([&] {
@@ -461,6 +483,29 @@ int bw_batch_get_normals(bw_batch *b, int i, double *vn, double *tn)
         if (!b || i < 0 || i >= b->batch->size())
             return fail(BW_ERR_ARG, "bad arguments");
         b->batch->normals(i, vn, tn);
+        return BW_OK;
+    });
+}
+
+int bw_batch_set_point_triangle_contacts(bw_batch *b, int i, int n, const int *ids,
+                                         const double *normal, const double *bary,
+                                         const float *depth)
+{
+    return guard([&] {
+        if (!b)
+            return fail(BW_ERR_ARG, "NULL batch");
+        b->batch->set_point_triangle_contacts(i, n, ids, normal, bary, depth);
+        return BW_OK;
+    });
+}
+
+int bw_batch_set_edge_edge_contacts(bw_batch *b, int i, int n, const int *ids,
+                                    const double *normal, const float *bary, const float *depth)
+{
+    return guard([&] {
+        if (!b)
+            return fail(BW_ERR_ARG, "NULL batch");
+        b->batch->set_edge_edge_contacts(i, n, ids, normal, bary, depth);
         return BW_OK;
     });
 }

@@ -218,6 +218,20 @@ void launch_gf_solve(const DeviceBatch &d, hipStream_t s);
 // render readback (normals.hip): triangle normals (3 per global triangle)
 // and vertex normals (3 per global vertex) at the current positions
 void launch_normals(const DeviceBatch &d, double *tn, double *vn, hipStream_t s);
+
+// collision response (contact.hip): doubles per repulsion record
+// {N[3], w[3], mass, den, d, coef, dt*K, dt}
+constexpr int CONTACT_REC = 12;
+// W += H for n summed 3x3 blocks; w_index = index of the block's entry 0 in
+// W64/W32 (entry q at + q*SLICE); diag_row = the row of a diagonal block
+// (its Jacobi diagonal is refreshed) or -1
+void launch_contact_hessian(const DeviceBatch &d, int n, const long long *w_index,
+                            const int *diag_row, const double *h9, hipStream_t s);
+// repulsion impulses on dx: workgroup g runs levels [wg_level[g],
+// wg_level[g+1]), level L the contacts [level_ptr[L], level_ptr[L+1]);
+// ids = {type (0 edge-edge, 1 cloth vertex, 2 cloth triangle), rows...}
+void launch_repulsion(const DeviceBatch &d, int n_groups, const int *wg_level,
+                      const int *level_ptr, const int4 *ids, const double *rec, hipStream_t s);
 // measurement helpers: set the solver residual to 1 on every row; one SpMV
 // launch of the solver's precision
 void launch_fill_residual(const DeviceBatch &d, bool fp32, hipStream_t s);

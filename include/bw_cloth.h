@@ -129,6 +129,32 @@ int bw_get_edges(const bw_cloth *c, int *edge5);
    areas; triangle_normals (3T doubles): normalised (x1-x0) x (x2-x1).
    Either may be NULL. */
 int bw_get_normals(const bw_cloth *c, double *vertex_normals, double *triangle_normals);
+/* Contacts for the next substep only, which clears them, as the scene's
+   collision detection hands them over before a step
+   (cSimScene::PerformCollisionDetection, SimScene.cpp:174-190):
+   cBaseObject::SetPointTriangleCollisionInfo (BaseObject.cpp:199-203) with
+   tPointTriangleCollisionInfo records (sim/collision/CollisionInfo.h:45-61).
+   n contacts; ids: n x {cloth_is_point (obj0_is_cloth_obj1_is_rb: the point
+   is a cloth vertex and the triangle a body's), vertex (mVertexId0),
+   triangle (mTriangleId1)}; normal: n x 3 (mOuterNormal); bary: n x 3
+   (mBary); depth: n (mDepth).  The step adds the collision Hessian to W
+   (cBaraffCloth::PrepareCollisionHessian, BaraffCloth.cpp:928-990, :293)
+   and the repulsion impulses to the solution (cBaraffCloth::Repulsion,
+   :626-926, :147), then forgets the contacts (:156-157).  The cloth-side
+   index (the vertex when cloth_is_point, else the triangle) must be in
+   range and the depth below 3e-3 (the reference asserts 3e-3 - depth > 0,
+   :832); the body-side index is not used.  n = 0 clears.  A "fem_gpu"
+   cloth keeps and ignores them, as the reference's GPU form does. */
+int bw_set_point_triangle_contacts(bw_cloth *c, int n, const int *ids, const double *normal,
+                                   const double *bary, const float *depth);
+/* The same for edge-edge contacts: cBaseObject::SetEdgeEdgeCollisionInfo
+   (BaseObject.cpp:204-207) with tEdgeEdgeCollisionInfo records
+   (CollisionInfo.h:62-66).  ids: n x {cloth_is_edge0, edge0 (mEdgeId0),
+   edge1 (mEdgeId1)}; normal: n x 3; bary: n x 2 (mBary0, mBary1); depth: n.
+   The cloth's edge (edge0 when cloth_is_edge0, else edge1) must be in
+   range.  Edge-edge contacts give repulsion impulses only. */
+int bw_set_edge_edge_contacts(bw_cloth *c, int n, const int *ids, const double *normal,
+                              const float *bary, const float *depth);
 /* ELL column structure in reference numbering: row v lists, ascending,
    {v} U every interior-edge stencil containing v, as
    cBaraffClothGpu::InitELLMatrix builds it (BaraffClothGpu.cpp:438-500), with
@@ -168,6 +194,13 @@ int bw_batch_set_state(bw_batch *b, int i, const double *x, const double *xpre);
 int bw_batch_get_stats(const bw_batch *b, int i, bw_step_stats *st);
 /* bw_get_normals for member i */
 int bw_batch_get_normals(bw_batch *b, int i, double *vertex_normals, double *triangle_normals);
+/* bw_set_point_triangle_contacts / bw_set_edge_edge_contacts for member i's
+   next step */
+int bw_batch_set_point_triangle_contacts(bw_batch *b, int i, int n, const int *ids,
+                                         const double *normal, const double *bary,
+                                         const float *depth);
+int bw_batch_set_edge_edge_contacts(bw_batch *b, int i, int n, const int *ids,
+                                    const double *normal, const float *bary, const float *depth);
 /* use HIP graph replay for the per-iteration launch chains (default on) */
 int bw_batch_set_graphs(bw_batch *b, int enable);
 /* Measurement (no reference counterpart): time `reps` back-to-back launches

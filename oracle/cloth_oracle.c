@@ -1088,8 +1088,7 @@ int oc_set_contacts(oc_cloth *c, int n_pt, const int *pt_ids, const double *pt_n
         c->pt_depth = xcalloc(n_pt, sizeof(float));
         memcpy(c->pt_ids, pt_ids, 3 * (size_t)n_pt * sizeof(int));
         memcpy(c->pt_normal, pt_normal, 3 * (size_t)n_pt * sizeof(double));
-        for (int k = 0; k < 3 * n_pt; k++)
-            c->pt_bary[k] = (double)(float)pt_bary[k]; /* tVector3f cast to double */
+        memcpy(c->pt_bary, pt_bary, 3 * (size_t)n_pt * sizeof(double));
         memcpy(c->pt_depth, pt_depth, (size_t)n_pt * sizeof(float));
         c->n_pt = n_pt;
     }

@@ -104,7 +104,7 @@ int oc_step(oc_cloth *c, int mode, double tol, int max_iters, oc_stats *st);
    BaseObject.cpp:199-207; cleared at BaraffCloth.cpp:156-157).
    Point-triangle contact k: pt_ids[3k..] = {cloth_is_point, vertex,
    triangle} (obj0_is_cloth_obj1_is_rb, mVertexId0, mTriangleId1), normal
-   (3), bary (3, float values), depth (float).  Edge-edge contact k:
+   (3), bary (3), depth (float).  Edge-edge contact k:
    ee_ids[3k..] = {cloth_is_edge0, edge0, edge1}, normal (3), bary (2:
    mBary0, mBary1), depth.  The collision Hessian enters W in oc_assemble
    (BaraffCloth.cpp:928-990, :293); the repulsion impulses are added to dx

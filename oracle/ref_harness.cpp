@@ -171,8 +171,7 @@ int main(int argc, char **argv)
             info->mTriangleId1 = (int)r[2];
             info->mDepth = (float)r[3];
             info->mOuterNormal = tVector3d(r[4], r[5], r[6]);
-            // the detector stores float barycentric coordinates
-            info->mBary = tVector3f((float)r[7], (float)r[8], (float)r[9]).cast<double>();
+            info->mBary = tVector3d(r[7], r[8], r[9]);
             pt_contacts.push_back(info);
         }
         for (int k = 0; k < n_ee; k++, r += 9)
