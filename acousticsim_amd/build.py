@@ -38,6 +38,8 @@ SOURCES = [
     ("cg.hip", DEVICE + ["-ffp-contract=fast"]),
     ("normals.hip", DEVICE + ["-ffp-contract=off"]),
     ("contact.hip", DEVICE + ["-ffp-contract=off"]),
+    ("collide.hip", DEVICE + ["-ffp-contract=off"]),
+    ("body.cpp", ["-ffp-contract=off"]),
     ("gpuform.hip", DEVICE + ["-ffp-contract=fast"]),
 ]
 

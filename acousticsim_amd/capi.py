@@ -86,6 +86,9 @@ EXPORTS = [
     "bw_batch_destroy", "bw_batch_size", "bw_batch_step", "bw_batch_num_vertices",
     "bw_batch_num_triangles", "bw_set_point_triangle_contacts", "bw_set_edge_edge_contacts",
     "bw_batch_set_point_triangle_contacts", "bw_batch_set_edge_edge_contacts",
+    "bw_get_point_triangle_contacts", "bw_get_edge_edge_contacts", "bw_body_create",
+    "bw_body_destroy", "bw_body_set_pos", "bw_body_num_edges", "bw_body_get_edges",
+    "bw_detect_contacts", "bw_batch_detect_contacts",
     "bw_batch_get_pos", "bw_batch_set_state", "bw_batch_get_stats", "bw_batch_set_graphs",
     "bw_batch_time_solver_kernel", "bw_batch_autotune", "bw_server_create", "bw_server_destroy", "bw_server_submit",
     "bw_server_wait", "bw_server_get_info", "bw_server_get_pos", "bw_server_release",
@@ -144,6 +147,15 @@ def lib():
         "bw_set_edge_edge_contacts": (i32, [vp, i32, ip, dp, fp, fp]),
         "bw_batch_set_point_triangle_contacts": (i32, [vp, i32, i32, ip, dp, dp, fp]),
         "bw_batch_set_edge_edge_contacts": (i32, [vp, i32, i32, ip, dp, fp, fp]),
+        "bw_get_point_triangle_contacts": (i32, [vp, i32, ip, dp, dp, fp]),
+        "bw_get_edge_edge_contacts": (i32, [vp, i32, ip, dp, fp, fp]),
+        "bw_body_create": (i32, [dp, i32, ip, i32, P(vp)]),
+        "bw_body_destroy": (None, [vp]),
+        "bw_body_set_pos": (i32, [vp, dp]),
+        "bw_body_num_edges": (i32, [vp]),
+        "bw_body_get_edges": (i32, [vp, ip]),
+        "bw_detect_contacts": (i32, [vp, vp, i32, ip, ip]),
+        "bw_batch_detect_contacts": (i32, [vp, i32, vp, i32, ip, ip]),
         "bw_batch_get_pos": (i32, [vp, i32, dp]),
         "bw_batch_set_state": (i32, [vp, i32, dp, dp]),
         "bw_batch_get_stats": (i32, [vp, i32, P(StepStats)]),
@@ -324,6 +336,29 @@ class BaraffClothHip:
                                                _dp(a["ee_normal"]), _fp(a["ee_bary"]),
                                                _fp(a["ee_depth"])))
 
+    def contacts(self):
+        """the contacts set for the next substep, as a contact dict"""
+        L = lib()
+        n = L.bw_get_point_triangle_contacts(self._h, 0, None, None, None, None)
+        _check(n)
+        m = L.bw_get_edge_edge_contacts(self._h, 0, None, None, None, None)
+        _check(m)
+        ct = {"pt_ids": np.zeros((n, 3), np.int32), "pt_normal": np.zeros((n, 3)),
+              "pt_bary": np.zeros((n, 3)), "pt_depth": np.zeros(n, np.float32),
+              "ee_ids": np.zeros((m, 3), np.int32), "ee_normal": np.zeros((m, 3)),
+              "ee_bary": np.zeros((m, 2), np.float32), "ee_depth": np.zeros(m, np.float32)}
+        _check(L.bw_get_point_triangle_contacts(self._h, n, _ip(ct["pt_ids"]), _dp(ct["pt_normal"]),
+                                                _dp(ct["pt_bary"]), _fp(ct["pt_depth"])))
+        _check(L.bw_get_edge_edge_contacts(self._h, m, _ip(ct["ee_ids"]), _dp(ct["ee_normal"]),
+                                           _fp(ct["ee_bary"]), _fp(ct["ee_depth"])))
+        return ct
+
+    def detect_contacts(self, body, cloth_first=True):
+        """collision detection against a KinematicBody; the contacts
+        replace this cloth's for its next substep; returns them"""
+        _check(lib().bw_detect_contacts(self._h, body._h, 1 if cloth_first else 0, None, None))
+        return self.contacts()
+
     def set_contacts(self, ct):
         """both kinds from one contact dict"""
         self.SetPointTriangleCollisionInfo(ct["pt_ids"], ct["pt_normal"], ct["pt_bary"],
@@ -475,10 +510,55 @@ class ClothBatch:
             self._h, i, a["ee_depth"].size, _ip(a["ee_ids"]), _dp(a["ee_normal"]),
             _fp(a["ee_bary"]), _fp(a["ee_depth"])))
 
+    def detect_contacts(self, i, body, cloth_first=True):
+        """collision detection of member i against a KinematicBody;
+        returns (point-triangle, edge-edge) counts"""
+        a, b = ctypes.c_int(), ctypes.c_int()
+        _check(lib().bw_batch_detect_contacts(self._h, i, body._h, 1 if cloth_first else 0,
+                                              ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
     def stats(self, i):
         st = StepStats()
         _check(lib().bw_batch_get_stats(self._h, i, ctypes.byref(st)))
         return st.as_dict()
+
+
+class KinematicBody:
+    """a kinematic body (triangle mesh) for collision detection
+    (bw_body_create)"""
+
+    def __init__(self, pos, tri):
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+        tri = np.ascontiguousarray(tri, dtype=np.int32).reshape(-1)
+        self.nv, self.nt = pos.size // 3, tri.size // 3
+        h = ctypes.c_void_p()
+        _check(lib().bw_body_create(_dp(pos), self.nv, _ip(tri), self.nt, ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().bw_body_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_pos(self, pos):
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+        if pos.size != 3 * self.nv:
+            raise ValueError("expected %d positions" % (3 * self.nv))
+        _check(lib().bw_body_set_pos(self._h, _dp(pos)))
+
+    def edges(self):
+        ne = lib().bw_body_num_edges(self._h)
+        _check(ne)
+        e = np.zeros(4 * ne, np.int32)
+        _check(lib().bw_body_get_edges(self._h, _ip(e)))
+        return e.reshape(-1, 4)
 
 
 class Server:

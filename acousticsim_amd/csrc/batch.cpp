@@ -5,6 +5,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
+#include <climits>
 #include <cstring>
 #include <stdexcept>
 #include <unordered_map>
@@ -183,6 +184,10 @@ Batch::~Batch()
         (void)hipHostFree(host_upd_list_);
     if (contact_dev_)
         (void)hipFree(contact_dev_);
+    if (detect_count_dev_)
+        (void)hipFree(detect_count_dev_);
+    if (detect_hit_dev_)
+        (void)hipFree(detect_hit_dev_);
     if (stream_ && own_stream_)
         (void)hipStreamDestroy(stream_);
 }
@@ -1355,6 +1360,163 @@ bool Batch::upload_contacts()
     c_ids_ = reinterpret_cast<const int4 *>(b + o_ids);
     c_rec_ = reinterpret_cast<const double *>(b + o_rec);
     return true;
+}
+
+namespace
+{
+// grow-only device scratch (the caller has synchronized the stream)
+void *ensure(void *&p, size_t &cap, size_t bytes)
+{
+    if (bytes > cap)
+    {
+        if (p)
+            check(hipFree(p), "hipFree");
+        p = nullptr;
+        cap = 0;
+        check(hipMalloc(&p, bytes), "hipMalloc detection");
+        cap = bytes;
+    }
+    return p;
+}
+} // namespace
+
+void Batch::detect_contacts(int i, const Body &body, bool cloth_first, int *n_pt, int *n_ee)
+{
+    if (i < 0 || i >= size())
+        throw std::invalid_argument("detect: bad cloth index");
+    const ClothModel &m = models_[i];
+    if (!vrow_dev_)
+    {
+        std::vector<int> vrow(std::max(1, d_.n_vert)), erow;
+        edge_base_.assign(size(), 0);
+        for (int k = 0; k < size(); k++)
+        {
+            for (int v = 0; v < models_[k].nv; v++)
+                vrow[vert_base_[k] + v] = row_base_[k] + row_of_[k][v];
+            edge_base_[k] = (int)erow.size() / 2;
+            for (const auto &e : models_[k].edges)
+            {
+                erow.push_back(row_base_[k] + row_of_[k][e.v0]);
+                erow.push_back(row_base_[k] + row_of_[k][e.v1]);
+            }
+        }
+        vrow_dev_ = dalloc<int>(vrow.size());
+        upload(vrow_dev_, vrow);
+        erow_dev_ = dalloc<int>(std::max<size_t>(erow.size(), 1));
+        upload(erow_dev_, erow);
+    }
+    if (!normals_dev_)
+        normals_dev_ = dalloc<double>(3 * ((size_t)d_.n_tri + (size_t)d_.n_vert));
+    double *tn_dev = normals_dev_, *vn_dev = normals_dev_ + 3 * (size_t)d_.n_tri;
+    // the cloth's triangle normals at its current positions
+    launch_normals(d_, tn_dev, vn_dev, stream_);
+    ClothView cv;
+    cv.nv = m.nv;
+    cv.nt = m.nt;
+    cv.ne = m.ne;
+    cv.vbase = vert_base_[i];
+    cv.tbase = tri_base_[i];
+    cv.vrow = vrow_dev_ + vert_base_[i];
+    cv.vt_ptr = d_.vt_ptr;
+    cv.tri_rows = d_.tri_rows;
+    cv.edge_rows = erow_dev_ + 2 * (size_t)edge_base_[i];
+    cv.X = d_.x;
+    cv.tn = tn_dev;
+    // pass 1: counts per cloth vertex, triangle and edge
+    const size_t N = (size_t)m.nv + m.nt + m.ne;
+    check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    int *cnt = static_cast<int *>(ensure(detect_count_dev_, detect_count_cap_, 2 * N * sizeof(int)));
+    int *c1 = cnt, *c2 = cnt + m.nv, *c3 = cnt + m.nv + m.nt;
+    launch_detect_points(cv, body, c1, nullptr, nullptr, stream_);
+    launch_detect_body_points(cv, body, c2, nullptr, nullptr, stream_);
+    launch_detect_edges(cv, body, cloth_first ? 1 : 0, c3, nullptr, nullptr, stream_);
+    check(hipGetLastError(), "detect launch");
+    check(hipStreamSynchronize(stream_), "detect sync");
+    std::vector<int> counts(N), offs(N);
+    check(hipMemcpy(counts.data(), cnt, N * sizeof(int), hipMemcpyDeviceToHost), "hipMemcpy counts");
+    // offsets: one exclusive scan per kernel, kernels back to back
+    long long run = 0;
+    size_t tot[3] = {0, 0, 0};
+    const size_t seg[4] = {0, (size_t)m.nv, (size_t)m.nv + m.nt, N};
+    for (int k = 0; k < 3; k++)
+    {
+        const long long start = run;
+        for (size_t q = seg[k]; q < seg[k + 1]; q++)
+        {
+            offs[q] = (int)(run - start);
+            run += counts[q];
+        }
+        tot[k] = (size_t)(run - start);
+    }
+    const size_t total = tot[0] + tot[1] + tot[2];
+    std::vector<Hit> hits(total);
+    if (total > 0)
+    {
+        if (total > (size_t)INT32_MAX)
+            throw std::runtime_error("capacity: too many contacts");
+        int *off_dev = cnt + N;
+        check(hipMemcpy(off_dev, offs.data(), N * sizeof(int), hipMemcpyHostToDevice),
+              "hipMemcpy offsets");
+        Hit *h = static_cast<Hit *>(ensure(detect_hit_dev_, detect_hit_cap_, total * sizeof(Hit)));
+        launch_detect_points(cv, body, nullptr, off_dev, h, stream_);
+        launch_detect_body_points(cv, body, nullptr, off_dev + m.nv, h + tot[0], stream_);
+        launch_detect_edges(cv, body, cloth_first ? 1 : 0, nullptr, off_dev + m.nv + m.nt,
+                            h + tot[0] + tot[1], stream_);
+        check(hipGetLastError(), "detect launch");
+        check(hipStreamSynchronize(stream_), "detect sync");
+        check(hipMemcpy(hits.data(), h, total * sizeof(Hit), hipMemcpyDeviceToHost),
+              "hipMemcpy hits");
+    }
+    auto by_key = [](const Hit &a, const Hit &b) {
+        return a.key0 != b.key0 ? a.key0 < b.key0 : a.key1 < b.key1;
+    };
+    // cloth-vertex hits come in (vertex, triangle) order; body-vertex hits in
+    // (cloth triangle, body vertex) order, so sort those by (vertex, triangle)
+    std::vector<Hit> cv_hits(hits.begin(), hits.begin() + tot[0]);
+    std::vector<Hit> bv_hits(hits.begin() + tot[0], hits.begin() + tot[0] + tot[1]);
+    std::vector<Hit> ee_hits(hits.begin() + tot[0] + tot[1], hits.end());
+    std::sort(bv_hits.begin(), bv_hits.end(), by_key);
+    if (!cloth_first)
+        std::sort(ee_hits.begin(), ee_hits.end(), by_key);
+    // obj0's points against obj1's triangles first (BVHCollisionDetecter.cpp:411-413)
+    std::vector<Hit> pt;
+    pt.reserve(tot[0] + tot[1]);
+    const std::vector<Hit> &first = cloth_first ? cv_hits : bv_hits;
+    const std::vector<Hit> &second = cloth_first ? bv_hits : cv_hits;
+    pt.insert(pt.end(), first.begin(), first.end());
+    pt.insert(pt.end(), second.begin(), second.end());
+    std::vector<int> pids(3 * pt.size()), eids(3 * ee_hits.size());
+    std::vector<double> pn(3 * pt.size()), pb(3 * pt.size()), en(3 * ee_hits.size());
+    std::vector<float> pd(pt.size()), eb(2 * ee_hits.size()), ed(ee_hits.size());
+    for (size_t k = 0; k < pt.size(); k++)
+    {
+        pids[3 * k] = pt[k].flag;
+        pids[3 * k + 1] = pt[k].key0;
+        pids[3 * k + 2] = pt[k].key1;
+        for (int a = 0; a < 3; a++)
+        {
+            pn[3 * k + a] = pt[k].n[a];
+            pb[3 * k + a] = (double)pt[k].bary[a];
+        }
+        pd[k] = pt[k].depth;
+    }
+    for (size_t k = 0; k < ee_hits.size(); k++)
+    {
+        eids[3 * k] = ee_hits[k].flag;
+        eids[3 * k + 1] = ee_hits[k].key0;
+        eids[3 * k + 2] = ee_hits[k].key1;
+        for (int a = 0; a < 3; a++)
+            en[3 * k + a] = ee_hits[k].n[a];
+        eb[2 * k] = ee_hits[k].bary[0];
+        eb[2 * k + 1] = ee_hits[k].bary[1];
+        ed[k] = ee_hits[k].depth;
+    }
+    set_point_triangle_contacts(i, (int)pt.size(), pids.data(), pn.data(), pb.data(), pd.data());
+    set_edge_edge_contacts(i, (int)ee_hits.size(), eids.data(), en.data(), eb.data(), ed.data());
+    if (n_pt)
+        *n_pt = (int)pt.size();
+    if (n_ee)
+        *n_ee = (int)ee_hits.size();
 }
 
 void Batch::step(int nsteps)

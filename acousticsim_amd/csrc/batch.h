@@ -5,6 +5,7 @@
 #include <string>
 #include <vector>
 
+#include "collide.h"
 #include "device.h"
 #include "model.h"
 
@@ -107,6 +108,14 @@ public:
     void set_edge_edge_contacts(int i, int n, const int *ids, const double *normal,
                                 const float *bary, const float *depth);
     int num_contacts(int i) const { return contacts_[i].size(); }
+    const Contacts &contacts(int i) const { return contacts_[i]; }
+    // collision detection of cloth i at its current positions against a
+    // kinematic body (cBVHCollisionDetecter::PerformCD for the pair,
+    // BVHCollisionDetecter.cpp:360-651); the contacts replace cloth i's for
+    // its next substep.  cloth_first: the cloth is obj0 of the pair (its
+    // object id is the lower one).  Returns the point-triangle and
+    // edge-edge counts.
+    void detect_contacts(int i, const Body &body, bool cloth_first, int *n_pt, int *n_ee);
     const StepStats &stats(int i) const { return stats_[i]; }
     void set_graphs(bool on) { use_graphs_ = on && !graphs_disabled_; }
     // mean microseconds per launch of the dominant solver kernel over reps
@@ -171,6 +180,11 @@ private:
     void *contact_dev_ = nullptr;    // device lists of the next step
     size_t contact_cap_ = 0;
     std::vector<char> contact_host_; // their host staging (kept until the step ends)
+    // detection: global vertex -> row, 2 rows per edge (all cloths), scratch
+    int *vrow_dev_ = nullptr, *erow_dev_ = nullptr;
+    std::vector<int> edge_base_;
+    void *detect_count_dev_ = nullptr, *detect_hit_dev_ = nullptr;
+    size_t detect_count_cap_ = 0, detect_hit_cap_ = 0;
     int contact_blocks_ = 0, contact_groups_ = 0;
     const long long *c_widx_ = nullptr;
     const int *c_diag_ = nullptr, *c_wgl_ = nullptr, *c_lvl_ = nullptr;

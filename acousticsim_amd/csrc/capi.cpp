@@ -2,6 +2,7 @@
 // point catches exceptions and returns a status; nothing exits the process.
 #include "../../include/bw_cloth.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -65,6 +66,11 @@ struct bw_cloth
 struct bw_batch
 {
     std::unique_ptr<bw::Batch> batch;
+};
+
+struct bw_body
+{
+    std::unique_ptr<bw::Body> body;
 };
 
 extern "C" {
@@ -275,6 +281,92 @@ int bw_set_edge_edge_contacts(bw_cloth *c, int n, const int *ids, const double *
         if (!c)
             return fail(BW_ERR_ARG, "NULL cloth");
         c->batch->set_edge_edge_contacts(0, n, ids, normal, bary, depth);
+        return BW_OK;
+    });
+}
+
+int bw_get_point_triangle_contacts(const bw_cloth *c, int cap, int *ids, double *normal,
+                                   double *bary, float *depth)
+{
+    return guard([&] {
+        if (!c || cap < 0)
+            return fail(BW_ERR_ARG, "bad arguments");
+        const bw::Contacts &ct = c->batch->contacts(0);
+        const int n = (int)ct.pt_depth.size(), m = std::min(n, cap);
+        if (m > 0 && (!ids || !normal || !bary || !depth))
+            return fail(BW_ERR_ARG, "NULL output array");
+        std::copy(ct.pt_ids.begin(), ct.pt_ids.begin() + 3 * (size_t)m, ids);
+        std::copy(ct.pt_normal.begin(), ct.pt_normal.begin() + 3 * (size_t)m, normal);
+        std::copy(ct.pt_bary.begin(), ct.pt_bary.begin() + 3 * (size_t)m, bary);
+        std::copy(ct.pt_depth.begin(), ct.pt_depth.begin() + m, depth);
+        return n;
+    });
+}
+
+int bw_get_edge_edge_contacts(const bw_cloth *c, int cap, int *ids, double *normal, float *bary,
+                              float *depth)
+{
+    return guard([&] {
+        if (!c || cap < 0)
+            return fail(BW_ERR_ARG, "bad arguments");
+        const bw::Contacts &ct = c->batch->contacts(0);
+        const int n = (int)ct.ee_depth.size(), m = std::min(n, cap);
+        if (m > 0 && (!ids || !normal || !bary || !depth))
+            return fail(BW_ERR_ARG, "NULL output array");
+        std::copy(ct.ee_ids.begin(), ct.ee_ids.begin() + 3 * (size_t)m, ids);
+        std::copy(ct.ee_normal.begin(), ct.ee_normal.begin() + 3 * (size_t)m, normal);
+        std::copy(ct.ee_bary.begin(), ct.ee_bary.begin() + 2 * (size_t)m, bary);
+        std::copy(ct.ee_depth.begin(), ct.ee_depth.begin() + m, depth);
+        return n;
+    });
+}
+
+int bw_body_create(const double *pos, int nv, const int *tri, int nt, bw_body **out)
+{
+    return guard([&] {
+        if (!out)
+            return fail(BW_ERR_ARG, "NULL output handle");
+        *out = nullptr;
+        auto b = std::make_unique<bw_body>();
+        b->body = std::make_unique<bw::Body>(pos, nv, tri, nt);
+        *out = b.release();
+        return BW_OK;
+    });
+}
+
+void bw_body_destroy(bw_body *b) { delete b; }
+
+int bw_body_set_pos(bw_body *b, const double *pos)
+{
+    return guard([&] {
+        if (!b)
+            return fail(BW_ERR_ARG, "NULL body");
+        b->body->set_pos(pos);
+        return BW_OK;
+    });
+}
+
+int bw_body_num_edges(const bw_body *b)
+{
+    if (!b)
+        return fail(BW_ERR_ARG, "NULL body");
+    return b->body->ne;
+}
+
+int bw_body_get_edges(const bw_body *b, int *edge4)
+{
+    if (!b || !edge4)
+        return fail(BW_ERR_ARG, "NULL argument");
+    std::copy(b->body->edge.begin(), b->body->edge.end(), edge4);
+    return BW_OK;
+}
+
+int bw_detect_contacts(bw_cloth *c, const bw_body *b, int cloth_first, int *n_pt, int *n_ee)
+{
+    return guard([&] {
+        if (!c || !b)
+            return fail(BW_ERR_ARG, "NULL cloth or body");
+        c->batch->detect_contacts(0, *b->body, cloth_first != 0, n_pt, n_ee);
         return BW_OK;
     });
 }
@@ -506,6 +598,17 @@ int bw_batch_set_edge_edge_contacts(bw_batch *b, int i, int n, const int *ids,
         if (!b)
             return fail(BW_ERR_ARG, "NULL batch");
         b->batch->set_edge_edge_contacts(i, n, ids, normal, bary, depth);
+        return BW_OK;
+    });
+}
+
+int bw_batch_detect_contacts(bw_batch *bt, int i, const bw_body *b, int cloth_first, int *n_pt,
+                             int *n_ee)
+{
+    return guard([&] {
+        if (!bt || !b)
+            return fail(BW_ERR_ARG, "NULL batch or body");
+        bt->batch->detect_contacts(i, *b->body, cloth_first != 0, n_pt, n_ee);
         return BW_OK;
     });
 }

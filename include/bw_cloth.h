@@ -44,6 +44,7 @@ extern "C" {
 
 typedef struct bw_cloth bw_cloth;
 typedef struct bw_batch bw_batch;
+typedef struct bw_body bw_body;
 
 typedef struct bw_step_stats
 {
@@ -155,6 +156,42 @@ int bw_set_point_triangle_contacts(bw_cloth *c, int n, const int *ids, const dou
    range.  Edge-edge contacts give repulsion impulses only. */
 int bw_set_edge_edge_contacts(bw_cloth *c, int n, const int *ids, const double *normal,
                               const float *bary, const float *depth);
+/* The contacts set for the next substep (by the calls above or by
+   bw_detect_contacts), in the layouts above; each returns the count and
+   copies at most cap records (cBaseObject's mPointTriangleCollisionInfo and
+   mEdgeEdgeCollisionInfo, BaseObject.h:101-102). */
+int bw_get_point_triangle_contacts(const bw_cloth *c, int cap, int *ids, double *normal,
+                                   double *bary, float *depth);
+int bw_get_edge_edge_contacts(const bw_cloth *c, int cap, int *ids, double *normal, float *bary,
+                              float *depth);
+
+/* A kinematic body for collision detection (cKinematicBody,
+   src/sim/kinematic/KinematicBody.cpp): a triangle mesh (positions 3*nv,
+   triangles 3*nt) whose edges are numbered as cObjUtil::BuildEdge numbers
+   them (ascending {min, max}; ObjUtil.cpp:141-224) and whose triangle and
+   vertex normals are those of the positions given here (cKinematicBody::Init
+   computes them once, KinematicBody.cpp:114-116).  BW_ERR_ARG for an index
+   out of range or an edge shared by more than two triangles. */
+int bw_body_create(const double *pos, int nv, const int *tri, int nt, bw_body **out);
+void bw_body_destroy(bw_body *b);
+/* move the body (cKinematicBody::SetMeshPos, KinematicBody.cpp:189-198);
+   its normals keep their creation values, as the reference's do */
+int bw_body_set_pos(bw_body *b, const double *pos);
+int bw_body_num_edges(const bw_body *b);
+/* edges as {id0, id1, tri0, tri1 (-1 on the boundary)}, E x 4 */
+int bw_body_get_edges(const bw_body *b, int *edge4);
+/* Collision detection of the cloth at its current positions against the
+   body, on the device (cBVHCollisionDetecter::PerformCD for the pair,
+   BVHCollisionDetecter.cpp:360-651; cSimScene::PerformCollisionDetection
+   hands the result to the cloth, SimScene.cpp:174-190): every cloth
+   vertex / body triangle, body vertex / cloth triangle and edge / edge pair
+   closer than 3e-3, in the reference's order, replaces the cloth's contacts
+   for its next substep.  cloth_first = 1 when the cloth's object id is
+   below the body's (the cloth is obj0 of the pair).  The counts of
+   point-triangle and edge-edge contacts go to n_point_triangle and
+   n_edge_edge (either may be NULL). */
+int bw_detect_contacts(bw_cloth *c, const bw_body *b, int cloth_first, int *n_point_triangle,
+                       int *n_edge_edge);
 /* ELL column structure in reference numbering: row v lists, ascending,
    {v} U every interior-edge stencil containing v, as
    cBaraffClothGpu::InitELLMatrix builds it (BaraffClothGpu.cpp:438-500), with
@@ -201,6 +238,9 @@ int bw_batch_set_point_triangle_contacts(bw_batch *b, int i, int n, const int *i
                                          const float *depth);
 int bw_batch_set_edge_edge_contacts(bw_batch *b, int i, int n, const int *ids,
                                     const double *normal, const float *bary, const float *depth);
+/* bw_detect_contacts for member i */
+int bw_batch_detect_contacts(bw_batch *bt, int i, const bw_body *b, int cloth_first,
+                             int *n_point_triangle, int *n_edge_edge);
 /* use HIP graph replay for the per-iteration launch chains (default on) */
 int bw_batch_set_graphs(bw_batch *b, int enable);
 /* Measurement (no reference counterpart): time `reps` back-to-back launches

@@ -1928,3 +1928,510 @@ void oc_gpu_form_step(oc_cloth *c, oc_stats *st)
     }
     free(tri); free(K); free(fs); free(W); free(b); free(Pinv); free(r); free(d); free(z);
 }
+
+/* ------------------------------------------------------------------ */
+/* collision detection against a kinematic body                        */
+/* ------------------------------------------------------------------ */
+
+struct oc_body
+{
+    int nv, nt, ne;
+    double *x;   /* 3 nv, current positions */
+    int *tri;    /* 3 nt */
+    int *edge;   /* 4 ne: id0, id1, tri0, tri1 (-1) */
+    double *tn;  /* 3 nt, normals at creation */
+    double *vn;  /* 3 nv, normals at creation */
+};
+
+/* Vector3d dot and norm: Eigen's vectorised sum (a0 b0 + a1 b1) + a2 b2 */
+static double d3(const double *a, const double *b) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
+
+oc_body *oc_body_create(const double *xyz, int nv, const int *tri, int nt)
+{
+    if (nv < 3 || nt < 1)
+        return NULL;
+    for (int k = 0; k < 3 * nt; k++)
+        if (tri[k] < 0 || tri[k] >= nv)
+            return NULL;
+    oc_body *b = xcalloc(1, sizeof(oc_body));
+    b->nv = nv;
+    b->nt = nt;
+    b->x = xcalloc(3 * (size_t)nv, sizeof(double));
+    memcpy(b->x, xyz, 3 * (size_t)nv * sizeof(double));
+    b->tri = xcalloc(3 * (size_t)nt, sizeof(int));
+    memcpy(b->tri, tri, 3 * (size_t)nt * sizeof(int));
+    /* edges as cObjUtil::BuildEdge numbers them (ObjUtil.cpp:141-224):
+       ascending {min, max}, tri0 the first triangle with the side */
+    int ns = 3 * nt;
+    oc_side *sd = xcalloc(ns, sizeof(oc_side));
+    for (int t = 0; t < nt; t++)
+        for (int k = 0; k < 3; k++)
+        {
+            int a = tri[3 * t + k], c = tri[3 * t + (k + 1) % 3];
+            sd[3 * t + k].a = a < c ? a : c;
+            sd[3 * t + k].b = a < c ? c : a;
+            sd[3 * t + k].side = 3 * t + k;
+        }
+    qsort(sd, ns, sizeof(oc_side), side_cmp);
+    b->edge = xcalloc(4 * (size_t)ns, sizeof(int));
+    for (int i = 0; i < ns;)
+    {
+        int j = i + 1;
+        while (j < ns && sd[j].a == sd[i].a && sd[j].b == sd[i].b)
+            j++;
+        if (j - i > 2)
+        {
+            free(sd);
+            oc_body_destroy(b);
+            return NULL;
+        }
+        int *e = b->edge + 4 * b->ne++;
+        e[0] = sd[i].a;
+        e[1] = sd[i].b;
+        e[2] = sd[i].side / 3;
+        e[3] = j - i == 2 ? sd[i + 1].side / 3 : -1;
+        i = j;
+    }
+    free(sd);
+    /* cKinematicBody::Init (KinematicBody.cpp:114-116): CalcTriangleInitArea
+       (BaseObject.cpp:291-302, float 0.5 |(p1 - p0) x (p2 - p0)|),
+       UpdateTriangleNormal, UpdateVertexNormalFromTriangleNormal */
+    b->tn = xcalloc(3 * (size_t)nt, sizeof(double));
+    b->vn = xcalloc(3 * (size_t)nv, sizeof(double));
+    float *area = xcalloc(nt, sizeof(float));
+    for (int t = 0; t < nt; t++)
+    {
+        const double *p0 = b->x + 3 * tri[3 * t], *p1 = b->x + 3 * tri[3 * t + 1];
+        const double *p2 = b->x + 3 * tri[3 * t + 2];
+        double u[3], v[3], w[3], *n = b->tn + 3 * t;
+        v3sub(p1, p0, u);
+        v3sub(p2, p1, v);
+        n[0] = u[1] * v[2] - u[2] * v[1];
+        n[1] = u[2] * v[0] - u[0] * v[2];
+        n[2] = u[0] * v[1] - u[1] * v[0];
+        oc_normalize3(n);
+        v3sub(p2, p0, v);
+        w[0] = u[1] * v[2] - u[2] * v[1];
+        w[1] = u[2] * v[0] - u[0] * v[2];
+        w[2] = u[0] * v[1] - u[1] * v[0];
+        area[t] = (float)(0.5 * sqrt(d3(w, w)));
+    }
+    for (int t = 0; t < nt; t++)
+        for (int k = 0; k < 3; k++)
+        {
+            double *v = b->vn + 3 * tri[3 * t + k];
+            for (int a = 0; a < 3; a++)
+                v[a] = v[a] + b->tn[3 * t + a] * (double)area[t];
+        }
+    for (int v = 0; v < nv; v++)
+        oc_normalize3(b->vn + 3 * v);
+    free(area);
+    return b;
+}
+
+void oc_body_destroy(oc_body *b)
+{
+    if (!b)
+        return;
+    free(b->x); free(b->tri); free(b->edge); free(b->tn); free(b->vn);
+    free(b);
+}
+
+void oc_body_set_pos(oc_body *b, const double *xyz) { memcpy(b->x, xyz, 3 * (size_t)b->nv * sizeof(double)); }
+int oc_body_num_edges(const oc_body *b) { return b->ne; }
+void oc_body_get_edges(const oc_body *b, int *e4) { memcpy(e4, b->edge, 4 * (size_t)b->ne * sizeof(int)); }
+void oc_body_normals(const oc_body *b, double *vn3, double *tn3)
+{
+    if (vn3)
+        memcpy(vn3, b->vn, 3 * (size_t)b->nv * sizeof(double));
+    if (tn3)
+        memcpy(tn3, b->tn, 3 * (size_t)b->nt * sizeof(double));
+}
+
+/* ClosestPtPointTriangle (BVHCollisionDetecter.cpp:17-83, Ericson's
+   region test): dot products in double rounded to float, region
+   arithmetic in float, the point in double */
+void oc_closest_pt_triangle(const double *p, const double *a, const double *b, const double *c,
+                            float *bary, double *out)
+{
+    double ab[3], ac[3], ap[3], bp[3], cp[3];
+    v3sub(b, a, ab);
+    v3sub(c, a, ac);
+    v3sub(p, a, ap);
+    float d1 = (float)d3(ab, ap), d2 = (float)d3(ac, ap);
+    bary[0] = bary[1] = bary[2] = 0.0f;
+    if (d1 <= 0.0f && d2 <= 0.0f)
+    {
+        bary[0] = 1;
+        memcpy(out, a, 3 * sizeof(double));
+        return;
+    }
+    v3sub(p, b, bp);
+    float d3f = (float)d3(ab, bp), d4 = (float)d3(ac, bp);
+    if (d3f >= 0.0f && d4 <= d3f)
+    {
+        bary[1] = 1;
+        memcpy(out, b, 3 * sizeof(double));
+        return;
+    }
+    float vc = d1 * d4 - d3f * d2;
+    if (vc <= 0.0f && d1 >= 0.0f && d3f <= 0.0f)
+    {
+        float v = d1 / (d1 - d3f);
+        bary[0] = 1 - v;
+        bary[1] = v;
+        for (int k = 0; k < 3; k++)
+            out[k] = a[k] + v * ab[k];
+        return;
+    }
+    v3sub(p, c, cp);
+    float d5 = (float)d3(ab, cp), d6 = (float)d3(ac, cp);
+    if (d6 >= 0.0f && d5 <= d6)
+    {
+        bary[2] = 1;
+        memcpy(out, c, 3 * sizeof(double));
+        return;
+    }
+    float vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f)
+    {
+        float w = d2 / (d2 - d6);
+        bary[0] = 1 - w;
+        bary[2] = w;
+        for (int k = 0; k < 3; k++)
+            out[k] = a[k] + w * ac[k];
+        return;
+    }
+    float va = d3f * d6 - d5 * d4;
+    if (va <= 0.0f && (d4 - d3f) >= 0.0f && (d5 - d6) >= 0.0f)
+    {
+        float w = (d4 - d3f) / ((d4 - d3f) + (d5 - d6));
+        bary[1] = 1 - w;
+        bary[2] = w;
+        for (int k = 0; k < 3; k++)
+            out[k] = b[k] + w * (c[k] - b[k]);
+        return;
+    }
+    float denom = 1.0f / (va + vb + vc);
+    float v = vb * denom, w = vc * denom;
+    bary[0] = 1 - v - w;
+    bary[1] = v;
+    bary[2] = w;
+    for (int k = 0; k < 3; k++)
+        out[k] = a[k] + ab[k] * v + ac[k] * w;
+}
+
+/* cMathUtil::Clamp (MathUtil.cpp:31-34): max(lo, min(v, hi)) in double */
+static float clampf01(float v)
+{
+    double m = (1.0 < (double)v) ? 1.0 : (double)v;
+    return (float)((0.0 < m) ? m : 0.0);
+}
+
+/* EdgeEdgeDist (BVHCollisionDetecter.cpp:99-177, Ericson's closest points
+   of two segments) with its float parameters: s on p1q1, t on p2q2 */
+void oc_edge_edge_params(const double *p1, const double *q1, const double *p2, const double *q2,
+                         float *s_out, float *t_out)
+{
+    double dd1[3], dd2[3], r[3];
+    v3sub(q1, p1, dd1);
+    v3sub(q2, p2, dd2);
+    v3sub(p1, p2, r);
+    float a = (float)d3(dd1, dd1), e = (float)d3(dd2, dd2), f = (float)d3(dd2, r);
+    const float EPSILON = 1e-10;
+    float s = 0, t = 0;
+    if (a <= EPSILON && e <= EPSILON)
+        s = t = 0.0f;
+    if (a <= EPSILON)
+    {
+        s = 0.0f;
+        t = f / e;
+        t = clampf01(t);
+    }
+    else
+    {
+        float c = (float)d3(dd1, r);
+        if (e <= EPSILON)
+        {
+            t = 0.0f;
+            s = clampf01(-c / a);
+        }
+        else
+        {
+            float bb = (float)d3(dd1, dd2);
+            float denom = a * e - bb * bb;
+            if (denom != 0.0f)
+                s = clampf01((bb * f - c * e) / denom);
+            else
+                s = 0.0f;
+            t = (bb * s + f) / e;
+            if (t < 0.0f)
+            {
+                t = 0.0f;
+                s = clampf01(-c / a);
+            }
+            else if (t > 1.0f)
+            {
+                t = 1.0f;
+                s = clampf01((bb - c) / a);
+            }
+        }
+    }
+    *s_out = s;
+    *t_out = t;
+}
+
+static void normalized3d(const double *u, double *o)
+{
+    double z = d3(u, u);
+    if (z > 0)
+    {
+        double n = sqrt(z);
+        for (int k = 0; k < 3; k++)
+            o[k] = u[k] / n;
+    }
+    else
+        memcpy(o, u, 3 * sizeof(double));
+}
+
+typedef struct
+{
+    int key0, key1;
+    int ids[3];
+    double normal[3], bary[3];
+    float depth, eb[2];
+} oc_hit;
+
+static int hit_cmp(const void *p, const void *q)
+{
+    const oc_hit *x = p, *y = q;
+    if (x->key0 != y->key0)
+        return x->key0 < y->key0 ? -1 : 1;
+    return x->key1 < y->key1 ? -1 : (x->key1 > y->key1);
+}
+
+/* cBVHCollisionDetecter::PerformCD for the pair {cloth, body}
+   (BVHCollisionDetecter.cpp:360-651).  The broadphase (leaf boxes of each
+   triangle grown by 5e-3, IntersectTwoBVHs :187-240) keeps every pair
+   closer than 1e-2, so the contacts are all pairs whose computed distance
+   is below 3e-3: (point of obj0, triangle of obj1), then (point of obj1,
+   triangle of obj0), each in ascending (point, triangle) order (std::set,
+   :291-331), then edge pairs in ascending (edge of obj0, edge of obj1).
+   obj0 is the object with the lower id: the cloth when cloth_first.
+   Vertices in no triangle are never candidates.  The cloth's triangle
+   normals are those at its current positions (cBaseCloth::Update refreshes
+   them after every substep); the body's are those of its creation
+   (cKinematicBody::Init).  The contacts replace the cloth's for its next
+   step; returns 0, -1 for a non-manifold body. */
+int oc_detect_contacts(oc_cloth *c, const oc_body *b, int cloth_first)
+{
+    double *ctn = xcalloc(3 * (size_t)c->nt, sizeof(double));
+    oc_normals(c, NULL, ctn);
+    unsigned char *c_in = xcalloc(c->nv, 1), *b_in = xcalloc(b->nv, 1);
+    for (int k = 0; k < 3 * c->nt; k++)
+        c_in[c->tri[k]] = 1;
+    for (int k = 0; k < 3 * b->nt; k++)
+        b_in[b->tri[k]] = 1;
+    size_t cap = 64, n01 = 0, n10 = 0, nee = 0;
+    oc_hit *h01 = xcalloc(cap, sizeof(oc_hit)), *h10 = xcalloc(cap, sizeof(oc_hit));
+    size_t cap01 = cap, cap10 = cap, capee = cap;
+    oc_hit *hee = xcalloc(cap, sizeof(oc_hit));
+#define PUSH(arr, n, capv, hit)                                      \
+    do                                                               \
+    {                                                                \
+        if (n == capv)                                               \
+        {                                                            \
+            capv *= 2;                                               \
+            arr = realloc(arr, capv * sizeof(oc_hit));               \
+        }                                                            \
+        arr[n++] = hit;                                              \
+    } while (0)
+    /* cloth vertex against body triangle: obj0_is_cloth_obj1_is_rb = true */
+    for (int p = 0; p < c->nv; p++)
+    {
+        if (!c_in[p])
+            continue;
+        const double *pt = c->x + 3 * p;
+        for (int t = 0; t < b->nt; t++)
+        {
+            const int *tv = b->tri + 3 * t;
+            float bary[3];
+            double cl[3], dv[3];
+            oc_closest_pt_triangle(pt, b->x + 3 * tv[0], b->x + 3 * tv[1], b->x + 3 * tv[2], bary, cl);
+            v3sub(pt, cl, dv);
+            if (sqrt(d3(dv, dv)) < 3e-3)
+            {
+                oc_hit hh;
+                memset(&hh, 0, sizeof hh);
+                hh.key0 = p;
+                hh.key1 = t;
+                hh.ids[0] = 1;
+                hh.ids[1] = p;
+                hh.ids[2] = t;
+                memcpy(hh.normal, b->tn + 3 * t, 3 * sizeof(double));
+                hh.depth = (float)d3(dv, hh.normal);
+                for (int k = 0; k < 3; k++)
+                    hh.bary[k] = (double)bary[k];
+                if (cloth_first)
+                    PUSH(h01, n01, cap01, hh);
+                else
+                    PUSH(h10, n10, cap10, hh);
+            }
+        }
+    }
+    /* body vertex against cloth triangle: obj0_is_cloth_obj1_is_rb = false */
+    for (int p = 0; p < b->nv; p++)
+    {
+        if (!b_in[p])
+            continue;
+        const double *pt = b->x + 3 * p;
+        for (int t = 0; t < c->nt; t++)
+        {
+            const int *tv = c->tri + 3 * t;
+            float bary[3];
+            double cl[3], dv[3], n[3];
+            oc_closest_pt_triangle(pt, c->x + 3 * tv[0], c->x + 3 * tv[1], c->x + 3 * tv[2], bary, cl);
+            memcpy(n, ctn + 3 * t, sizeof n);
+            if (d3(n, b->vn + 3 * p) < 0)
+                for (int k = 0; k < 3; k++)
+                    n[k] = n[k] * -1;
+            v3sub(cl, pt, dv);
+            if (sqrt(d3(dv, dv)) < 3e-3)
+            {
+                oc_hit hh;
+                memset(&hh, 0, sizeof hh);
+                hh.key0 = p;
+                hh.key1 = t;
+                hh.ids[0] = 0;
+                hh.ids[1] = p;
+                hh.ids[2] = t;
+                memcpy(hh.normal, n, sizeof n);
+                hh.depth = (float)d3(dv, n);
+                for (int k = 0; k < 3; k++)
+                    hh.bary[k] = (double)bary[k];
+                if (cloth_first)
+                    PUSH(h10, n10, cap10, hh);
+                else
+                    PUSH(h01, n01, cap01, hh);
+            }
+        }
+    }
+    /* edge pairs: e0 of obj0, e1 of obj1 */
+    for (int ce = 0; ce < c->ne; ce++)
+        for (int be = 0; be < b->ne; be++)
+        {
+            const int *cev = c->edge + 5 * ce, *bev = b->edge + 4 * be;
+            /* p0 p1 = obj0's edge, p2 p3 = obj1's */
+            const double *p0, *p1, *p2, *p3;
+            if (cloth_first)
+            {
+                p0 = c->x + 3 * cev[0]; p1 = c->x + 3 * cev[1];
+                p2 = b->x + 3 * bev[0]; p3 = b->x + 3 * bev[1];
+            }
+            else
+            {
+                p0 = b->x + 3 * bev[0]; p1 = b->x + 3 * bev[1];
+                p2 = c->x + 3 * cev[0]; p3 = c->x + 3 * cev[1];
+            }
+            float s, t;
+            oc_edge_edge_params(p0, p1, p2, p3, &s, &t);
+            double e0p[3], e1p[3], u[3], v[3], cr[3], outer[3], on[3], dv[3];
+            for (int k = 0; k < 3; k++)
+            {
+                e0p[k] = (double)(1 - s) * p0[k] + (double)s * p1[k];
+                e1p[k] = (double)(1 - t) * p2[k] + (double)t * p3[k];
+            }
+            v3sub(p1, p0, u);
+            v3sub(p3, p2, v);
+            cr[0] = u[1] * v[2] - u[2] * v[1];
+            cr[1] = u[2] * v[0] - u[0] * v[2];
+            cr[2] = u[0] * v[1] - u[1] * v[0];
+            normalized3d(cr, outer);
+            /* the body edge's triangle normals set the side */
+            memcpy(on, b->tn + 3 * bev[2], sizeof on);
+            if (bev[3] != -1)
+                for (int k = 0; k < 3; k++)
+                    on[k] = on[k] + b->tn[3 * bev[3] + k];
+            normalized3d(on, on);
+            if (d3(on, outer) < 0)
+                for (int k = 0; k < 3; k++)
+                    outer[k] = outer[k] * -1;
+            /* from the body to the cloth */
+            if (cloth_first)
+                v3sub(e0p, e1p, dv);
+            else
+                v3sub(e1p, e0p, dv);
+            if (sqrt(d3(dv, dv)) < 3e-3)
+            {
+                oc_hit hh;
+                memset(&hh, 0, sizeof hh);
+                hh.key0 = cloth_first ? ce : be;
+                hh.key1 = cloth_first ? be : ce;
+                hh.ids[0] = cloth_first ? 1 : 0;
+                hh.ids[1] = hh.key0;
+                hh.ids[2] = hh.key1;
+                memcpy(hh.normal, outer, sizeof outer);
+                hh.depth = (float)d3(dv, outer);
+                hh.eb[0] = s;
+                hh.eb[1] = t;
+                PUSH(hee, nee, capee, hh);
+            }
+        }
+#undef PUSH
+    qsort(h01, n01, sizeof(oc_hit), hit_cmp);
+    qsort(h10, n10, sizeof(oc_hit), hit_cmp);
+    qsort(hee, nee, sizeof(oc_hit), hit_cmp);
+    int npt = (int)(n01 + n10), ne = (int)nee;
+    int *pid = xcalloc(3 * (size_t)npt + 1, sizeof(int)), *eid = xcalloc(3 * (size_t)ne + 1, sizeof(int));
+    double *pn = xcalloc(3 * (size_t)npt + 1, sizeof(double)), *pb = xcalloc(3 * (size_t)npt + 1, sizeof(double));
+    double *en = xcalloc(3 * (size_t)ne + 1, sizeof(double));
+    float *pd = xcalloc(npt + 1, sizeof(float)), *eb = xcalloc(2 * (size_t)ne + 1, sizeof(float));
+    float *ed = xcalloc(ne + 1, sizeof(float));
+    for (int k = 0; k < npt; k++)
+    {
+        const oc_hit *hh = k < (int)n01 ? &h01[k] : &h10[k - n01];
+        memcpy(pid + 3 * k, hh->ids, 3 * sizeof(int));
+        memcpy(pn + 3 * k, hh->normal, 3 * sizeof(double));
+        memcpy(pb + 3 * k, hh->bary, 3 * sizeof(double));
+        pd[k] = hh->depth;
+    }
+    for (int k = 0; k < ne; k++)
+    {
+        memcpy(eid + 3 * k, hee[k].ids, 3 * sizeof(int));
+        memcpy(en + 3 * k, hee[k].normal, 3 * sizeof(double));
+        eb[2 * k] = hee[k].eb[0];
+        eb[2 * k + 1] = hee[k].eb[1];
+        ed[k] = hee[k].depth;
+    }
+    /* the repulsion asserts 3e-3 - depth > 0 for point-triangle contacts;
+       the detector only records distances below 3e-3, and |depth| <= distance */
+    oc_set_contacts(c, npt, pid, pn, pb, pd, ne, eid, en, eb, ed);
+    free(pid); free(eid); free(pn); free(pb); free(en); free(pd); free(eb); free(ed);
+    free(h01); free(h10); free(hee); free(ctn); free(c_in); free(b_in);
+    return 0;
+}
+
+void oc_num_contacts(const oc_cloth *c, int *n_pt, int *n_ee)
+{
+    *n_pt = c->n_pt;
+    *n_ee = c->n_ee;
+}
+
+void oc_get_contacts(const oc_cloth *c, int *pt_ids, double *pt_normal, double *pt_bary,
+                     float *pt_depth, int *ee_ids, double *ee_normal, float *ee_bary, float *ee_depth)
+{
+    if (c->n_pt)
+    {
+        memcpy(pt_ids, c->pt_ids, 3 * (size_t)c->n_pt * sizeof(int));
+        memcpy(pt_normal, c->pt_normal, 3 * (size_t)c->n_pt * sizeof(double));
+        memcpy(pt_bary, c->pt_bary, 3 * (size_t)c->n_pt * sizeof(double));
+        memcpy(pt_depth, c->pt_depth, (size_t)c->n_pt * sizeof(float));
+    }
+    if (c->n_ee)
+    {
+        memcpy(ee_ids, c->ee_ids, 3 * (size_t)c->n_ee * sizeof(int));
+        memcpy(ee_normal, c->ee_normal, 3 * (size_t)c->n_ee * sizeof(double));
+        memcpy(ee_bary, c->ee_bary, 2 * (size_t)c->n_ee * sizeof(float));
+        memcpy(ee_depth, c->ee_depth, (size_t)c->n_ee * sizeof(float));
+    }
+}

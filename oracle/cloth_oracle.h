@@ -113,6 +113,33 @@ int oc_set_contacts(oc_cloth *c, int n_pt, const int *pt_ids, const double *pt_n
                     const double *pt_bary, const float *pt_depth, int n_ee, const int *ee_ids,
                     const double *ee_normal, const float *ee_bary, const float *ee_depth);
 
+void oc_num_contacts(const oc_cloth *c, int *n_pt, int *n_ee);
+void oc_get_contacts(const oc_cloth *c, int *pt_ids, double *pt_normal, double *pt_bary,
+                     float *pt_depth, int *ee_ids, double *ee_normal, float *ee_bary, float *ee_depth);
+
+/* A kinematic body (cKinematicBody, KinematicBody.cpp): a triangle mesh
+   with edges numbered by cObjUtil::BuildEdge and triangle and vertex normals
+   fixed at creation (:114-116).  NULL for an index out of range or an edge
+   with more than two triangles. */
+typedef struct oc_body oc_body;
+oc_body *oc_body_create(const double *xyz, int nv, const int *tri, int nt);
+void oc_body_destroy(oc_body *b);
+void oc_body_set_pos(oc_body *b, const double *xyz);
+int oc_body_num_edges(const oc_body *b);
+void oc_body_get_edges(const oc_body *b, int *e4);
+void oc_body_normals(const oc_body *b, double *vn3, double *tn3);
+/* collision detection of the cloth at its current positions against the
+   body (cBVHCollisionDetecter::PerformCD, BVHCollisionDetecter.cpp:360-651);
+   the contacts replace the cloth's for its next step.  cloth_first: the
+   cloth has the lower object id (it is obj0 of the pair). */
+int oc_detect_contacts(oc_cloth *c, const oc_body *b, int cloth_first);
+/* the detector's two distance routines (BVHCollisionDetecter.cpp:17-83,
+   99-177) */
+void oc_closest_pt_triangle(const double *p, const double *a, const double *b, const double *c,
+                            float *bary, double *out);
+void oc_edge_edge_params(const double *p1, const double *q1, const double *p2, const double *q2,
+                         float *s, float *t);
+
 /* stage access: assemble K, f, W, b at the current state without stepping */
 void oc_assemble(oc_cloth *c);
 void oc_get_K(const oc_cloth *c, double *k9, int which); /* 0 total,1 stretch,2 bend */

@@ -87,6 +87,20 @@ def lib():
         fp = P(ctypes.c_float)
         L.oc_set_contacts.argtypes = [vp, ctypes.c_int, ip, dp, dp, fp, ctypes.c_int, ip, dp, fp, fp]
         L.oc_set_contacts.restype = ctypes.c_int
+        L.oc_num_contacts.argtypes = [vp, ip, ip]
+        L.oc_get_contacts.argtypes = [vp, ip, dp, dp, fp, ip, dp, fp, fp]
+        L.oc_body_create.restype = vp
+        L.oc_body_create.argtypes = [dp, ctypes.c_int, ip, ctypes.c_int]
+        L.oc_body_destroy.argtypes = [vp]
+        L.oc_body_set_pos.argtypes = [vp, dp]
+        L.oc_body_num_edges.argtypes = [vp]
+        L.oc_body_num_edges.restype = ctypes.c_int
+        L.oc_body_get_edges.argtypes = [vp, ip]
+        L.oc_body_normals.argtypes = [vp, dp, dp]
+        L.oc_detect_contacts.argtypes = [vp, vp, ctypes.c_int]
+        L.oc_detect_contacts.restype = ctypes.c_int
+        L.oc_closest_pt_triangle.argtypes = [dp, dp, dp, dp, fp, dp]
+        L.oc_edge_edge_params.argtypes = [dp, dp, dp, dp, fp, fp]
         L.oc_gpu_form_step.argtypes = [vp, P(OcStats)]
         L.oc_gpu_form_bending.argtypes = [vp, dp]
         L.oc_get_bending_rest.argtypes = [vp, dp, dp, dp]
@@ -358,6 +372,29 @@ class OracleCloth:
         if rc != 0:
             raise ValueError("contact index out of range")
 
+    def contacts(self):
+        """the contacts set for the next step, as a dict of arrays"""
+        n_pt, n_ee = ctypes.c_int(), ctypes.c_int()
+        lib().oc_num_contacts(self.h, ctypes.byref(n_pt), ctypes.byref(n_ee))
+        a, e = n_pt.value, n_ee.value
+        ct = {"pt_ids": np.zeros((a, 3), np.int32), "pt_normal": np.zeros((a, 3)),
+              "pt_bary": np.zeros((a, 3)), "pt_depth": np.zeros(a, np.float32),
+              "ee_ids": np.zeros((e, 3), np.int32), "ee_normal": np.zeros((e, 3)),
+              "ee_bary": np.zeros((e, 2), np.float32), "ee_depth": np.zeros(e, np.float32)}
+        fp = ctypes.POINTER(ctypes.c_float)
+        lib().oc_get_contacts(self.h, _ip(ct["pt_ids"]), _dp(ct["pt_normal"]), _dp(ct["pt_bary"]),
+                              ct["pt_depth"].ctypes.data_as(fp), _ip(ct["ee_ids"]),
+                              _dp(ct["ee_normal"]), ct["ee_bary"].ctypes.data_as(fp),
+                              ct["ee_depth"].ctypes.data_as(fp))
+        return ct
+
+    def detect_contacts(self, body, cloth_first=True):
+        """collision detection against an OracleBody; the contacts replace
+        this cloth's for its next step"""
+        if lib().oc_detect_contacts(self.h, body.h, 1 if cloth_first else 0) != 0:
+            raise ValueError("detection failed")
+        return self.contacts()
+
     def gpu_form_step(self):
         """one substep of the reference's GPU form (cloth_type "fem_gpu")"""
         st = OcStats()
@@ -410,3 +447,55 @@ def dbeta_dx(v0, v1, v2, v3):
     g = np.zeros(12)
     lib().oc_dbeta_dx(_dp(a[0]), _dp(a[1]), _dp(a[2]), _dp(a[3]), _dp(beta), _dp(g))
     return float(beta[0]), g
+
+
+class OracleBody:
+    """a kinematic body (triangle mesh) for collision detection"""
+
+    def __init__(self, xyz, tri):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1)
+        tri = np.ascontiguousarray(tri, dtype=np.int32).reshape(-1)
+        self.nv, self.nt = xyz.size // 3, tri.size // 3
+        self.h = lib().oc_body_create(_dp(xyz), self.nv, _ip(tri), self.nt)
+        if not self.h:
+            raise ValueError("bad body mesh")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oc_body_destroy(self.h)
+            self.h = None
+
+    def set_pos(self, xyz):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1)
+        assert xyz.size == 3 * self.nv
+        lib().oc_body_set_pos(self.h, _dp(xyz))
+
+    def edges(self):
+        ne = lib().oc_body_num_edges(self.h)
+        e = np.zeros(4 * ne, np.int32)
+        lib().oc_body_get_edges(self.h, _ip(e))
+        return e.reshape(-1, 4)
+
+    def normals(self):
+        vn, tn = np.zeros(3 * self.nv), np.zeros(3 * self.nt)
+        lib().oc_body_normals(self.h, _dp(vn), _dp(tn))
+        return vn.reshape(-1, 3), tn.reshape(-1, 3)
+
+
+def closest_pt_triangle(p, a, b, c):
+    """(closest point, float barycentric coordinates) as the detector computes them"""
+    args = [np.ascontiguousarray(v, dtype=np.float64) for v in (p, a, b, c)]
+    bary = np.zeros(3, np.float32)
+    out = np.zeros(3)
+    lib().oc_closest_pt_triangle(*[_dp(v) for v in args],
+                                 bary.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), _dp(out))
+    return out, bary
+
+
+def edge_edge_params(p1, q1, p2, q2):
+    """(s, t): the closest points p1 + s (q1 - p1) and p2 + t (q2 - p2), as
+    the detector computes them (floats)"""
+    args = [np.ascontiguousarray(v, dtype=np.float64) for v in (p1, q1, p2, q2)]
+    s, t = ctypes.c_float(), ctypes.c_float()
+    lib().oc_edge_edge_params(*[_dp(v) for v in args], ctypes.byref(s), ctypes.byref(t))
+    return s.value, t.value
