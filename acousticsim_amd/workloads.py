@@ -266,3 +266,47 @@ def contacts_file_records(ct):
     ee = np.concatenate([ct["ee_ids"].astype(np.float64), ct["ee_depth"][:, None].astype(np.float64),
                          ct["ee_bary"].astype(np.float64), ct["ee_normal"]], axis=1) if n_ee else np.zeros((0, 9))
     return np.concatenate([[float(n_pt), float(n_ee)], pt.reshape(-1), ee.reshape(-1)])
+
+
+def uv_sphere(center, radius, n_lat=12, n_lon=24):
+    """(positions (V, 3), triangles (T, 3)) of a latitude-longitude sphere
+    with outward winding: a body for collision detection"""
+    pts = [[0.0, radius, 0.0]]
+    for i in range(1, n_lat):
+        th = np.pi * i / n_lat
+        for j in range(n_lon):
+            ph = 2 * np.pi * j / n_lon
+            pts.append([radius * np.sin(th) * np.cos(ph), radius * np.cos(th),
+                        radius * np.sin(th) * np.sin(ph)])
+    pts.append([0.0, -radius, 0.0])
+    pts = np.array(pts) + np.asarray(center, dtype=np.float64)
+
+    def ring(i, j):
+        return 1 + (i - 1) * n_lon + (j % n_lon)
+    tri = [[0, ring(1, j + 1), ring(1, j)] for j in range(n_lon)]
+    for i in range(1, n_lat - 1):
+        for j in range(n_lon):
+            a, b, c, d = ring(i, j), ring(i, j + 1), ring(i + 1, j), ring(i + 1, j + 1)
+            tri.append([a, b, d])
+            tri.append([a, d, c])
+    last = len(pts) - 1
+    tri += [[last, ring(n_lat - 1, j), ring(n_lat - 1, j + 1)] for j in range(n_lon)]
+    return pts, np.array(tri, np.int32)
+
+
+# Draping: a horizontal square cloth released just above a sphere, collision
+# detection before every step (contacts feed the step's collision response)
+DRAPE = dict(side=129, size=0.5, radius=0.15, gap=1.5e-3, n_lat=48, n_lon=96, dt=0.01)
+
+
+def drape_config(side=DRAPE["side"]):
+    """unpinned horizontal cloth centred above the drape sphere (at the origin)"""
+    c = grid_config(side, size=DRAPE["size"], dt=DRAPE["dt"])
+    c["cloth_init_orientation"] = [-np.pi / 2, 0.0, 0.0, 0.0]
+    c["cloth_init_pos"] = [0.0, DRAPE["radius"] + DRAPE["gap"], 0.0]
+    c["constraints"] = {"static_point": []}
+    return c
+
+
+def drape_body():
+    return uv_sphere([0.0, 0.0, 0.0], DRAPE["radius"], DRAPE["n_lat"], DRAPE["n_lon"])

@@ -36,29 +36,7 @@ def floor_body(y, half=2.0):
 
 
 def sphere_body(center, radius, n_lat=12, n_lon=24):
-    """UV sphere, outward normals (counter-clockwise seen from outside)"""
-    pts = [[0.0, radius, 0.0]]
-    for i in range(1, n_lat):
-        th = np.pi * i / n_lat
-        for j in range(n_lon):
-            ph = 2 * np.pi * j / n_lon
-            pts.append([radius * np.sin(th) * np.cos(ph), radius * np.cos(th),
-                        radius * np.sin(th) * np.sin(ph)])
-    pts.append([0.0, -radius, 0.0])
-    pts = np.array(pts) + np.asarray(center)
-    tri = []
-    ring = lambda i, j: 1 + (i - 1) * n_lon + (j % n_lon)  # noqa: E731
-    for j in range(n_lon):
-        tri.append([0, ring(1, j + 1), ring(1, j)])
-    for i in range(1, n_lat - 1):
-        for j in range(n_lon):
-            a, b, c, d = ring(i, j), ring(i, j + 1), ring(i + 1, j), ring(i + 1, j + 1)
-            tri.append([a, b, d])
-            tri.append([a, d, c])
-    last = len(pts) - 1
-    for j in range(n_lon):
-        tri.append([last, ring(n_lat - 1, j), ring(n_lat - 1, j + 1)])
-    return pts, np.array(tri, np.int32)
+    return wl.uv_sphere(center, radius, n_lat, n_lon)
 
 
 def flat_cloth(side=9, height=0.0, precision="fp64", mode="reference"):
