@@ -110,7 +110,7 @@ Body::Body(const double *xyz, int nv_, const int *tri_, int nt_) : nv(nv_), nt(n
     for (int v = 0; v < nv; v++)
         normalize4(&vn[3 * (size_t)v]);
     // each edge's side normal: the normalised sum of its triangles' normals
-    // (BVHCollisionDetecter.cpp:566-576, 594-604)
+    // (BVHCollisionDetecter.cpp:565-576, 592-604)
     edge_n.assign(3 * (size_t)ne, 0.0);
     for (int e = 0; e < ne; e++)
     {

@@ -119,7 +119,7 @@ __device__ inline float clamp01(float v)
     return (float)((0.0 < m) ? m : 0.0);
 }
 
-// EdgeEdgeDist's parameters (BVHCollisionDetecter.cpp:99-177)
+// EdgeEdgeDist's parameters (BVHCollisionDetecter.cpp:104-177)
 __device__ void edge_edge_params(const double *p1, const double *q1, const double *p2,
                                  const double *q2, float &s, float &t)
 {

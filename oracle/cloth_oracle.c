@@ -2128,7 +2128,7 @@ static float clampf01(float v)
     return (float)((0.0 < m) ? m : 0.0);
 }
 
-/* EdgeEdgeDist (BVHCollisionDetecter.cpp:99-177, Ericson's closest points
+/* EdgeEdgeDist (BVHCollisionDetecter.cpp:104-177, Ericson's closest points
    of two segments) with its float parameters: s on p1q1, t on p2q2 */
 void oc_edge_edge_params(const double *p1, const double *q1, const double *p2, const double *q2,
                          float *s_out, float *t_out)

@@ -134,7 +134,7 @@ void oc_body_normals(const oc_body *b, double *vn3, double *tn3);
    cloth has the lower object id (it is obj0 of the pair). */
 int oc_detect_contacts(oc_cloth *c, const oc_body *b, int cloth_first);
 /* the detector's two distance routines (BVHCollisionDetecter.cpp:17-83,
-   99-177) */
+   104-177) */
 void oc_closest_pt_triangle(const double *p, const double *a, const double *b, const double *c,
                             float *bary, double *out);
 void oc_edge_edge_params(const double *p1, const double *q1, const double *p2, const double *q2,
