@@ -41,6 +41,51 @@ void normalize4(double *u)
 }
 } // namespace
 
+namespace
+{
+// boxes of tiles of DETECT_TILE consecutive items, each item's points given
+// by pts(item, k) for k < npts
+template <typename F> std::vector<double> tile_boxes(int n, int npts, F pts)
+{
+    const int nt = (n + DETECT_TILE - 1) / DETECT_TILE;
+    std::vector<double> box(6 * (size_t)nt);
+    for (int t = 0; t < nt; t++)
+    {
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int i = t * DETECT_TILE; i < std::min(n, (t + 1) * DETECT_TILE); i++)
+            for (int k = 0; k < npts; k++)
+            {
+                const double *p = pts(i, k);
+                for (int a = 0; a < 3; a++)
+                {
+                    lo[a] = std::min(lo[a], p[a]);
+                    hi[a] = std::max(hi[a], p[a]);
+                }
+            }
+        for (int a = 0; a < 3; a++)
+        {
+            box[6 * (size_t)t + a] = lo[a];
+            box[6 * (size_t)t + 3 + a] = hi[a];
+        }
+    }
+    return box;
+}
+} // namespace
+
+void Body::refresh_tiles()
+{
+    tile_box_tri = tile_boxes(nt, 3, [&](int i, int k) { return &x[3 * (size_t)tri[3 * (size_t)i + k]]; });
+    tile_box_vert = tile_boxes(nv, 1, [&](int i, int) { return &x[3 * (size_t)i]; });
+    tile_box_edge = tile_boxes(ne, 2, [&](int i, int k) { return &x[3 * (size_t)edge[4 * (size_t)i + k]]; });
+    std::vector<double> all(tile_box_tri);
+    all.insert(all.end(), tile_box_vert.begin(), tile_box_vert.end());
+    all.insert(all.end(), tile_box_edge.begin(), tile_box_edge.end());
+    if (!tile_box_dev)
+        hcheck(hipMalloc(&tile_box_dev, all.size() * sizeof(double)), "hipMalloc body");
+    hcheck(hipMemcpy(tile_box_dev, all.data(), all.size() * sizeof(double), hipMemcpyHostToDevice),
+           "hipMemcpy body");
+}
+
 Body::Body(const double *xyz, int nv_, const int *tri_, int nt_) : nv(nv_), nt(nt_)
 {
     if (nv < 3 || nt < 1 || !xyz || !tri_)
@@ -140,6 +185,7 @@ Body::Body(const double *xyz, int nv_, const int *tri_, int nt_) : nv(nv_), nt(n
     vn_dev = to_device(vn);
     edge_n_dev = to_device(edge_n);
     in_tri_dev = to_device(in_tri);
+    refresh_tiles();
 }
 
 Body::~Body()
@@ -151,6 +197,7 @@ Body::~Body()
     (void)hipFree(vn_dev);
     (void)hipFree(edge_n_dev);
     (void)hipFree(in_tri_dev);
+    (void)hipFree(tile_box_dev);
 }
 
 void Body::set_pos(const double *xyz)
@@ -163,6 +210,7 @@ void Body::set_pos(const double *xyz)
     x.assign(xyz, xyz + 3 * (size_t)nv);
     hcheck(hipMemcpy(x_dev, x.data(), x.size() * sizeof(double), hipMemcpyHostToDevice),
            "hipMemcpy body");
+    refresh_tiles();
 }
 
 } // namespace bw

@@ -9,6 +9,9 @@
 namespace bw
 {
 
+// body primitives per LDS tile of the detection kernels
+constexpr int DETECT_TILE = 256;
+
 // A kinematic body: a triangle mesh whose edges are numbered as
 // cObjUtil::BuildEdge numbers them (ascending {min, max}, tri0 the first
 // triangle with the side, ObjUtil.cpp:141-224) and whose triangle and vertex
@@ -23,6 +26,7 @@ public:
     Body(const Body &) = delete;
     Body &operator=(const Body &) = delete;
     void set_pos(const double *xyz);
+    int tiles(int n) const { return (n + DETECT_TILE - 1) / DETECT_TILE; }
 
     int nv = 0, nt = 0, ne = 0;
     std::vector<double> x;      // 3 nv
@@ -31,10 +35,17 @@ public:
     std::vector<double> tn, vn; // 3 nt, 3 nv
     std::vector<double> edge_n; // 3 ne: normalised sum of the edge's triangle normals
     std::vector<unsigned char> in_tri; // per vertex: used by a triangle
+    // boxes (lo xyz, hi xyz) of each tile of DETECT_TILE consecutive
+    // triangles, vertices and edges, refreshed with the positions
+    std::vector<double> tile_box_tri, tile_box_vert, tile_box_edge;
+    double *tile_box_dev = nullptr; // the three lists back to back
     // device copies
     double *x_dev = nullptr, *tn_dev = nullptr, *vn_dev = nullptr, *edge_n_dev = nullptr;
     int *tri_dev = nullptr, *edge_dev = nullptr;
     unsigned char *in_tri_dev = nullptr;
+
+private:
+    void refresh_tiles();
 };
 
 // one detected pair as the kernels write it

@@ -9,7 +9,9 @@
 // scans the body's triangles, vertices or edges through LDS in tiles,
 // skipping those whose box lies more than 4e-3 away, and applies the
 // reference's narrow-phase arithmetic (float region tests, double points) to
-// the rest.  Two passes: counts, then writes at host-scanned offsets, so the
+// the rest; a workgroup skips every tile whose box (kept per tile of 256
+// body items) lies more than 4e-3 from the box of its own cloth items.  Two
+// passes: counts, then writes at host-scanned offsets, so the
 // output order is fixed by the thread index and the scan order.
 //
 // Contraction is off for this file (the narrow phase rounds as the
@@ -22,7 +24,7 @@ namespace bw
 namespace
 {
 
-constexpr int TILE = 256;
+constexpr int TILE = DETECT_TILE;
 constexpr double REJECT = 4e-3; // beyond it no computed distance is below 3e-3
 constexpr double CONTACT = 3e-3;
 
@@ -170,6 +172,44 @@ __device__ inline bool outside(const double *p, const double *lo, const double *
            p[1] > hi[1] + REJECT || p[2] < lo[2] - REJECT || p[2] > hi[2] + REJECT;
 }
 
+// the workgroup's box: union of its threads' boxes (lo = +huge, hi = -huge
+// for threads without work), reduced through LDS
+__device__ void block_box(double (*red)[TILE], const double *lo, const double *hi, double *blo,
+                          double *bhi)
+{
+    const int t = threadIdx.x;
+    for (int a = 0; a < 3; a++)
+    {
+        red[a][t] = lo[a];
+        red[3 + a][t] = hi[a];
+    }
+    __syncthreads();
+    for (int w = TILE / 2; w > 0; w >>= 1)
+    {
+        if (t < w)
+            for (int a = 0; a < 3; a++)
+            {
+                red[a][t] = fmin(red[a][t], red[a][t + w]);
+                red[3 + a][t] = fmax(red[3 + a][t], red[3 + a][t + w]);
+            }
+        __syncthreads();
+    }
+    for (int a = 0; a < 3; a++)
+    {
+        blo[a] = red[a][0];
+        bhi[a] = red[3 + a][0];
+    }
+    __syncthreads();
+}
+
+// a tile of body items wholly outside the workgroup's box (grown by the
+// reject distance) holds no pair for it
+__device__ inline bool tile_apart(const double *tb, const double *blo, const double *bhi)
+{
+    return tb[0] > bhi[0] + REJECT || tb[3] < blo[0] - REJECT || tb[1] > bhi[1] + REJECT ||
+           tb[4] < blo[1] - REJECT || tb[2] > bhi[2] + REJECT || tb[5] < blo[2] - REJECT;
+}
+
 __device__ inline void box3(const double *a, const double *b, const double *c, double *lo,
                             double *hi)
 {
@@ -184,11 +224,12 @@ __device__ inline void box3(const double *a, const double *b, const double *c, d
 // outer normal = the body triangle's, depth = (p - closest) . normal
 __global__ void __launch_bounds__(TILE)
 k_detect_points(ClothView c, int nbt, const double *__restrict__ bx, const int *__restrict__ btri,
-                const double *__restrict__ btn, int *__restrict__ count,
-                const int *__restrict__ offset, Hit *__restrict__ out)
+                const double *__restrict__ btn, const double *__restrict__ tile_box,
+                int *__restrict__ count, const int *__restrict__ offset, Hit *__restrict__ out)
 {
     __shared__ double s_v[TILE][9];
     __shared__ double s_box[TILE][6];
+    __shared__ double s_red[6][TILE];
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = v < c.nv && c.vt_ptr[c.vbase + v + 1] > c.vt_ptr[c.vbase + v];
     double p[3] = {0, 0, 0};
@@ -199,10 +240,19 @@ k_detect_points(ClothView c, int nbt, const double *__restrict__ bx, const int *
         p[1] = x[1];
         p[2] = x[2];
     }
+    double blo[3], bhi[3];
+    {
+        const double big = 1e300;
+        const double lo[3] = {active ? p[0] : big, active ? p[1] : big, active ? p[2] : big};
+        const double hi[3] = {active ? p[0] : -big, active ? p[1] : -big, active ? p[2] : -big};
+        block_box(s_red, lo, hi, blo, bhi);
+    }
     int n = 0;
     const int o = (out && active) ? offset[v] : 0;
     for (int t0 = 0; t0 < nbt; t0 += TILE)
     {
+        if (tile_apart(tile_box + 6 * (size_t)(t0 / TILE), blo, bhi))
+            continue; // the same decision in every thread
         __syncthreads();
         const int k = threadIdx.x;
         if (t0 + k < nbt)
@@ -256,12 +306,13 @@ k_detect_points(ClothView c, int nbt, const double *__restrict__ bx, const int *
 __global__ void __launch_bounds__(TILE)
 k_detect_body_points(ClothView c, int nbv, const double *__restrict__ bx,
                      const double *__restrict__ bvn, const unsigned char *__restrict__ b_in,
-                     int *__restrict__ count, const int *__restrict__ offset,
-                     Hit *__restrict__ out)
+                     const double *__restrict__ tile_box, int *__restrict__ count,
+                     const int *__restrict__ offset, Hit *__restrict__ out)
 {
     __shared__ double s_p[TILE][3];
     __shared__ double s_n[TILE][3];
     __shared__ int s_in[TILE];
+    __shared__ double s_red[6][TILE];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = t < c.nt;
     double a[3] = {0, 0, 0}, b[3] = {0, 0, 0}, cc[3] = {0, 0, 0}, lo[3], hi[3], tn[3] = {0, 0, 0};
@@ -277,10 +328,19 @@ k_detect_body_points(ClothView c, int nbv, const double *__restrict__ bx,
         }
     }
     box3(a, b, cc, lo, hi);
+    double blo[3], bhi[3];
+    {
+        const double big = 1e300;
+        const double l2[3] = {active ? lo[0] : big, active ? lo[1] : big, active ? lo[2] : big};
+        const double h2[3] = {active ? hi[0] : -big, active ? hi[1] : -big, active ? hi[2] : -big};
+        block_box(s_red, l2, h2, blo, bhi);
+    }
     int n = 0;
     const int o = (out && active) ? offset[t] : 0;
     for (int p0 = 0; p0 < nbv; p0 += TILE)
     {
+        if (tile_apart(tile_box + 6 * (size_t)(p0 / TILE), blo, bhi))
+            continue; // the same decision in every thread
         __syncthreads();
         const int k = threadIdx.x;
         if (p0 + k < nbv)
@@ -338,11 +398,13 @@ k_detect_body_points(ClothView c, int nbv, const double *__restrict__ bx,
 // body's point to the cloth's
 __global__ void __launch_bounds__(TILE)
 k_detect_edges(ClothView c, int nbe, const double *__restrict__ bx, const int *__restrict__ bedge,
-               const double *__restrict__ bedge_n, int cloth_first, int *__restrict__ count,
-               const int *__restrict__ offset, Hit *__restrict__ out)
+               const double *__restrict__ bedge_n, const double *__restrict__ tile_box,
+               int cloth_first, int *__restrict__ count, const int *__restrict__ offset,
+               Hit *__restrict__ out)
 {
     __shared__ double s_e[TILE][6];
     __shared__ double s_n[TILE][3];
+    __shared__ double s_red[6][TILE];
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = e < c.ne;
     double ca[3] = {0, 0, 0}, cb[3] = {0, 0, 0};
@@ -358,10 +420,22 @@ k_detect_edges(ClothView c, int nbe, const double *__restrict__ bx, const int *_
         lo[k] = fmin(ca[k], cb[k]) - REJECT;
         hi[k] = fmax(ca[k], cb[k]) + REJECT;
     }
+    double blo[3], bhi[3];
+    {
+        // lo / hi are already grown by the reject distance
+        const double big = 1e300;
+        const double l2[3] = {active ? lo[0] + REJECT : big, active ? lo[1] + REJECT : big,
+                              active ? lo[2] + REJECT : big};
+        const double h2[3] = {active ? hi[0] - REJECT : -big, active ? hi[1] - REJECT : -big,
+                              active ? hi[2] - REJECT : -big};
+        block_box(s_red, l2, h2, blo, bhi);
+    }
     int n = 0;
     const int o = (out && active) ? offset[e] : 0;
     for (int e0 = 0; e0 < nbe; e0 += TILE)
     {
+        if (tile_apart(tile_box + 6 * (size_t)(e0 / TILE), blo, bhi))
+            continue; // the same decision in every thread
         __syncthreads();
         const int k = threadIdx.x;
         if (e0 + k < nbe)
@@ -447,7 +521,7 @@ void launch_detect_points(const ClothView &c, const Body &b, int *count, const i
 {
     if (c.nv > 0)
         hipLaunchKernelGGL(k_detect_points, dim3((c.nv + TILE - 1) / TILE), dim3(TILE), 0, s, c,
-                           b.nt, b.x_dev, b.tri_dev, b.tn_dev, count, offset, out);
+                           b.nt, b.x_dev, b.tri_dev, b.tn_dev, b.tile_box_dev, count, offset, out);
 }
 
 void launch_detect_body_points(const ClothView &c, const Body &b, int *count, const int *offset,
@@ -455,7 +529,8 @@ void launch_detect_body_points(const ClothView &c, const Body &b, int *count, co
 {
     if (c.nt > 0)
         hipLaunchKernelGGL(k_detect_body_points, dim3((c.nt + TILE - 1) / TILE), dim3(TILE), 0, s,
-                           c, b.nv, b.x_dev, b.vn_dev, b.in_tri_dev, count, offset, out);
+                           c, b.nv, b.x_dev, b.vn_dev, b.in_tri_dev,
+                           b.tile_box_dev + 6 * (size_t)b.tiles(b.nt), count, offset, out);
 }
 
 void launch_detect_edges(const ClothView &c, const Body &b, int cloth_first, int *count,
@@ -463,8 +538,9 @@ void launch_detect_edges(const ClothView &c, const Body &b, int cloth_first, int
 {
     if (c.ne > 0)
         hipLaunchKernelGGL(k_detect_edges, dim3((c.ne + TILE - 1) / TILE), dim3(TILE), 0, s, c,
-                           b.ne, b.x_dev, b.edge_dev, b.edge_n_dev, cloth_first, count, offset,
-                           out);
+                           b.ne, b.x_dev, b.edge_dev, b.edge_n_dev,
+                           b.tile_box_dev + 6 * (size_t)(b.tiles(b.nt) + b.tiles(b.nv)),
+                           cloth_first, count, offset, out);
 }
 
 } // namespace bw
