@@ -1143,7 +1143,7 @@ bool Batch::upload_contacts()
     std::vector<long long> widx;
     std::vector<int> diag;
     std::vector<double> h9;
-    std::vector<int> wg_level(1, 0), level_ptr(1, 0);
+    std::vector<int> wg_level(1, 0), level_ptr(1, 0), wg_rows(1, 0), touched_rows;
     std::vector<int4> ids;
     std::vector<double> rec;
     std::vector<int> last_level;
@@ -1236,14 +1236,29 @@ bool Batch::upload_contacts()
         std::vector<int4> cid;
         std::vector<double> crec;
         last_level.assign(m.nv, -1);
-        auto add = [&](int4 id, const int *verts, int nvert, const double *r) {
+        // slots: this cloth's touched rows in order of first use
+        std::unordered_map<int, int> slot_of;
+        const int row0 = (int)touched_rows.size();
+        auto slot = [&](int v) {
+            auto it = slot_of.find(v);
+            if (it != slot_of.end())
+                return it->second;
+            const int sl = (int)touched_rows.size() - row0;
+            slot_of.emplace(v, sl);
+            touched_rows.push_back(row(v));
+            return sl;
+        };
+        auto add = [&](int type, const int *verts, int nvert, const double *r) {
             int L = 0;
             for (int l = 0; l < nvert; l++)
                 L = std::max(L, last_level[verts[l]] + 1);
             for (int l = 0; l < nvert; l++)
                 last_level[verts[l]] = L;
             lvl.push_back(L);
-            cid.push_back(id);
+            int sl[3] = {0, 0, 0};
+            for (int l = 0; l < nvert; l++)
+                sl[l] = slot(verts[l]);
+            cid.push_back(make_int4(type, sl[0], sl[1], sl[2]));
             crec.insert(crec.end(), r, r + CONTACT_REC);
         };
         const int n_ee = (int)c.ee_depth.size();
@@ -1260,7 +1275,7 @@ bool Batch::upload_contacts()
                 (double)(1 - bc), (double)bc, 0.0,
                 m.mass[v[0]] + m.mass[v[1]], (double)den, 3e-3 - (double)c.ee_depth[k],
                 (double)0.1f, dtK, dt};
-            add(make_int4(0, row(v[0]), row(v[1]), 0), v, 2, r);
+            add(0, v, 2, r);
         }
         for (int k = 0; k < n_pt; k++)
         {
@@ -1273,7 +1288,7 @@ bool Batch::upload_contacts()
                 const double r[CONTACT_REC] = {c.pt_normal[3 * k], c.pt_normal[3 * k + 1],
                                                c.pt_normal[3 * k + 2], 0.0, 0.0, 0.0,
                                                m.mass[v], den, d, 0.1, dtK, dt};
-                add(make_int4(1, row(v), 0, 0), &v, 1, r);
+                add(1, &v, 1, r);
             }
             else
             {
@@ -1283,7 +1298,7 @@ bool Batch::upload_contacts()
                                                c.pt_normal[3 * k + 2], bary[0], bary[1], bary[2],
                                                m.mass[v[0]] + m.mass[v[1]] + m.mass[v[2]], den, d,
                                                0.1, dtK, dt};
-                add(make_int4(2, row(v[0]), row(v[1]), row(v[2])), v, 3, r);
+                add(2, v, 3, r);
             }
         }
         // group by level, contact order kept within a level
@@ -1307,6 +1322,7 @@ bool Batch::upload_contacts()
         for (int L = 0; L < nl; L++)
             level_ptr.push_back(base + count[L + 1]);
         wg_level.push_back((int)level_ptr.size() - 1);
+        wg_rows.push_back((int)touched_rows.size());
     }
     contact_blocks_ = (int)widx.size();
     contact_groups_ = (int)wg_level.size() - 1;
@@ -1322,6 +1338,10 @@ bool Batch::upload_contacts()
     off = align256(off + wg_level.size() * sizeof(int));
     const size_t o_lvl = off;
     off = align256(off + level_ptr.size() * sizeof(int));
+    const size_t o_wgr = off;
+    off = align256(off + wg_rows.size() * sizeof(int));
+    const size_t o_rows = off;
+    off = align256(off + touched_rows.size() * sizeof(int));
     const size_t o_ids = off;
     off = align256(off + ids.size() * sizeof(int4));
     const size_t o_rec = off;
@@ -1336,6 +1356,8 @@ bool Batch::upload_contacts()
     put(o_h9, h9.data(), h9.size() * sizeof(double));
     put(o_wgl, wg_level.data(), wg_level.size() * sizeof(int));
     put(o_lvl, level_ptr.data(), level_ptr.size() * sizeof(int));
+    put(o_wgr, wg_rows.data(), wg_rows.size() * sizeof(int));
+    put(o_rows, touched_rows.data(), touched_rows.size() * sizeof(int));
     put(o_ids, ids.data(), ids.size() * sizeof(int4));
     put(o_rec, rec.data(), rec.size() * sizeof(double));
     if (off > contact_cap_)
@@ -1357,6 +1379,8 @@ bool Batch::upload_contacts()
     c_h9_ = reinterpret_cast<const double *>(b + o_h9);
     c_wgl_ = reinterpret_cast<const int *>(b + o_wgl);
     c_lvl_ = reinterpret_cast<const int *>(b + o_lvl);
+    c_wgr_ = reinterpret_cast<const int *>(b + o_wgr);
+    c_rows_ = reinterpret_cast<const int *>(b + o_rows);
     c_ids_ = reinterpret_cast<const int4 *>(b + o_ids);
     c_rec_ = reinterpret_cast<const double *>(b + o_rec);
     return true;
@@ -1540,7 +1564,8 @@ void Batch::step(int nsteps)
             else
                 solve_cg();
             if (ct) // repulsion on dx (BaraffCloth.cpp:147)
-                launch_repulsion(d_, contact_groups_, c_wgl_, c_lvl_, c_ids_, c_rec_, stream_);
+                launch_repulsion(d_, contact_groups_, c_wgl_, c_lvl_, c_wgr_, c_rows_, c_ids_,
+                                 c_rec_, stream_);
             launch_update_positions(d_, stream_);
         }
         check(hipGetLastError(), "step launch");

@@ -13,7 +13,8 @@
 //     so the host assigns each contact a level one above the last earlier
 //     contact that shares a cloth vertex; contacts of one level touch
 //     disjoint vertices and run in parallel, levels in order.  The result is
-//     the serial one, bit for bit.
+//     the serial one, bit for bit.  The touched rows of dx stay in LDS for
+//     the pass when they fit (chains can be thousands of levels deep).
 //
 // Contraction is off for this file so that every product and sum rounds as
 // in the reference.
@@ -54,17 +55,15 @@ k_contact_hessian(int n, const long long *__restrict__ w_index, const int *__res
 }
 
 // one contact's impulse (BaraffCloth.cpp:640-780 edge-edge, :781-905
-// point-triangle); rec = {N[3], w[3], mass, den, d, coef, dt*K, dt}
-__device__ void apply_contact(double *__restrict__ dx, int4 id, const double *__restrict__ rec)
+// point-triangle) on the solution rows x0, x1, x2 (the contact's vertices);
+// rec = {N[3], w[3], mass, den, d, coef, dt*K, dt}
+__device__ void apply_contact(double *x0, double *x1, double *x2, int type,
+                              const double *__restrict__ rec)
 {
     const double N[3] = {rec[0], rec[1], rec[2]};
     const double w0 = rec[3], w1 = rec[4], w2 = rec[5];
     const double mass = rec[6], den = rec[7], d = rec[8], coef = rec[9], dtK = rec[10],
                  dt = rec[11];
-    const int type = id.x;
-    double *x0 = dx + 3 * (size_t)id.y;
-    double *x1 = dx + 3 * (size_t)(type == 1 ? id.y : id.z);
-    double *x2 = dx + 3 * (size_t)(type == 2 ? id.w : id.y);
     double vrel[3];
     for (int a = 0; a < 3; a++)
     {
@@ -113,20 +112,45 @@ __device__ void apply_contact(double *__restrict__ dx, int4 id, const double *__
     }
 }
 
-// one workgroup per cloth with contacts: its levels in order, the contacts
-// of a level spread over the workgroup's threads
+// One workgroup per cloth with contacts: its levels in order, the contacts
+// of a level spread over the workgroup's threads.  A contact names its
+// vertices by slot in the cloth's list of touched rows; when that list fits,
+// the touched rows of dx live in LDS for the whole pass (a level then costs
+// a barrier instead of a round trip to memory: contact chains through shared
+// vertices can be thousands of levels deep), else they are used in place.
 __global__ void __launch_bounds__(256)
 k_repulsion(double *__restrict__ dx, const int *__restrict__ wg_level,
-            const int *__restrict__ level_ptr, const int4 *__restrict__ ids,
+            const int *__restrict__ level_ptr, const int *__restrict__ wg_rows,
+            const int *__restrict__ rows, const int4 *__restrict__ ids,
             const double *__restrict__ rec)
 {
+    __shared__ double s_dx[3 * REPULSION_LDS_ROWS];
     const int g = blockIdx.x;
+    const int *my_rows = rows + wg_rows[g];
+    const int nr = wg_rows[g + 1] - wg_rows[g];
+    const bool lds = nr <= REPULSION_LDS_ROWS;
+    if (lds)
+    {
+        for (int k = threadIdx.x; k < 3 * nr; k += blockDim.x)
+            s_dx[k] = dx[3 * (size_t)my_rows[k / 3] + k % 3];
+        __syncthreads();
+    }
     for (int L = wg_level[g]; L < wg_level[g + 1]; L++)
     {
         for (int k = level_ptr[L] + (int)threadIdx.x; k < level_ptr[L + 1]; k += blockDim.x)
-            apply_contact(dx, ids[k], rec + CONTACT_REC * (size_t)k);
-        __syncthreads(); // this level's dx writes before the next level reads
+        {
+            const int4 id = ids[k];
+            const int s0 = id.y, s1 = id.x == 1 ? id.y : id.z, s2 = id.x == 2 ? id.w : id.y;
+            double *x0 = lds ? s_dx + 3 * s0 : dx + 3 * (size_t)my_rows[s0];
+            double *x1 = lds ? s_dx + 3 * s1 : dx + 3 * (size_t)my_rows[s1];
+            double *x2 = lds ? s_dx + 3 * s2 : dx + 3 * (size_t)my_rows[s2];
+            apply_contact(x0, x1, x2, id.x, rec + CONTACT_REC * (size_t)k);
+        }
+        __syncthreads(); // this level's writes before the next level reads
     }
+    if (lds)
+        for (int k = threadIdx.x; k < 3 * nr; k += blockDim.x)
+            dx[3 * (size_t)my_rows[k / 3] + k % 3] = s_dx[k];
 }
 
 } // namespace
@@ -140,11 +164,12 @@ void launch_contact_hessian(const DeviceBatch &d, int n, const long long *w_inde
 }
 
 void launch_repulsion(const DeviceBatch &d, int n_groups, const int *wg_level,
-                      const int *level_ptr, const int4 *ids, const double *rec, hipStream_t s)
+                      const int *level_ptr, const int *wg_rows, const int *rows, const int4 *ids,
+                      const double *rec, hipStream_t s)
 {
     if (n_groups > 0)
         hipLaunchKernelGGL(k_repulsion, dim3(n_groups), dim3(256), 0, s, d.dx, wg_level,
-                           level_ptr, ids, rec);
+                           level_ptr, wg_rows, rows, ids, rec);
 }
 
 } // namespace bw

@@ -229,9 +229,13 @@ void launch_contact_hessian(const DeviceBatch &d, int n, const long long *w_inde
                             const int *diag_row, const double *h9, hipStream_t s);
 // repulsion impulses on dx: workgroup g runs levels [wg_level[g],
 // wg_level[g+1]), level L the contacts [level_ptr[L], level_ptr[L+1]);
-// ids = {type (0 edge-edge, 1 cloth vertex, 2 cloth triangle), rows...}
+// ids = {type (0 edge-edge, 1 cloth vertex, 2 cloth triangle), slots...},
+// a slot indexing the workgroup's touched rows rows[wg_rows[g] ...
+// wg_rows[g+1]); up to REPULSION_LDS_ROWS of them are kept in LDS
+constexpr int REPULSION_LDS_ROWS = 2560;
 void launch_repulsion(const DeviceBatch &d, int n_groups, const int *wg_level,
-                      const int *level_ptr, const int4 *ids, const double *rec, hipStream_t s);
+                      const int *level_ptr, const int *wg_rows, const int *rows, const int4 *ids,
+                      const double *rec, hipStream_t s);
 // measurement helpers: set the solver residual to 1 on every row; one SpMV
 // launch of the solver's precision
 void launch_fill_residual(const DeviceBatch &d, bool fp32, hipStream_t s);

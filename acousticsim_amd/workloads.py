@@ -305,6 +305,9 @@ def drape_config(side=DRAPE["side"]):
     c["cloth_init_orientation"] = [-np.pi / 2, 0.0, 0.0, 0.0]
     c["cloth_init_pos"] = [0.0, DRAPE["radius"] + DRAPE["gap"], 0.0]
     c["constraints"] = {"static_point": []}
+    # the unpinned cloth's first contact steps need up to ~40k Jacobi-PCG
+    # iterations to reach 1e-6
+    c["solver_max_iters"] = 100000
     return c
 
 
