@@ -1143,7 +1143,8 @@ bool Batch::upload_contacts()
     std::vector<long long> widx;
     std::vector<int> diag;
     std::vector<double> h9;
-    std::vector<int> wg_level(1, 0), level_ptr(1, 0), wg_rows(1, 0), touched_rows;
+    std::vector<int> wg_chunk(1, 0), chunk_level(1, 0), level_ptr(1, 0), wg_rows(1, 0),
+        touched_rows;
     std::vector<int4> ids;
     std::vector<double> rec;
     std::vector<int> last_level;
@@ -1319,13 +1320,24 @@ bool Batch::upload_contacts()
             std::copy(crec.begin() + CONTACT_REC * k, crec.begin() + CONTACT_REC * (k + 1),
                       rec.begin() + CONTACT_REC * (size_t)at);
         }
+        // levels, then chunks of whole levels with at most REPULSION_CHUNK
+        // contacts (a wider level alone)
+        const int lbase = (int)level_ptr.size() - 1;
         for (int L = 0; L < nl; L++)
             level_ptr.push_back(base + count[L + 1]);
-        wg_level.push_back((int)level_ptr.size() - 1);
+        for (int L = 0; L < nl;)
+        {
+            int L1 = L + 1;
+            while (L1 < nl && count[L1 + 1] - count[L] <= REPULSION_CHUNK)
+                L1++;
+            chunk_level.push_back(lbase + L1);
+            L = L1;
+        }
+        wg_chunk.push_back((int)chunk_level.size() - 1);
         wg_rows.push_back((int)touched_rows.size());
     }
     contact_blocks_ = (int)widx.size();
-    contact_groups_ = (int)wg_level.size() - 1;
+    contact_groups_ = (int)wg_chunk.size() - 1;
     // one staging buffer, one device buffer, one copy
     size_t off = 0;
     const size_t o_widx = off;
@@ -1335,7 +1347,9 @@ bool Batch::upload_contacts()
     const size_t o_h9 = off;
     off = align256(off + h9.size() * sizeof(double));
     const size_t o_wgl = off;
-    off = align256(off + wg_level.size() * sizeof(int));
+    off = align256(off + wg_chunk.size() * sizeof(int));
+    const size_t o_chl = off;
+    off = align256(off + chunk_level.size() * sizeof(int));
     const size_t o_lvl = off;
     off = align256(off + level_ptr.size() * sizeof(int));
     const size_t o_wgr = off;
@@ -1354,7 +1368,8 @@ bool Batch::upload_contacts()
     put(o_widx, widx.data(), widx.size() * sizeof(long long));
     put(o_diag, diag.data(), diag.size() * sizeof(int));
     put(o_h9, h9.data(), h9.size() * sizeof(double));
-    put(o_wgl, wg_level.data(), wg_level.size() * sizeof(int));
+    put(o_wgl, wg_chunk.data(), wg_chunk.size() * sizeof(int));
+    put(o_chl, chunk_level.data(), chunk_level.size() * sizeof(int));
     put(o_lvl, level_ptr.data(), level_ptr.size() * sizeof(int));
     put(o_wgr, wg_rows.data(), wg_rows.size() * sizeof(int));
     put(o_rows, touched_rows.data(), touched_rows.size() * sizeof(int));
@@ -1378,6 +1393,7 @@ bool Batch::upload_contacts()
     c_diag_ = reinterpret_cast<const int *>(b + o_diag);
     c_h9_ = reinterpret_cast<const double *>(b + o_h9);
     c_wgl_ = reinterpret_cast<const int *>(b + o_wgl);
+    c_chl_ = reinterpret_cast<const int *>(b + o_chl);
     c_lvl_ = reinterpret_cast<const int *>(b + o_lvl);
     c_wgr_ = reinterpret_cast<const int *>(b + o_wgr);
     c_rows_ = reinterpret_cast<const int *>(b + o_rows);
@@ -1564,8 +1580,8 @@ void Batch::step(int nsteps)
             else
                 solve_cg();
             if (ct) // repulsion on dx (BaraffCloth.cpp:147)
-                launch_repulsion(d_, contact_groups_, c_wgl_, c_lvl_, c_wgr_, c_rows_, c_ids_,
-                                 c_rec_, stream_);
+                launch_repulsion(d_, contact_groups_, c_wgl_, c_chl_, c_lvl_, c_wgr_, c_rows_,
+                                 c_ids_, c_rec_, stream_);
             launch_update_positions(d_, stream_);
         }
         check(hipGetLastError(), "step launch");

@@ -188,7 +188,7 @@ private:
     int contact_blocks_ = 0, contact_groups_ = 0;
     const long long *c_widx_ = nullptr;
     const int *c_diag_ = nullptr, *c_wgl_ = nullptr, *c_lvl_ = nullptr;
-    const int *c_wgr_ = nullptr, *c_rows_ = nullptr;
+    const int *c_wgr_ = nullptr, *c_rows_ = nullptr, *c_chl_ = nullptr;
     const double *c_h9_ = nullptr, *c_rec_ = nullptr;
     const int4 *c_ids_ = nullptr;
 };

@@ -14,7 +14,8 @@
 //     contact that shares a cloth vertex; contacts of one level touch
 //     disjoint vertices and run in parallel, levels in order.  The result is
 //     the serial one, bit for bit.  The touched rows of dx stay in LDS for
-//     the pass when they fit (chains can be thousands of levels deep).
+//     the pass when they fit, and levels are staged through LDS in chunks
+//     (chains can be thousands of levels deep).
 //
 // Contraction is off for this file so that every product and sum rounds as
 // in the reference.
@@ -112,44 +113,73 @@ __device__ void apply_contact(double *x0, double *x1, double *x2, int type,
     }
 }
 
-// One workgroup per cloth with contacts: its levels in order, the contacts
-// of a level spread over the workgroup's threads.  A contact names its
+// One single-wave workgroup per cloth with contacts: its levels in order,
+// the contacts of a level spread over the wave's lanes.  A contact names its
 // vertices by slot in the cloth's list of touched rows; when that list fits,
-// the touched rows of dx live in LDS for the whole pass (a level then costs
-// a barrier instead of a round trip to memory: contact chains through shared
-// vertices can be thousands of levels deep), else they are used in place.
-__global__ void __launch_bounds__(256)
-k_repulsion(double *__restrict__ dx, const int *__restrict__ wg_level,
-            const int *__restrict__ level_ptr, const int *__restrict__ wg_rows,
-            const int *__restrict__ rows, const int4 *__restrict__ ids,
-            const double *__restrict__ rec)
+// the touched rows of dx live in LDS for the whole pass, else they are used
+// in place.  Levels come in chunks of whole levels holding at most
+// REPULSION_CHUNK contacts, whose records are staged in LDS with one
+// coalesced load, so that a level costs its arithmetic and a barrier
+// (contact chains through shared vertices can be thousands of levels deep);
+// a level wider than a chunk runs from memory.
+__global__ void __launch_bounds__(64)
+k_repulsion(double *__restrict__ dx, const int *__restrict__ wg_chunk,
+            const int *__restrict__ chunk_level, const int *__restrict__ level_ptr,
+            const int *__restrict__ wg_rows, const int *__restrict__ rows,
+            const int4 *__restrict__ ids, const double *__restrict__ rec)
 {
     __shared__ double s_dx[3 * REPULSION_LDS_ROWS];
-    const int g = blockIdx.x;
+    __shared__ double s_rec[REPULSION_CHUNK * CONTACT_REC];
+    __shared__ int4 s_id[REPULSION_CHUNK];
+    __shared__ int s_lp[REPULSION_CHUNK + 1];
+    const int g = blockIdx.x, lane = threadIdx.x, nl = blockDim.x;
     const int *my_rows = rows + wg_rows[g];
     const int nr = wg_rows[g + 1] - wg_rows[g];
     const bool lds = nr <= REPULSION_LDS_ROWS;
     if (lds)
     {
-        for (int k = threadIdx.x; k < 3 * nr; k += blockDim.x)
+        for (int k = lane; k < 3 * nr; k += nl)
             s_dx[k] = dx[3 * (size_t)my_rows[k / 3] + k % 3];
         __syncthreads();
     }
-    for (int L = wg_level[g]; L < wg_level[g + 1]; L++)
+    auto row = [&](int sl) { return lds ? s_dx + 3 * sl : dx + 3 * (size_t)my_rows[sl]; };
+    for (int c = wg_chunk[g]; c < wg_chunk[g + 1]; c++)
     {
-        for (int k = level_ptr[L] + (int)threadIdx.x; k < level_ptr[L + 1]; k += blockDim.x)
+        const int L0 = chunk_level[c], L1 = chunk_level[c + 1];
+        const int k0 = level_ptr[L0], k1 = level_ptr[L1];
+        if (k1 - k0 <= REPULSION_CHUNK)
         {
-            const int4 id = ids[k];
-            const int s0 = id.y, s1 = id.x == 1 ? id.y : id.z, s2 = id.x == 2 ? id.w : id.y;
-            double *x0 = lds ? s_dx + 3 * s0 : dx + 3 * (size_t)my_rows[s0];
-            double *x1 = lds ? s_dx + 3 * s1 : dx + 3 * (size_t)my_rows[s1];
-            double *x2 = lds ? s_dx + 3 * s2 : dx + 3 * (size_t)my_rows[s2];
-            apply_contact(x0, x1, x2, id.x, rec + CONTACT_REC * (size_t)k);
+            for (int k = lane; k < k1 - k0; k += nl)
+                s_id[k] = ids[k0 + k];
+            for (int k = lane; k < (k1 - k0) * CONTACT_REC; k += nl)
+                s_rec[k] = rec[CONTACT_REC * (size_t)k0 + k];
+            for (int L = L0 + lane; L <= L1; L += nl)
+                s_lp[L - L0] = level_ptr[L] - k0;
+            __syncthreads();
+            for (int L = 0; L < L1 - L0; L++)
+            {
+                for (int k = s_lp[L] + lane; k < s_lp[L + 1]; k += nl)
+                {
+                    const int4 id = s_id[k];
+                    const int s1 = id.x == 1 ? id.y : id.z, s2 = id.x == 2 ? id.w : id.y;
+                    apply_contact(row(id.y), row(s1), row(s2), id.x, s_rec + CONTACT_REC * k);
+                }
+                __syncthreads(); // this level's writes before the next level reads
+            }
         }
-        __syncthreads(); // this level's writes before the next level reads
+        else // one level wider than a chunk
+        {
+            for (int k = k0 + lane; k < k1; k += nl)
+            {
+                const int4 id = ids[k];
+                const int s1 = id.x == 1 ? id.y : id.z, s2 = id.x == 2 ? id.w : id.y;
+                apply_contact(row(id.y), row(s1), row(s2), id.x, rec + CONTACT_REC * (size_t)k);
+            }
+            __syncthreads();
+        }
     }
     if (lds)
-        for (int k = threadIdx.x; k < 3 * nr; k += blockDim.x)
+        for (int k = lane; k < 3 * nr; k += nl)
             dx[3 * (size_t)my_rows[k / 3] + k % 3] = s_dx[k];
 }
 
@@ -163,13 +193,13 @@ void launch_contact_hessian(const DeviceBatch &d, int n, const long long *w_inde
                            diag_row, h9, d.W64, d.W32, d.P64, d.P32);
 }
 
-void launch_repulsion(const DeviceBatch &d, int n_groups, const int *wg_level,
-                      const int *level_ptr, const int *wg_rows, const int *rows, const int4 *ids,
-                      const double *rec, hipStream_t s)
+void launch_repulsion(const DeviceBatch &d, int n_groups, const int *wg_chunk,
+                      const int *chunk_level, const int *level_ptr, const int *wg_rows,
+                      const int *rows, const int4 *ids, const double *rec, hipStream_t s)
 {
     if (n_groups > 0)
-        hipLaunchKernelGGL(k_repulsion, dim3(n_groups), dim3(256), 0, s, d.dx, wg_level,
-                           level_ptr, wg_rows, rows, ids, rec);
+        hipLaunchKernelGGL(k_repulsion, dim3(n_groups), dim3(64), 0, s, d.dx, wg_chunk,
+                           chunk_level, level_ptr, wg_rows, rows, ids, rec);
 }
 
 } // namespace bw

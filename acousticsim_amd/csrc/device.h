@@ -227,15 +227,18 @@ constexpr int CONTACT_REC = 12;
 // (its Jacobi diagonal is refreshed) or -1
 void launch_contact_hessian(const DeviceBatch &d, int n, const long long *w_index,
                             const int *diag_row, const double *h9, hipStream_t s);
-// repulsion impulses on dx: workgroup g runs levels [wg_level[g],
-// wg_level[g+1]), level L the contacts [level_ptr[L], level_ptr[L+1]);
-// ids = {type (0 edge-edge, 1 cloth vertex, 2 cloth triangle), slots...},
-// a slot indexing the workgroup's touched rows rows[wg_rows[g] ...
-// wg_rows[g+1]); up to REPULSION_LDS_ROWS of them are kept in LDS
-constexpr int REPULSION_LDS_ROWS = 2560;
-void launch_repulsion(const DeviceBatch &d, int n_groups, const int *wg_level,
-                      const int *level_ptr, const int *wg_rows, const int *rows, const int4 *ids,
-                      const double *rec, hipStream_t s);
+// repulsion impulses on dx: workgroup g runs chunks [wg_chunk[g],
+// wg_chunk[g+1]), chunk c the levels [chunk_level[c], chunk_level[c+1]),
+// level L the contacts [level_ptr[L], level_ptr[L+1]); a chunk holds at most
+// REPULSION_CHUNK contacts unless it is one wider level.  ids = {type (0
+// edge-edge, 1 cloth vertex, 2 cloth triangle), slots...}, a slot indexing
+// the workgroup's touched rows rows[wg_rows[g] ... wg_rows[g+1]); up to
+// REPULSION_LDS_ROWS of them are kept in LDS
+constexpr int REPULSION_LDS_ROWS = 1536;
+constexpr int REPULSION_CHUNK = 192;
+void launch_repulsion(const DeviceBatch &d, int n_groups, const int *wg_chunk,
+                      const int *chunk_level, const int *level_ptr, const int *wg_rows,
+                      const int *rows, const int4 *ids, const double *rec, hipStream_t s);
 // measurement helpers: set the solver residual to 1 on every row; one SpMV
 // launch of the solver's precision
 void launch_fill_residual(const DeviceBatch &d, bool fp32, hipStream_t s);
