@@ -22,15 +22,20 @@ void check(hipError_t e, const char *what)
 }
 } // namespace
 
-// slices above which a cloth's solver sums go through group sums
-// (BW_GROUP_MIN overrides the default for tuning runs)
-static int group_min()
+// Slices above which a cloth's solver sums go through group sums
+// (BW_GROUP_MIN overrides the choice for tuning runs).  The group-sum
+// launches cost every iteration a little latency and save each large cloth
+// a redundant pass over its partials that grows with the square of its
+// slice count.  A batch's cloth may be solved alone and pay the latency in
+// full: 2048 (a 257 x 257 cloth, 1,032 slices, runs 23.7 steps/s without
+// group sums and 21.0 with).  The server's cloths always share their rounds'
+// launches: 512 (the serving mix runs 42.6 cloth-steps/s at 512 against
+// 32.0 at 2048).
+static int group_min(bool shared_launches)
 {
-    const char *e = std::getenv("BW_GROUP_MIN");
-    // 2048: the two extra launches per iteration cost a 257 x 257 cloth
-    // (1,032 slices) more than its SpMV saves (20.0 against 18.1 us per
-    // iteration); from a few thousand slices the redundant pass dominates
-    return e ? std::max(0, std::atoi(e)) : 2048;
+    if (const char *e = std::getenv("BW_GROUP_MIN"))
+        return std::max(0, std::atoi(e));
+    return shared_launches ? 512 : 2048;
 }
 
 template <typename T> T *Batch::dalloc(size_t n)
@@ -280,6 +285,9 @@ void Batch::build_layout()
     std::vector<double> ie_factor(nie);
     std::vector<InstParams> params(n);
     std::vector<int> group_inst;
+    // a batch's own rule does not depend on its other members, so that a
+    // cloth gives the same result alone and in a mix
+    const int gmin = group_min(opt_.shared_launches);
     for (int i = 0; i < n; i++)
     {
         const ClothModel &m = models_[i];
@@ -381,7 +389,7 @@ void Batch::build_layout()
         // group sums for cloths whose per-workgroup pass over all slice
         // partials would cost more than one extra small launch
         P.group_begin = (int)group_inst.size();
-        P.group_count = nslices_[i] > group_min() ? (nslices_[i] + GROUP - 1) / GROUP : 0;
+        P.group_count = nslices_[i] > gmin ? (nslices_[i] + GROUP - 1) / GROUP : 0;
         group_inst.insert(group_inst.end(), P.group_count, i);
     }
     d_.n_groups = (int)group_inst.size();

@@ -56,6 +56,9 @@ struct BatchOptions
 {
     hipStream_t stream = nullptr;
     CgState *cg_states = nullptr;
+    // the batch's solver launches are shared with other batches (a server's
+    // rounds): group sums pay off from smaller cloths
+    bool shared_launches = false;
 };
 
 class Batch

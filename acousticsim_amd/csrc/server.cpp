@@ -213,6 +213,7 @@ void Server::builder_loop()
             BatchOptions bo;
             bo.stream = stream_;
             bo.cg_states = states_dev_ + 2 * (size_t)slot;
+            bo.shared_launches = true;
             std::vector<ClothModel> ms;
             ms.push_back(std::move(m));
             auto r = std::make_unique<Resident>();
