@@ -485,6 +485,8 @@ void Batch::build_layout()
         upload(d_.gf_grav, grav);
     }
     else
+    d_.cg_fresh = dalloc<unsigned char>(n);
+    upload(d_.cg_fresh, std::vector<unsigned char>(n, 1));
     {
         if (kind_ == SolverKind::CG32)
         {
@@ -558,6 +560,7 @@ void Batch::refresh_pins(int i, const std::vector<int> &pins)
     ClothModel &m = models_[i];
     m.pins = pins;
     m.rebuild_pins_from_list();
+    mark_fresh(i);
     std::vector<uint8_t> rows((size_t)nslices_[i] * SLICE, 0);
     for (int v = 0; v < m.nv; v++)
         rows[row_of_[i][v]] = m.pinned[v];
@@ -565,8 +568,20 @@ void Batch::refresh_pins(int i, const std::vector<int> &pins)
           "hipMemcpy pins");
 }
 
+void Batch::mark_fresh(int i)
+{
+    if (!d_.cg_fresh)
+        return;
+    const unsigned char one[1] = {1};
+    if (i >= 0)
+        check(hipMemcpy(d_.cg_fresh + i, one, 1, hipMemcpyHostToDevice), "hipMemcpy fresh");
+    else
+        upload(d_.cg_fresh, std::vector<unsigned char>(size(), 1));
+}
+
 void Batch::upload_state(int i, const double *src, double *dst_dev)
 {
+    mark_fresh(i);
     std::vector<double> rows(3 * (size_t)nslices_[i] * SLICE, 0.0);
     for (int v = 0; v < models_[i].nv; v++)
         for (int a = 0; a < 3; a++)

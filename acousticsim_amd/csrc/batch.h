@@ -130,6 +130,8 @@ public:
 private:
     void build_layout();
     void upload_state(int i, const double *src, double *dst_dev);
+    // cloth i (all for -1) starts its next solve on the FP32 matrix
+    void mark_fresh(int i);
     void download(int i, const double *src_dev, double *dst) const;
     void read_states();
     // one chunk of solver launches; slices: the k_cg_spmv grid

@@ -570,15 +570,25 @@ k_cg_group_multi(const DeviceBatch *__restrict__ table, const int2 *__restrict__
 }
 
 // both slots to the solve start (true residual of the warm start first)
-__global__ void k_cg_reset(CgState *st, int n)
+// Solver states for a new solve, one thread per cloth.  A cloth whose last
+// solve moved to the FP64 matrix starts on it: its stiffness has not
+// changed, so the FP32 matrix would stall again, and finding that out costs
+// every solve up to max_since iterations.  fresh[i] (set by the host when
+// the cloth's state or pins are set from outside) starts it on the FP32
+// matrix instead, as a new cloth.
+__global__ void k_cg_reset(CgState *st, unsigned char *fresh, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= 2 * n)
+    if (i >= n)
         return;
+    const int keep = fresh[i] ? 0 : (st[2 * i].w64 | st[2 * i + 1].w64);
+    fresh[i] = 0;
     CgState z = {};
     z.start = 1;
     z.first = 1;
-    st[i] = z;
+    z.w64 = keep;
+    st[2 * i] = z;
+    st[2 * i + 1] = z;
 }
 
 // timing set-up: every cloth mid-solve, at a k_cg_spmv with residual 1
@@ -622,7 +632,8 @@ constexpr int SB = BW_SB;
 
 void launch_cg_reset(const DeviceBatch &d, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_cg_reset, dim3((2 * d.n_inst + 63) / 64), dim3(64), 0, s, d.cg, d.n_inst);
+    hipLaunchKernelGGL(k_cg_reset, dim3((d.n_inst + 63) / 64), dim3(64), 0, s, d.cg, d.cg_fresh,
+                       d.n_inst);
 }
 
 template <typename T, int L>

@@ -155,6 +155,9 @@ struct DeviceBatch
     // accumulated in FP32 and folded into the FP64 dx by the k_cg_update
     // that follows each true residual (the solution is dx + dx32)
     float *dx32 = nullptr;
+    // per cloth: 1 until the next k_cg_reset, which then starts the cloth
+    // on the FP32 matrix (else a cloth keeps the FP64 matrix it moved to)
+    unsigned char *cg_fresh = nullptr;
     double *z64 = nullptr;
     // GPU form (gpuform.hip): bending scalar per ELL entry, per-triangle
     // FP32 force and Hessian (90), block-Jacobi inverse (9 per row), the
