@@ -436,6 +436,8 @@ void Batch::build_layout()
     d_.P64 = dalloc<double>(nv3);
     d_.P32 = dalloc<float>(nv3);
     d_.params = dalloc<InstParams>(n);
+    d_.cg_fresh = dalloc<unsigned char>(n);
+    upload(d_.cg_fresh, std::vector<unsigned char>(n, 1));
     if (kind_ == SolverKind::RefPCG64)
     {
         d_.r64 = dalloc<double>(nv3);
@@ -485,8 +487,6 @@ void Batch::build_layout()
         upload(d_.gf_grav, grav);
     }
     else
-    d_.cg_fresh = dalloc<unsigned char>(n);
-    upload(d_.cg_fresh, std::vector<unsigned char>(n, 1));
     {
         if (kind_ == SolverKind::CG32)
         {
