@@ -161,7 +161,7 @@ def lib():
         "bw_batch_get_stats": (i32, [vp, i32, P(StepStats)]),
         "bw_batch_set_graphs": (i32, [vp, i32]),
         "bw_batch_time_solver_kernel": (i32, [vp, i32, dp, dp]),
-        "bw_batch_autotune": (i32, [vp, ip, dp]),
+        "bw_batch_autotune": (i32, [vp, ip, ip, ip, dp]),
         "bw_server_create": (i32, [ctypes.c_double, i32, cp, P(vp)]),
         "bw_server_destroy": (None, [vp]),
         "bw_server_submit": (i32, [vp, cp, dp, i32, i32, P(ctypes.c_longlong)]),
@@ -480,11 +480,25 @@ class ClothBatch:
         return float(us[0]), float(nbytes[0])
 
     def autotune(self):
-        """(lanes per row chosen, {lanes: mean SpMV launch us})"""
+        """(lanes per row chosen, {lanes: mean SpMV launch us}) for the
+        direction form and load hint chosen; autotune_all() has every form"""
+        lanes, staged, nt, us = self.autotune_all()
+        return lanes, us[("staged" if staged else "gathered") + ("_nt" if nt else "")]
+
+    def autotune_all(self):
+        """(lanes per row, staged (bool), nontemporal (bool),
+        {"gathered"|"staged"|"gathered_nt"|"staged_nt": {lanes: us}}); a
+        variant not timed has 0 us"""
         lanes = ctypes.c_int()
-        us = np.zeros(3)
-        _check(lib().bw_batch_autotune(self._h, ctypes.byref(lanes), _dp(us)))
-        return lanes.value, {1: us[0], 2: us[1], 4: us[2]}
+        staged = ctypes.c_int()
+        nt = ctypes.c_int()
+        us = np.zeros(12)
+        _check(lib().bw_batch_autotune(self._h, ctypes.byref(lanes), ctypes.byref(staged),
+                                       ctypes.byref(nt), _dp(us)))
+        forms = ("gathered", "staged", "gathered_nt", "staged_nt")
+        return (lanes.value, bool(staged.value), bool(nt.value),
+                {f: {1: float(us[3 * i]), 2: float(us[3 * i + 1]), 4: float(us[3 * i + 2])}
+                 for i, f in enumerate(forms)})
 
     def set_state(self, i, x, xpre):
         x = np.ascontiguousarray(x, dtype=np.float64)

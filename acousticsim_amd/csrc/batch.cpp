@@ -407,6 +407,39 @@ void Batch::build_layout()
     d_.row_pinned = dalloc<uint8_t>(d_.n_rows);
     d_.row_mass = dalloc<double>(d_.n_rows);
     d_.col = dalloc<int>(entries);
+    // the staged SpMV's lists: the distinct columns of each slice, ascending,
+    // and each entry's position in its slice's list
+    std::vector<int> halo_off(slices + 1, 0), halo_row;
+    std::vector<uint16_t> lcol(entries);
+    {
+        int hmax = 0;
+        std::vector<int> u;
+        for (int s = 0; s < slices; s++)
+        {
+            const size_t e0 = slice_off[s], e1 = e0 + (size_t)slice_w[s] * SLICE;
+            u.assign(col.begin() + e0, col.begin() + e1);
+            std::sort(u.begin(), u.end());
+            u.erase(std::unique(u.begin(), u.end()), u.end());
+            hmax = std::max(hmax, (int)u.size());
+            if (hmax > HALO_CAP)
+                break;
+            for (size_t e = e0; e < e1; e++)
+                lcol[e] = (uint16_t)(std::lower_bound(u.begin(), u.end(), col[e]) - u.begin());
+            halo_row.insert(halo_row.end(), u.begin(), u.end());
+            halo_off[s + 1] = (int)halo_row.size();
+        }
+        if (hmax <= HALO_CAP && slices > 0)
+        {
+            d_.halo_max = hmax;
+            n_halo_ = (long long)halo_row.size();
+            d_.halo_off = dalloc<int>(slices + 1);
+            d_.halo_row = dalloc<int>(halo_row.size());
+            d_.lcol = dalloc<uint16_t>(entries);
+            upload(d_.halo_off, halo_off);
+            upload(d_.halo_row, halo_row);
+            upload(d_.lcol, lcol);
+        }
+    }
     d_.W64 = dalloc<double>(9 * (size_t)entries);
     d_.W32 = dalloc<float>(9 * (size_t)entries);
     d_.vert_row = dalloc<int>(nvert);
@@ -713,8 +746,12 @@ double Batch::time_solver_kernel(int reps, double *alg_bytes)
         for (int k = 0; k < reps; k++)
             launch_cg_spmv_only(all, fp32, stream_);
         check(hipEventRecord(e1, stream_), "hipEventRecord");
-        // per block: W + column; per row: z, d_prev read, d, W d written
-        bytes = fp32 ? 40.0 * nnzb + 48.0 * nv : 76.0 * nnzb + 96.0 * nv;
+        // per block: W + column (4 B index, or the staged variant's 2 B
+        // position plus 4 B per entry of the slices' column lists); per row:
+        // z, d_prev read, d, W d written
+        const double wb = fp32 ? 36.0 : 72.0, vb = fp32 ? 48.0 : 96.0;
+        bytes = d_.spmv_stage ? (wb + 2.0) * nnzb + 4.0 * (double)n_halo_ + vb * nv
+                              : (wb + 4.0) * nnzb + vb * nv;
     }
     check(hipEventSynchronize(e1), "hipEventSynchronize");
     float ms = 0;
@@ -1034,9 +1071,11 @@ void Batch::end_step(const CgState *states, double step_us)
         throw std::runtime_error("step: non-finite residual (NaN/Inf in the state)");
 }
 
-// Picks the SpMV variant (lanes per row, see cg.hip) that runs this batch's
-// matrix fastest, timing each on the system assembled at the current state.
-// BW_SPMV_LPR=1|2|4 in the environment fixes the choice instead.
+// Picks the SpMV variant (lanes per row; directions gathered from memory or
+// staged in LDS; matrix loads with or without the nontemporal hint; see
+// cg.hip) that runs this batch's matrix fastest, timing each on the system
+// assembled at the current state.  BW_SPMV_LPR=1|2|4, BW_SPMV_STAGE=0|1 and
+// BW_SPMV_NT=0|1 in the environment fix those choices instead.
 int Batch::autotune()
 {
     if (kind_ == SolverKind::RefPCG64 || kind_ == SolverKind::GpuForm)
@@ -1045,30 +1084,55 @@ int Batch::autotune()
         (void)hipGraphExecDestroy(g.exec);
     graphs_.clear();
     tuned_ = true;
+    int fixed_lpr = 0, fixed_stage = -1, fixed_nt = -1;
     if (const char *e = std::getenv("BW_SPMV_LPR"))
     {
         const int v = std::atoi(e);
         if (v == 1 || v == 2 || v == 4)
-        {
-            d_.spmv_lpr = v;
-            return v;
-        }
+            fixed_lpr = v;
     }
+    if (const char *e = std::getenv("BW_SPMV_STAGE"))
+        fixed_stage = std::atoi(e) != 0 ? 1 : 0;
+    if (const char *e = std::getenv("BW_SPMV_NT"))
+        fixed_nt = std::atoi(e) != 0 ? 1 : 0;
+    if (d_.halo_max == 0)
+        fixed_stage = 0;
     double best = 1e300;
-    int pick = 4;
+    int pick = 0;
     const int choices[3] = {1, 2, 4};
-    for (int k = 0; k < 3; k++)
+    // variant v: lanes choices[v % 3], staged (v / 3) % 2, nontemporal v / 6
+    auto set = [&](int v) {
+        d_.spmv_lpr = choices[v % 3];
+        d_.spmv_stage = (v / 3) % 2;
+        d_.spmv_nt = v / 6;
+    };
+    int candidates = 0;
+    for (int v = 0; v < TUNE_VARIANTS; v++)
     {
-        d_.spmv_lpr = choices[k];
-        tune_us_[k] = time_solver_kernel(20, nullptr);
-        if (tune_us_[k] < best)
-        {
-            best = tune_us_[k];
-            pick = choices[k];
-        }
+        tune_us_[v] = 0.0;
+        if ((fixed_lpr && choices[v % 3] != fixed_lpr) ||
+            (fixed_stage >= 0 && (v / 3) % 2 != fixed_stage) || (fixed_nt >= 0 && v / 6 != fixed_nt))
+            continue;
+        pick = v;
+        candidates++;
     }
-    d_.spmv_lpr = pick;
-    return pick;
+    if (candidates > 1)
+        for (int v = 0; v < TUNE_VARIANTS; v++)
+        {
+            if ((fixed_lpr && choices[v % 3] != fixed_lpr) ||
+                (fixed_stage >= 0 && (v / 3) % 2 != fixed_stage) ||
+                (fixed_nt >= 0 && v / 6 != fixed_nt))
+                continue;
+            set(v);
+            tune_us_[v] = time_solver_kernel(20, nullptr);
+            if (tune_us_[v] < best)
+            {
+                best = tune_us_[v];
+                pick = v;
+            }
+        }
+    set(pick);
+    return d_.spmv_lpr;
 }
 
 // ---------------------------------------------------------------------------

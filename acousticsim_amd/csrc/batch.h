@@ -85,10 +85,16 @@ public:
     void end_step(const CgState *states, double step_us);
     size_t device_bytes() const { return device_bytes_; }
     // choose the SpMV variant by timing (done by the first step()); returns
-    // the lanes per row picked; tune_us() has the times of 1, 2, 4
+    // the lanes per row picked; tune_us() has the times of variant v =
+    // lanes index (1, 2, 4) + 3 * staged + 6 * nontemporal (0: not timed;
+    // the staged form needs every slice to have at most HALO_CAP columns)
+    static constexpr int TUNE_VARIANTS = 12;
     int autotune();
     const double *tune_us() const { return tune_us_; }
     int spmv_lpr() const { return d_.spmv_lpr; }
+    bool spmv_stage() const { return d_.spmv_stage != 0; }
+    bool spmv_nt() const { return d_.spmv_nt != 0; }
+    int halo_max() const { return d_.halo_max; }
     // assemble W and rhs at the current state (no solve) and copy them out in
     // the model's block order
     void assemble_system(int i, double *W, double *rhs);
@@ -157,7 +163,8 @@ private:
     BatchOptions opt_;
     bool own_stream_ = false;
     bool tuned_ = false;
-    double tune_us_[3] = {0, 0, 0};
+    double tune_us_[TUNE_VARIANTS] = {};
+    long long n_halo_ = 0; // entries of the staged SpMV's column lists
     size_t device_bytes_ = 0;
     SolverKind kind_ = SolverKind::CG32;
     InstState *host_state_ = nullptr; // pinned, reference PCG state

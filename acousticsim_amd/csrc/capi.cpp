@@ -637,14 +637,18 @@ int bw_batch_time_solver_kernel(bw_batch *b, int reps, double *avg_us, double *a
     });
 }
 
-int bw_batch_autotune(bw_batch *b, int *lanes, double *us)
+int bw_batch_autotune(bw_batch *b, int *lanes, int *staged, int *nontemporal, double *us)
 {
     return guard([&] {
         if (!b || !lanes)
             return fail(BW_ERR_ARG, "NULL argument");
         *lanes = b->batch->autotune();
+        if (staged)
+            *staged = b->batch->spmv_stage() ? 1 : 0;
+        if (nontemporal)
+            *nontemporal = b->batch->spmv_nt() ? 1 : 0;
         if (us)
-            for (int k = 0; k < 3; k++)
+            for (int k = 0; k < bw::Batch::TUNE_VARIANTS; k++)
                 us[k] = b->batch->tune_us()[k];
         return BW_OK;
     });

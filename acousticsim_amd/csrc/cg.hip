@@ -168,37 +168,166 @@ __device__ inline void solver_sums(const DeviceBatch &d, const InstParams &ip, i
                          ip.slice_count, comp, out);
 }
 
+// Loads of data that one launch reads once (matrix planes, column indices),
+// with the nontemporal hint when NT: a matrix larger than the Infinity Cache
+// then streams past the caches instead of evicting the direction vectors
+// that the gathers and the staging read many times.  A matrix that fits the
+// Infinity Cache is read from it by the next launch without the hint, so the
+// hint is one more autotuned dimension of the SpMV.
+template <bool NT, typename V> __device__ inline V stream_load(const G<V> *p)
+{
+    if constexpr (NT)
+        return __builtin_nontemporal_load(p);
+    else
+        return *p;
+}
+
 // q = W d over the rows of one slice for lane phase j: W of type TW (the
 // solver's matrix, or W64 in the mixed form after a stall), d = z + beta
-// d_prev formed on the fly in T, sums in TW
-template <typename TW, typename T, int L>
+// d_prev formed on the fly in T, sums in TW.
+// The slots go in chunks of U per lane, software-pipelined: the column
+// indices of the next chunk are loaded after the gathers and blocks of the
+// current one, so each chunk waits on one memory round trip (gathers and
+// blocks together) rather than two (indices, then gathers), and the
+// indices' loads, issued last, stay in flight across the chunk's arithmetic.
+template <typename TW, typename T, int L, bool NT>
 __device__ inline void slice_product(const G<TW> *__restrict__ Wl, const G<int> *__restrict__ cl,
                                      int w, int j, const G<T> *__restrict__ z,
                                      const G<T> *__restrict__ dold, T bt, bool first, TW (&y)[3])
 {
+    constexpr int U = sizeof(TW) == sizeof(T) ? 4 / L : 1; // slots per lane and chunk
+    constexpr int STEP = U * L;
     y[0] = y[1] = y[2] = 0;
-#pragma unroll(sizeof(TW) == sizeof(T) ? 4 / L : 1)
-    for (int k = j; k < w; k += L)
+    int c[U];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+        c[u] = j + u * L < w ? stream_load<NT>(cl + (j + u * L) * SLICE) : -1;
+    for (int k0 = j; k0 < w; k0 += STEP)
     {
-        const size_t c3 = 3 * (size_t)cl[k * SLICE];
-        const TW v0 = (TW)dir_value<T>(z[c3], dold[c3], bt, first);
-        const TW v1 = (TW)dir_value<T>(z[c3 + 1], dold[c3 + 1], bt, first);
-        const TW v2 = (TW)dir_value<T>(z[c3 + 2], dold[c3 + 2], bt, first);
-        const G<TW> *wk = Wl + 9 * (size_t)k * SLICE;
-        y[0] += wk[0 * SLICE] * v0 + wk[1 * SLICE] * v1 + wk[2 * SLICE] * v2;
-        y[1] += wk[3 * SLICE] * v0 + wk[4 * SLICE] * v1 + wk[5 * SLICE] * v2;
-        y[2] += wk[6 * SLICE] * v0 + wk[7 * SLICE] * v1 + wk[8 * SLICE] * v2;
+        T zv[U][3], dv[U][3];
+        TW m[U][9];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (c[u] >= 0)
+            {
+                const size_t c3 = 3 * (size_t)c[u];
+                for (int a = 0; a < 3; a++)
+                {
+                    zv[u][a] = z[c3 + a];
+                    dv[u][a] = dold[c3 + a];
+                }
+                const G<TW> *wk = Wl + 9 * (size_t)(k0 + u * L) * SLICE;
+                for (int e = 0; e < 9; e++)
+                    m[u][e] = stream_load<NT>(wk + e * SLICE);
+            }
+        int cn[U];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+        {
+            const int k = k0 + STEP + u * L;
+            cn[u] = k < w ? stream_load<NT>(cl + k * SLICE) : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+        {
+            if (c[u] >= 0)
+            {
+                const TW v0 = (TW)dir_value<T>(zv[u][0], dv[u][0], bt, first);
+                const TW v1 = (TW)dir_value<T>(zv[u][1], dv[u][1], bt, first);
+                const TW v2 = (TW)dir_value<T>(zv[u][2], dv[u][2], bt, first);
+                y[0] += m[u][0] * v0 + m[u][1] * v1 + m[u][2] * v2;
+                y[1] += m[u][3] * v0 + m[u][4] * v1 + m[u][5] * v2;
+                y[2] += m[u][6] * v0 + m[u][7] * v1 + m[u][8] * v2;
+            }
+            c[u] = cn[u];
+        }
     }
     for (int a = 0; a < 3; a++)
         y[a] = phase_sum<L>(y[a]);
 }
 
-// k_cg_spmv on slice sl of batch d
+// The staged variant: the workgroup first forms d = z + beta d_prev once for
+// every distinct column of its slice (the slice's halo list, ascending rows,
+// so neighbouring lanes load neighbouring rows) into LDS, 4 values per
+// column; the slot loop then reads each column's d from LDS through its
+// 16-bit position in the list.  A column shared by several rows of the
+// slice is loaded from memory once instead of once per row, and the loop's
+// only memory loads are the matrix planes and the 2-byte positions, which
+// do not wait on one another.
 template <typename T, int L>
+__device__ inline void stage_directions(const DeviceBatch &d, int sl, const G<T> *__restrict__ z,
+                                        const G<T> *__restrict__ dold, T bt, bool first, T *vs)
+{
+    const int h0 = glob(d.halo_off)[sl];
+    const int H = glob(d.halo_off)[sl + 1] - h0;
+    const G<int> *hr = glob(d.halo_row) + h0;
+    for (int i = threadIdx.x; i < H; i += L * 64)
+    {
+        const size_t c3 = 3 * (size_t)hr[i];
+        for (int a = 0; a < 3; a++)
+            vs[4 * i + a] = dir_value<T>(z[c3 + a], dold[c3 + a], bt, first);
+    }
+    __syncthreads();
+}
+
+template <typename TW, typename T, int L, bool NT>
+__device__ inline void slice_product_staged(const DeviceBatch &d, int sl,
+                                            const G<TW> *__restrict__ Wl,
+                                            const G<uint16_t> *__restrict__ lc, int w, int j,
+                                            const G<T> *__restrict__ z,
+                                            const G<T> *__restrict__ dold, T bt, bool first,
+                                            T *vs, TW (&y)[3])
+{
+    constexpr int U = sizeof(TW) == sizeof(T) ? 4 / L : 1; // slots per lane and chunk
+    constexpr int STEP = U * L;
+    y[0] = y[1] = y[2] = 0;
+    // the first chunk's blocks and positions are in flight while the
+    // directions are staged
+    TW m[U][9];
+    int c[U];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+    {
+        const int k = j + u * L;
+        c[u] = k < w ? (int)stream_load<NT>(lc + k * SLICE) : -1;
+        if (k < w)
+            for (int e = 0; e < 9; e++)
+                m[u][e] = stream_load<NT>(Wl + 9 * (size_t)k * SLICE + e * SLICE);
+    }
+    stage_directions<T, L>(d, sl, z, dold, bt, first, vs);
+    for (int k0 = j; k0 < w; k0 += STEP)
+    {
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (c[u] >= 0)
+            {
+                const int cc = c[u];
+                const TW v0 = (TW)vs[4 * cc], v1 = (TW)vs[4 * cc + 1], v2 = (TW)vs[4 * cc + 2];
+                y[0] += m[u][0] * v0 + m[u][1] * v1 + m[u][2] * v2;
+                y[1] += m[u][3] * v0 + m[u][4] * v1 + m[u][5] * v2;
+                y[2] += m[u][6] * v0 + m[u][7] * v1 + m[u][8] * v2;
+            }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+        {
+            const int k = k0 + STEP + u * L;
+            c[u] = k < w ? (int)stream_load<NT>(lc + k * SLICE) : -1;
+            if (k < w)
+                for (int e = 0; e < 9; e++)
+                    m[u][e] = stream_load<NT>(Wl + 9 * (size_t)k * SLICE + e * SLICE);
+        }
+    }
+    for (int a = 0; a < 3; a++)
+        y[a] = phase_sum<L>(y[a]);
+}
+
+// k_cg_spmv on slice sl of batch d; S: the staged variant (vs: its LDS)
+template <typename T, int L, bool S, bool NT>
 __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> *__restrict__ W,
                                  const G<T> *__restrict__ P, G<T> *__restrict__ r,
                                  G<T> *__restrict__ z, G<T> *__restrict__ q,
-                                 G<T> *__restrict__ dbuf0, G<T> *__restrict__ dbuf1, double *lds)
+                                 G<T> *__restrict__ dbuf0, G<T> *__restrict__ dbuf1, double *lds,
+                                 T *vs)
 {
     const int inst = glob(d.slice_inst)[sl];
     const InstParams ip = glob(d.params)[inst];
@@ -334,10 +463,32 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
     bool wide = false; // the FP64 matrix under FP32 vectors
     if constexpr (sizeof(T) == sizeof(float))
         wide = st.w64 != 0;
-    if (wide)
+    if constexpr (S)
+    {
+        const G<uint16_t> *lc = glob(d.lcol) + base + row;
+        if (wide)
+        {
+            double y[3];
+            slice_product_staged<double, T, L, NT>(d, sl, glob(d.W64) + 9 * base + row, lc, w, j, z,
+                                               dold, bt, first, vs, y);
+            y0 = (T)y[0];
+            y1 = (T)y[1];
+            y2 = (T)y[2];
+        }
+        else
+        {
+            T y[3];
+            slice_product_staged<T, T, L, NT>(d, sl, W + 9 * base + row, lc, w, j, z, dold, bt,
+                                          first, vs, y);
+            y0 = y[0];
+            y1 = y[1];
+            y2 = y[2];
+        }
+    }
+    else if (wide)
     {
         double y[3];
-        slice_product<double, T, L>(glob(d.W64) + 9 * base + row, cl, w, j, z, dold, bt, first, y);
+        slice_product<double, T, L, NT>(glob(d.W64) + 9 * base + row, cl, w, j, z, dold, bt, first, y);
         y0 = (T)y[0];
         y1 = (T)y[1];
         y2 = (T)y[2];
@@ -345,7 +496,7 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
     else
     {
         T y[3];
-        slice_product<T, T, L>(W + 9 * base + row, cl, w, j, z, dold, bt, first, y);
+        slice_product<T, T, L, NT>(W + 9 * base + row, cl, w, j, z, dold, bt, first, y);
         y0 = y[0];
         y1 = y[1];
         y2 = y[2];
@@ -470,7 +621,11 @@ __device__ inline void update_body(const DeviceBatch &d, const int blk,
         store_state(st_out, st);
 }
 
-template <typename T, int L>
+// the staged variant's LDS: 4 values per halo entry (HALO_CAP at most),
+// sized at launch
+extern __shared__ __align__(16) unsigned char stage_lds[];
+
+template <typename T, int L, bool S, bool NT>
 __global__ void __launch_bounds__(L * 64)
 k_cg_spmv(DeviceBatch d, const T *__restrict__ W, const T *__restrict__ P, T *__restrict__ r,
           T *__restrict__ z, T *__restrict__ q, T *__restrict__ dbuf0, T *__restrict__ dbuf1)
@@ -484,8 +639,8 @@ k_cg_spmv(DeviceBatch d, const T *__restrict__ W, const T *__restrict__ P, T *__
         if (sl < 0)
             return;
     }
-    spmv_body<T, L>(d, sl, glob(W), glob(P), glob(r), glob(z), glob(q), glob(dbuf0),
-                    glob(dbuf1), lds);
+    spmv_body<T, L, S, NT>(d, sl, glob(W), glob(P), glob(r), glob(z), glob(q), glob(dbuf0),
+                       glob(dbuf1), lds, reinterpret_cast<T *>(stage_lds));
 }
 
 template <typename T, int SB>
@@ -533,7 +688,7 @@ __global__ void __launch_bounds__(64) k_cg_group(DeviceBatch d, int slot)
 
 // Many batches in one launch: entry (batch, slice) of `list` per workgroup,
 // batches in the device table `table`; entries with batch < 0 are padding.
-template <typename T, int L>
+template <typename T, int L, bool S>
 __global__ void __launch_bounds__(L * 64)
 k_cg_spmv_multi(const DeviceBatch *__restrict__ table, const int2 *__restrict__ list, int xcd_map)
 {
@@ -542,9 +697,9 @@ k_cg_spmv_multi(const DeviceBatch *__restrict__ table, const int2 *__restrict__ 
     if (e.x < 0)
         return;
     const DeviceBatch &d = table[e.x];
-    spmv_body<T, L>(d, e.y, glob(Vecs<T>::W(d)), glob(Vecs<T>::P(d)), glob(Vecs<T>::r(d)),
-                 glob(Vecs<T>::z(d)), glob(Vecs<T>::q(d)), glob(Vecs<T>::d0(d)),
-                 glob(Vecs<T>::d1(d)), lds);
+    spmv_body<T, L, S, false>(d, e.y, glob(Vecs<T>::W(d)), glob(Vecs<T>::P(d)), glob(Vecs<T>::r(d)),
+                       glob(Vecs<T>::z(d)), glob(Vecs<T>::q(d)), glob(Vecs<T>::d0(d)),
+                       glob(Vecs<T>::d1(d)), lds, reinterpret_cast<T *>(stage_lds));
 }
 
 template <typename T, int SB>
@@ -636,12 +791,29 @@ void launch_cg_reset(const DeviceBatch &d, hipStream_t s)
                        d.n_inst);
 }
 
+// bytes of LDS the staged variant needs for lists of up to halo_max entries
+template <typename T> size_t stage_bytes(int halo_max) { return 4 * sizeof(T) * (size_t)halo_max; }
+
+template <typename T, int L, bool NT>
+void launch_spmv_nt(const DeviceBatch &d, int slices, const T *W, const T *P, T *r, T *z, T *q,
+                    T *d0, T *d1, hipStream_t s)
+{
+    if (d.spmv_stage)
+        hipLaunchKernelGGL((k_cg_spmv<T, L, true, NT>), dim3(slices), dim3(L * 64),
+                           stage_bytes<T>(d.halo_max), s, d, W, P, r, z, q, d0, d1);
+    else
+        hipLaunchKernelGGL((k_cg_spmv<T, L, false, NT>), dim3(slices), dim3(L * 64), 0, s, d, W,
+                           P, r, z, q, d0, d1);
+}
+
 template <typename T, int L>
 void launch_spmv(const DeviceBatch &d, int slices, const T *W, const T *P, T *r, T *z, T *q,
                  T *d0, T *d1, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_cg_spmv<T, L>), dim3(slices), dim3(L * 64), 0, s, d, W, P, r, z, q, d0,
-                       d1);
+    if (d.spmv_nt)
+        launch_spmv_nt<T, L, true>(d, slices, W, P, r, z, q, d0, d1, s);
+    else
+        launch_spmv_nt<T, L, false>(d, slices, W, P, r, z, q, d0, d1, s);
 }
 
 template <typename T>
@@ -656,19 +828,29 @@ void launch_spmv_lpr(const DeviceBatch &d, int slices, const T *W, const T *P, T
         launch_spmv<T, 4>(d, slices, W, P, r, z, q, d0, d1, s);
 }
 
-template <typename T>
-void launch_spmv_multi(const DeviceBatch *table, const int2 *list, int grid, int lpr, int xcd_map,
-                       hipStream_t s)
+template <typename T, bool S>
+void launch_spmv_multi_v(const DeviceBatch *table, const int2 *list, int grid, int lpr,
+                         int xcd_map, size_t lds, hipStream_t s)
 {
     if (lpr == 1)
-        hipLaunchKernelGGL((k_cg_spmv_multi<T, 1>), dim3(grid), dim3(64), 0, s, table, list,
+        hipLaunchKernelGGL((k_cg_spmv_multi<T, 1, S>), dim3(grid), dim3(64), lds, s, table, list,
                            xcd_map);
     else if (lpr == 2)
-        hipLaunchKernelGGL((k_cg_spmv_multi<T, 2>), dim3(grid), dim3(128), 0, s, table, list,
+        hipLaunchKernelGGL((k_cg_spmv_multi<T, 2, S>), dim3(grid), dim3(128), lds, s, table, list,
                            xcd_map);
     else
-        hipLaunchKernelGGL((k_cg_spmv_multi<T, 4>), dim3(grid), dim3(256), 0, s, table, list,
+        hipLaunchKernelGGL((k_cg_spmv_multi<T, 4, S>), dim3(grid), dim3(256), lds, s, table, list,
                            xcd_map);
+}
+
+template <typename T>
+void launch_spmv_multi(const DeviceBatch *table, const int2 *list, int grid, int lpr,
+                       int halo_max, int xcd_map, hipStream_t s)
+{
+    if (halo_max > 0)
+        launch_spmv_multi_v<T, true>(table, list, grid, lpr, xcd_map, stage_bytes<T>(halo_max), s);
+    else
+        launch_spmv_multi_v<T, false>(table, list, grid, lpr, xcd_map, 0, s);
 }
 
 void launch_group_sums(const DeviceBatch &d, int slot, hipStream_t s)
@@ -714,12 +896,12 @@ void launch_cg_step(const DeviceBatch &d, bool fp32, int phase, int slices, hipS
 
 void launch_cg_step_multi(const DeviceBatch *table, const int2 *spmv_list, int spmv_grid,
                           const int2 *upd_list, int upd_grid, const int2 *grp_list, int grp_grid,
-                          bool fp32, int phase, int lpr, int xcd_map, hipStream_t s)
+                          bool fp32, int phase, int lpr, int halo_max, int xcd_map, hipStream_t s)
 {
     if (fp32)
     {
         if (phase == 0)
-            launch_spmv_multi<float>(table, spmv_list, spmv_grid, lpr, xcd_map, s);
+            launch_spmv_multi<float>(table, spmv_list, spmv_grid, lpr, halo_max, xcd_map, s);
         else
             hipLaunchKernelGGL((k_cg_update_multi<float, SB>), dim3(upd_grid), dim3(SB * 64), 0, s,
                                table, upd_list, xcd_map);
@@ -727,7 +909,7 @@ void launch_cg_step_multi(const DeviceBatch *table, const int2 *spmv_list, int s
     else
     {
         if (phase == 0)
-            launch_spmv_multi<double>(table, spmv_list, spmv_grid, lpr, xcd_map, s);
+            launch_spmv_multi<double>(table, spmv_list, spmv_grid, lpr, halo_max, xcd_map, s);
         else
             hipLaunchKernelGGL((k_cg_update_multi<double, SB>), dim3(upd_grid), dim3(SB * 64), 0,
                                s, table, upd_list, xcd_map);

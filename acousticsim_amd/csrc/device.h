@@ -17,6 +17,9 @@ namespace bw
 {
 
 constexpr int SLICE = 64;
+// longest per-slice column list the staged SpMV keeps in LDS (4 values of
+// the solver's precision per entry: 32 KB in FP32, 64 KB in FP64)
+constexpr int HALO_CAP = 2048;
 
 // Global-address-space views of device pointers.  Pointers that kernels read
 // from device memory (a table of batches) are generic to the compiler, and
@@ -106,6 +109,8 @@ struct DeviceBatch
     int n_inst = 0, n_slices = 0, n_rows = 0;
     int xcd_map = 1;   // solver workgroups remapped so each XCD owns a contiguous range
     int spmv_lpr = 4;  // k_cg_spmv variant: lanes per row (1, 2 or 4)
+    int spmv_stage = 0; // k_cg_spmv variant: directions staged in LDS (1) or gathered (0)
+    int spmv_nt = 0;    // k_cg_spmv variant: matrix loads with the nontemporal hint (1)
     long long n_entries = 0;
     int n_tri = 0, n_ie = 0, n_vert = 0, n_blocks = 0;
     int n_blocks_wg = 0; // solver workgroups
@@ -117,6 +122,13 @@ struct DeviceBatch
     double *row_mass = nullptr;
     // matrix
     int *col = nullptr;
+    // the staged SpMV: per slice the ascending distinct column rows of its
+    // entries (halo_row[halo_off[s] ...], n_slices + 1 offsets), each
+    // entry's position in its slice's list (lcol, indexed like col), and the
+    // longest list (halo_max; 0: the staged variant is unavailable)
+    int *halo_off = nullptr, *halo_row = nullptr;
+    uint16_t *lcol = nullptr;
+    int halo_max = 0;
     double *W64 = nullptr;
     float *W32 = nullptr;
     // assembly inputs (global vertex ids map to physical rows)
@@ -207,10 +219,12 @@ void launch_cg_spmv_only(const DeviceBatch &d, bool fp32, hipStream_t s);
 // spmv_list one (batch, slice) entry per k_cg_spmv workgroup, upd_list one
 // (batch, solver workgroup) entry per k_cg_update workgroup, grp_list one
 // (batch, group) entry per group-sum wavefront (grp_grid 0: none); batch < 0
-// pads
+// pads; halo_max > 0 runs the staged SpMV variant (every batch in the table
+// must have its lists) with LDS for that many entries
 void launch_cg_step_multi(const DeviceBatch *table, const int2 *spmv_list, int spmv_grid,
                           const int2 *upd_list, int upd_grid, const int2 *grp_list, int grp_grid,
-                          bool fp32, int phase, int lpr, int xcd_map, hipStream_t s);
+                          bool fp32, int phase, int lpr, int halo_max, int xcd_map,
+                          hipStream_t s);
 void launch_cg_timing_setup(const DeviceBatch &d, bool fp32, hipStream_t s);
 // group sums of the solver partials in `slot` for cloths that have them
 void launch_group_sums(const DeviceBatch &d, int slot, hipStream_t s);

@@ -289,7 +289,8 @@ void Server::launch_chain(int length, int spmv_grid, int grp_grid)
     const int ugrid = (spmv_grid + BW_SB - 1) / BW_SB;
     for (int k = 0; k < length; k++)
         launch_cg_step_multi(table_dev_, spmv_list_dev_, spmv_grid, upd_list_dev_, ugrid,
-                             grp_list_dev_, grp_grid, opt_.fp32, k & 1, opt_.spmv_lpr, 1, stream_);
+                             grp_list_dev_, grp_grid, opt_.fp32, k & 1, opt_.spmv_lpr, 0, 1,
+                             stream_);
 }
 
 void Server::run_chunk(int length, int spmv_grid, int grp_grid)

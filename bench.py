@@ -52,7 +52,17 @@ from acousticsim_amd import workloads as wl  # noqa: E402
 
 METRIC = "cloth steps/s & p50/p99 step latency at 1e-6 PCG residual; SpMV HBM GB/s"
 HBM_PEAK_GBS = 8000.0
-KERNEL = "k_cg_spmv<float, %d> (PCG SpMV: d = P r + beta d_prev, q = W d, partial d.q)"
+KERNEL = ("k_cg_spmv<float, %d, %s, %s> (PCG SpMV: d = P r + beta d_prev, q = W d, partial "
+          "d.q; directions %s; matrix loads %s)")
+
+
+def kernel_name(lanes, staged, nt):
+    return KERNEL % (lanes, "true" if staged else "false", "true" if nt else "false",
+                     "staged in LDS per distinct slice column" if staged
+                     else "gathered per slot",
+                     "nontemporal" if nt else "cached")
+
+
 # HBM bytes per launch of the k_cg_spmv variant (lanes per row) the autotuner
 # picks, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE over the
 # --roofline-only command, scaled by the same counters' reading for a 256 MiB
@@ -60,8 +70,8 @@ KERNEL = "k_cg_spmv<float, %d> (PCG SpMV: d = P r + beta d_prev, q = W d, partia
 # variants time within 1% of each other on both workloads, so either may be
 # picked; the counter passes abort inside the profiler on the irregular
 # workload, so it has none)
-TRAFFIC = {"batch64": {1: 213810572.0, 2: 209697107.0},
-           "single257": {1: 33077660.0, 2: 32754087.0}}
+TRAFFIC = {"batch64": {(1, False, False): 213810572.0, (2, False, False): 209697107.0},
+           "single257": {(1, False, False): 33077660.0, (2, False, False): 32754087.0}}
 
 
 def members(workload, precision):
@@ -105,7 +115,7 @@ def describe(workload, ms):
 def time_batch(ms, steps, warmup, graphs=True):
     b = capi.ClothBatch([c for c, _ in ms], [x for _, x in ms])
     b.set_graphs(graphs)
-    b.tuning = b.autotune()
+    b.tuning = b.autotune_all()
     for _ in range(warmup):
         b.step(1)
     lat = []
@@ -215,11 +225,15 @@ def roofline_only(args, ms):
         torch.cuda.synchronize()
         print("bench: copied %d bytes x3" % (4 * n), file=sys.stderr, flush=True)
     b = capi.ClothBatch([c for c, _ in ms], [x for _, x in ms])
-    lanes, tune = b.autotune()
+    lanes, staged, nt, tune = b.autotune_all()
     us, nbytes = b.time_solver_kernel(reps=200)
     b.close()
     achieved = nbytes / us / 1e3
-    print(json.dumps({"workload": args.workload, "kernel": KERNEL % lanes, "launches": 222,
+    print(json.dumps({"workload": args.workload, "kernel": kernel_name(lanes, staged, nt),
+                      "tuning_us": {f: {str(k): float(v) for k, v in t.items()}
+                                    for f, t in tune.items()},
+                      "launches": 20 * sum(int(v > 0) for t in tune.values()
+                                           for v in t.values()) + 201,
                       "launch_us": us, "alg_bytes": nbytes, "achieved_GBs": achieved,
                       "frac": achieved / HBM_PEAK_GBS}), flush=True)
 
@@ -411,13 +425,15 @@ def main():
         "p50_ms": 1000 * float(np.percentile(lat, 50)),
         "p99_ms": 1000 * float(np.percentile(lat, 99)),
         "pcg_iterations_mean": iters,
-        "autotune": {"spmv_lanes_per_row": b.tuning[0],
-                     "spmv_us_by_lanes": {str(k): v for k, v in b.tuning[1].items()}},
+        "autotune": {"spmv_lanes_per_row": b.tuning[0], "spmv_staged": b.tuning[1],
+                     "spmv_nontemporal": b.tuning[2],
+                     "spmv_us": {f: {str(k): float(v) for k, v in t.items()}
+                                 for f, t in b.tuning[3].items()}},
         "max_rel_residual": worst,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": TRAFFIC.get(args.workload, {}).get(b.tuning[0]),
-                     "kernel": KERNEL % b.tuning[0],
+                     "traffic": TRAFFIC.get(args.workload, {}).get(tuple(b.tuning[:3])),
+                     "kernel": kernel_name(*b.tuning[:3]),
                      "launch_us": us, "alg_bytes": nbytes},
     }
     b.close()

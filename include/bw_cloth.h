@@ -251,16 +251,25 @@ int bw_batch_set_graphs(bw_batch *b, int enable);
    (FP32 matrix) or 72 + 4 (FP64) bytes per true 3x3 block plus the vector
    values per vertex, each read or written once (tolerance modes: P r and
    d_prev read, d and q written, 12 values; reference mode: r, P, d_prev
-   read, d and q written, 15 values).  The system is
+   read, d and q written, 15 values).  With the staged SpMV variant the
+   column index is a 2-byte position (36 + 2 bytes per block) and each
+   entry of the slices' column lists adds 4 bytes.  The system is
    assembled at the current state first; the solver's vectors are
    overwritten (positions are not). */
 int bw_batch_time_solver_kernel(bw_batch *b, int reps, double *avg_us, double *alg_bytes);
-/* Autotuner (no reference counterpart): time the SpMV variants (1, 2 or 4
-   lanes per matrix row) on the batch's own system and keep the fastest; the
-   first bw_batch_step does this by itself.  lanes gets the choice, us (3
-   doubles, may be NULL) the mean launch times of the 1-, 2- and 4-lane
-   variants.  BW_SPMV_LPR=1|2|4 in the environment fixes the choice. */
-int bw_batch_autotune(bw_batch *b, int *lanes, double *us);
+/* Autotuner (no reference counterpart): time the SpMV variants on the
+   batch's own system and keep the fastest; the first bw_batch_step does
+   this by itself.  A variant is 1, 2 or 4 lanes per matrix row; the
+   direction vector gathered from memory per slot, or staged in LDS once
+   per distinct column of a 64-row slice; matrix loads with or without the
+   nontemporal hint.  lanes gets the lanes per row chosen, staged and
+   nontemporal (each may be NULL) 1 if that form was chosen, us (12
+   doubles, may be NULL) the mean launch time of variant
+   v = (0, 1, 2 for 1, 2, 4 lanes) + 3 * staged + 6 * nontemporal (0 for a
+   variant not timed: the staged form needs every slice to have at most
+   2048 distinct columns).  BW_SPMV_LPR=1|2|4, BW_SPMV_STAGE=0|1 and
+   BW_SPMV_NT=0|1 in the environment fix those choices. */
+int bw_batch_autotune(bw_batch *b, int *lanes, int *staged, int *nontemporal, double *us);
 
 /* ---- serving (no reference counterpart): step requests of different
    cloths arrive at any time, are admitted against a device-memory budget,

@@ -31,15 +31,18 @@ def _confs(precision):
 
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
 @pytest.mark.parametrize("graphs", [True, False])
-def test_alone_equals_in_mix(precision, graphs, monkeypatch):
+@pytest.mark.parametrize("stage", ["0", "1"])
+def test_alone_equals_in_mix(precision, graphs, stage, monkeypatch):
     """Bit-identical for one SpMV variant (the autotuner may pick different
     variants for different batches, and variants sum a row's blocks in
-    different orders), so the variant is fixed here."""
+    different orders), so the variant is fixed here: 2 lanes per row, the
+    directions gathered per slot (stage 0) or staged in LDS (stage 1)."""
     from acousticsim_amd import capi
 
     monkeypatch.setenv("BW_SPMV_LPR", "2")
+    monkeypatch.setenv("BW_SPMV_STAGE", stage)
     cs = _confs(precision)
-    say("case batch", precision, "graphs", graphs)
+    say("case batch", precision, "graphs", graphs, "stage", stage)
     b = capi.ClothBatch([c for c, _ in cs], [x for _, x in cs])
     b.set_graphs(graphs)
     b.step(5)
@@ -81,26 +84,41 @@ def test_batched_step_matches_oracle(precision):
 
 
 def test_autotuner_variants_agree(monkeypatch):
-    """The autotuner times the 1-, 2- and 4-lane SpMV variants and keeps the
-    fastest; every variant gives the same step within the solve tolerance."""
+    """The autotuner times the 1-, 2- and 4-lane SpMV variants, each with the
+    directions gathered per slot or staged in LDS and the matrix loaded with
+    or without the nontemporal hint, and keeps the fastest; every variant
+    gives the same step within the solve tolerance (the staged and gathered
+    forms sum in the same order, but the compiler contracts their products
+    into FMAs differently), and the hint changes no result bit."""
     from acousticsim_amd import capi
 
     cs = _confs("mixed")
     b = capi.ClothBatch([c for c, _ in cs], [x for _, x in cs])
-    lanes, us = b.autotune()
-    say("case autotune: picked %d, us %s" % (lanes, us))
-    assert lanes in (1, 2, 4) and us[lanes] == min(us.values())
+    lanes, staged, nt, us = b.autotune_all()
+    say("case autotune: picked %d staged %s nt %s, us %s" % (lanes, staged, nt, us))
+    timed = [v for f in us.values() for v in f.values()]
+    assert len(timed) == 12 and min(timed) > 0
+    form = ("staged" if staged else "gathered") + ("_nt" if nt else "")
+    assert lanes in (1, 2, 4) and us[form][lanes] == min(timed)
     out = {}
-    for lpr in ("1", "2", "4"):
-        monkeypatch.setenv("BW_SPMV_LPR", lpr)
-        b = capi.ClothBatch([c for c, _ in cs], [x for _, x in cs])
-        b.step(1)
-        out[lpr] = [b.get_pos(i) for i in range(len(cs))]
-        assert max(b.stats(i)["rel_residual"] for i in range(len(cs))) <= 1e-6
+    for hint in ("0", "1"):
+        for stage in ("0", "1"):
+            for lpr in ("1", "2", "4"):
+                monkeypatch.setenv("BW_SPMV_LPR", lpr)
+                monkeypatch.setenv("BW_SPMV_STAGE", stage)
+                monkeypatch.setenv("BW_SPMV_NT", hint)
+                b = capi.ClothBatch([c for c, _ in cs], [x for _, x in cs])
+                b.step(1)
+                out[hint, stage, lpr] = [b.get_pos(i) for i in range(len(cs))]
+                assert max(b.stats(i)["rel_residual"] for i in range(len(cs))) <= 1e-6
     for i, (c, x0) in enumerate(cs):
-        d = np.abs(out["4"][i] - x0).max()
-        for lpr in ("1", "2"):
-            assert np.abs(out[lpr][i] - out["4"][i]).max() <= 1e-4 * d
+        ref = out["0", "0", "4"][i]
+        d = np.abs(ref - x0).max()
+        for key, pos in out.items():
+            assert np.abs(pos[i] - ref).max() <= 1e-4 * d, key
+        for stage in ("0", "1"):
+            for lpr in ("1", "2", "4"):
+                assert np.array_equal(out["0", stage, lpr][i], out["1", stage, lpr][i])
 
 
 @pytest.mark.parametrize("graphs", [True, False])
@@ -112,6 +130,7 @@ def test_group_sums_alone_equals_mix_and_oracle(graphs, monkeypatch):
     from acousticsim_amd import capi
 
     monkeypatch.setenv("BW_SPMV_LPR", "2")
+    monkeypatch.setenv("BW_SPMV_STAGE", "0")
     monkeypatch.setenv("BW_GROUP_MIN", "0")
     cs = _confs("mixed")
     say("case group sums, graphs", graphs)
@@ -146,6 +165,7 @@ def test_stalled_solves_move_to_fp64_matrix(monkeypatch):
     from acousticsim_amd import capi
 
     monkeypatch.setenv("BW_SPMV_LPR", "2")
+    monkeypatch.setenv("BW_SPMV_STAGE", "0")
     monkeypatch.setenv("BW_MAX_SINCE", "20")
     monkeypatch.setenv("BW_STALL", "1e-30")
     cs = _confs("mixed")

@@ -136,6 +136,7 @@ def test_batch_member_detection(capi, monkeypatch):
     """detection on one batch member sets only that member's contacts; the
     member then steps as the same cloth alone"""
     monkeypatch.setenv("BW_SPMV_LPR", "2")
+    monkeypatch.setenv("BW_SPMV_STAGE", "0")
     confs = [flat_cloth(9, 2e-3, "mixed", "tolerance"), flat_cloth(13, 2e-3, "mixed", "tolerance")]
     xs = [of.OracleCloth(c).rest_pos() for c in confs]
     xyz, tri = floor_body(0.0)

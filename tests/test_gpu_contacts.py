@@ -193,6 +193,7 @@ def test_contacts_last_one_step_and_batch_members(capi, monkeypatch):
     """contacts apply to the next step only; in a batch they reach only
     their member, which then equals the same cloth alone"""
     monkeypatch.setenv("BW_SPMV_LPR", "2")
+    monkeypatch.setenv("BW_SPMV_STAGE", "0")
     g = load_golden("grid21_contacts")
     ct = contacts_of(g)
     confs, x0s = [], []

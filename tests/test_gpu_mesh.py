@@ -105,6 +105,7 @@ def test_delaunay_twenty_tolerance_steps(capi, precision):
 
 def test_mesh_entry_equals_delaunay_geometry(capi, monkeypatch):
     monkeypatch.setenv("BW_SPMV_LPR", "1")  # the two objects autotune apart
+    monkeypatch.setenv("BW_SPMV_STAGE", "0")
     conf, pos, tri, x0 = _case(500, 6)
     say("case bw_create_mesh vs delaunay")
     a = capi.BaraffClothHip(conf, x0)
@@ -120,6 +121,7 @@ def test_mesh_entry_equals_delaunay_geometry(capi, monkeypatch):
 
 def test_delaunay_in_batch_equals_alone(capi, monkeypatch):
     monkeypatch.setenv("BW_SPMV_LPR", "2")  # the two batches autotune apart
+    monkeypatch.setenv("BW_SPMV_STAGE", "0")
     conf, pos, tri, x0 = _case(800, 7)
     g = wl.grid_config(33)
     gx = wl.jittered_positions(g, 8)
@@ -137,6 +139,7 @@ def test_normals_match_oracle(capi, kind, monkeypatch):
     the oracle's at the same positions to 1e-14, and a batch member's equal
     the single cloth's (one SpMV variant fixed, as the two autotune apart)."""
     monkeypatch.setenv("BW_SPMV_LPR", "1")
+    monkeypatch.setenv("BW_SPMV_STAGE", "0")
     say("case normals", kind)
     if kind == "grid":
         conf = wl.grid_config(21)

@@ -54,6 +54,7 @@ def test_served_alone_equals_batch_alone(monkeypatch):
     from acousticsim_amd import capi
 
     monkeypatch.setenv("BW_SPMV_LPR", "1")
+    monkeypatch.setenv("BW_SPMV_STAGE", "0")
     c, x0 = REQS[0]
     say("case served alone vs batch alone")
     info, x = _served([(c, x0)])[0][0]
@@ -142,6 +143,7 @@ def test_served_with_group_sums_equals_batch(monkeypatch):
     from acousticsim_amd import capi
 
     monkeypatch.setenv("BW_SPMV_LPR", "1")
+    monkeypatch.setenv("BW_SPMV_STAGE", "0")
     monkeypatch.setenv("BW_GROUP_MIN", "0")
     say("case served group sums")
     mix, counters = _served(REQS, stagger=0.01)
