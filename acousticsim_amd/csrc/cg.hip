@@ -270,30 +270,35 @@ __device__ inline void stage_directions(const DeviceBatch &d, int sl, const G<T>
     __syncthreads();
 }
 
-template <typename TW, typename T, int L, bool NT>
-__device__ inline void slice_product_staged(const DeviceBatch &d, int sl,
-                                            const G<TW> *__restrict__ Wl,
-                                            const G<uint16_t> *__restrict__ lc, int w, int j,
-                                            const G<T> *__restrict__ z,
-                                            const G<T> *__restrict__ dold, T bt, bool first,
-                                            T *vs, TW (&y)[3])
+// blocks and positions of the chunk of slots k0, k0 + L, ... (U of them)
+template <typename TW, int L, int U, bool NT>
+__device__ inline void staged_chunk(const G<TW> *__restrict__ Wl,
+                                    const G<uint16_t> *__restrict__ lc, int k0, int w,
+                                    TW (&m)[U][9], int (&c)[U])
 {
-    constexpr int U = sizeof(TW) == sizeof(T) ? 4 / L : 1; // slots per lane and chunk
-    constexpr int STEP = U * L;
-    y[0] = y[1] = y[2] = 0;
-    // the first chunk's blocks and positions are in flight while the
-    // directions are staged
-    TW m[U][9];
-    int c[U];
 #pragma unroll
     for (int u = 0; u < U; u++)
     {
-        const int k = j + u * L;
+        const int k = k0 + u * L;
         c[u] = k < w ? (int)stream_load<NT>(lc + k * SLICE) : -1;
         if (k < w)
             for (int e = 0; e < 9; e++)
                 m[u][e] = stream_load<NT>(Wl + 9 * (size_t)k * SLICE + e * SLICE);
     }
+}
+
+// the staged product given its first chunk (m, c), which the caller loaded
+// early so that it is in flight while the directions are staged
+template <typename TW, typename T, int L, int U, bool NT>
+__device__ inline void staged_product_from(const DeviceBatch &d, int sl,
+                                           const G<TW> *__restrict__ Wl,
+                                           const G<uint16_t> *__restrict__ lc, int w, int j,
+                                           const G<T> *__restrict__ z,
+                                           const G<T> *__restrict__ dold, T bt, bool first,
+                                           T *vs, TW (&m)[U][9], int (&c)[U], TW (&y)[3])
+{
+    constexpr int STEP = U * L;
+    y[0] = y[1] = y[2] = 0;
     stage_directions<T, L>(d, sl, z, dold, bt, first, vs);
     for (int k0 = j; k0 < w; k0 += STEP)
     {
@@ -307,15 +312,7 @@ __device__ inline void slice_product_staged(const DeviceBatch &d, int sl,
                 y[1] += m[u][3] * v0 + m[u][4] * v1 + m[u][5] * v2;
                 y[2] += m[u][6] * v0 + m[u][7] * v1 + m[u][8] * v2;
             }
-#pragma unroll
-        for (int u = 0; u < U; u++)
-        {
-            const int k = k0 + STEP + u * L;
-            c[u] = k < w ? (int)stream_load<NT>(lc + k * SLICE) : -1;
-            if (k < w)
-                for (int e = 0; e < 9; e++)
-                    m[u][e] = stream_load<NT>(Wl + 9 * (size_t)k * SLICE + e * SLICE);
-        }
+        staged_chunk<TW, L, U, NT>(Wl, lc, k0 + STEP, w, m, c);
     }
     for (int a = 0; a < 3; a++)
         y[a] = phase_sum<L>(y[a]);
@@ -329,6 +326,22 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
                                  G<T> *__restrict__ dbuf0, G<T> *__restrict__ dbuf1, double *lds,
                                  T *vs)
 {
+    const int lane = threadIdx.x & 63;
+    constexpr int RPW = SLICE / L;                                  // rows per wavefront
+    const int row = (threadIdx.x >> 6) * RPW + (lane & (RPW - 1)); // row within the slice
+    const int j = lane / RPW;                                        // lane phase
+    const int p = sl * SLICE + row;
+    const size_t p3 = 3 * (size_t)p;
+    const int w = glob(d.row_w)[p];
+    const size_t base = (size_t)glob(d.slice_off)[sl];
+    // staged variant: the first chunk of the solver's matrix is loaded
+    // before anything else, so that it is in flight during the prologue
+    // (state, solver sums) and the staging of the directions
+    constexpr int U = 4 / L;
+    T m0[U][9];
+    int c0[U];
+    if constexpr (S)
+        staged_chunk<T, L, U, NT>(W + 9 * base + row, glob(d.lcol) + base + row, j, w, m0, c0);
     const int inst = glob(d.slice_inst)[sl];
     const InstParams ip = glob(d.params)[inst];
     const bool lead = sl == ip.slice_begin && threadIdx.x == 0;
@@ -393,14 +406,6 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
             st.rho = rz;
         }
     }
-    const int lane = threadIdx.x & 63;
-    constexpr int RPW = SLICE / L;                                  // rows per wavefront
-    const int row = (threadIdx.x >> 6) * RPW + (lane & (RPW - 1)); // row within the slice
-    const int j = lane / RPW;                                        // lane phase
-    const int p = sl * SLICE + row;
-    const size_t p3 = 3 * (size_t)p;
-    const int w = glob(d.row_w)[p];
-    const size_t base = (size_t)glob(d.slice_off)[sl];
     const G<int> *cl = glob(d.col) + base + row;
     if (residual)
     {
@@ -468,9 +473,12 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
         const G<uint16_t> *lc = glob(d.lcol) + base + row;
         if (wide)
         {
-            double y[3];
-            slice_product_staged<double, T, L, NT>(d, sl, glob(d.W64) + 9 * base + row, lc, w, j, z,
-                                               dold, bt, first, vs, y);
+            double y[3], mw[1][9];
+            int cw[1];
+            const G<double> *Wl = glob(d.W64) + 9 * base + row;
+            staged_chunk<double, L, 1, NT>(Wl, lc, j, w, mw, cw);
+            staged_product_from<double, T, L, 1, NT>(d, sl, Wl, lc, w, j, z, dold, bt, first, vs,
+                                                     mw, cw, y);
             y0 = (T)y[0];
             y1 = (T)y[1];
             y2 = (T)y[2];
@@ -478,8 +486,8 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
         else
         {
             T y[3];
-            slice_product_staged<T, T, L, NT>(d, sl, W + 9 * base + row, lc, w, j, z, dold, bt,
-                                          first, vs, y);
+            staged_product_from<T, T, L, U, NT>(d, sl, W + 9 * base + row, lc, w, j, z, dold, bt,
+                                                first, vs, m0, c0, y);
             y0 = y[0];
             y1 = y[1];
             y2 = y[2];
