@@ -53,6 +53,8 @@ Server::Server(const ServerOptions &opt) : opt_(opt), t0_(std::chrono::steady_cl
 {
     if (opt_.max_resident < 1 || opt_.max_resident > 65536)
         throw std::runtime_error("config: max_resident must be in [1, 65536]");
+    if (const char *e = std::getenv("BW_SPMV_STAGE"))
+        opt_.spmv_stage = std::atoi(e) != 0;
     check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     const size_t ns = 2 * (size_t)opt_.max_resident;
     check(hipMalloc((void **)&states_dev_, ns * sizeof(CgState)), "hipMalloc");
@@ -282,6 +284,19 @@ void Server::upload_table(const std::vector<std::unique_ptr<Resident>> &res)
         check(hipMemcpyAsync(table_dev_, table_host_, top * sizeof(DeviceBatch),
                              hipMemcpyHostToDevice, stream_),
               "hipMemcpyAsync table");
+    // one SpMV form per launch: staged when every resident request has its
+    // column lists
+    int hmax = 0;
+    bool all = opt_.spmv_stage != 0;
+    for (auto &r : res)
+    {
+        all = all && r->batch->halo_max() > 0;
+        hmax = std::max(hmax, r->batch->halo_max());
+    }
+    halo_ = 0;
+    if (all && hmax > 0)
+        for (halo_ = 64; halo_ < hmax;)
+            halo_ *= 2;
 }
 
 void Server::launch_chain(int length, int spmv_grid, int grp_grid)
@@ -289,7 +304,7 @@ void Server::launch_chain(int length, int spmv_grid, int grp_grid)
     const int ugrid = (spmv_grid + BW_SB - 1) / BW_SB;
     for (int k = 0; k < length; k++)
         launch_cg_step_multi(table_dev_, spmv_list_dev_, spmv_grid, upd_list_dev_, ugrid,
-                             grp_list_dev_, grp_grid, opt_.fp32, k & 1, opt_.spmv_lpr, 0, 1,
+                             grp_list_dev_, grp_grid, opt_.fp32, k & 1, opt_.spmv_lpr, halo_, 1,
                              stream_);
 }
 
@@ -314,7 +329,8 @@ void Server::run_chunk(int length, int spmv_grid, int grp_grid)
         ++graph_clock_;
         hipGraphExec_t exec = nullptr;
         for (auto &g : graphs_)
-            if (g.length == piece && g.spmv_grid == spmv_grid && g.grp_grid == grp_grid)
+            if (g.length == piece && g.spmv_grid == spmv_grid && g.grp_grid == grp_grid &&
+                g.halo == halo_)
             {
                 exec = g.exec;
                 g.last_use = graph_clock_;
@@ -341,7 +357,7 @@ void Server::run_chunk(int length, int spmv_grid, int grp_grid)
             }
             check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "GraphInstantiate");
             (void)hipGraphDestroy(graph);
-            graphs_.push_back({piece, spmv_grid, grp_grid, exec, graph_clock_});
+            graphs_.push_back({piece, spmv_grid, grp_grid, halo_, exec, graph_clock_});
         }
         check(hipGraphLaunch(exec, stream_), "GraphLaunch");
     }

@@ -37,6 +37,10 @@ struct ServerOptions
     bool fp32 = true;                        // "mixed" (true) or "fp64" requests
     int builders = 4;                        // host threads building requests
     int spmv_lpr = 1;                        // SpMV lanes per row (fastest on serving mixes)
+    // SpMV directions staged in LDS (1) or gathered (0); BW_SPMV_STAGE
+    // overrides.  Staged: 42.1 against 38.9 cloth-steps/s on the serving mix
+    // of 64 requests (half Delaunay cloths), same requests, same device
+    int spmv_stage = 1;
 };
 
 struct RequestInfo
@@ -139,12 +143,15 @@ private:
 
     struct GraphEntry
     {
-        int length, spmv_grid, grp_grid;
+        int length, spmv_grid, grp_grid, halo;
         hipGraphExec_t exec;
         unsigned long long last_use;
     };
     std::vector<GraphEntry> graphs_;
     unsigned long long graph_clock_ = 0;
+    // staged SpMV: LDS entries per workgroup (the longest column list among
+    // the resident requests, rounded up to a power of two; 0: gathered)
+    int halo_ = 0;
     void run_chunk(int length, int spmv_grid, int grp_grid);
     void launch_chain(int length, int spmv_grid, int grp_grid);
 

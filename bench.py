@@ -63,14 +63,14 @@ def kernel_name(lanes, staged, nt):
                      "nontemporal" if nt else "cached")
 
 
-# HBM bytes per launch of the k_cg_spmv variant (lanes per row) the autotuner
-# picks, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE over the
-# --roofline-only command, scaled by the same counters' reading for a 256 MiB
-# device-to-device copy (profiles/k_cg_spmv_<workload>.md; the 1- and 2-lane
-# variants time within 1% of each other on both workloads, so either may be
-# picked; the counter passes abort inside the profiler on the irregular
-# workload, so it has none)
-TRAFFIC = {"batch64": {(1, False, False): 213810572.0, (2, False, False): 209697107.0},
+# HBM bytes per launch of the k_cg_spmv variant (lanes per row, staged,
+# nontemporal) the autotuner picks, from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE over the --roofline-only command, scaled by the same counters'
+# reading for a 256 MiB device-to-device copy (profiles/k_cg_spmv_<workload>.md;
+# the gathered 1- and 2-lane variants time within 1-3% of each other on both
+# workloads, so either may be picked; the counter passes abort inside the
+# profiler on the irregular workload, so it has none)
+TRAFFIC = {"batch64": {(1, False, False): 210387018.0, (2, False, False): 209697107.0},
            "single257": {(1, False, False): 33077660.0, (2, False, False): 32754087.0}}
 
 

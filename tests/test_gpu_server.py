@@ -49,14 +49,18 @@ def _served(reqs, stagger=0.0, **kw):
     return out, counters
 
 
-def test_served_alone_equals_batch_alone(monkeypatch):
-    """The server runs the 1-lane SpMV variant; the batch is held to it."""
+@pytest.mark.parametrize("stage", ["0", "1"])
+def test_served_alone_equals_batch_alone(stage, monkeypatch):
+    """The server runs the 1-lane SpMV variant with cached matrix loads, the
+    directions staged in LDS (its default) or gathered; the batch is held to
+    the same variant."""
     from acousticsim_amd import capi
 
     monkeypatch.setenv("BW_SPMV_LPR", "1")
-    monkeypatch.setenv("BW_SPMV_STAGE", "0")
+    monkeypatch.setenv("BW_SPMV_STAGE", stage)
+    monkeypatch.setenv("BW_SPMV_NT", "0")
     c, x0 = REQS[0]
-    say("case served alone vs batch alone")
+    say("case served alone vs batch alone, stage", stage)
     info, x = _served([(c, x0)])[0][0]
     assert info["status"] == 0 and info["steps_done"] == STEPS
     b = capi.ClothBatch([c], [x0])
@@ -136,16 +140,18 @@ def test_bad_requests_report_errors():
     s.close()
 
 
-def test_served_with_group_sums_equals_batch(monkeypatch):
+@pytest.mark.parametrize("stage", ["0", "1"])
+def test_served_with_group_sums_equals_batch(stage, monkeypatch):
     """Requests whose cloths reduce through group sums (forced for every
     cloth here) give, in a mix, the same result as a batch stepping each one
     alone."""
     from acousticsim_amd import capi
 
     monkeypatch.setenv("BW_SPMV_LPR", "1")
-    monkeypatch.setenv("BW_SPMV_STAGE", "0")
+    monkeypatch.setenv("BW_SPMV_STAGE", stage)
+    monkeypatch.setenv("BW_SPMV_NT", "0")
     monkeypatch.setenv("BW_GROUP_MIN", "0")
-    say("case served group sums")
+    say("case served group sums, stage", stage)
     mix, counters = _served(REQS, stagger=0.01)
     for i, (c, x0) in enumerate(REQS):
         info, x = mix[i]
