@@ -254,18 +254,88 @@ __device__ inline void slice_product(const G<TW> *__restrict__ Wl, const G<int> 
 // slice is loaded from memory once instead of once per row, and the loop's
 // only memory loads are the matrix planes and the 2-byte positions, which
 // do not wait on one another.
+// Staging in rounds of R = 4/L entries per lane (256 per workgroup): the
+// first round's column rows, z and d_prev are loaded at the very start of
+// the kernel (they do not depend on beta), so that they are in flight
+// during the prologue; each later round loads its rows, then all of its
+// gathers, then writes, so a round costs two memory round trips rather than
+// two per entry.
+template <typename T, int L> struct StageRound
+{
+    static constexpr int R = 4 / L;
+    int row[R];
+    T zv[R][3], dv[R][3];
+};
+
 template <typename T, int L>
-__device__ inline void stage_directions(const DeviceBatch &d, int sl, const G<T> *__restrict__ z,
-                                        const G<T> *__restrict__ dold, T bt, bool first, T *vs)
+__device__ inline void stage_round_rows(const G<int> *__restrict__ hr, int i0, int H,
+                                        StageRound<T, L> &b)
+{
+    constexpr int R = StageRound<T, L>::R;
+#pragma unroll
+    for (int u = 0; u < R; u++)
+    {
+        const int i = i0 + u * L * 64;
+        b.row[u] = i < H ? hr[i] : -1;
+    }
+}
+
+template <typename T, int L>
+__device__ inline void stage_round_gather(const G<T> *__restrict__ z,
+                                          const G<T> *__restrict__ dold, StageRound<T, L> &b)
+{
+    constexpr int R = StageRound<T, L>::R;
+#pragma unroll
+    for (int u = 0; u < R; u++)
+        if (b.row[u] >= 0)
+        {
+            const size_t c3 = 3 * (size_t)b.row[u];
+            for (int a = 0; a < 3; a++)
+            {
+                b.zv[u][a] = z[c3 + a];
+                b.dv[u][a] = dold[c3 + a];
+            }
+        }
+}
+
+template <typename T, int L>
+__device__ inline void stage_round_store(int i0, T bt, bool first, const StageRound<T, L> &b,
+                                         T *vs)
+{
+    constexpr int R = StageRound<T, L>::R;
+#pragma unroll
+    for (int u = 0; u < R; u++)
+        if (b.row[u] >= 0)
+        {
+            const int i = i0 + u * L * 64;
+            for (int a = 0; a < 3; a++)
+                vs[4 * i + a] = dir_value<T>(b.zv[u][a], b.dv[u][a], bt, first);
+        }
+}
+
+// the slice's column list: offset and length
+__device__ inline int2 stage_list(const DeviceBatch &d, int sl)
 {
     const int h0 = glob(d.halo_off)[sl];
-    const int H = glob(d.halo_off)[sl + 1] - h0;
-    const G<int> *hr = glob(d.halo_row) + h0;
-    for (int i = threadIdx.x; i < H; i += L * 64)
+    return make_int2(h0, glob(d.halo_off)[sl + 1] - h0);
+}
+
+// the rest of the staging after the first round (b, loaded early), and the
+// barrier after which the slot loop reads the LDS
+template <typename T, int L>
+__device__ inline void stage_directions(const DeviceBatch &d, int2 list, const G<T> *__restrict__ z,
+                                        const G<T> *__restrict__ dold, T bt, bool first,
+                                        const StageRound<T, L> &b, T *vs)
+{
+    constexpr int PER = StageRound<T, L>::R * L * 64; // entries per round
+    stage_round_store<T, L>(threadIdx.x, bt, first, b, vs);
+    const G<int> *hr = glob(d.halo_row) + list.x;
+    for (int i0 = threadIdx.x + PER; i0 - (int)threadIdx.x < list.y; i0 += PER)
     {
-        const size_t c3 = 3 * (size_t)hr[i];
-        for (int a = 0; a < 3; a++)
-            vs[4 * i + a] = dir_value<T>(z[c3 + a], dold[c3 + a], bt, first);
+        StageRound<T, L> nb;
+        stage_round_rows<T, L>(hr, i0, list.y, nb);
+        stage_round_gather<T, L>(z, dold, nb);
+        stage_round_store<T, L>(i0, bt, first, nb, vs);
     }
     __syncthreads();
 }
@@ -295,11 +365,12 @@ __device__ inline void staged_product_from(const DeviceBatch &d, int sl,
                                            const G<uint16_t> *__restrict__ lc, int w, int j,
                                            const G<T> *__restrict__ z,
                                            const G<T> *__restrict__ dold, T bt, bool first,
-                                           T *vs, TW (&m)[U][9], int (&c)[U], TW (&y)[3])
+                                           T *vs, const StageRound<T, L> &sb, int2 list,
+                                           TW (&m)[U][9], int (&c)[U], TW (&y)[3])
 {
     constexpr int STEP = U * L;
     y[0] = y[1] = y[2] = 0;
-    stage_directions<T, L>(d, sl, z, dold, bt, first, vs);
+    stage_directions<T, L>(d, list, z, dold, bt, first, sb, vs);
     for (int k0 = j; k0 < w; k0 += STEP)
     {
 #pragma unroll
@@ -340,8 +411,16 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
     constexpr int U = 4 / L;
     T m0[U][9];
     int c0[U];
+    StageRound<T, L> sb;
+    int2 list = make_int2(0, 0);
     if constexpr (S)
+    {
         staged_chunk<T, L, U, NT>(W + 9 * base + row, glob(d.lcol) + base + row, j, w, m0, c0);
+        // the first staging round's column rows; its gathers of z and
+        // d_prev follow the state load, which selects d_prev's buffer
+        list = stage_list(d, sl);
+        stage_round_rows<T, L>(glob(d.halo_row) + list.x, threadIdx.x, list.y, sb);
+    }
     const int inst = glob(d.slice_inst)[sl];
     const InstParams ip = glob(d.params)[inst];
     const bool lead = sl == ip.slice_begin && threadIdx.x == 0;
@@ -353,6 +432,8 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
             store_state(st_out, st);
         return;
     }
+    if constexpr (S)
+        stage_round_gather<T, L>(z, (st.iter & 1) ? dbuf0 : dbuf1, sb);
     G<double> *part_out = glob(d.cg_partial) + (size_t)PARTS * d.n_slices;
     const double tol2 = ip.tol * ip.tol;
     bool residual = false;
@@ -478,7 +559,7 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
             const G<double> *Wl = glob(d.W64) + 9 * base + row;
             staged_chunk<double, L, 1, NT>(Wl, lc, j, w, mw, cw);
             staged_product_from<double, T, L, 1, NT>(d, sl, Wl, lc, w, j, z, dold, bt, first, vs,
-                                                     mw, cw, y);
+                                                     sb, list, mw, cw, y);
             y0 = (T)y[0];
             y1 = (T)y[1];
             y2 = (T)y[2];
@@ -487,7 +568,7 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
         {
             T y[3];
             staged_product_from<T, T, L, U, NT>(d, sl, W + 9 * base + row, lc, w, j, z, dold, bt,
-                                                first, vs, m0, c0, y);
+                                                first, vs, sb, list, m0, c0, y);
             y0 = y[0];
             y1 = y[1];
             y2 = y[2];
