@@ -74,22 +74,23 @@ std::vector<int> slice_row_order(const ClothModel &m)
     {
         // Irregular meshes: vertex ids carry no locality, so rows follow a
         // Hilbert curve over the rest positions; inside every window of
-        // 16384 rows they are sorted by width class (widths in classes of
-        // 4 slots, widest first, curve order kept within a class), so that
-        // a 64-row slice holds rows of similar width (ELL padding 9% on
-        // Delaunay meshes with widths 4..25, 0.2% with exact widths, 39% in
-        // vertex order) whose columns stay close along the curve: the
-        // locality is worth more than the padding.  The window keeps a
-        // slice's gathered entries within a stretch of the curve that fits
-        // one XCD's L2.  SpMV on a 1.5M-point cloth: 226 us (classes of 4,
-        // 16384-row windows), 237 us (exact widths), 256 us (exact widths,
-        // 1024-row windows), 255 us (curve order only), 355 us (global
+        // 65536 rows they are sorted by width class (widths in classes of
+        // 2 slots, widest first, curve order kept within a class), so that
+        // a 64-row slice holds rows of similar width (ELL padding 9% with
+        // classes of 4 on Delaunay meshes with widths 4..25, 0.2% with exact
+        // widths, 39% in vertex order) whose columns stay close along the
+        // curve.  SpMV on a 1.5M-point cloth, staged directions (the
+        // autotuner's pick on such meshes): 165 us (classes of 2, 65536-row
+        // windows), 168 us (2, 16384), 170 us (4, 16384), 172 us (1, 16384
+        // or 2, 131072), 186 us (4, 256).  With gathered directions the
+        // locality is worth more than the padding: 226 us (4, 16384),
+        // 237 us (exact widths), 255 us (curve order only), 355 us (global
         // width sort).
         std::vector<int> order = hilbert_order(m.flat_xy);
-        int window = 16384; // BW_SORT_WINDOW overrides it for tuning runs
+        int window = 65536; // BW_SORT_WINDOW overrides it for tuning runs
         if (const char *e = std::getenv("BW_SORT_WINDOW"))
             window = std::max(64, std::atoi(e));
-        int bucket = 4; // widths compared in classes of this many slots
+        int bucket = 2; // widths compared in classes of this many slots
         if (const char *e = std::getenv("BW_WIDTH_BUCKET"))
             bucket = std::max(1, std::atoi(e));
         for (size_t s = 0; s < order.size(); s += window)
@@ -430,6 +431,27 @@ void Batch::build_layout()
         }
         if (hmax <= HALO_CAP && slices > 0)
         {
+            // LDS for every list, or, for batches that share launches with
+            // others (the server sizes the launch's LDS by the longest of
+            // its batches), for the 99th percentile of the list lengths (at
+            // least 256 entries, a multiple of 64); the rest of the slices
+            // take the gathered product.  Serving mix: 47.5 against 36.1
+            // cloth-steps/s; one 1.5M-point cloth: 165 us with every list
+            // staged against 172 us with the percentile.  BW_STAGE_LDS=max
+            // or p99 fixes the rule (a batch held to the server's form)
+            d_.halo_lds = hmax;
+            bool p99 = opt_.shared_launches;
+            if (const char *e = std::getenv("BW_STAGE_LDS"))
+                p99 = std::string(e) == "p99";
+            if (p99)
+            {
+                std::vector<int> len(slices);
+                for (int s = 0; s < slices; s++)
+                    len[s] = halo_off[s + 1] - halo_off[s];
+                const int k = (slices - 1) * 99 / 100;
+                std::nth_element(len.begin(), len.begin() + k, len.end());
+                d_.halo_lds = std::min(hmax, std::max(256, (len[k] + 63) / 64 * 64));
+            }
             d_.halo_max = hmax;
             n_halo_ = (long long)halo_row.size();
             d_.halo_off = dalloc<int>(slices + 1);

@@ -95,6 +95,7 @@ public:
     bool spmv_stage() const { return d_.spmv_stage != 0; }
     bool spmv_nt() const { return d_.spmv_nt != 0; }
     int halo_max() const { return d_.halo_max; }
+    int halo_lds() const { return d_.halo_lds; }
     // assemble W and rhs at the current state (no solve) and copy them out in
     // the model's block order
     void assemble_system(int i, double *W, double *rhs);

@@ -549,7 +549,12 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
     bool wide = false; // the FP64 matrix under FP32 vectors
     if constexpr (sizeof(T) == sizeof(float))
         wide = st.w64 != 0;
+    // staged variant: a slice whose column list exceeds the LDS provided
+    // (uniform over the workgroup) takes the gathered product
+    bool staged = false;
     if constexpr (S)
+        staged = list.y <= d.halo_lds;
+    if (staged)
     {
         const G<uint16_t> *lc = glob(d.lcol) + base + row;
         if (wide)
@@ -889,7 +894,7 @@ void launch_spmv_nt(const DeviceBatch &d, int slices, const T *W, const T *P, T 
 {
     if (d.spmv_stage)
         hipLaunchKernelGGL((k_cg_spmv<T, L, true, NT>), dim3(slices), dim3(L * 64),
-                           stage_bytes<T>(d.halo_max), s, d, W, P, r, z, q, d0, d1);
+                           stage_bytes<T>(d.halo_lds), s, d, W, P, r, z, q, d0, d1);
     else
         hipLaunchKernelGGL((k_cg_spmv<T, L, false, NT>), dim3(slices), dim3(L * 64), 0, s, d, W,
                            P, r, z, q, d0, d1);

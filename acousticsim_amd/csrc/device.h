@@ -129,6 +129,11 @@ struct DeviceBatch
     int *halo_off = nullptr, *halo_row = nullptr;
     uint16_t *lcol = nullptr;
     int halo_max = 0;
+    // LDS entries the staged variant provides per workgroup: a slice whose
+    // list is longer takes the gathered product instead (a few slices with
+    // long lists would otherwise size every workgroup's LDS, and LDS bounds
+    // how many workgroups a CU holds)
+    int halo_lds = 0;
     double *W64 = nullptr;
     float *W32 = nullptr;
     // assembly inputs (global vertex ids map to physical rows)

@@ -291,7 +291,7 @@ void Server::upload_table(const std::vector<std::unique_ptr<Resident>> &res)
     for (auto &r : res)
     {
         all = all && r->batch->halo_max() > 0;
-        hmax = std::max(hmax, r->batch->halo_max());
+        hmax = std::max(hmax, r->batch->halo_lds());
     }
     halo_ = 0;
     if (all && hmax > 0)

@@ -84,9 +84,13 @@ def test_delaunay_reference_semantics_first_steps(capi):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_delaunay_twenty_tolerance_steps(capi, precision):
+@pytest.mark.parametrize("stage", ["0", "1"])
+def test_delaunay_twenty_tolerance_steps(capi, precision, stage, monkeypatch):
+    """20 tolerance-mode steps against the oracle, with the SpMV's directions
+    gathered per slot (stage 0) or staged in LDS per slice column (stage 1)."""
+    monkeypatch.setenv("BW_SPMV_STAGE", stage)
     conf, pos, tri, x0 = _case(1200, 5)
-    say("case delaunay 20 steps", precision)
+    say("case delaunay 20 steps", precision, "stage", stage)
     cl = capi.BaraffClothHip(_mode(conf, precision), x0)
     oc = of.OracleCloth(conf, mesh=(pos, tri))
     oc.set_state(x0, x0)
@@ -119,13 +123,14 @@ def test_mesh_entry_equals_delaunay_geometry(capi, monkeypatch):
     assert np.array_equal(a.GetPos(), b.GetPos())
 
 
-def test_delaunay_in_batch_equals_alone(capi, monkeypatch):
+@pytest.mark.parametrize("stage", ["0", "1"])
+def test_delaunay_in_batch_equals_alone(capi, stage, monkeypatch):
     monkeypatch.setenv("BW_SPMV_LPR", "2")  # the two batches autotune apart
-    monkeypatch.setenv("BW_SPMV_STAGE", "0")
+    monkeypatch.setenv("BW_SPMV_STAGE", stage)
     conf, pos, tri, x0 = _case(800, 7)
     g = wl.grid_config(33)
     gx = wl.jittered_positions(g, 8)
-    say("case delaunay in batch")
+    say("case delaunay in batch, stage", stage)
     alone = capi.ClothBatch([conf], [x0])
     alone.step(3)
     mix = capi.ClothBatch([g, conf], [gx, x0])

@@ -60,6 +60,8 @@ def test_served_alone_equals_batch_alone(stage, monkeypatch):
     monkeypatch.setenv("BW_SPMV_STAGE", stage)
     monkeypatch.setenv("BW_SPMV_NT", "0")
     c, x0 = REQS[0]
+    # the server stages the lists of 99% of a request's slices; the batch too
+    monkeypatch.setenv("BW_STAGE_LDS", "p99")
     say("case served alone vs batch alone, stage", stage)
     info, x = _served([(c, x0)])[0][0]
     assert info["status"] == 0 and info["steps_done"] == STEPS
@@ -151,6 +153,8 @@ def test_served_with_group_sums_equals_batch(stage, monkeypatch):
     monkeypatch.setenv("BW_SPMV_STAGE", stage)
     monkeypatch.setenv("BW_SPMV_NT", "0")
     monkeypatch.setenv("BW_GROUP_MIN", "0")
+    # the server stages the lists of 99% of a request's slices; the batch too
+    monkeypatch.setenv("BW_STAGE_LDS", "p99")
     say("case served group sums, stage", stage)
     mix, counters = _served(REQS, stagger=0.01)
     for i, (c, x0) in enumerate(REQS):
