@@ -782,7 +782,7 @@ __global__ void __launch_bounds__(64) k_cg_group(DeviceBatch d, int slot)
 
 // Many batches in one launch: entry (batch, slice) of `list` per workgroup,
 // batches in the device table `table`; entries with batch < 0 are padding.
-template <typename T, int L, bool S>
+template <typename T, int L, bool S, bool NT>
 __global__ void __launch_bounds__(L * 64)
 k_cg_spmv_multi(const DeviceBatch *__restrict__ table, const int2 *__restrict__ list, int xcd_map)
 {
@@ -791,7 +791,7 @@ k_cg_spmv_multi(const DeviceBatch *__restrict__ table, const int2 *__restrict__ 
     if (e.x < 0)
         return;
     const DeviceBatch &d = table[e.x];
-    spmv_body<T, L, S, false>(d, e.y, glob(Vecs<T>::W(d)), glob(Vecs<T>::P(d)), glob(Vecs<T>::r(d)),
+    spmv_body<T, L, S, NT>(d, e.y, glob(Vecs<T>::W(d)), glob(Vecs<T>::P(d)), glob(Vecs<T>::r(d)),
                        glob(Vecs<T>::z(d)), glob(Vecs<T>::q(d)), glob(Vecs<T>::d0(d)),
                        glob(Vecs<T>::d1(d)), lds, reinterpret_cast<T *>(stage_lds));
 }
@@ -922,29 +922,34 @@ void launch_spmv_lpr(const DeviceBatch &d, int slices, const T *W, const T *P, T
         launch_spmv<T, 4>(d, slices, W, P, r, z, q, d0, d1, s);
 }
 
-template <typename T, bool S>
+template <typename T, bool S, bool NT>
 void launch_spmv_multi_v(const DeviceBatch *table, const int2 *list, int grid, int lpr,
                          int xcd_map, size_t lds, hipStream_t s)
 {
     if (lpr == 1)
-        hipLaunchKernelGGL((k_cg_spmv_multi<T, 1, S>), dim3(grid), dim3(64), lds, s, table, list,
-                           xcd_map);
+        hipLaunchKernelGGL((k_cg_spmv_multi<T, 1, S, NT>), dim3(grid), dim3(64), lds, s, table,
+                           list, xcd_map);
     else if (lpr == 2)
-        hipLaunchKernelGGL((k_cg_spmv_multi<T, 2, S>), dim3(grid), dim3(128), lds, s, table, list,
-                           xcd_map);
+        hipLaunchKernelGGL((k_cg_spmv_multi<T, 2, S, NT>), dim3(grid), dim3(128), lds, s, table,
+                           list, xcd_map);
     else
-        hipLaunchKernelGGL((k_cg_spmv_multi<T, 4, S>), dim3(grid), dim3(256), lds, s, table, list,
-                           xcd_map);
+        hipLaunchKernelGGL((k_cg_spmv_multi<T, 4, S, NT>), dim3(grid), dim3(256), lds, s, table,
+                           list, xcd_map);
 }
 
 template <typename T>
 void launch_spmv_multi(const DeviceBatch *table, const int2 *list, int grid, int lpr,
-                       int halo_max, int xcd_map, hipStream_t s)
+                       int halo_max, bool nt, int xcd_map, hipStream_t s)
 {
-    if (halo_max > 0)
-        launch_spmv_multi_v<T, true>(table, list, grid, lpr, xcd_map, stage_bytes<T>(halo_max), s);
+    const size_t lds = stage_bytes<T>(halo_max);
+    if (halo_max > 0 && nt)
+        launch_spmv_multi_v<T, true, true>(table, list, grid, lpr, xcd_map, lds, s);
+    else if (halo_max > 0)
+        launch_spmv_multi_v<T, true, false>(table, list, grid, lpr, xcd_map, lds, s);
+    else if (nt)
+        launch_spmv_multi_v<T, false, true>(table, list, grid, lpr, xcd_map, 0, s);
     else
-        launch_spmv_multi_v<T, false>(table, list, grid, lpr, xcd_map, 0, s);
+        launch_spmv_multi_v<T, false, false>(table, list, grid, lpr, xcd_map, 0, s);
 }
 
 void launch_group_sums(const DeviceBatch &d, int slot, hipStream_t s)
@@ -990,12 +995,13 @@ void launch_cg_step(const DeviceBatch &d, bool fp32, int phase, int slices, hipS
 
 void launch_cg_step_multi(const DeviceBatch *table, const int2 *spmv_list, int spmv_grid,
                           const int2 *upd_list, int upd_grid, const int2 *grp_list, int grp_grid,
-                          bool fp32, int phase, int lpr, int halo_max, int xcd_map, hipStream_t s)
+                          bool fp32, int phase, int lpr, int halo_max, bool nt, int xcd_map,
+                          hipStream_t s)
 {
     if (fp32)
     {
         if (phase == 0)
-            launch_spmv_multi<float>(table, spmv_list, spmv_grid, lpr, halo_max, xcd_map, s);
+            launch_spmv_multi<float>(table, spmv_list, spmv_grid, lpr, halo_max, nt, xcd_map, s);
         else
             hipLaunchKernelGGL((k_cg_update_multi<float, SB>), dim3(upd_grid), dim3(SB * 64), 0, s,
                                table, upd_list, xcd_map);
@@ -1003,7 +1009,7 @@ void launch_cg_step_multi(const DeviceBatch *table, const int2 *spmv_list, int s
     else
     {
         if (phase == 0)
-            launch_spmv_multi<double>(table, spmv_list, spmv_grid, lpr, halo_max, xcd_map, s);
+            launch_spmv_multi<double>(table, spmv_list, spmv_grid, lpr, halo_max, nt, xcd_map, s);
         else
             hipLaunchKernelGGL((k_cg_update_multi<double, SB>), dim3(upd_grid), dim3(SB * 64), 0,
                                s, table, upd_list, xcd_map);

@@ -225,10 +225,11 @@ void launch_cg_spmv_only(const DeviceBatch &d, bool fp32, hipStream_t s);
 // (batch, solver workgroup) entry per k_cg_update workgroup, grp_list one
 // (batch, group) entry per group-sum wavefront (grp_grid 0: none); batch < 0
 // pads; halo_max > 0 runs the staged SpMV variant (every batch in the table
-// must have its lists) with LDS for that many entries
+// must have its lists) with LDS for that many entries; nt: matrix loads with
+// the nontemporal hint
 void launch_cg_step_multi(const DeviceBatch *table, const int2 *spmv_list, int spmv_grid,
                           const int2 *upd_list, int upd_grid, const int2 *grp_list, int grp_grid,
-                          bool fp32, int phase, int lpr, int halo_max, int xcd_map,
+                          bool fp32, int phase, int lpr, int halo_max, bool nt, int xcd_map,
                           hipStream_t s);
 void launch_cg_timing_setup(const DeviceBatch &d, bool fp32, hipStream_t s);
 // group sums of the solver partials in `slot` for cloths that have them

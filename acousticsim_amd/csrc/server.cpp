@@ -55,6 +55,8 @@ Server::Server(const ServerOptions &opt) : opt_(opt), t0_(std::chrono::steady_cl
         throw std::runtime_error("config: max_resident must be in [1, 65536]");
     if (const char *e = std::getenv("BW_SPMV_STAGE"))
         opt_.spmv_stage = std::atoi(e) != 0;
+    if (const char *e = std::getenv("BW_SPMV_NT"))
+        opt_.spmv_nt = std::atoi(e) != 0;
     check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     const size_t ns = 2 * (size_t)opt_.max_resident;
     check(hipMalloc((void **)&states_dev_, ns * sizeof(CgState)), "hipMalloc");
@@ -304,8 +306,8 @@ void Server::launch_chain(int length, int spmv_grid, int grp_grid)
     const int ugrid = (spmv_grid + BW_SB - 1) / BW_SB;
     for (int k = 0; k < length; k++)
         launch_cg_step_multi(table_dev_, spmv_list_dev_, spmv_grid, upd_list_dev_, ugrid,
-                             grp_list_dev_, grp_grid, opt_.fp32, k & 1, opt_.spmv_lpr, halo_, 1,
-                             stream_);
+                             grp_list_dev_, grp_grid, opt_.fp32, k & 1, opt_.spmv_lpr, halo_,
+                             opt_.spmv_nt != 0, 1, stream_);
 }
 
 void Server::run_chunk(int length, int spmv_grid, int grp_grid)

@@ -41,6 +41,10 @@ struct ServerOptions
     // overrides.  Staged: 42.1 against 38.9 cloth-steps/s on the serving mix
     // of 64 requests (half Delaunay cloths), same requests, same device
     int spmv_stage = 1;
+    // matrix loads with the nontemporal hint; BW_SPMV_NT overrides.  The
+    // resident requests' matrices together exceed the Infinity Cache: 48.7-49.5
+    // against 46.1-46.4 cloth-steps/s on the serving mix
+    int spmv_nt = 1;
 };
 
 struct RequestInfo

@@ -51,9 +51,9 @@ def _served(reqs, stagger=0.0, **kw):
 
 @pytest.mark.parametrize("stage", ["0", "1"])
 def test_served_alone_equals_batch_alone(stage, monkeypatch):
-    """The server runs the 1-lane SpMV variant with cached matrix loads, the
-    directions staged in LDS (its default) or gathered; the batch is held to
-    the same variant."""
+    """The server runs the 1-lane SpMV variant, the directions staged in LDS
+    (its default) or gathered; the batch is held to the same variant (the
+    nontemporal hint changes no result bit)."""
     from acousticsim_amd import capi
 
     monkeypatch.setenv("BW_SPMV_LPR", "1")
