@@ -438,11 +438,21 @@ void Batch::build_layout()
             // take the gathered product.  Serving mix: 47.5 against 36.1
             // cloth-steps/s; one 1.5M-point cloth: 165 us with every list
             // staged against 172 us with the percentile.  BW_STAGE_LDS=max
-            // or p99 fixes the rule (a batch held to the server's form)
+            // or p99 fixes the rule (a batch held to the server's form), a
+            // number of entries (>= 64) fixes the LDS itself
             d_.halo_lds = hmax;
             bool p99 = opt_.shared_launches;
+            int fixed = 0;
             if (const char *e = std::getenv("BW_STAGE_LDS"))
+            {
                 p99 = std::string(e) == "p99";
+                fixed = std::atoi(e);
+            }
+            if (fixed >= 64)
+            {
+                p99 = false;
+                d_.halo_lds = std::min(hmax, fixed);
+            }
             if (p99)
             {
                 std::vector<int> len(slices);

@@ -84,11 +84,15 @@ def test_delaunay_reference_semantics_first_steps(capi):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-@pytest.mark.parametrize("stage", ["0", "1"])
+@pytest.mark.parametrize("stage", ["0", "1", "1/192"])
 def test_delaunay_twenty_tolerance_steps(capi, precision, stage, monkeypatch):
     """20 tolerance-mode steps against the oracle, with the SpMV's directions
-    gathered per slot (stage 0) or staged in LDS per slice column (stage 1)."""
-    monkeypatch.setenv("BW_SPMV_STAGE", stage)
+    gathered per slot (stage 0) or staged in LDS per slice column (stage 1);
+    1/192: LDS for 192 list entries, so that the slices with longer lists
+    take the gathered product within the staged kernel."""
+    monkeypatch.setenv("BW_SPMV_STAGE", stage[0])
+    if "/" in stage:
+        monkeypatch.setenv("BW_STAGE_LDS", stage.split("/")[1])
     conf, pos, tri, x0 = _case(1200, 5)
     say("case delaunay 20 steps", precision, "stage", stage)
     cl = capi.BaraffClothHip(_mode(conf, precision), x0)
@@ -123,10 +127,12 @@ def test_mesh_entry_equals_delaunay_geometry(capi, monkeypatch):
     assert np.array_equal(a.GetPos(), b.GetPos())
 
 
-@pytest.mark.parametrize("stage", ["0", "1"])
+@pytest.mark.parametrize("stage", ["0", "1", "1/192"])
 def test_delaunay_in_batch_equals_alone(capi, stage, monkeypatch):
     monkeypatch.setenv("BW_SPMV_LPR", "2")  # the two batches autotune apart
-    monkeypatch.setenv("BW_SPMV_STAGE", stage)
+    monkeypatch.setenv("BW_SPMV_STAGE", stage[0])
+    if "/" in stage:
+        monkeypatch.setenv("BW_STAGE_LDS", stage.split("/")[1])
     conf, pos, tri, x0 = _case(800, 7)
     g = wl.grid_config(33)
     gx = wl.jittered_positions(g, 8)
