@@ -17,14 +17,17 @@
 // and their workgroups read the group sums instead (a 1.5M-vertex cloth
 // reads 366 sums rather than 23k partials per workgroup).
 //
-//   k_cg_spmv    one 64-row slice per workgroup, four lanes per row (lane
-//                phase j takes ELL slots j, j+4, ...; the four partial row
-//                sums meet by lane shuffles), which cuts the serial chain of
-//                dependent loads per lane from ~13 blocks to ~4: a cloth
-//                solved alone, late in a batch, is latency-bound.
+//   k_cg_spmv    one 64-row slice per workgroup, L = 1, 2 or 4 lanes per
+//                row (lane phase j takes ELL slots j, j+L, ...; the partial
+//                row sums meet by lane shuffles; more lanes shorten the
+//                serial chain of dependent loads per lane, which bounds a
+//                cloth solved alone), the direction gathered per slot or
+//                staged in LDS per distinct column of the slice, matrix
+//                loads cached or nontemporal: twelve variants, autotuned.
 //                Prologue: reduce r.z and r.r of the previous update;
 //                beta = rho_new / rho; d = z + beta d_prev, formed on the
-//                fly for every gathered column; q = W d; partial d.q.
+//                fly for every gathered (or staged) column; q = W d;
+//                partial d.q.
 //                When a check is due it computes the true residual instead,
 //                r = b - W64 dx in FP64, z = P r (partials r.z, r.r, b.b),
 //                and the following k_cg_update idles for that cloth.
@@ -255,10 +258,11 @@ __device__ inline void slice_product(const G<TW> *__restrict__ Wl, const G<int> 
 // only memory loads are the matrix planes and the 2-byte positions, which
 // do not wait on one another.
 // Staging in rounds of R = 8/L entries per lane (512 per workgroup; 4/L
-// measured 1% slower on the 1.5M-point cloth, 2/L 7% slower): the
-// first round's column rows, z and d_prev are loaded at the very start of
-// the kernel (they do not depend on beta), so that they are in flight
-// during the prologue; each later round loads its rows, then all of its
+// measured 1% slower on the 1.5M-point cloth, 2/L 7% slower): the first
+// round's column rows are loaded at the very start of the kernel and its z
+// and d_prev right after the state load (none of them depends on beta), so
+// that they are in flight during the prologue; each later round loads its
+// rows, then all of its
 // gathers, then writes, so a round costs two memory round trips rather than
 // two per entry.
 template <typename T, int L> struct StageRound
