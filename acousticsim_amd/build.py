@@ -23,8 +23,7 @@ ARCH = "gfx950"
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-I" + CSRC, "-I" + INCLUDE, "-Wall",
           "-Wno-unused-result"]
 DEVICE = ["-x", "hip", "--offload-arch=" + ARCH,
-          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-          "-munsafe-fp-atomics"]
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 SOURCES = [
     ("json_lite.cpp", ["-ffp-contract=off"]),

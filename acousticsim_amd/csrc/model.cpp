@@ -47,10 +47,18 @@ ClothConfig ClothConfig::from_json(const Json &j)
         throw std::runtime_error("config: cloth_size and subdivision need 2 values");
     c.cloth_size[0] = size[0];
     c.cloth_size[1] = size[1];
+    // range-checked as doubles before the cast (a cast of an out-of-range
+    // double is undefined); a grid's vertex, triangle and edge counts must
+    // fit in int, which 2^28 cells bound (edges < 3 cells + sides + 1)
+    for (int k = 0; k < 2; k++)
+        if (!(sub[k] >= 1.0 && sub[k] <= (double)(1 << 28)))
+            throw std::runtime_error("config: subdivision must be in [1, 2^28]");
     c.subdivision[0] = (int)sub[0];
     c.subdivision[1] = (int)sub[1];
-    if (c.subdivision[0] < 1 || c.subdivision[1] < 1 || !(size[0] > 0) || !(size[1] > 0))
-        throw std::runtime_error("config: cloth_size and subdivision must be positive");
+    if ((long long)c.subdivision[0] * c.subdivision[1] > (1LL << 28))
+        throw std::runtime_error("config: subdivision[0] * subdivision[1] must be at most 2^28");
+    if (!(size[0] > 0) || !(size[1] > 0) || !std::isfinite(size[0]) || !std::isfinite(size[1]))
+        throw std::runtime_error("config: cloth_size must be positive and finite");
     if (j.has("cloth_init_pos"))
     {
         auto p = j.at("cloth_init_pos").as_vector();

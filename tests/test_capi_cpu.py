@@ -71,3 +71,18 @@ def test_create_without_device_fails_loudly():
     with pytest.raises(capi.BwError) as e:
         capi.BaraffClothHip(wl.grid_config(5), None)
     assert e.value.code == -2
+
+
+@pytest.mark.parametrize("sub", [[65536, 65536], [1e300, 2], [-1, 5], [0.5, 4], [16385, 16385]])
+def test_out_of_range_grid_is_a_config_error(sub):
+    """Grid sizes whose vertex, triangle or edge counts would overflow are
+    rejected while the configuration is read, before any device call, with
+    BW_ERR_ARG (the host arrays are never sized from a wrapped count)."""
+    if not os.path.exists(capi.LIB_PATH):
+        pytest.skip("libbwcloth.so not built")
+    c = wl.grid_config(5)
+    c["subdivision"] = sub
+    with pytest.raises(capi.BwError) as e:
+        capi.BaraffClothHip(c, None)
+    assert e.value.code == -1, e.value
+    assert "subdivision" in str(e.value)
