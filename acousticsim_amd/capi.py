@@ -82,7 +82,10 @@ EXPORTS = [
     "bw_get_default_timestep", "bw_num_vertices", "bw_num_triangles", "bw_num_edges",
     "bw_get_constraint_static_vertices", "bw_add_constraint_static_vertices",
     "bw_clear_constraint_static_vertices", "bw_get_triangles", "bw_get_edges",
-    "bw_get_ell_map", "bw_get_normals", "bw_batch_get_normals", "bw_assemble_system", "bw_get_last_stats", "bw_last_error", "bw_batch_create",
+    "bw_get_ell_map", "bw_get_normals", "bw_batch_get_normals",
+    "bw_apply_perturb", "bw_apply_user_perturb_force_once", "bw_set_y_drag_force",
+    "bw_batch_apply_perturb", "bw_batch_apply_user_perturb_force_once",
+    "bw_batch_set_y_drag_force", "bw_assemble_system", "bw_get_last_stats", "bw_last_error", "bw_batch_create",
     "bw_batch_destroy", "bw_batch_size", "bw_batch_step", "bw_batch_num_vertices",
     "bw_batch_num_triangles", "bw_set_point_triangle_contacts", "bw_set_edge_edge_contacts",
     "bw_batch_set_point_triangle_contacts", "bw_batch_set_edge_edge_contacts",
@@ -133,6 +136,12 @@ def lib():
         "bw_get_edges": (i32, [vp, ip]),
         "bw_get_ell_map": (i32, [vp, ip, ip]),
         "bw_get_normals": (i32, [vp, dp, dp]),
+        "bw_apply_perturb": (i32, [vp, i32, dp]),
+        "bw_apply_user_perturb_force_once": (i32, [vp, i32, dp]),
+        "bw_set_y_drag_force": (i32, [vp, ctypes.c_float]),
+        "bw_batch_apply_perturb": (i32, [vp, i32, i32, dp]),
+        "bw_batch_apply_user_perturb_force_once": (i32, [vp, i32, i32, dp]),
+        "bw_batch_set_y_drag_force": (i32, [vp, i32, ctypes.c_float]),
         "bw_batch_get_normals": (i32, [vp, i32, dp, dp]),
         "bw_assemble_system": (i32, [vp, dp, dp]),
         "bw_get_last_stats": (i32, [vp, P(StepStats)]),
@@ -315,6 +324,22 @@ class BaraffClothHip:
         tn = np.empty(3 * nt)
         _check(lib().bw_get_normals(self._h, _dp(vn), _dp(tn)))
         return vn.reshape(-1, 3), tn.reshape(-1, 3)
+
+    def ApplyPerturb(self, triangle, force):
+        """cBaseCloth::ApplyPerturb: force / 3 on each vertex of the triangle,
+        as user force of the next substep"""
+        f = np.ascontiguousarray(force, dtype=np.float64)
+        _check(lib().bw_apply_perturb(self._h, int(triangle), _dp(f)))
+
+    def ApplyUserPerturbForceOnce(self, triangle, goal):
+        """cBaraffCloth::ApplyUserPerturbForceOnce: the triangle's first vertex
+        moves to goal and is fixed in the next substep"""
+        g = np.ascontiguousarray(goal, dtype=np.float64)
+        _check(lib().bw_apply_user_perturb_force_once(self._h, int(triangle), _dp(g)))
+
+    def SetYDragForce(self, force):
+        """cBaseCloth::mYDragForce, applied by every substep"""
+        _check(lib().bw_set_y_drag_force(self._h, float(force)))
 
     def SetPointTriangleCollisionInfo(self, ids, normal, bary, depth):
         """contacts for the next substep (cBaseObject::SetPointTriangleCollisionInfo):
@@ -511,6 +536,17 @@ class ClothBatch:
         tn = np.empty(3 * lib().bw_batch_num_triangles(self._h, i))
         _check(lib().bw_batch_get_normals(self._h, i, _dp(vn), _dp(tn)))
         return vn.reshape(-1, 3), tn.reshape(-1, 3)
+
+    def apply_perturb(self, i, triangle, force):
+        f = np.ascontiguousarray(force, dtype=np.float64)
+        _check(lib().bw_batch_apply_perturb(self._h, i, int(triangle), _dp(f)))
+
+    def apply_user_perturb_force_once(self, i, triangle, goal):
+        g = np.ascontiguousarray(goal, dtype=np.float64)
+        _check(lib().bw_batch_apply_user_perturb_force_once(self._h, i, int(triangle), _dp(g)))
+
+    def set_y_drag_force(self, i, force):
+        _check(lib().bw_batch_set_y_drag_force(self._h, i, float(force)))
 
     def set_contacts(self, i, ct):
         """contacts (a dict as workloads.random_contacts returns) for member

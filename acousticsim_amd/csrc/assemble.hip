@@ -14,7 +14,7 @@
 //     duplicate triplets (BaraffMaterial.cpp:297-351, DihedralBending.cpp:449-489),
 //     so W equals the reference's W = (1 + h a) M + h (b - h) K
 //     (BaraffCloth.cpp:274-275) entry for entry.  The same lane sums the
-//     vertex force and forms rhs = h^2 (g + f) + h M (x - x_pre) / h
+//     vertex force and forms rhs = h^2 (g + f_user + f) + h M (x - x_pre) / h
 //     (BaraffCloth.cpp:282-283), the Jacobi diagonal and the warm start
 //     dx0 = x - x_pre (BaraffCloth.cpp:284).  Rows and columns of pinned
 //     vertices are zeroed, which is the reference's filter in ApplyMatmul
@@ -378,7 +378,8 @@ k_assemble_rows(DeviceBatch d)
         const double f = fs[a] + fb[a];
         const double grav = a == 1 ? -mass * 9.8 * P.grav : 0.0;
         const double x = d.x[r3 + a], xp = d.xpre[r3 + a];
-        const double t1 = dt2 * ((grav + 0.0) + f);
+        const double u = d.fuser ? d.fuser[r3 + a] : 0.0;
+        const double t1 = dt2 * ((grav + u) + f);
         const double t2 = ((dt * mass) * (x - xp)) / dt;
         d.b[r3 + a] = pin_p ? 0.0 : t1 + t2;
         d.dx[r3 + a] = pin_p ? 0.0 : x - xp;

@@ -587,6 +587,15 @@ void Batch::build_layout()
     upload(d_.slice_w, slice_w);
     host_slice_off_ = slice_off;
     contacts_.assign(models_.size(), Contacts());
+    user_force_.assign(models_.size(), std::vector<double>());
+    sched_bounds_.reserve(models_.size());
+    sched_pairs_.reserve(models_.size());
+    active_.reserve(models_.size());
+    pred_launches_.reserve(models_.size());
+    drag_vertex_.assign(models_.size(), -1);
+    ydrag_.assign(models_.size(), 0.0f);
+    fuser_dirty_.assign(models_.size(), 0);
+    drag_pinned_.assign(models_.size(), 0);
     upload(d_.slice_inst, slice_inst);
     upload(d_.blk_inst, blk_inst);
     upload(d_.row_vertex, row_vertex);
@@ -969,12 +978,16 @@ void Batch::solve_ref()
 // Chunk boundaries (launch counts) for this solve from the launches every
 // cloth needed in the previous one: a boundary wherever the cloths expected
 // to be still solving own at most half the slices of the previous boundary.
-std::vector<long long> Batch::chunk_schedule() const
+// (The host buffers are the batch's own, sized at construction, so a solve
+// allocates no host memory either.)
+void Batch::chunk_schedule()
 {
-    std::vector<long long> bounds;
+    std::vector<long long> &bounds = sched_bounds_;
+    bounds.clear();
     if (pred_launches_.empty())
-        return bounds;
-    std::vector<std::pair<long long, int>> v;
+        return;
+    std::vector<std::pair<long long, int>> &v = sched_pairs_;
+    v.clear();
     long long remaining = 0;
     for (int i = 0; i < size(); i++)
     {
@@ -993,7 +1006,6 @@ std::vector<long long> Batch::chunk_schedule() const
                 level /= 2;
         }
     }
-    return bounds;
 }
 
 // Lists of the active cloths' slices and solver workgroups (padded with -1
@@ -1030,8 +1042,10 @@ void Batch::solve_cg()
     long long cap = 16;
     for (auto &m : models_)
         cap = std::max<long long>(cap, 4LL * m.cfg.solver_max_iters + 64);
-    const std::vector<long long> bounds = chunk_schedule();
-    std::vector<char> active(size(), 1);
+    chunk_schedule();
+    const std::vector<long long> &bounds = sched_bounds_;
+    std::vector<char> &active = active_;
+    active.assign(size(), 1);
     long long launched = 0;
     size_t next = 0;
     int blind = 0; // chunks run without a predicted boundary
@@ -1103,6 +1117,25 @@ void Batch::end_step(const CgState *states, double step_us)
         throw std::runtime_error("step: non-finite residual (NaN/Inf in the state)");
 }
 
+void Batch::fixed_variant(int &lpr, int &stage, int &nt) const
+{
+    lpr = 0;
+    stage = -1;
+    nt = -1;
+    if (const char *e = std::getenv("BW_SPMV_LPR"))
+    {
+        const int v = std::atoi(e);
+        if (v == 1 || v == 2 || v == 4)
+            lpr = v;
+    }
+    if (const char *e = std::getenv("BW_SPMV_STAGE"))
+        stage = std::atoi(e) != 0 ? 1 : 0;
+    if (const char *e = std::getenv("BW_SPMV_NT"))
+        nt = std::atoi(e) != 0 ? 1 : 0;
+    if (d_.halo_max == 0)
+        stage = 0;
+}
+
 // Picks the SpMV variant (lanes per row; directions gathered from memory or
 // staged in LDS; matrix loads with or without the nontemporal hint; see
 // cg.hip) that runs this batch's matrix fastest, timing each on the system
@@ -1117,18 +1150,7 @@ int Batch::autotune()
     graphs_.clear();
     tuned_ = true;
     int fixed_lpr = 0, fixed_stage = -1, fixed_nt = -1;
-    if (const char *e = std::getenv("BW_SPMV_LPR"))
-    {
-        const int v = std::atoi(e);
-        if (v == 1 || v == 2 || v == 4)
-            fixed_lpr = v;
-    }
-    if (const char *e = std::getenv("BW_SPMV_STAGE"))
-        fixed_stage = std::atoi(e) != 0 ? 1 : 0;
-    if (const char *e = std::getenv("BW_SPMV_NT"))
-        fixed_nt = std::atoi(e) != 0 ? 1 : 0;
-    if (d_.halo_max == 0)
-        fixed_stage = 0;
+    fixed_variant(fixed_lpr, fixed_stage, fixed_nt);
     double best = 1e300;
     int pick = 0;
     const int choices[3] = {1, 2, 4};
@@ -1233,6 +1255,156 @@ void Batch::set_edge_edge_contacts(int i, int n, const int *ids, const double *n
     contact_total_ += c.size();
 }
 
+// ---------------------------------------------------------------------------
+// user inputs (cBaseCloth::ApplyPerturb, cBaraffCloth::ApplyUserPerturbForceOnce,
+// cBaseCloth::mYDragForce)
+
+namespace
+{
+void check_plain(SolverKind k)
+{
+    if (k == SolverKind::GpuForm)
+        throw std::runtime_error("state: user perturbations of a fem_gpu cloth are not supported "
+                                 "(its drag point follows the barycentric argmax, "
+                                 "BaraffClothGpu.cpp:266-279)");
+}
+} // namespace
+
+void Batch::apply_perturb(int i, int tri, const double force[3])
+{
+    check_plain(kind_);
+    const ClothModel &m = models_[i];
+    if (tri < 0 || tri >= m.nt)
+        throw std::invalid_argument("perturb: triangle out of range");
+    for (int a = 0; a < 3; a++)
+        if (!std::isfinite(force[a]))
+            throw std::invalid_argument("perturb: force has NaN/Inf");
+    std::vector<double> &u = user_force_[i];
+    if (u.empty())
+        u.assign(3 * (size_t)m.nv, 0.0);
+    // mUserForce.segment(3 id, 3) += force / 3 for mId0, mId1, mId2
+    for (int k = 0; k < 3; k++)
+        for (int a = 0; a < 3; a++)
+        {
+            double &f = u[3 * (size_t)m.tri[tri][k] + a];
+            f = f + force[a] / 3;
+        }
+}
+
+void Batch::apply_user_perturb_force_once(int i, int tri, const double goal[3])
+{
+    check_plain(kind_);
+    const ClothModel &m = models_[i];
+    if (tri < 0 || tri >= m.nt)
+        throw std::invalid_argument("perturb: triangle out of range");
+    for (int a = 0; a < 3; a++)
+        if (!std::isfinite(goal[a]))
+            throw std::invalid_argument("perturb: goal has NaN/Inf");
+    const int v0 = m.tri[tri][0];
+    const size_t at = 3 * ((size_t)row_base_[i] + row_of_[i][v0]);
+    check(hipMemcpy(d_.x + at, goal, 3 * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy goal");
+    check(hipMemcpy(d_.xpre + at, goal, 3 * sizeof(double), hipMemcpyHostToDevice),
+          "hipMemcpy goal");
+    drag_vertex_[i] = v0;
+}
+
+void Batch::set_y_drag_force(int i, float f)
+{
+    check_plain(kind_);
+    if (!std::isfinite(f))
+        throw std::invalid_argument("perturb: drag force has NaN/Inf");
+    ydrag_[i] = f;
+}
+
+void Batch::clear_user_force(int i) { user_force_[i].clear(); }
+
+// The step's user force per row: the pending perturbation forces, then the y
+// drag force (ApplyYDragForce runs inside UpdatePos, after them): +F/n_up on
+// every vertex whose float v is within 1e-5 of the largest v, -F/n_low on the
+// smallest, F/n in float.  Rows of cloths without inputs are zero.  The drag
+// vertex's row is marked pinned for the step (its row and column leave the
+// solve; its rhs and warm start become zero).
+void Batch::prepare_user_inputs()
+{
+    const uint8_t one = 1;
+    for (int i = 0; i < size(); i++)
+    {
+        const ClothModel &m = models_[i];
+        const bool has = !user_force_[i].empty() || ydrag_[i] != 0.0f;
+        const size_t rows = (size_t)nslices_[i] * SLICE;
+        if (has)
+        {
+            std::vector<double> u = user_force_[i];
+            if (u.empty())
+                u.assign(3 * (size_t)m.nv, 0.0);
+            if (ydrag_[i] != 0.0f)
+            {
+                double vmax = -1e5, vmin = 1e5;
+                for (int v = 0; v < m.nv; v++)
+                {
+                    vmax = std::max(vmax, (double)m.uv[2 * v + 1]);
+                    vmin = std::min(vmin, (double)m.uv[2 * v + 1]);
+                }
+                int n_up = 0, n_low = 0;
+                for (int v = 0; v < m.nv; v++)
+                {
+                    n_up += std::fabs(vmax - (double)m.uv[2 * v + 1]) < 1e-5;
+                    n_low += std::fabs(vmin - (double)m.uv[2 * v + 1]) < 1e-5;
+                }
+                const float up = ydrag_[i] / (float)n_up, low = -ydrag_[i] / (float)n_low;
+                for (int v = 0; v < m.nv; v++)
+                    if (std::fabs(vmax - (double)m.uv[2 * v + 1]) < 1e-5)
+                        u[3 * (size_t)v + 1] = u[3 * (size_t)v + 1] + (double)up;
+                for (int v = 0; v < m.nv; v++)
+                    if (std::fabs(vmin - (double)m.uv[2 * v + 1]) < 1e-5)
+                        u[3 * (size_t)v + 1] = u[3 * (size_t)v + 1] + (double)low;
+            }
+            if (!d_.fuser)
+                d_.fuser = dalloc<double>(3 * (size_t)d_.n_rows);
+            std::vector<double> r(3 * rows, 0.0);
+            for (int v = 0; v < m.nv; v++)
+                for (int a = 0; a < 3; a++)
+                    r[3 * (size_t)row_of_[i][v] + a] = u[3 * (size_t)v + a];
+            check(hipMemcpy(d_.fuser + 3 * (size_t)row_base_[i], r.data(), r.size() * sizeof(double),
+                            hipMemcpyHostToDevice),
+                  "hipMemcpy user force");
+            fuser_dirty_[i] = 1;
+        }
+        else if (fuser_dirty_[i])
+        {
+            check(hipMemset(d_.fuser + 3 * (size_t)row_base_[i], 0, 3 * rows * sizeof(double)),
+                  "hipMemset user force");
+            check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+            fuser_dirty_[i] = 0;
+        }
+        const int dv = drag_vertex_[i];
+        if (dv >= 0 && !m.pinned[dv])
+        {
+            check(hipMemcpy(d_.row_pinned + row_base_[i] + row_of_[i][dv], &one, 1,
+                            hipMemcpyHostToDevice),
+                  "hipMemcpy drag");
+            drag_pinned_[i] = 1;
+        }
+    }
+}
+
+void Batch::finish_user_inputs()
+{
+    const uint8_t zero = 0;
+    for (int i = 0; i < size(); i++)
+    {
+        if (drag_pinned_[i])
+        {
+            check(hipMemcpy(d_.row_pinned + row_base_[i] + row_of_[i][drag_vertex_[i]], &zero, 1,
+                            hipMemcpyHostToDevice),
+                  "hipMemcpy drag");
+            drag_pinned_[i] = 0;
+        }
+        drag_vertex_[i] = -1;
+        user_force_[i].clear();
+    }
+}
+
 void Batch::clear_contacts()
 {
     if (contact_total_ == 0)
@@ -1334,7 +1506,9 @@ bool Batch::upload_contacts()
             for (size_t blk = 0; blk < key.size(); blk++)
             {
                 const int v = key[blk].first, col = key[blk].second;
-                if (m.pinned[v] || m.pinned[col])
+                // the drag vertex is fixed in this step's solve like a pin
+                const int dv = drag_vertex_[i];
+                if (m.pinned[v] || m.pinned[col] || v == dv || col == dv)
                     continue;
                 int ks = -1;
                 for (int k = m.row_ptr[v]; k < m.row_ptr[v + 1]; k++)
@@ -1689,6 +1863,7 @@ void Batch::step(int nsteps)
             solve_gpu_form(); // assembly, solve and position update
         else
         {
+            prepare_user_inputs();
             const bool ct = upload_contacts();
             launch_elements(d_, stream_);
             launch_assemble_rows(d_, stream_);
@@ -1706,6 +1881,8 @@ void Batch::step(int nsteps)
         check(hipGetLastError(), "step launch");
         check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
         clear_contacts(); // contacts last one substep (BaraffCloth.cpp:156-157)
+        if (kind_ != SolverKind::GpuForm)
+            finish_user_inputs(); // mUserForce and mDragVertexIdx too (:154-155)
         const double us =
             std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
         bool bad = false;

@@ -90,6 +90,9 @@ public:
     // the staged form needs every slice to have at most HALO_CAP columns)
     static constexpr int TUNE_VARIANTS = 12;
     int autotune();
+    // the SpMV choices BW_SPMV_LPR / BW_SPMV_STAGE / BW_SPMV_NT fix (0 / -1:
+    // free)
+    void fixed_variant(int &lpr, int &stage, int &nt) const;
     const double *tune_us() const { return tune_us_; }
     int spmv_lpr() const { return d_.spmv_lpr; }
     bool spmv_stage() const { return d_.spmv_stage != 0; }
@@ -118,6 +121,21 @@ public:
     void set_edge_edge_contacts(int i, int n, const int *ids, const double *normal,
                                 const float *bary, const float *depth);
     int num_contacts(int i) const { return contacts_[i].size(); }
+    // User inputs of cloth i's next substep, which clears them (the plain
+    // method only).  apply_perturb: cBaseCloth::ApplyPerturb
+    // (BaseCloth.cpp:76-87), a third of `force` added to the user force of
+    // each vertex of triangle `tri` (mUserForce, which enters the rhs
+    // dt^2 (g + f_user + f), BaraffCloth.cpp:282).
+    // apply_user_perturb_force_once: cBaraffCloth::ApplyUserPerturbForceOnce
+    // (BaraffCloth.cpp:165-178), the triangle's first vertex moved to `goal`
+    // (current and previous position) and fixed in the next solve
+    // (mDragVertexIdx, :291-292).  set_y_drag_force: cBaseCloth::mYDragForce,
+    // applied by every substep (ApplyYDragForce, BaseCloth.cpp:592-623).
+    // clear_user_force: cBaseCloth::ClearForce's user-force part (Reset).
+    void apply_perturb(int i, int tri, const double force[3]);
+    void apply_user_perturb_force_once(int i, int tri, const double goal[3]);
+    void set_y_drag_force(int i, float f);
+    void clear_user_force(int i);
     const Contacts &contacts(int i) const { return contacts_[i]; }
     // collision detection of cloth i at its current positions against a
     // kinematic body (cBVHCollisionDetecter::PerformCD for the pair,
@@ -145,12 +163,25 @@ private:
     void launch_chain(int length, int slices);
     void run_chunk(int length, int slices); // as replayed graphs when enabled
     void run_graph(int length, int slices);
-    std::vector<long long> chunk_schedule() const;
+    // chunk boundaries of this solve into sched_bounds_
+    void chunk_schedule();
+    std::vector<long long> sched_bounds_;
+    std::vector<std::pair<long long, int>> sched_pairs_;
+    std::vector<char> active_;
     int upload_lists(const std::vector<char> &active);
     void solve_ref();
     void solve_cg();
     // contacts: device lists for the next step (false if there are none)
     bool upload_contacts();
+    // user inputs: upload the next step's user forces and drag pins before
+    // it, and take the drag pins back and forget the inputs after it
+    void prepare_user_inputs();
+    void finish_user_inputs();
+    std::vector<std::vector<double>> user_force_; // per cloth: 3 nv, or empty
+    std::vector<int> drag_vertex_;                // per cloth: -1 or the vertex
+    std::vector<float> ydrag_;                    // per cloth
+    std::vector<char> fuser_dirty_;               // d_.fuser rows of the cloth hold values
+    std::vector<char> drag_pinned_;               // row_pinned set for the drag vertex
     void clear_contacts();
 
     std::vector<ClothModel> models_;

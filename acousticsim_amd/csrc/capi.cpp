@@ -73,6 +73,19 @@ struct bw_body
     std::unique_ptr<bw::Body> body;
 };
 
+namespace
+{
+template <typename F> int with_member(bw_batch *b, int i, F &&f)
+{
+    return guard([&] {
+        if (!b || i < 0 || i >= b->batch->size())
+            return fail(BW_ERR_ARG, "bad batch or member index");
+        f();
+        return BW_OK;
+    });
+}
+} // namespace
+
 extern "C" {
 
 const char *bw_last_error(void) { return g_error.c_str(); }
@@ -189,6 +202,37 @@ int bw_reset(bw_cloth *c)
         c->batch->set_pos(0, c->init_pos.data());
         c->batch->set_pre_pos(0, c->init_pos.data());
         c->batch->clear_warm_start(0);
+        c->batch->clear_user_force(0);
+        return BW_OK;
+    });
+}
+
+int bw_apply_perturb(bw_cloth *c, int triangle, const double force[3])
+{
+    return guard([&] {
+        if (!c || !force)
+            return fail(BW_ERR_ARG, "NULL argument");
+        c->batch->apply_perturb(0, triangle, force);
+        return BW_OK;
+    });
+}
+
+int bw_apply_user_perturb_force_once(bw_cloth *c, int triangle, const double goal[3])
+{
+    return guard([&] {
+        if (!c || !goal)
+            return fail(BW_ERR_ARG, "NULL argument");
+        c->batch->apply_user_perturb_force_once(0, triangle, goal);
+        return BW_OK;
+    });
+}
+
+int bw_set_y_drag_force(bw_cloth *c, float force)
+{
+    return guard([&] {
+        if (!c)
+            return fail(BW_ERR_ARG, "NULL cloth");
+        c->batch->set_y_drag_force(0, force);
         return BW_OK;
     });
 }
@@ -600,6 +644,27 @@ int bw_batch_set_edge_edge_contacts(bw_batch *b, int i, int n, const int *ids,
         b->batch->set_edge_edge_contacts(i, n, ids, normal, bary, depth);
         return BW_OK;
     });
+}
+
+int bw_batch_apply_perturb(bw_batch *b, int i, int triangle, const double force[3])
+{
+    if (!force)
+        return fail(BW_ERR_ARG, "NULL force");
+    return with_member(b, i, [&] { b->batch->apply_perturb(i, triangle, force); });
+}
+
+int bw_batch_apply_user_perturb_force_once(bw_batch *b, int i, int triangle,
+                                           const double goal[3])
+{
+    if (!goal)
+        return fail(BW_ERR_ARG, "NULL goal");
+    return with_member(b, i,
+                       [&] { b->batch->apply_user_perturb_force_once(i, triangle, goal); });
+}
+
+int bw_batch_set_y_drag_force(bw_batch *b, int i, float force)
+{
+    return with_member(b, i, [&] { b->batch->set_y_drag_force(i, force); });
 }
 
 int bw_batch_detect_contacts(bw_batch *bt, int i, const bw_body *b, int cloth_first, int *n_pt,

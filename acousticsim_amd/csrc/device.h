@@ -152,6 +152,9 @@ struct DeviceBatch
     double *tri_ing = nullptr, *ie_ing = nullptr;  // 25 doubles per element
     // state and vectors (3 per row)
     double *x = nullptr, *xpre = nullptr, *dx = nullptr, *b = nullptr, *P64 = nullptr;
+    // user force of the next step (3 per row; the reference's mUserForce),
+    // allocated on first use; null: zero everywhere
+    double *fuser = nullptr;
     double *r64 = nullptr, *d64[2] = {nullptr, nullptr}, *q64 = nullptr;
     float *P32 = nullptr;
     double *partial = nullptr; // PARTS per solver workgroup

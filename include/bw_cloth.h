@@ -94,7 +94,8 @@ void bw_destroy(bw_cloth *c);
 /* cBaraffCloth::UpdatePos(double dt) (BaraffCloth.cpp:90-163); like the
    reference, dt is ignored and "default_timestep" is used. */
 int bw_update_pos(bw_cloth *c, double dt);
-/* cBaseObject::Reset -> cBaseCloth::Reset (BaseCloth.cpp:57-64) */
+/* cBaseObject::Reset -> cBaseCloth::Reset (BaseCloth.cpp:57-64): initial
+   positions, and the pending user force cleared (ClearForce, :66-73) */
 int bw_reset(bw_cloth *c);
 /* cBaseCloth::GetPos (BaseCloth.cpp:112): copies 3V doubles */
 int bw_get_pos(const bw_cloth *c, double *x);
@@ -192,6 +193,28 @@ int bw_body_get_edges(const bw_body *b, int *edge4);
    n_edge_edge (either may be NULL). */
 int bw_detect_contacts(bw_cloth *c, const bw_body *b, int cloth_first, int *n_point_triangle,
                        int *n_edge_edge);
+/* User perturbation (the scene's mouse drag, tPerturb, sim/Perturb.h), for
+   the plain method; a "fem_gpu" cloth returns BW_ERR_STATE.
+   cBaseCloth::ApplyPerturb(tPerturb*) (BaseCloth.cpp:76-87): a third of
+   `force` (tPerturb::GetPerturbForce, xyz) is added to the user force of
+   each vertex of `triangle` (tPerturb::mAffectedTriId).  The user force
+   (mUserForce) enters the next substep's rhs as dt^2 (g + f_user + f)
+   (BaraffCloth.cpp:282) and that substep clears it (:154). */
+int bw_apply_perturb(bw_cloth *c, int triangle, const double force[3]);
+/* cBaraffCloth::ApplyUserPerturbForceOnce(tPerturb*) (BaraffCloth.cpp:165-178):
+   the first vertex of `triangle` moves to `goal` (tPerturb::GetGoalPos, xyz;
+   current and previous position) and is fixed in the next substep's solve
+   like a static vertex (mDragVertexIdx, :291-292), which then releases it
+   (:155).  The scene calls it before each frame's update while the user
+   drags (cSimScene::UpdateObjects, SimScene.cpp:148-169). */
+int bw_apply_user_perturb_force_once(bw_cloth *c, int triangle, const double goal[3]);
+/* cBaseCloth::mYDragForce (BaseCloth.h:66; the ImGui slider of
+   BaraffCloth.cpp:384, default 0): every substep adds +F/n_up to the y force
+   of the n_up vertices on the cloth's top UV edge and -F/n_low on the bottom
+   edge (cBaseCloth::ApplyYDragForce, BaseCloth.cpp:592-623, called by
+   UpdatePos, BaraffCloth.cpp:129).  Kept until changed. */
+int bw_set_y_drag_force(bw_cloth *c, float force);
+
 /* ELL column structure in reference numbering: row v lists, ascending,
    {v} U every interior-edge stencil containing v, as
    cBaraffClothGpu::InitELLMatrix builds it (BaraffClothGpu.cpp:438-500), with
@@ -238,6 +261,12 @@ int bw_batch_set_point_triangle_contacts(bw_batch *b, int i, int n, const int *i
                                          const float *depth);
 int bw_batch_set_edge_edge_contacts(bw_batch *b, int i, int n, const int *ids,
                                     const double *normal, const float *bary, const float *depth);
+/* bw_apply_perturb, bw_apply_user_perturb_force_once and
+   bw_set_y_drag_force for member i */
+int bw_batch_apply_perturb(bw_batch *b, int i, int triangle, const double force[3]);
+int bw_batch_apply_user_perturb_force_once(bw_batch *b, int i, int triangle,
+                                           const double goal[3]);
+int bw_batch_set_y_drag_force(bw_batch *b, int i, float force);
 /* bw_detect_contacts for member i */
 int bw_batch_detect_contacts(bw_batch *bt, int i, const bw_body *b, int cloth_first,
                              int *n_point_triangle, int *n_edge_edge);

@@ -165,6 +165,12 @@ struct oc_cloth
     int *ee_ids;              /* 3 n_ee: cloth_is_edge0, edge0, edge1 */
     double *ee_normal;        /* 3 n_ee */
     float *ee_bary, *ee_depth; /* 2 n_ee, n_ee */
+    /* user inputs: cBaseCloth::mUserForce of the next step (cleared by
+       oc_step, BaraffCloth.cpp:154), cBaraffCloth::mDragVertexIdx (-1: none;
+       cleared by oc_step, :155) and cBaseCloth::mYDragForce (kept) */
+    double *fuser; /* 3nv */
+    int drag;
+    float ydrag;
 };
 
 static void *xcalloc(size_t n, size_t s)
@@ -874,6 +880,9 @@ static oc_cloth *complete(oc_cloth *c, const oc_params *p)
     c->dx = xcalloc(3 * c->nv, sizeof(double));
     c->x = xcalloc(3 * c->nv, sizeof(double));
     c->xpre = xcalloc(3 * c->nv, sizeof(double));
+    c->fuser = xcalloc(3 * c->nv, sizeof(double));
+    c->drag = -1;
+    c->ydrag = 0.0f;
     memcpy(c->x, c->xrest, 3 * c->nv * sizeof(double));
     memcpy(c->xpre, c->xrest, 3 * c->nv * sizeof(double));
     return c;
@@ -891,7 +900,7 @@ void oc_destroy(oc_cloth *c)
     free(c->edge_len); free(c->heron); free(c->height);
     free(c->row_ptr); free(c->cols); free(c->diag);
     free(c->Ks); free(c->Kb); free(c->K); free(c->W);
-    free(c->fs); free(c->fb); free(c->f); free(c->b); free(c->dx);
+    free(c->fs); free(c->fb); free(c->f); free(c->b); free(c->dx); free(c->fuser);
     oc_set_contacts(c, 0, NULL, NULL, NULL, NULL, 0, NULL, NULL, NULL, NULL);
     free(c);
 }
@@ -1361,15 +1370,45 @@ void oc_assemble(oc_cloth *c)
                 w[8] = mm + w[8];
             }
         }
+    /* cBaseCloth::ApplyYDragForce (BaseCloth.cpp:592-623), called by
+       UpdatePos after the internal force (BaraffCloth.cpp:129): the vertices
+       whose float v coordinate is within 1e-5 of the largest (smallest) v
+       get +F/n (-F/n) in y, F/n in float; added to mUserForce, which the
+       step then clears */
+    double *u = xcalloc(3 * nv, sizeof(double));
+    memcpy(u, c->fuser, 3 * nv * sizeof(double));
+    {
+        double vmax = -1e5, vmin = 1e5;
+        for (int v = 0; v < nv; v++)
+        {
+            double cv = (double)c->uv[2 * v + 1];
+            vmax = cv > vmax ? cv : vmax;
+            vmin = cv < vmin ? cv : vmin;
+        }
+        int n_up = 0, n_low = 0;
+        for (int v = 0; v < nv; v++)
+        {
+            float cv = c->uv[2 * v + 1];
+            n_up += fabs(vmax - (double)cv) < 1e-5;
+            n_low += fabs(vmin - (double)cv) < 1e-5;
+        }
+        for (int v = 0; v < nv; v++)
+            if (fabs(vmax - (double)c->uv[2 * v + 1]) < 1e-5)
+                u[3 * v + 1] = u[3 * v + 1] + (double)(c->ydrag / (float)n_up);
+        for (int v = 0; v < nv; v++)
+            if (fabs(vmin - (double)c->uv[2 * v + 1]) < 1e-5)
+                u[3 * v + 1] = u[3 * v + 1] + (double)(-c->ydrag / (float)n_low);
+    }
     double dt2 = dt * dt;
     for (int v = 0; v < nv; v++)
         for (int a = 0; a < 3; a++)
         {
             int q = 3 * v + a;
-            double t1 = dt2 * ((c->gravity[q] + 0.0) + c->f[q]);
+            double t1 = dt2 * ((c->gravity[q] + u[q]) + c->f[q]);
             double t2 = ((dt * c->mass[v]) * (c->x[q] - c->xpre[q])) / dt;
             c->b[q] = t1 + t2;
         }
+    free(u);
     add_contact_hessian(c);
 }
 
@@ -1416,6 +1455,12 @@ static int solve(oc_cloth *c, int mode, double tol, int max_iters, oc_stats *st)
         {
             r[3 * c->pins[j] + a] = 0.0;
             x[3 * c->pins[j] + a] = 0.0;
+        }
+    if (c->drag >= 0) /* fix_array.push_back(mDragVertexIdx) (BaraffCloth.cpp:291-292) */
+        for (int a = 0; a < 3; a++)
+        {
+            r[3 * c->drag + a] = 0.0;
+            x[3 * c->drag + a] = 0.0;
         }
     for (int v = 0; v < c->nv; v++)
         for (int a = 0; a < 3; a++)
@@ -1488,7 +1533,17 @@ static int solve(oc_cloth *c, int mode, double tol, int max_iters, oc_stats *st)
 int oc_step(oc_cloth *c, int mode, double tol, int max_iters, oc_stats *st)
 {
     oc_assemble(c);
+    /* the drag vertex is fixed for this solve only: its row and column are
+       filtered like a static vertex's */
+    const int drag_added = c->drag >= 0 && !c->pinned[c->drag];
+    if (drag_added)
+        c->pinned[c->drag] = 1;
     solve(c, mode, tol, max_iters, st);
+    if (drag_added)
+        c->pinned[c->drag] = 0;
+    /* cleared after the step (BaraffCloth.cpp:154-155) */
+    c->drag = -1;
+    memset(c->fuser, 0, 3 * c->nv * sizeof(double));
     repulsion(c);
     oc_set_contacts(c, 0, NULL, NULL, NULL, NULL, 0, NULL, NULL, NULL, NULL);
     int n = 3 * c->nv;
@@ -1542,6 +1597,44 @@ void oc_get_pattern(const oc_cloth *c, int *rp, int *cols)
     memcpy(rp, c->row_ptr, (c->nv + 1) * sizeof(int));
     memcpy(cols, c->cols, c->nnzb * sizeof(int));
 }
+/* cBaseCloth::ApplyPerturb (BaseCloth.cpp:76-87): a third of the force on
+   each vertex of the triangle, into the next step's user force */
+int oc_apply_perturb(oc_cloth *c, int tri, const double *force)
+{
+    if (tri < 0 || tri >= c->nt)
+        return -1;
+    for (int k = 0; k < 3; k++)
+    {
+        int v = c->tri[3 * tri + k];
+        for (int a = 0; a < 3; a++)
+            c->fuser[3 * v + a] = c->fuser[3 * v + a] + force[a] / 3;
+    }
+    return 0;
+}
+
+/* cBaraffCloth::ApplyUserPerturbForceOnce (BaraffCloth.cpp:165-178): the
+   triangle's first vertex moves to the goal (current and previous position)
+   and is fixed in the next step's solve */
+int oc_apply_user_perturb_force_once(oc_cloth *c, int tri, const double *goal)
+{
+    if (tri < 0 || tri >= c->nt)
+        return -1;
+    int v0 = c->tri[3 * tri];
+    for (int a = 0; a < 3; a++)
+    {
+        c->x[3 * v0 + a] = goal[a];
+        c->xpre[3 * v0 + a] = goal[a];
+    }
+    c->drag = v0;
+    return 0;
+}
+
+/* cBaseCloth::mYDragForce (BaseCloth.h:66) */
+void oc_set_y_drag_force(oc_cloth *c, float f) { c->ydrag = f; }
+
+/* cBaseCloth::ClearForce (BaseCloth.cpp:66-73) for the user force */
+void oc_clear_user_force(oc_cloth *c) { memset(c->fuser, 0, 3 * c->nv * sizeof(double)); }
+
 void oc_set_state(oc_cloth *c, const double *x, const double *xpre)
 {
     memcpy(c->x, x, 3 * c->nv * sizeof(double));

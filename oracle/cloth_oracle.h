@@ -92,6 +92,13 @@ int oc_get_ref_ell(const oc_cloth *c, int *row_ptr, int *cols);
 
 /* state */
 void oc_set_state(oc_cloth *c, const double *x, const double *xpre);
+/* user inputs of the next oc_step (cBaseCloth::ApplyPerturb,
+   cBaraffCloth::ApplyUserPerturbForceOnce, cBaseCloth::mYDragForce); the
+   first two return -1 for a triangle out of range */
+int oc_apply_perturb(oc_cloth *c, int tri, const double *force);
+int oc_apply_user_perturb_force_once(oc_cloth *c, int tri, const double *goal);
+void oc_set_y_drag_force(oc_cloth *c, float f);
+void oc_clear_user_force(oc_cloth *c);
 void oc_get_pos(const oc_cloth *c, double *x);
 void oc_get_pre_pos(const oc_cloth *c, double *x);
 

@@ -108,6 +108,12 @@ def lib():
         L.oc_get_ref_ell.argtypes = [vp, ip, ip]
         L.oc_get_ref_ell.restype = ctypes.c_int
         L.oc_set_state.argtypes = [vp, dp, dp]
+        L.oc_apply_perturb.argtypes = [vp, ctypes.c_int, dp]
+        L.oc_apply_perturb.restype = ctypes.c_int
+        L.oc_apply_user_perturb_force_once.argtypes = [vp, ctypes.c_int, dp]
+        L.oc_apply_user_perturb_force_once.restype = ctypes.c_int
+        L.oc_set_y_drag_force.argtypes = [vp, ctypes.c_float]
+        L.oc_clear_user_force.argtypes = [vp]
         L.oc_step.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, P(OcStats)]
         L.oc_step.restype = ctypes.c_int
         L.oc_assemble.argtypes = [vp]
@@ -335,6 +341,26 @@ class OracleCloth:
         xpre = np.ascontiguousarray(xpre, dtype=np.float64)
         assert x.size == 3 * self.nv and xpre.size == 3 * self.nv
         lib().oc_set_state(self.h, _dp(x), _dp(xpre))
+
+    def apply_perturb(self, tri, force):
+        """cBaseCloth::ApplyPerturb: force/3 on each vertex of the triangle,
+        for the next step"""
+        f = np.ascontiguousarray(force, dtype=np.float64)
+        if lib().oc_apply_perturb(self.h, int(tri), _dp(f)) != 0:
+            raise ValueError("triangle out of range")
+
+    def apply_user_perturb_force_once(self, tri, goal):
+        """cBaraffCloth::ApplyUserPerturbForceOnce: the triangle's first
+        vertex moves to goal and is fixed in the next step"""
+        g = np.ascontiguousarray(goal, dtype=np.float64)
+        if lib().oc_apply_user_perturb_force_once(self.h, int(tri), _dp(g)) != 0:
+            raise ValueError("triangle out of range")
+
+    def set_y_drag_force(self, f):
+        lib().oc_set_y_drag_force(self.h, float(f))
+
+    def clear_user_force(self):
+        lib().oc_clear_user_force(self.h)
 
     def pos(self):
         return self._f64("oc_get_pos", 3 * self.nv)

@@ -25,10 +25,20 @@
 //   (every step by default) through cBaseObject::SetPointTriangleCollisionInfo
 //   and SetEdgeEdgeCollisionInfo, as cSimScene::PerformCollisionDetection
 //   does (src/scenes/SimScene.cpp:174-190).
+//                  [--perturb perturb.bin] [--ydrag F]
+// perturb.bin (float64 little-endian): records {step, kind, triangle, v[3]};
+//   before that step, kind 0 hands a tPerturb with force v on the triangle to
+//   cBaseCloth::ApplyPerturb (BaseCloth.cpp:76-87), kind 1 one with goal
+//   position v to cBaraffCloth::ApplyUserPerturbForceOnce
+//   (BaraffCloth.cpp:165-178), in file order, as cSimScene::UpdateObjects
+//   calls the latter before each object's update (SimScene.cpp:148-169).
+//   --ydrag sets cBaseCloth::mYDragForce (the ImGui slider,
+//   BaraffCloth.cpp:384), which every step's ApplyYDragForce reads.
 #include "sim/cloth/BaraffCloth.h"
 #include "sim/collision/CollisionInfo.h"
 #include "sim/cloth/BaraffMaterial.h"
 #include "sim/cloth/BaseBendingMaterial.h"
+#include "sim/Perturb.h"
 #include "geometries/Primitives.h"
 #include "utils/JsonUtil.h"
 #include <chrono>
@@ -57,6 +67,24 @@ public:
     cBaraffMaterialPtr Stretch() const { return mMaterial; }
     cBaseBendingMaterialPtr Bending() const { return mBendingMaterial; }
     const tEigenArr<tVector4i> &Stencils() const { return mEdgeAffectVertexId; }
+    void SetYDragForce(float f) { mYDragForce = f; }
+};
+
+// A user perturbation whose force and goal are given directly (the scene
+// derives them from a mouse ray, tPerturb::UpdatePerturbPos, Perturb.cpp).
+struct tProbePerturb : public tPerturb
+{
+    void Set(const tVector &force, const tVector &goal)
+    {
+        mPerturbForce = force;
+        mGoalPos = goal;
+    }
+};
+
+struct PerturbRecord
+{
+    int step, kind, tri;
+    double v[3];
 };
 
 FILE *gOut = nullptr;
@@ -126,7 +154,9 @@ int main(int argc, char **argv)
     int nsteps = atoi(argv[3]);
     std::set<int> dump_steps, contact_steps;
     bool stages = false, timing = false;
-    std::string contacts_path;
+    std::string contacts_path, perturb_path;
+    double ydrag = 0.0;
+    bool has_ydrag = false;
     for (int i = 5; i < argc; i++)
     {
         if (!strcmp(argv[i], "--dump-steps") && i + 1 < argc)
@@ -140,6 +170,13 @@ int main(int argc, char **argv)
             timing = true;
         else if (!strcmp(argv[i], "--contacts") && i + 1 < argc)
             contacts_path = argv[++i];
+        else if (!strcmp(argv[i], "--perturb") && i + 1 < argc)
+            perturb_path = argv[++i];
+        else if (!strcmp(argv[i], "--ydrag") && i + 1 < argc)
+        {
+            ydrag = atof(argv[++i]);
+            has_ydrag = true;
+        }
         else if (!strcmp(argv[i], "--contact-steps") && i + 1 < argc)
         {
             for (int s : ParseList(argv[++i]))
@@ -188,6 +225,17 @@ int main(int argc, char **argv)
             ee_contacts.push_back(info);
         }
     }
+    std::vector<PerturbRecord> perturbs;
+    if (!perturb_path.empty())
+    {
+        FILE *f = fopen(perturb_path.c_str(), "rb");
+        if (!f)
+            return 9;
+        double r[6];
+        while (fread(r, sizeof(double), 6, f) == 6)
+            perturbs.push_back({(int)r[0], (int)r[1], (int)r[2], {r[3], r[4], r[5]}});
+        fclose(f);
+    }
     dump_steps.insert(nsteps);
 
     Json::Value conf;
@@ -202,6 +250,8 @@ int main(int argc, char **argv)
 
     auto cloth = std::make_shared<cProbeCloth>(0);
     cloth->Init(conf);
+    if (has_ydrag)
+        cloth->SetYDragForce((float)ydrag);
     int nv = cloth->GetNumOfVertices();
     tVectorXd x0 = cloth->GetPos();
     if (x0_path != "-")
@@ -262,12 +312,31 @@ int main(int argc, char **argv)
     std::vector<double> step_seconds;
     for (int step = 1; step <= nsteps; step++)
     {
-        tVectorXd xbefore = cloth->GetPos();
         if (!contacts_path.empty() && (contact_steps.empty() || contact_steps.count(step)))
         {
             cloth->SetPointTriangleCollisionInfo(pt_contacts);
             cloth->SetEdgeEdgeCollisionInfo(ee_contacts);
         }
+        for (const PerturbRecord &p : perturbs)
+        {
+            if (p.step != step)
+                continue;
+            tProbePerturb pert;
+            pert.mObject = cloth;
+            pert.mAffectedTriId = p.tri;
+            tVector v(p.v[0], p.v[1], p.v[2], 0.0);
+            if (p.kind == 0)
+            {
+                pert.Set(v, tVector::Zero());
+                cloth->ApplyPerturb(&pert);
+            }
+            else
+            {
+                pert.Set(tVector::Zero(), v);
+                cloth->ApplyUserPerturbForceOnce(&pert);
+            }
+        }
+        tVectorXd xbefore = cloth->GetPos();
         auto t0 = std::chrono::steady_clock::now();
         cloth->UpdatePos(cloth->GetDefaultTimestep());
         auto t1 = std::chrono::steady_clock::now();

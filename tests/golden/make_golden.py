@@ -66,10 +66,47 @@ def case_defs():
                                    full_k=True, contacts=dict(seed=6, n_point=10, n_tri=6,
                                                               n_edge=8, repeat=3))
 
+    # user inputs before given steps: forces split over a triangle's vertices
+    # (cBaseCloth::ApplyPerturb), a dragged vertex moved to a goal and fixed
+    # for the step (cBaraffCloth::ApplyUserPerturbForceOnce), and the y drag
+    # force on the top and bottom UV edges every step (mYDragForce)
+    c = workloads.grid_config(21, dt=0.01)
+    cases["grid21_perturb"] = dict(conf=c, seed=23, steps=12, dump=[1, 2, 3, 5, 8, 12],
+                                   stages=True, full_k=False, perturb=perturb_records(c),
+                                   ydrag=0.35)
+    c = workloads.grid_config(5, dt=0.005)
+    c.update({"rayleigh_damping_a": 0.1, "max_iters": 200, "cloth_uv_rotation": 45})
+    cases["grid5_perturb"] = dict(conf=c, seed=24, steps=4, dump=[1, 2, 3, 4], stages=True,
+                                  full_k=True, perturb=[(1, 0, 3, (0.01, -0.02, 0.004)),
+                                                        (1, 1, 5, (0.0, 0.0, 0.003)),
+                                                        (2, 0, 5, (0.0, 0.0, 0.02)),
+                                                        (2, 0, 5, (0.001, 0.0, 0.0)),
+                                                        (3, 1, 0, (-0.125, -0.125, -0.002))],
+                                  ydrag=0.0)
+
     c = workloads.grid_config(257, dt=0.005)
     cases["grid257"] = dict(conf=c, seed=workloads.SINGLE_LARGE["seed"], steps=5,
                             dump=[1, 5], stages=False, full_k=False, stride=17)
     return cases
+
+
+def perturb_records(conf):
+    """[(step, kind, triangle, vector)] for the 21 x 21 cloth: kind 0 is a
+    force on the triangle (ApplyPerturb), kind 1 the goal of the triangle's
+    first vertex (ApplyUserPerturbForceOnce); goals are set relative to the
+    vertex's rest position"""
+    rest = workloads.grid_rest_positions(conf).reshape(-1, 3)
+    hd, wd = conf["subdivision"]
+    # first vertex of triangle t is the cell's lower-left or upper-left corner;
+    # the harness reads the actual vertex from the reference's triangle array
+    # and these goals only need to be near the cloth
+    goal_a = rest[(wd // 2) * (hd + 1) + hd // 2] + (0.0, 0.0, 0.02)
+    goal_b = rest[(wd // 3) * (hd + 1) + hd // 4] + (0.01, -0.01, -0.015)
+    return [(1, 0, 100, (0.02, -0.01, 0.05)), (1, 0, 101, (0.02, -0.01, 0.05)),
+            (1, 0, 100, (0.0, 0.0, -0.01)),
+            (2, 1, 400, tuple(goal_a)), (3, 1, 400, tuple(goal_a + (0.0, 0.0, 0.005))),
+            (5, 0, 250, (0.0, 0.1, 0.0)), (5, 1, 150, tuple(goal_b)),
+            (8, 1, 799, tuple(rest[-1] + (0.0, 0.0, 0.01)))]
 
 
 def initial_positions(conf, seed):
@@ -102,6 +139,11 @@ def run_case(name, d):
             ctpath = os.path.join(td, "contacts.bin")
             workloads.contacts_file_records(ct).astype("<f8").tofile(ctpath)
             cmd += ["--contacts", ctpath]
+        if "perturb" in d:
+            ppath = os.path.join(td, "perturb.bin")
+            np.array([[st, kind, tri, v[0], v[1], v[2]] for st, kind, tri, v in d["perturb"]],
+                     dtype="<f8").tofile(ppath)
+            cmd += ["--perturb", ppath, "--ydrag", repr(d["ydrag"])]
         subprocess.run(cmd, check=True)
         dump = read_dump(opath)
     # the injected x0 must be a jitter of the reference's own rest positions
@@ -121,6 +163,10 @@ def run_case(name, d):
     if ct is not None:
         for k, v in ct.items():
             out["contacts_" + k] = v
+    if "perturb" in d:
+        out["perturb"] = np.array([[st, kind, tri, v[0], v[1], v[2]]
+                                   for st, kind, tri, v in d["perturb"]], dtype=np.float64)
+        out["ydrag"] = np.array(d["ydrag"], dtype=np.float32)
     out["pins"] = dump["pins"]
     out["mass"] = dump["mass_diag"][0::3][sel]
     for s in d["dump"]:
