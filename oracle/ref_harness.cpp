@@ -25,7 +25,19 @@
 //   (every step by default) through cBaseObject::SetPointTriangleCollisionInfo
 //   and SetEdgeEdgeCollisionInfo, as cSimScene::PerformCollisionDetection
 //   does (src/scenes/SimScene.cpp:174-190).
-//                  [--perturb perturb.bin] [--ydrag F]
+//                  [--perturb perturb.bin] [--ydrag F] [--xpre xpre.bin]
+//                  [--qbend] [--normals]
+// --qbend dumps a cQBendingMaterial built on the cloth at its rest positions
+//   with unit stiffness (QBendingMaterial.cpp:47-118, the constant bending
+//   matrix of the reference's GPU form before its per-edge stiffness): the
+//   assembled matrix (qbend_K) and each interior edge's 12x12 element matrix
+//   (qbend_ele, in edge order) with the edge ids (qbend_edges).
+// --normals dumps the vertex and triangle normals after the last step, as
+//   cBaseCloth::Update refreshes them (BaseCloth.cpp:585-590:
+//   cBaseObject::UpdateTriangleNormal, UpdateVertexNormalFromTriangleNormal,
+//   BaseObject.cpp:119-160).
+// --xpre gives the previous positions separately (SetPrePos), so that a step
+//   starts from any state of a trajectory (x0.bin is then only SetPos).
 // perturb.bin (float64 little-endian): records {step, kind, triangle, v[3]};
 //   before that step, kind 0 hands a tPerturb with force v on the triangle to
 //   cBaseCloth::ApplyPerturb (BaseCloth.cpp:76-87), kind 1 one with goal
@@ -39,6 +51,7 @@
 #include "sim/cloth/BaraffMaterial.h"
 #include "sim/cloth/BaseBendingMaterial.h"
 #include "sim/Perturb.h"
+#include "sim/cloth/QBendingMaterial.h"
 #include "geometries/Primitives.h"
 #include "utils/JsonUtil.h"
 #include <chrono>
@@ -68,6 +81,12 @@ public:
     cBaseBendingMaterialPtr Bending() const { return mBendingMaterial; }
     const tEigenArr<tVector4i> &Stencils() const { return mEdgeAffectVertexId; }
     void SetYDragForce(float f) { mYDragForce = f; }
+    // the normal refresh of cBaseCloth::Update (BaseCloth.cpp:585-590)
+    void RefreshNormals()
+    {
+        UpdateTriangleNormal();
+        UpdateVertexNormalFromTriangleNormal();
+    }
 };
 
 // A user perturbation whose force and goal are given directly (the scene
@@ -153,8 +172,8 @@ int main(int argc, char **argv)
     std::string conf_path = argv[1], x0_path = argv[2], out_path = argv[4];
     int nsteps = atoi(argv[3]);
     std::set<int> dump_steps, contact_steps;
-    bool stages = false, timing = false;
-    std::string contacts_path, perturb_path;
+    bool stages = false, timing = false, qbend = false, normals = false;
+    std::string contacts_path, perturb_path, xpre_path;
     double ydrag = 0.0;
     bool has_ydrag = false;
     for (int i = 5; i < argc; i++)
@@ -170,6 +189,12 @@ int main(int argc, char **argv)
             timing = true;
         else if (!strcmp(argv[i], "--contacts") && i + 1 < argc)
             contacts_path = argv[++i];
+        else if (!strcmp(argv[i], "--qbend"))
+            qbend = true;
+        else if (!strcmp(argv[i], "--normals"))
+            normals = true;
+        else if (!strcmp(argv[i], "--xpre") && i + 1 < argc)
+            xpre_path = argv[++i];
         else if (!strcmp(argv[i], "--perturb") && i + 1 < argc)
             perturb_path = argv[++i];
         else if (!strcmp(argv[i], "--ydrag") && i + 1 < argc)
@@ -266,6 +291,18 @@ int main(int argc, char **argv)
         cloth->SetPos(x0);
         cloth->SetPrePos(x0);
     }
+    if (!xpre_path.empty())
+    {
+        tVectorXd xpre = x0;
+        FILE *f = fopen(xpre_path.c_str(), "rb");
+        if (!f)
+            return 4;
+        size_t got = fread(xpre.data(), sizeof(double), 3 * nv, f);
+        fclose(f);
+        if (got != size_t(3 * nv))
+            return 5;
+        cloth->SetPrePos(xpre);
+    }
 
     gOut = fopen(out_path.c_str(), "wb");
     if (!gOut)
@@ -307,6 +344,31 @@ int main(int argc, char **argv)
         WriteVec("x0", x0);
         WriteVec("mass_diag", cloth->MassDiag());
         WriteVec("gravity", cloth->GravityForce());
+    }
+
+    if (qbend)
+    {
+        // at the initial positions, where the GPU form builds its matrix
+        // (cBaraffClothGpu::Init, before any SetPos)
+        tVectorXd xnow = cloth->GetPos();
+        cloth->SetPos(cloth->GetInitPos());
+        auto qb = std::make_shared<cQBendingMaterial>();
+        qb->Init(cloth->GetVertexArray(), cloth->GetEdgeArray(), cloth->GetTriangleArray(),
+                 tVector3d::Ones());
+        WriteSparse("qbend_K", qb->GetStiffnessMatrix());
+        std::vector<double> ele;
+        for (const tMatrix12f &k : qb->GetEleStiffnessMatrixLst())
+            for (int r = 0; r < 12; r++)
+                for (int c = 0; c < 12; c++)
+                    ele.push_back((double)k(r, c));
+        WriteF8("qbend_ele", ele.data(), ele.size());
+        std::vector<int> ids;
+        const auto &edges = cloth->GetEdgeArray();
+        for (int e = 0; e < (int)edges.size(); e++)
+            if (!edges[e]->mIsBoundary)
+                ids.push_back(e);
+        WriteI4("qbend_edges", ids.data(), ids.size());
+        cloth->SetPos(xnow);
     }
 
     std::vector<double> step_seconds;
@@ -359,6 +421,19 @@ int main(int argc, char **argv)
     }
     if (timing)
         WriteF8("step_seconds", step_seconds.data(), step_seconds.size());
+    if (normals)
+    {
+        cloth->RefreshNormals();
+        std::vector<double> vn, tn;
+        for (auto &v : cloth->GetVertexArray())
+            for (int a = 0; a < 3; a++)
+                vn.push_back(v->mNormal[a]);
+        for (auto &t : cloth->GetTriangleArray())
+            for (int a = 0; a < 3; a++)
+                tn.push_back(t->mNormal[a]);
+        WriteF8("vertex_normals", vn.data(), vn.size());
+        WriteF8("triangle_normals", tn.data(), tn.size());
+    }
     fclose(gOut);
     fflush(stdout);
     dup2(saved_stdout, 1);

@@ -1,4 +1,6 @@
-"""Reader for the tagged binary dumps written by oracle/ref_harness.cpp."""
+"""Reader for the tagged binary dumps written by oracle/ref_harness.cpp (test
+infrastructure: used by tests/golden/make_golden.py and bench.py's CPU-baseline
+leg)."""
 import numpy as np
 
 

@@ -24,7 +24,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
-sys.path.insert(0, HERE)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 from acousticsim_amd import workloads  # noqa: E402
 from refdump import read_dump  # noqa: E402
@@ -41,17 +42,20 @@ def case_defs():
     c.update({"cloth_uv_rotation": 45, "rayleigh_damping_a": 0.1, "rayleigh_damping_b": -2e-3,
               "cloth_stretch_stiffness": [500, 800, 50], "cloth_bending_stiffness": [3e-4, 1e-4],
               "cloth_density": 0.3, "max_iters": 200})
-    cases["grid5_rot45"] = dict(conf=c, seed=12, steps=5, dump=[1, 5], stages=True, full_k=True)
+    cases["grid5_rot45"] = dict(conf=c, seed=12, steps=5, dump=[1, 5], stages=True, full_k=True,
+                                qbend=True)
 
     c = workloads.grid_config(7)
     c.update({"subdivision": [6, 11], "cloth_size": [0.3, 0.5], "cloth_init_pos": [0, 0.4, 0],
               "constraints": {"static_point": [[-0.25, 0.15], [0.25, 0.15], [0.0, 0.15]]},
               "cloth_uv_rotation": 90, "max_iters": 300})
-    cases["rect7x12"] = dict(conf=c, seed=13, steps=8, dump=[1, 8], stages=True, full_k=True)
+    cases["rect7x12"] = dict(conf=c, seed=13, steps=8, dump=[1, 8], stages=True, full_k=True,
+                             qbend=True, normals=True)
 
     c = workloads.grid_config(21, dt=0.01)
     cases["grid21"] = dict(conf=c, seed=workloads.SINGLE_SMALL["seed"], steps=100,
-                           dump=[1, 10, 50, 100], stages=True, full_k=False)
+                           dump=[1, 10, 50, 100], stages=True, full_k=False, qbend=True,
+                           normals=True)
 
     # contacts handed to the cloth before every step (collision Hessian in W,
     # repulsion impulses on dx); stiffness values whose float mean depends on
@@ -84,6 +88,11 @@ def case_defs():
                                                         (3, 1, 0, (-0.125, -0.125, -0.002))],
                                   ydrag=0.0)
 
+    # an OBJ cloth of jittered vertices (triangle areas from 0.3x to 2x the
+    # mean), so that area-weighted vertex normals differ from unweighted ones
+    cases["obj_irregular"] = dict(conf=None, seed=None, steps=6, dump=[1, 6], stages=True,
+                                  full_k=False, normals=True, obj=dict(seed=7))
+
     c = workloads.grid_config(257, dt=0.005)
     cases["grid257"] = dict(conf=c, seed=workloads.SINGLE_LARGE["seed"], steps=5,
                             dump=[1, 5], stages=False, full_k=False, stride=17)
@@ -109,6 +118,31 @@ def perturb_records(conf):
             (8, 1, 799, tuple(rest[-1] + (0.0, 0.0, 0.01)))]
 
 
+def irregular_sheet(seed):
+    """(positions, uv, triangles) of the OBJ fixture sheet with its interior
+    vertices jittered in the plane (positions and texture coordinates alike)"""
+    import obj_fixture as ofx
+
+    rng = np.random.default_rng(seed)
+    pos, uv, tri = ofx.sheet()
+    inner = [k for k in range(len(pos))
+             if 0 < k % ofx.NX < ofx.NX - 1 and 0 < k // ofx.NX < ofx.NY - 1]
+    d = rng.uniform(-0.4, 0.4, (len(inner), 2))
+    pos[inner, 0] += d[:, 0] * 0.05
+    pos[inner, 1] += d[:, 1] * 0.04
+    uv[inner, 0] += d[:, 0] * 0.05
+    uv[inner, 1] += d[:, 1] * 0.05
+    return pos, uv, tri
+
+
+def write_sheet_obj(path, pos, uv, tri):
+    lines = ["o sheet"] + ["v %.17g %.17g %.17g" % tuple(p) for p in pos]
+    lines += ["vt %.17g %.17g" % tuple(t) for t in uv]
+    lines += ["f " + " ".join("%d/%d" % (v + 1, v + 1) for v in t) for t in tri]
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
 def initial_positions(conf, seed):
     x = workloads.jittered_positions(conf, seed).reshape(-1, 3)
     x += np.array((list(conf.get("cloth_init_pos", [0, 0, 0])) + [0, 0, 0])[:3])
@@ -117,18 +151,33 @@ def initial_positions(conf, seed):
 
 def run_case(name, d):
     conf = d["conf"]
-    x0 = initial_positions(conf, d["seed"])
     with tempfile.TemporaryDirectory() as td:
+        sheet = None
+        if "obj" in d:
+            import obj_fixture as ofx
+
+            sheet = irregular_sheet(d["obj"]["seed"])
+            objpath = os.path.join(td, "sheet.obj")
+            write_sheet_obj(objpath, *sheet)
+            conf = ofx.obj_config(objpath)
+            x0 = None
+        else:
+            x0 = initial_positions(conf, d["seed"])
         cpath = os.path.join(td, "conf.json")
         xpath = os.path.join(td, "x0.bin")
         opath = os.path.join(td, "out.bin")
         with open(cpath, "w") as f:
             json.dump(conf, f)
-        x0.astype("<f8").tofile(xpath)
-        cmd = [REF_BIN, cpath, xpath, str(d["steps"]), opath,
+        if x0 is not None:
+            x0.astype("<f8").tofile(xpath)
+        cmd = [REF_BIN, cpath, xpath if x0 is not None else "-", str(d["steps"]), opath,
                "--dump-steps", ",".join(str(s) for s in d["dump"]), "--time"]
         if d["stages"]:
             cmd.append("--stages")
+        if d.get("qbend"):
+            cmd.append("--qbend")
+        if d.get("normals"):
+            cmd.append("--normals")
         ct = None
         if "contacts" in d:
             side = conf["subdivision"]
@@ -146,11 +195,19 @@ def run_case(name, d):
             cmd += ["--perturb", ppath, "--ydrag", repr(d["ydrag"])]
         subprocess.run(cmd, check=True)
         dump = read_dump(opath)
-    # the injected x0 must be a jitter of the reference's own rest positions
     xi = dump["x_init"]
-    assert np.max(np.abs((x0 - xi).reshape(-1, 3)[:, :2])) == 0.0, name
-    out = {"config_json": np.array(json.dumps(conf)), "seed": np.array(d["seed"]),
+    if x0 is None:
+        x0 = dump["x0"]
+    else:
+        # the injected x0 must be a jitter of the reference's own rest positions
+        assert np.max(np.abs((x0 - xi).reshape(-1, 3)[:, :2])) == 0.0, name
+    out = {"config_json": np.array(json.dumps(conf)),
+           "seed": np.array(d["seed"] if d["seed"] is not None else -1),
            "steps": np.array(d["dump"], dtype=np.int32)}
+    if sheet is not None:
+        # the OBJ's content (the test writes it again; the config's path is
+        # a placeholder)
+        out["obj_pos"], out["obj_uv"], out["obj_tri"] = sheet
     stride = d.get("stride", 1)
     sel = np.arange(0, len(xi) // 3, stride)
     out["vertex_subset"] = sel.astype(np.int32)
@@ -175,6 +232,15 @@ def run_case(name, d):
         out["norm_step%d" % s] = np.array(np.linalg.norm(x))
         out["sum_step%d" % s] = x.sum(axis=0)
     out["step_seconds"] = dump["step_seconds"]
+    if d.get("qbend"):
+        # the reference's quadratic bending matrix at unit stiffness
+        # (assembled COO and per-interior-edge 12x12 element matrices)
+        for k in ("qbend_K_row", "qbend_K_col", "qbend_K_val", "qbend_ele", "qbend_edges"):
+            out[k] = dump[k]
+    if d.get("normals"):
+        # normals after the last step (cBaseCloth::Update's refresh)
+        out["vertex_normals"] = dump["vertex_normals"]
+        out["triangle_normals"] = dump["triangle_normals"]
     if d["stages"]:
         out["triangles"] = dump["triangles"]
         out["edges"] = dump["edges"]

@@ -12,6 +12,10 @@ instead by what the mathematics fixes:
   * the contact set against a brute-force distance test in numpy, its order
     against the sorted (point, triangle) / (edge, edge) keys, normals and
     depths against their definitions;
+  * edge-edge contacts against a closed form on a tent-shaped body: the
+    normal is the edges' cross product turned towards the body edge's
+    summed triangle normals, the depth the cloth-minus-body vector along it
+    at the segments' closest points, for the cloth as either object;
   * the response to detected contacts: a cloth dropped onto a floor body
     stops penetrating it, unlike the same cloth without detection.
 """
@@ -217,3 +221,87 @@ def test_detected_contacts_hold_the_cloth_above_a_floor():
     assert n_contacts > 0
     assert low_free < -5e-3
     assert low > low_free + 3e-3
+
+
+def _seg_seg_params(p1, q1, p2, q2):
+    """(s, t) of the closest points of two segments: the minimum of the
+    convex quadratic |p1 + s d1 - p2 - t d2|^2 over the unit square, from
+    its interior stationary point and the minimisers along the four sides"""
+    d1, d2, r = q1 - p1, q2 - p2, p1 - p2
+    a, e, b, c, f = d1 @ d1, d2 @ d2, d1 @ d2, d1 @ r, d2 @ r
+    cands = []
+    den = a * e - b * b
+    if den > 1e-14:
+        cands.append(((b * f - c * e) / den, (a * f - b * c) / den))
+    cands += [(s, (b * s + f) / e) for s in (0.0, 1.0)]
+    cands += [((b * t - c) / a, t) for t in (0.0, 1.0)]
+    best = None
+    for s, t in cands:
+        s, t = float(np.clip(s, 0, 1)), float(np.clip(t, 0, 1))
+        d = np.linalg.norm(p1 + s * d1 - p2 - t * d2)
+        if best is None or d < best[0]:
+            best = (d, s, t)
+    return best[1], best[2]
+
+
+@pytest.mark.parametrize("cloth_first", [True, False])
+@pytest.mark.parametrize("tilt", [0.0, 0.05])
+def test_edge_edge_normal_and_depth_closed_form(cloth_first, tilt):
+    """A tent-shaped body (ridge along z, slopes at 45 degrees) under a flat
+    cloth 1.2 mm above the ridge, level or tilted about z.  Every edge-edge
+    contact (BVHCollisionDetecter.cpp:555-629) has the normal of the two
+    edges' cross product turned towards the body edge's summed triangle
+    normals (here: up), and the depth (cloth point - body point) along it,
+    at the closest points of the two segments; the barycentric parameters
+    are those closest points; edge0 is the object with the lower id"""
+    L = 0.4
+    xyz = np.array([[0.0, 0.0, -L], [0.0, 0.0, L], [-0.1, -0.1, 0.0], [0.1, -0.1, 0.0]])
+    tri = np.array([[0, 2, 1], [0, 1, 3]], np.int32)
+    body = of.OracleBody(xyz, tri)
+    _, btn = body.normals()
+    assert np.all(btn[:, 1] > 0)  # outward: up and to the sides
+    conf = flat_cloth(9, height=1.2e-3)
+    # no cloth vertex on the ridge; tilted, the cloth still passes about
+    # 0.4 mm above it
+    conf["cloth_init_pos"] = [0.03125 / 2, 1.2e-3, 0.0]
+    conf["cloth_init_orientation"] = [-np.pi / 2, 0.0, tilt, 0.0]
+    oc = of.OracleCloth(conf)
+    x = oc.rest_pos()
+    oc.set_state(x, x)
+    ct = oc.detect_contacts(body, cloth_first)
+    cx = x.reshape(-1, 3)
+    cedges = oc.edges()
+    bedges = body.edges()
+    ee = ct["ee_ids"]
+    assert len(ee) >= 6
+    ridge_hits = 0
+    for k, (flag, e0, e1) in enumerate(ee):
+        assert flag == (1 if cloth_first else 0)
+        ce, be = (e0, e1) if cloth_first else (e1, e0)
+        bq = bedges[be]
+        c_pts = cx[cedges[ce][0]], cx[cedges[ce][1]]
+        b_pts = xyz[bq[0]], xyz[bq[1]]
+        p0, p1 = c_pts if cloth_first else b_pts
+        p2, p3 = b_pts if cloth_first else c_pts
+        n = np.cross(p1 - p0, p3 - p2)
+        n /= np.linalg.norm(n)
+        nb = btn[bq[2]] + (btn[bq[3]] if bq[3] >= 0 else 0.0)
+        if nb @ n < 0:
+            n = -n
+        s, t = _seg_seg_params(p0, p1, p2, p3)
+        q0, q1 = p0 + s * (p1 - p0), p2 + t * (p3 - p2)
+        cloth_pt, body_pt = (q0, q1) if cloth_first else (q1, q0)
+        depth = (cloth_pt - body_pt) @ n
+        assert np.allclose(ct["ee_normal"][k], n, atol=1e-12), k
+        assert ct["ee_depth"][k] == pytest.approx(depth, abs=2e-8), k
+        assert np.linalg.norm(cloth_pt - body_pt) < 3e-3
+        assert ct["ee_bary"][k][0] == pytest.approx(s, abs=1e-5)
+        assert ct["ee_bary"][k][1] == pytest.approx(t, abs=1e-5)
+        ridge = bq[0] == 0 and bq[1] == 1
+        if ridge:  # the cloth lies above the ridge, outside the body
+            assert n[1] > 0 and depth > 0
+            ridge_hits += 1
+        if tilt == 0.0 and ridge:
+            assert np.allclose(n, [0.0, 1.0, 0.0], atol=1e-12)
+            assert depth == pytest.approx(1.2e-3, abs=1e-9)
+    assert ridge_hits >= 4
