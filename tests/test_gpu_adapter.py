@@ -1,0 +1,54 @@
+"""The call sequence of the documented reference-side adapter
+(cBaraffClothHip::UpdatePos: bw_update_pos, then bw_get_pos, and only the
+host-side cBaseCloth::SetPos) keeps the device solver's state across steps:
+it equals stepping the cloth without host calls in between, bit for bit,
+while re-uploading the positions after every step (bw_set_pos, the virtual
+SetPos) restarts each solve on the FP32 matrix and changes the result."""
+import sys
+
+import numpy as np
+import pytest
+
+from acousticsim_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_adapter_sequence_keeps_the_fp64_matrix_choice(monkeypatch):
+    from acousticsim_amd import capi
+
+    # every solve's first forced residual check counts as stalled, so each
+    # cloth moves to the FP64 matrix in step 1 and keeps it afterwards
+    monkeypatch.setenv("BW_SPMV_LPR", "1")
+    monkeypatch.setenv("BW_SPMV_STAGE", "0")
+    monkeypatch.setenv("BW_MAX_SINCE", "20")
+    monkeypatch.setenv("BW_STALL", "1e-30")
+    c = wl.grid_config(33, stiffness_scale=2.0)
+    c["precision"] = "mixed"
+    x0 = wl.jittered_positions(c, 9)
+    say("case adapter call sequence")
+    plain = capi.BaraffClothHip(c, x0)
+    adapter = capi.BaraffClothHip(c, x0)
+    upload = capi.BaraffClothHip(c, x0)
+    iters = {"plain": [], "adapter": [], "upload": []}
+    for _ in range(3):
+        plain.UpdatePos()
+        iters["plain"].append(plain.last_stats()["pcg_iterations"])
+        adapter.UpdatePos()
+        x = adapter.GetPos()  # then cBaseCloth::SetPos(x) on the host only
+        iters["adapter"].append(adapter.last_stats()["pcg_iterations"])
+        upload.UpdatePos()
+        upload.SetPos(upload.GetPos())  # the virtual SetPos: uploads again
+        iters["upload"].append(upload.last_stats()["pcg_iterations"])
+        assert plain.last_stats()["matrix_fp64"] == 1
+    say("  iterations", iters)
+    assert np.array_equal(adapter.GetPos(), plain.GetPos())
+    assert iters["adapter"] == iters["plain"]
+    assert not np.array_equal(upload.GetPos(), plain.GetPos())
+    assert iters["upload"][1:] != iters["plain"][1:]
+    assert np.array_equal(x, adapter.GetPos())

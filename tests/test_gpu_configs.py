@@ -212,7 +212,7 @@ def test_irregular_step_solves_oracle_system():
     c["precision"] = "mixed"
     pos, tri = wl.delaunay_rest_positions(c)
     x0 = wl.jitter(pos, wl.IRREGULAR["seed"])
-    say("case irregular: %d vertices, %d triangles" % (len(pos) // 3, len(tri)))
+    say("case irregular: %d vertices, %d triangles" % (np.asarray(pos).size // 3, len(tri)))
     with pool() as ex:
         fut = ex.submit(of.OracleCloth, c, (pos, tri))
         cl = capi.BaraffClothHip(c, x0)
