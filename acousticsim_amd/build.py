@@ -40,6 +40,7 @@ SOURCES = [
     ("collide.hip", DEVICE + ["-ffp-contract=off"]),
     ("body.cpp", ["-ffp-contract=off"]),
     ("gpuform.hip", DEVICE + ["-ffp-contract=fast"]),
+    ("noalloc.hip", DEVICE + ["-ffp-contract=off"]),
 ]
 
 

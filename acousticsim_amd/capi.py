@@ -86,7 +86,7 @@ EXPORTS = [
     "bw_apply_perturb", "bw_apply_user_perturb_force_once", "bw_set_y_drag_force",
     "bw_batch_apply_perturb", "bw_batch_apply_user_perturb_force_once",
     "bw_batch_set_y_drag_force", "bw_assemble_system", "bw_get_last_stats", "bw_last_error", "bw_batch_create",
-    "bw_batch_destroy", "bw_batch_size", "bw_batch_step", "bw_batch_num_vertices",
+    "bw_batch_destroy", "bw_batch_size", "bw_batch_step", "bw_batch_step_noalloc", "bw_batch_num_vertices",
     "bw_batch_num_triangles", "bw_set_point_triangle_contacts", "bw_set_edge_edge_contacts",
     "bw_batch_set_point_triangle_contacts", "bw_batch_set_edge_edge_contacts",
     "bw_get_point_triangle_contacts", "bw_get_edge_edge_contacts", "bw_body_create",
@@ -150,6 +150,7 @@ def lib():
         "bw_batch_destroy": (None, [vp]),
         "bw_batch_size": (i32, [vp]),
         "bw_batch_step": (i32, [vp, i32]),
+        "bw_batch_step_noalloc": (i32, [vp, i32]),
         "bw_batch_num_vertices": (i32, [vp, i32]),
         "bw_batch_num_triangles": (i32, [vp, i32]),
         "bw_set_point_triangle_contacts": (i32, [vp, i32, ip, dp, dp, fp]),
@@ -487,6 +488,11 @@ class ClothBatch:
 
     def step(self, nsteps=1):
         _check(lib().bw_batch_step(self._h, nsteps))
+
+    def step_noalloc(self, nsteps=1):
+        """bw_batch_step_noalloc: the same steps with the solver scratch in the
+        library's static device array (no allocation)"""
+        _check(lib().bw_batch_step_noalloc(self._h, nsteps))
 
     def set_graphs(self, on):
         _check(lib().bw_batch_set_graphs(self._h, 1 if on else 0))

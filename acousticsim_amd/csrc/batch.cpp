@@ -1,5 +1,6 @@
 #include "batch.h"
 #include "mesh_gen.h"
+#include "noalloc.h"
 
 #include <algorithm>
 #include <chrono>
@@ -1850,6 +1851,95 @@ void Batch::detect_contacts(int i, const Body &body, bool cloth_first, int *n_pt
         *n_pt = (int)pt.size();
     if (n_ee)
         *n_ee = (int)ee_hits.size();
+}
+
+// bw_batch_step_noalloc.  The solver states are copied into the scratch
+// before the steps (they carry each cloth's FP64-matrix choice from step to
+// step) and back after them; the partial and group sums are zeroed first.
+// Scratch layout: 2 CgState per cloth | 2 x PARTS doubles per slice | 2 x
+// PARTS doubles per group, each part 256-byte aligned.
+void Batch::step_noalloc(int nsteps)
+{
+    if (kind_ != SolverKind::CG32 && kind_ != SolverKind::CG64)
+        throw std::runtime_error("state: the no-allocation step covers the tolerance-mode "
+                                 "solvers of \"fem\" cloths only");
+    if (contact_total_ > 0)
+        throw std::runtime_error("state: contacts are pending; they need bw_batch_step, "
+                                 "whose contact lists are allocated per step");
+    for (int i = 0; i < size(); i++)
+        if (!user_force_[i].empty() || drag_vertex_[i] >= 0 || ydrag_[i] != 0.0f)
+            throw std::runtime_error("state: user perturbations are set; they need "
+                                     "bw_batch_step");
+    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t st_bytes = align(2 * (size_t)size() * sizeof(CgState));
+    const size_t part_bytes = align(2 * (size_t)PARTS * d_.n_slices * sizeof(double));
+    const size_t grp_bytes = 2 * (size_t)PARTS * d_.n_groups * sizeof(double);
+    if (st_bytes + part_bytes + grp_bytes > NOALLOC_SCRATCH_BYTES)
+        throw std::runtime_error("capacity: this batch's solver scratch (" +
+                                 std::to_string(st_bytes + part_bytes + grp_bytes) +
+                                 " bytes) exceeds the " + std::to_string(NOALLOC_SCRATCH_BYTES) +
+                                 " bytes of the no-allocation entry point");
+    if (!tuned_)
+    {
+        int lpr, stage, nt;
+        fixed_variant(lpr, stage, nt);
+        if (lpr)
+            d_.spmv_lpr = lpr;
+        if (stage >= 0)
+            d_.spmv_stage = stage;
+        if (nt >= 0)
+            d_.spmv_nt = nt;
+    }
+    char *base = static_cast<char *>(noalloc_scratch());
+    CgState *own_cg = d_.cg;
+    double *own_partial = d_.cg_partial, *own_group = d_.cg_group;
+    const bool own_graphs = use_graphs_;
+    struct Restore
+    {
+        Batch *b;
+        CgState *cg;
+        double *partial, *group;
+        bool graphs;
+        ~Restore()
+        {
+            b->d_.cg = cg;
+            b->d_.cg_partial = partial;
+            b->d_.cg_group = group;
+            b->use_graphs_ = graphs;
+        }
+    } restore{this, own_cg, own_partial, own_group, own_graphs};
+    const size_t st_copy = 2 * (size_t)size() * sizeof(CgState);
+    check(hipMemcpyAsync(base, own_cg, st_copy, hipMemcpyDeviceToDevice, stream_),
+          "hipMemcpyAsync states");
+    noalloc_clear(base + st_bytes, part_bytes + grp_bytes, stream_);
+    d_.cg = reinterpret_cast<CgState *>(base);
+    d_.cg_partial = reinterpret_cast<double *>(base + st_bytes);
+    d_.cg_group = d_.n_groups > 0 ? reinterpret_cast<double *>(base + st_bytes + part_bytes)
+                                  : nullptr;
+    use_graphs_ = false;
+    for (int n = 0; n < nsteps; n++)
+    {
+        const auto t0 = std::chrono::steady_clock::now();
+        launch_elements(d_, stream_);
+        launch_assemble_rows(d_, stream_);
+        solve_cg();
+        launch_update_positions(d_, stream_);
+        check(hipGetLastError(), "step launch");
+        check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+        const double us =
+            std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+        bool bad = false;
+        for (auto &st : stats_)
+        {
+            st.step_us = us;
+            bad |= !std::isfinite(st.rel_residual);
+        }
+        if (bad)
+            throw std::runtime_error("step: non-finite residual (NaN/Inf in the state)");
+    }
+    check(hipMemcpyAsync(own_cg, base, st_copy, hipMemcpyDeviceToDevice, stream_),
+          "hipMemcpyAsync states");
+    check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
 }
 
 void Batch::step(int nsteps)

@@ -76,6 +76,12 @@ public:
     void refresh_pins(int i, const std::vector<int> &pins);
 
     void step(int nsteps);
+    // step() for callers that cannot allocate: the tolerance solvers only,
+    // no contacts or user inputs pending, direct launches (no graphs), no
+    // autotuning, and the solver's scratch (states, partial and group sums)
+    // in the statically sized device array of noalloc.hip instead of the
+    // batch's own memory; allocates no device or host memory
+    void step_noalloc(int nsteps);
     // One step driven from outside (tolerance solvers): begin_step queues the
     // element, assembly and solver-reset kernels; the caller then runs the
     // solver launches (launch_cg_step_multi) until every cloth's state says

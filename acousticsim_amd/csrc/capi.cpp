@@ -646,6 +646,16 @@ int bw_batch_set_edge_edge_contacts(bw_batch *b, int i, int n, const int *ids,
     });
 }
 
+int bw_batch_step_noalloc(bw_batch *b, int nsteps)
+{
+    return guard([&] {
+        if (!b || nsteps < 0)
+            return fail(BW_ERR_ARG, "bad batch or step count");
+        b->batch->step_noalloc(nsteps);
+        return BW_OK;
+    });
+}
+
 int bw_batch_apply_perturb(bw_batch *b, int i, int triangle, const double force[3])
 {
     if (!force)

@@ -270,6 +270,23 @@ int bw_batch_set_y_drag_force(bw_batch *b, int i, float force);
 /* bw_detect_contacts for member i */
 int bw_batch_detect_contacts(bw_batch *bt, int i, const bw_body *b, int cloth_first,
                              int *n_point_triangle, int *n_edge_edge);
+/* No reference counterpart.  bw_batch_step for callers that cannot
+   allocate (code under HIP graph capture, tight serving loops, harnesses
+   that forbid hipMalloc): the same kernels, launched directly on the batch's
+   stream, with the solver's scratch (two solver states per cloth, two slots
+   of FP64 partial sums per 64-row slice, group sums of large cloths) in a
+   statically sized device array of 4 MiB per GPU instead of the batch's own
+   memory; it allocates no device or host memory (the batch, created
+   beforehand, holds everything else).  The solver states are carried in and
+   out, so results are those of bw_batch_step with the same SpMV variant:
+   the autotuner's pick if bw_batch_step or bw_batch_autotune ran before,
+   else the variant the BW_SPMV_* environment variables fix or the default;
+   it never tunes.  Limits: tolerance-mode "fem" batches (BW_ERR_STATE
+   otherwise, and while contacts or user perturbations are pending);
+   BW_ERR_CAPACITY when the batch's scratch exceeds 4 MiB (beyond about
+   52,000 slices, 3.3M vertices).  Every call on a GPU shares that GPU's
+   array: calls that overlap on one GPU race. */
+int bw_batch_step_noalloc(bw_batch *b, int nsteps);
 /* use HIP graph replay for the per-iteration launch chains (default on) */
 int bw_batch_set_graphs(bw_batch *b, int enable);
 /* Measurement (no reference counterpart): time `reps` back-to-back launches
