@@ -87,7 +87,7 @@ EXPORTS = [
     "bw_batch_apply_perturb", "bw_batch_apply_user_perturb_force_once",
     "bw_batch_set_y_drag_force", "bw_assemble_system", "bw_get_last_stats", "bw_last_error", "bw_batch_create",
     "bw_batch_destroy", "bw_batch_size", "bw_batch_step", "bw_batch_step_noalloc", "bw_batch_num_vertices",
-    "bw_batch_num_triangles", "bw_set_point_triangle_contacts", "bw_set_edge_edge_contacts",
+    "bw_batch_num_triangles", "bw_batch_num_blocks", "bw_set_point_triangle_contacts", "bw_set_edge_edge_contacts",
     "bw_batch_set_point_triangle_contacts", "bw_batch_set_edge_edge_contacts",
     "bw_get_point_triangle_contacts", "bw_get_edge_edge_contacts", "bw_body_create",
     "bw_body_destroy", "bw_body_set_pos", "bw_body_num_edges", "bw_body_get_edges",
@@ -153,6 +153,7 @@ def lib():
         "bw_batch_step_noalloc": (i32, [vp, i32]),
         "bw_batch_num_vertices": (i32, [vp, i32]),
         "bw_batch_num_triangles": (i32, [vp, i32]),
+        "bw_batch_num_blocks": (i32, [vp, i32]),
         "bw_set_point_triangle_contacts": (i32, [vp, i32, ip, dp, dp, fp]),
         "bw_set_edge_edge_contacts": (i32, [vp, i32, ip, dp, fp, fp]),
         "bw_batch_set_point_triangle_contacts": (i32, [vp, i32, i32, ip, dp, dp, fp]),
@@ -488,6 +489,13 @@ class ClothBatch:
 
     def step(self, nsteps=1):
         _check(lib().bw_batch_step(self._h, nsteps))
+
+    def num_blocks(self, i):
+        """true 3x3 blocks of member i's matrix"""
+        n = lib().bw_batch_num_blocks(self._h, i)
+        if n < 0:
+            _check(n)
+        return n
 
     def step_noalloc(self, nsteps=1):
         """bw_batch_step_noalloc: the same steps with the solver scratch in the

@@ -395,6 +395,22 @@ void Batch::build_layout()
         group_inst.insert(group_inst.end(), P.group_count, i);
     }
     d_.n_groups = (int)group_inst.size();
+    // per slice: cloth, solver-sum range (negative count: group sums) and
+    // first ELL entry, so that a solver kernel needs one load to find them
+    std::vector<int4> slice_info(slices);
+    for (int i = 0; i < n; i++)
+    {
+        const InstParams &P = params[i];
+        for (int s = 0; s < nslices_[i]; s++)
+        {
+            const int gs = slice_base_[i] + s;
+            slice_info[gs] = P.group_count > 0
+                                 ? make_int4(i, P.group_begin, -P.group_count, slice_off[gs])
+                                 : make_int4(i, P.slice_begin, P.slice_count, slice_off[gs]);
+        }
+    }
+    d_.slice_info = dalloc<int4>(slices);
+    upload(d_.slice_info, slice_info);
 
     d_.slice_off = dalloc<int>(slices);
     d_.slice_w = dalloc<int>(slices);
@@ -824,8 +840,17 @@ void Batch::launch_chain(int length, int slices)
         for (int k = 0; k < length; k++)
             launch_pcg_iter64(d_, stream_);
     else
+    {
+        DeviceBatch dl = d_;
+        if (launch_first_ >= 0)
+        {
+            dl.spmv_list = nullptr;
+            dl.upd_list = nullptr;
+            dl.slice_first = launch_first_;
+        }
         for (int k = 0; k < length; k++)
-            launch_cg_step(d_, kind_ == SolverKind::CG32, k & 1, slices, stream_);
+            launch_cg_step(dl, kind_ == SolverKind::CG32, k & 1, slices, stream_);
+    }
 }
 
 void Batch::run_chunk(int length, int slices)
@@ -851,7 +876,7 @@ void Batch::run_graph(int length, int slices)
     ++graph_clock_;
     hipGraphExec_t exec = nullptr;
     for (auto &g : graphs_)
-        if (g.length == length && g.slices == slices)
+        if (g.length == length && g.slices == slices && g.first == launch_first_)
         {
             exec = g.exec;
             g.last_use = graph_clock_;
@@ -876,7 +901,7 @@ void Batch::run_graph(int length, int slices)
         check(hipStreamEndCapture(stream_, &graph), "EndCapture");
         check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "GraphInstantiate");
         (void)hipGraphDestroy(graph);
-        graphs_.push_back({length, slices, exec, graph_clock_});
+        graphs_.push_back({length, slices, launch_first_, exec, graph_clock_});
     }
     check(hipGraphLaunch(exec, stream_), "GraphLaunch");
 }
@@ -1009,10 +1034,39 @@ void Batch::chunk_schedule()
     }
 }
 
-// Lists of the active cloths' slices and solver workgroups (padded with -1
-// to the bucketed grid), uploaded on the stream; returns the k_cg_spmv grid.
+// The slices the next chunk's launches cover.  With every cloth active the
+// launches cover all slices directly (launch_first_ = 0): no list, so no
+// dependent load of a list entry before a workgroup can start.  Otherwise
+// lists of the active cloths' slices and solver workgroups (padded with -1
+// to the bucketed grid) are uploaded on the stream: a replayed graph reads
+// whatever list the buffer holds, so one graph serves every active set of a
+// bucket, where a range baked into the launches would need a graph per
+// range.  BW_RANGE=2 also runs any contiguous active range directly, 0 never
+// (measurement runs).  Returns the k_cg_spmv grid.
 int Batch::upload_lists(const std::vector<char> &active)
 {
+    static const int range_rule = [] {
+        const char *e = std::getenv("BW_RANGE");
+        return e ? std::atoi(e) : 1;
+    }();
+    int first = -1, last = -1;
+    bool contiguous = true;
+    for (int i = 0; i < size() && contiguous; i++)
+        if (active[i])
+        {
+            if (first < 0)
+                first = slice_base_[i];
+            else if (slice_base_[i] != last)
+                contiguous = false;
+            last = slice_base_[i] + nslices_[i];
+        }
+    const bool whole = first == 0 && last == d_.n_slices;
+    if (first >= 0 && contiguous && (range_rule == 2 || (range_rule == 1 && whole)))
+    {
+        launch_first_ = first;
+        return last - first;
+    }
+    launch_first_ = -1;
     int n = 0, nb = 0;
     for (int i = 0; i < size(); i++)
         if (active[i])

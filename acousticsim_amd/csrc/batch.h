@@ -214,9 +214,12 @@ private:
     int last_launches_ = 0;
     std::vector<long long> pred_launches_;      // per cloth, from the last solve
     int *host_spmv_list_ = nullptr, *host_upd_list_ = nullptr; // pinned
+    // first slice of the contiguous range the next launches cover, or -1
+    // when they run from the slice lists
+    int launch_first_ = -1;
     struct GraphEntry
     {
-        int length, slices;
+        int length, slices, first;
         hipGraphExec_t exec;
         unsigned long long last_use;
     };

@@ -688,6 +688,13 @@ int bw_batch_detect_contacts(bw_batch *bt, int i, const bw_body *b, int cloth_fi
     });
 }
 
+int bw_batch_num_blocks(const bw_batch *b, int i)
+{
+    if (!b || i < 0 || i >= b->batch->size())
+        return fail(BW_ERR_ARG, "bad arguments");
+    return b->batch->model(i).nnzb();
+}
+
 int bw_batch_get_stats(const bw_batch *b, int i, bw_step_stats *st)
 {
     if (!b || !st || i < 0 || i >= b->batch->size())

@@ -57,6 +57,9 @@
 // (beta = r_new.z_new / rho_old).  Convergence is always decided on the FP64
 // true residual: ||b - W64 dx|| <= tol ||b||.  "fp64" runs the same machine
 // with T = double.
+#include <algorithm>
+#include <cstdlib>
+
 #include "device.h"
 #include "reduce.h"
 
@@ -118,10 +121,25 @@ template <typename SP> __device__ inline void store_state(SP p, const CgState &s
 // loads, which is what bounds a cloth solved alone.  The batch picks the
 // fastest on its own matrix (Batch::autotune).
 
+// The solver's arithmetic is written out operation by operation, fused
+// multiply-adds included, and this file is compiled without floating-point
+// contraction: every kernel and variant that computes the same quantity
+// rounds it the same way, whatever the surrounding code (a cloth's result
+// cannot depend on which launch shape or kernel variant computed it).
+__device__ inline float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ inline double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 // d = z + beta d_prev, z = P r
 template <typename T> __device__ inline T dir_value(T z, T dold, T beta, bool first)
 {
-    return first ? z : z + beta * dold;
+    return first ? z : fma_t(beta, dold, z);
+}
+
+// one row of a 3x3 block times a 3-vector: (m0 v0 + m1 v1) + m2 v2 with the
+// two additions fused
+template <typename T> __device__ inline T block_row(T m0, T m1, T m2, T v0, T v1, T v2)
+{
+    return fma_t(m2, v2, fma_t(m1, v1, m0 * v0));
 }
 
 // sum over the L lane phases of a row (lanes l, l+64/L, ...)
@@ -158,17 +176,19 @@ template <> struct Vecs<double>
 // Sums of components comp[] of the previous launch's partials of a cloth
 // (slot 0: written by k_cg_update, 1: by k_cg_spmv), in every wavefront.
 // Large cloths read their group sums, which k_cg_group formed after that
-// launch; the others read their per-slice partials.
+// launch; the others read their per-slice partials.  si is the slice's
+// entry of d.slice_info, so the loads need nothing but the slice id and go
+// out together with the state's.
 template <int N>
-__device__ inline void solver_sums(const DeviceBatch &d, const InstParams &ip, int slot,
-                                   const int (&comp)[N], double (&out)[N])
+__device__ inline void solver_sums(const DeviceBatch &d, int4 si, int slot, const int (&comp)[N],
+                                   double (&out)[N])
 {
-    if (ip.group_count > 0)
-        wave_sets_sum<N>(glob(d.cg_group) + (size_t)slot * PARTS * d.n_groups, ip.group_begin,
-                         ip.group_count, comp, out);
+    if (si.z < 0)
+        wave_sets_sum<N>(glob(d.cg_group) + (size_t)slot * PARTS * d.n_groups, si.y, -si.z, comp,
+                         out);
     else
-        wave_sets_sum<N>(glob(d.cg_partial) + (size_t)slot * PARTS * d.n_slices, ip.slice_begin,
-                         ip.slice_count, comp, out);
+        wave_sets_sum<N>(glob(d.cg_partial) + (size_t)slot * PARTS * d.n_slices, si.y, si.z, comp,
+                         out);
 }
 
 // Loads of data that one launch reads once (matrix planes, column indices),
@@ -238,9 +258,78 @@ __device__ inline void slice_product(const G<TW> *__restrict__ Wl, const G<int> 
                 const TW v0 = (TW)dir_value<T>(zv[u][0], dv[u][0], bt, first);
                 const TW v1 = (TW)dir_value<T>(zv[u][1], dv[u][1], bt, first);
                 const TW v2 = (TW)dir_value<T>(zv[u][2], dv[u][2], bt, first);
-                y[0] += m[u][0] * v0 + m[u][1] * v1 + m[u][2] * v2;
-                y[1] += m[u][3] * v0 + m[u][4] * v1 + m[u][5] * v2;
-                y[2] += m[u][6] * v0 + m[u][7] * v1 + m[u][8] * v2;
+                y[0] += block_row<TW>(m[u][0], m[u][1], m[u][2], v0, v1, v2);
+                y[1] += block_row<TW>(m[u][3], m[u][4], m[u][5], v0, v1, v2);
+                y[2] += block_row<TW>(m[u][6], m[u][7], m[u][8], v0, v1, v2);
+            }
+            c[u] = cn[u];
+        }
+    }
+    for (int a = 0; a < 3; a++)
+        y[a] = phase_sum<L>(y[a]);
+}
+
+// column indices of the gathered product's first chunk (slots j, j + L, ...,
+// U of them), loaded before the prologue: the chain column -> gather is the
+// product's critical path, the blocks go out with the gathers
+template <int L, int U, bool NT>
+__device__ inline void gathered_columns(const G<int> *__restrict__ cl, int j, int w, int (&c)[U])
+{
+#pragma unroll
+    for (int u = 0; u < U; u++)
+    {
+        const int k = j + u * L;
+        c[u] = k < w ? stream_load<NT>(cl + k * SLICE) : -1;
+    }
+}
+
+// slice_product with the column indices of its first chunk already loaded;
+// the same loads otherwise and the same arithmetic in the same order
+template <typename T, int L, bool NT>
+__device__ inline void slice_product_from(const G<T> *__restrict__ Wl, const G<int> *__restrict__ cl,
+                                          int w, int j, const G<T> *__restrict__ z,
+                                          const G<T> *__restrict__ dold, T bt, bool first,
+                                          int (&c)[4 / L], T (&y)[3])
+{
+    constexpr int U = 4 / L;
+    constexpr int STEP = U * L;
+    y[0] = y[1] = y[2] = 0;
+    for (int k0 = j; k0 < w; k0 += STEP)
+    {
+        T zv[U][3], dv[U][3];
+        T m[U][9];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (c[u] >= 0)
+            {
+                const size_t c3 = 3 * (size_t)c[u];
+                for (int a = 0; a < 3; a++)
+                {
+                    zv[u][a] = z[c3 + a];
+                    dv[u][a] = dold[c3 + a];
+                }
+                const G<T> *wk = Wl + 9 * (size_t)(k0 + u * L) * SLICE;
+                for (int e = 0; e < 9; e++)
+                    m[u][e] = stream_load<NT>(wk + e * SLICE);
+            }
+        int cn[U];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+        {
+            const int k = k0 + STEP + u * L;
+            cn[u] = k < w ? stream_load<NT>(cl + k * SLICE) : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+        {
+            if (c[u] >= 0)
+            {
+                const T v0 = dir_value<T>(zv[u][0], dv[u][0], bt, first);
+                const T v1 = dir_value<T>(zv[u][1], dv[u][1], bt, first);
+                const T v2 = dir_value<T>(zv[u][2], dv[u][2], bt, first);
+                y[0] += block_row<T>(m[u][0], m[u][1], m[u][2], v0, v1, v2);
+                y[1] += block_row<T>(m[u][3], m[u][4], m[u][5], v0, v1, v2);
+                y[2] += block_row<T>(m[u][6], m[u][7], m[u][8], v0, v1, v2);
             }
             c[u] = cn[u];
         }
@@ -384,9 +473,9 @@ __device__ inline void staged_product_from(const DeviceBatch &d, int sl,
             {
                 const int cc = c[u];
                 const TW v0 = (TW)vs[4 * cc], v1 = (TW)vs[4 * cc + 1], v2 = (TW)vs[4 * cc + 2];
-                y[0] += m[u][0] * v0 + m[u][1] * v1 + m[u][2] * v2;
-                y[1] += m[u][3] * v0 + m[u][4] * v1 + m[u][5] * v2;
-                y[2] += m[u][6] * v0 + m[u][7] * v1 + m[u][8] * v2;
+                y[0] += block_row<TW>(m[u][0], m[u][1], m[u][2], v0, v1, v2);
+                y[1] += block_row<TW>(m[u][3], m[u][4], m[u][5], v0, v1, v2);
+                y[2] += block_row<TW>(m[u][6], m[u][7], m[u][8], v0, v1, v2);
             }
         staged_chunk<TW, L, U, NT>(Wl, lc, k0 + STEP, w, m, c);
     }
@@ -408,11 +497,16 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
     const int j = lane / RPW;                                        // lane phase
     const int p = sl * SLICE + row;
     const size_t p3 = 3 * (size_t)p;
+    // Loads in dependent rounds, as few as the data allow: (1) the slice's
+    // info and the row's width; (2) the cloth's parameters and state, its
+    // solver sums and the first chunk of the matrix (column indices and
+    // blocks; staged: the positions and the first staging round's rows);
+    // (3) the direction gathers.  A cloth solved alone is bound by these
+    // round trips, not by bandwidth.
+    const int4 si = glob(d.slice_info)[sl];
     const int w = glob(d.row_w)[p];
-    const size_t base = (size_t)glob(d.slice_off)[sl];
-    // staged variant: the first chunk of the solver's matrix is loaded
-    // before anything else, so that it is in flight during the prologue
-    // (state, solver sums) and the staging of the directions
+    const int inst = si.x;
+    const size_t base = (size_t)si.w;
     constexpr int U = 4 / L;
     T m0[U][9];
     int c0[U];
@@ -426,11 +520,17 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
         list = stage_list(d, sl);
         stage_round_rows<T, L>(glob(d.halo_row) + list.x, threadIdx.x, list.y, sb);
     }
-    const int inst = glob(d.slice_inst)[sl];
+    else
+        gathered_columns<L, U, NT>(glob(d.col) + base + row, j, w, c0);
     const InstParams ip = glob(d.params)[inst];
     const bool lead = sl == ip.slice_begin && threadIdx.x == 0;
     CgState st = load_state(glob(d.cg) + 2 * inst);
     G<CgState> *st_out = glob(d.cg) + 2 * inst + 1;
+    // the solver sums of the previous update (not used by the first launch
+    // of a solve, whose partials are stale but allocated)
+    double sums[3];
+    const int sum_comp[3] = {P_RZ, P_RR, P_BB};
+    solver_sums<3>(d, si, 0, sum_comp, sums);
     if (st.done)
     {
         if (lead)
@@ -447,9 +547,7 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
         residual = true; // r0 = b - W dx0 (BaraffCloth.cpp:515)
     else
     {
-        double v[3];
-        const int comp[3] = {P_RZ, P_RR, P_BB};
-        solver_sums<3>(d, ip, 0, comp, v);
+        const double *v = sums;
         const double rz = v[0], rr = v[1];
         if (st.pending)
         {
@@ -512,8 +610,8 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
                 x2 += (double)dx32[c3 + 2];
             }
             for (int a = 0; a < 3; a++)
-                y[a] += wk[(3 * a) * SLICE] * x0 + wk[(3 * a + 1) * SLICE] * x1 +
-                        wk[(3 * a + 2) * SLICE] * x2;
+                y[a] += block_row<double>(wk[(3 * a) * SLICE], wk[(3 * a + 1) * SLICE],
+                                          wk[(3 * a + 2) * SLICE], x0, x1, x2);
         }
         for (int a = 0; a < 3; a++)
             y[a] = phase_sum<L>(y[a]);
@@ -527,9 +625,9 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
                 const T zt = P[p3 + a] * rt;
                 r[p3 + a] = rt;
                 z[p3 + a] = zt;
-                v[0] += (double)rt * (double)zt;
-                v[1] += rv * rv;
-                v[2] += bv * bv;
+                v[0] = fma_t((double)rt, (double)zt, v[0]);
+                v[1] = fma_t(rv, rv, v[1]);
+                v[2] = fma_t(bv, bv, v[2]);
             }
         block_sum<L, 3>(v, lds);
         const int comp[3] = {P_RZ, P_RR, P_BB};
@@ -595,7 +693,12 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
     else
     {
         T y[3];
-        slice_product<T, T, L, NT>(W + 9 * base + row, cl, w, j, z, dold, bt, first, y);
+        // the staged variant's slices with lists too long for its LDS take
+        // the gathered product; their c0 holds list positions, not columns
+        if constexpr (S)
+            slice_product<T, T, L, NT>(W + 9 * base + row, cl, w, j, z, dold, bt, first, y);
+        else
+            slice_product_from<T, L, NT>(W + 9 * base + row, cl, w, j, z, dold, bt, first, c0, y);
         y0 = y[0];
         y1 = y[1];
         y2 = y[2];
@@ -612,7 +715,8 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
         q[p3] = y0;
         q[p3 + 1] = y1;
         q[p3 + 2] = y2;
-        v[0] = (double)dn[0] * (double)y0 + (double)dn[1] * (double)y1 + (double)dn[2] * (double)y2;
+        v[0] = block_row<double>((double)dn[0], (double)dn[1], (double)dn[2], (double)y0,
+                                 (double)y1, (double)y2);
     }
     block_sum<L, 1>(v, lds);
     const int comp[1] = {P_DQ};
@@ -621,14 +725,54 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
         store_state(st_out, st);
 }
 
+// The row vectors an update reads: loaded before the prologue by the
+// prefetching variant (PF: both direction buffers, since the state that
+// picks one is not known yet; 12 more bytes per row, for small grids that
+// latency bounds), else after it.
+template <typename T> struct UpdateRow
+{
+    T r[3], q[3], P[3], d0[3], d1[3];
+    float x32[3];
+    double x64[3];
+};
+
+template <typename T>
+__device__ inline void load_update_row(const DeviceBatch &d, size_t p3, const G<T> *P,
+                                       const G<T> *r, const G<T> *q, const G<T> *dbuf0,
+                                       const G<T> *dbuf1, int which, UpdateRow<T> &u)
+{
+    for (int a = 0; a < 3; a++)
+    {
+        u.r[a] = r[p3 + a];
+        u.q[a] = q[p3 + a];
+        u.P[a] = P[p3 + a];
+        if (which != 1)
+            u.d0[a] = dbuf0[p3 + a];
+        if (which != 0)
+            u.d1[a] = dbuf1[p3 + a];
+        if constexpr (sizeof(T) == sizeof(float))
+            u.x32[a] = glob(d.dx32)[p3 + a];
+        else
+            u.x64[a] = glob(d.dx)[p3 + a];
+    }
+}
+
 // k_cg_update on solver workgroup blk (SB slices) of batch d
-template <typename T, int SB>
+template <typename T, int SB, bool PF>
 __device__ inline void update_body(const DeviceBatch &d, const int blk,
                                    const G<T> *__restrict__ P, G<T> *__restrict__ r,
                                    G<T> *__restrict__ z, const G<T> *__restrict__ q,
                                    const G<T> *__restrict__ dbuf0, const G<T> *__restrict__ dbuf1)
 {
-    const int inst = glob(d.blk_inst)[blk];
+    // the SB slices of a workgroup belong to one cloth
+    const int4 si = glob(d.slice_info)[blk * SB];
+    const int inst = si.x;
+    const int lane = threadIdx.x & 63;
+    const int sl = blk * SB + (threadIdx.x >> 6);
+    const size_t p3 = 3 * (size_t)(sl * SLICE + lane);
+    UpdateRow<T> row;
+    if constexpr (PF)
+        load_update_row<T>(d, p3, P, r, q, dbuf0, dbuf1, 2, row);
     const InstParams ip = glob(d.params)[inst];
     const bool lead = blk == ip.blk_begin && threadIdx.x == 0;
     CgState st = load_state(glob(d.cg) + 2 * inst + 1);
@@ -636,6 +780,13 @@ __device__ inline void update_body(const DeviceBatch &d, const int blk,
     const G<double> *part_in = glob(d.cg_partial) + (size_t)PARTS * d.n_slices;
     G<double> *part_out = glob(d.cg_partial);
     const size_t part0 = (size_t)PARTS * SB * blk; // this workgroup's slices
+    // d.Wd of the preceding k_cg_spmv (loaded with the state; unused when
+    // the cloth is done or that launch formed a true residual)
+    double v1[1];
+    {
+        const int comp[1] = {P_DQ};
+        solver_sums<1>(d, si, 1, comp, v1);
+    }
     if (st.done)
     {
         if (lead)
@@ -663,11 +814,6 @@ __device__ inline void update_body(const DeviceBatch &d, const int blk,
             store_state(st_out, st);
         return;
     }
-    double v1[1];
-    {
-        const int comp[1] = {P_DQ};
-        solver_sums<1>(d, ip, 1, comp, v1);
-    }
     const double dq = v1[0];
     if (!(dq > 0.0) || !isfinite(dq))
     {
@@ -684,25 +830,27 @@ __device__ inline void update_body(const DeviceBatch &d, const int blk,
         return;
     }
     const double alpha = st.rho / dq;
-    const int lane = threadIdx.x & 63;
-    const int sl = blk * SB + (threadIdx.x >> 6);
-    const size_t p3 = 3 * (size_t)(sl * SLICE + lane);
-    const G<T> *dn = (st.iter & 1) ? dbuf1 : dbuf0;
+    const int which = (st.iter & 1) ? 1 : 0; // the direction buffer k_cg_spmv wrote
+    if constexpr (!PF)
+        load_update_row<T>(d, p3, P, r, q, dbuf0, dbuf1, which, row);
+    T dn[3];
+    for (int a = 0; a < 3; a++)
+        dn[a] = which ? row.d1[a] : row.d0[a];
     G<double> *dx = glob(d.dx);
     G<float> *dx32 = glob(d.dx32);
     double v[2] = {0.0, 0.0};
     for (int a = 0; a < 3; a++)
     {
         if constexpr (sizeof(T) == sizeof(float))
-            dx32[p3 + a] = dx32[p3 + a] + (float)(alpha * (double)dn[p3 + a]);
+            dx32[p3 + a] = row.x32[a] + (float)(alpha * (double)dn[a]);
         else
-            dx[p3 + a] = dx[p3 + a] + alpha * (double)dn[p3 + a];
-        const T rn = (T)((double)r[p3 + a] - alpha * (double)q[p3 + a]);
-        const T zn = P[p3 + a] * rn;
+            dx[p3 + a] = fma_t(alpha, (double)dn[a], row.x64[a]);
+        const T rn = (T)fma_t(-alpha, (double)row.q[a], (double)row.r[a]);
+        const T zn = row.P[a] * rn;
         r[p3 + a] = rn;
         z[p3 + a] = zn;
-        v[0] += (double)rn * (double)zn;
-        v[1] += (double)rn * (double)rn;
+        v[0] = fma_t((double)rn, (double)zn, v[0]);
+        v[1] = fma_t((double)rn, (double)rn, v[1]);
     }
     // one partial set per slice (wavefront)
     v[0] = wave_sum(v[0]);
@@ -730,7 +878,8 @@ k_cg_spmv(DeviceBatch d, const T *__restrict__ W, const T *__restrict__ P, T *__
           T *__restrict__ z, T *__restrict__ q, T *__restrict__ dbuf0, T *__restrict__ dbuf1)
 {
     __shared__ double lds[8 * L];
-    // one slice per workgroup, from the list of active slices when given
+    // one slice per workgroup, from the list of active slices when given,
+    // else from the contiguous range at slice_first
     int sl = solver_block(d.xcd_map);
     if (d.spmv_list)
     {
@@ -738,11 +887,13 @@ k_cg_spmv(DeviceBatch d, const T *__restrict__ W, const T *__restrict__ P, T *__
         if (sl < 0)
             return;
     }
+    else
+        sl += d.slice_first;
     spmv_body<T, L, S, NT>(d, sl, glob(W), glob(P), glob(r), glob(z), glob(q), glob(dbuf0),
                        glob(dbuf1), lds, reinterpret_cast<T *>(stage_lds));
 }
 
-template <typename T, int SB>
+template <typename T, int SB, bool PF>
 __global__ void __launch_bounds__(SB * 64)
 k_cg_update(DeviceBatch d, const T *__restrict__ P, T *__restrict__ r, T *__restrict__ z,
             const T *__restrict__ q, const T *__restrict__ dbuf0, const T *__restrict__ dbuf1)
@@ -754,7 +905,9 @@ k_cg_update(DeviceBatch d, const T *__restrict__ P, T *__restrict__ r, T *__rest
         if (blk < 0)
             return;
     }
-    update_body<T, SB>(d, blk, glob(P), glob(r), glob(z), glob(q), glob(dbuf0), glob(dbuf1));
+    else
+        blk += d.slice_first / SB;
+    update_body<T, SB, PF>(d, blk, glob(P), glob(r), glob(z), glob(q), glob(dbuf0), glob(dbuf1));
 }
 
 // group sums of the partials in `slot` (just written by the launch before):
@@ -801,7 +954,7 @@ k_cg_spmv_multi(const DeviceBatch *__restrict__ table, const int2 *__restrict__ 
                        glob(Vecs<T>::d1(d)), lds, reinterpret_cast<T *>(stage_lds));
 }
 
-template <typename T, int SB>
+template <typename T, int SB, bool PF>
 __global__ void __launch_bounds__(SB * 64)
 k_cg_update_multi(const DeviceBatch *__restrict__ table, const int2 *__restrict__ list,
                   int xcd_map)
@@ -810,7 +963,7 @@ k_cg_update_multi(const DeviceBatch *__restrict__ table, const int2 *__restrict_
     if (e.x < 0)
         return;
     const DeviceBatch &d = table[e.x];
-    update_body<T, SB>(d, e.y, glob(Vecs<T>::P(d)), glob(Vecs<T>::r(d)), glob(Vecs<T>::z(d)),
+    update_body<T, SB, PF>(d, e.y, glob(Vecs<T>::P(d)), glob(Vecs<T>::r(d)), glob(Vecs<T>::z(d)),
                        glob(Vecs<T>::q(d)), glob(Vecs<T>::d0(d)), glob(Vecs<T>::d1(d)));
 }
 
@@ -973,17 +1126,39 @@ void launch_cg_spmv_only(const DeviceBatch &d, bool fp32, hipStream_t s)
                                 d.d64[1], s);
 }
 
-void launch_cg_step(const DeviceBatch &d, bool fp32, int phase, int slices, hipStream_t s)
+// Grids of at most this many slices run the update kernel that loads its rows
+// before its prologue (latency over bytes; BW_PF_SLICES overrides)
+static int prefetch_slices()
+{
+    static const int v = [] {
+        const char *e = std::getenv("BW_PF_SLICES");
+        return e ? std::max(0, std::atoi(e)) : 2048;
+    }();
+    return v;
+}
+
+template <typename T>
+void launch_update(const DeviceBatch &d, int slices, const T *P, T *r, T *z, const T *q,
+                   const T *d0, const T *d1, hipStream_t s)
 {
     const dim3 upd_grid((slices + SB - 1) / SB), upd_block(SB * 64);
+    if (slices <= prefetch_slices())
+        hipLaunchKernelGGL((k_cg_update<T, SB, true>), upd_grid, upd_block, 0, s, d, P, r, z, q,
+                           d0, d1);
+    else
+        hipLaunchKernelGGL((k_cg_update<T, SB, false>), upd_grid, upd_block, 0, s, d, P, r, z, q,
+                           d0, d1);
+}
+
+void launch_cg_step(const DeviceBatch &d, bool fp32, int phase, int slices, hipStream_t s)
+{
     if (fp32)
     {
         if (phase == 0)
             launch_spmv_lpr<float>(d, slices, d.W32, d.P32, d.r32, d.z32, d.q32, d.d32[0], d.d32[1],
                                    s);
         else
-            hipLaunchKernelGGL((k_cg_update<float, SB>), upd_grid, upd_block, 0, s, d, d.P32,
-                               d.r32, d.z32, d.q32, d.d32[0], d.d32[1]);
+            launch_update<float>(d, slices, d.P32, d.r32, d.z32, d.q32, d.d32[0], d.d32[1], s);
     }
     else
     {
@@ -991,8 +1166,7 @@ void launch_cg_step(const DeviceBatch &d, bool fp32, int phase, int slices, hipS
             launch_spmv_lpr<double>(d, slices, d.W64, d.P64, d.r64, d.z64, d.q64, d.d64[0],
                                     d.d64[1], s);
         else
-            hipLaunchKernelGGL((k_cg_update<double, SB>), upd_grid, upd_block, 0, s, d, d.P64,
-                               d.r64, d.z64, d.q64, d.d64[0], d.d64[1]);
+            launch_update<double>(d, slices, d.P64, d.r64, d.z64, d.q64, d.d64[0], d.d64[1], s);
     }
     if (d.n_groups > 0)
         hipLaunchKernelGGL(k_cg_group, dim3(d.n_groups), dim3(64), 0, s, d, phase == 0 ? 1 : 0);
@@ -1007,17 +1181,23 @@ void launch_cg_step_multi(const DeviceBatch *table, const int2 *spmv_list, int s
     {
         if (phase == 0)
             launch_spmv_multi<float>(table, spmv_list, spmv_grid, lpr, halo_max, nt, xcd_map, s);
+        else if (upd_grid * SB <= prefetch_slices())
+            hipLaunchKernelGGL((k_cg_update_multi<float, SB, true>), dim3(upd_grid), dim3(SB * 64),
+                               0, s, table, upd_list, xcd_map);
         else
-            hipLaunchKernelGGL((k_cg_update_multi<float, SB>), dim3(upd_grid), dim3(SB * 64), 0, s,
-                               table, upd_list, xcd_map);
+            hipLaunchKernelGGL((k_cg_update_multi<float, SB, false>), dim3(upd_grid),
+                               dim3(SB * 64), 0, s, table, upd_list, xcd_map);
     }
     else
     {
         if (phase == 0)
             launch_spmv_multi<double>(table, spmv_list, spmv_grid, lpr, halo_max, nt, xcd_map, s);
+        else if (upd_grid * SB <= prefetch_slices())
+            hipLaunchKernelGGL((k_cg_update_multi<double, SB, true>), dim3(upd_grid),
+                               dim3(SB * 64), 0, s, table, upd_list, xcd_map);
         else
-            hipLaunchKernelGGL((k_cg_update_multi<double, SB>), dim3(upd_grid), dim3(SB * 64), 0,
-                               s, table, upd_list, xcd_map);
+            hipLaunchKernelGGL((k_cg_update_multi<double, SB, false>), dim3(upd_grid),
+                               dim3(SB * 64), 0, s, table, upd_list, xcd_map);
     }
     if (grp_grid > 0)
         hipLaunchKernelGGL(k_cg_group_multi, dim3(grp_grid), dim3(64), 0, s, table, grp_list,

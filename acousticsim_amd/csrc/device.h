@@ -187,8 +187,14 @@ struct DeviceBatch
     float *gf_target = nullptr, *gf_mass = nullptr, *gf_grav = nullptr;
     GfState *gf_state = nullptr;
     // optional lists of the slices (k_cg_spmv) and solver workgroups
-    // (k_cg_update) a launch runs; null: all of them
+    // (k_cg_update) a launch runs; null: the contiguous range of slices from
+    // slice_first (a multiple of BW_SB) that the grid covers
     int *spmv_list = nullptr, *upd_list = nullptr;
+    int slice_first = 0;
+    // per slice, what the solver kernels need before anything else, in one
+    // load: {cloth, first partial set of the cloth's solver sums, their
+    // count (negative: -count group sums), the slice's first ELL entry}
+    int4 *slice_info = nullptr;
 };
 
 // Workgroup b of a G-workgroup launch runs on XCD b % 8 (the dispatcher deals

@@ -61,22 +61,25 @@ __device__ inline void write_partials(PP part, int blk, const int (&comp)[N], co
 
 // Sum of components comp[] over partial sets b0 .. b0+nb-1, computed by
 // every wavefront on its own (identical in all of them, whatever the
-// workgroup size): lane l adds sets l, l+64, l+128, ... in that order, four
-// loads in flight, then the wavefront butterfly.
+// workgroup size): lane l adds sets l, l+64, l+128, ... in that order, then
+// the wavefront butterfly.  F sets per lane are loaded at a time (8 for one
+// component: a cloth of up to 512 slices sums in one memory round trip; 4
+// for three, to keep the registers of a kernel's prologue few).
 template <int N, typename PP>
 __device__ inline void wave_sets_sum(PP sets, int b0, int nb, const int (&comp)[N], double (&out)[N])
 {
+    constexpr int F = N >= 3 ? 4 : 8;
     const int lane = threadIdx.x & 63;
     for (int j = 0; j < N; j++)
         out[j] = 0.0;
     int i = lane;
-    for (; i + 192 < nb; i += 256)
+    for (; i + 64 * (F - 1) < nb; i += 64 * F)
     {
-        double a[4][N];
-        for (int u = 0; u < 4; u++)
+        double a[F][N];
+        for (int u = 0; u < F; u++)
             for (int j = 0; j < N; j++)
                 a[u][j] = sets[PARTS * (size_t)(b0 + i + 64 * u) + comp[j]];
-        for (int u = 0; u < 4; u++)
+        for (int u = 0; u < F; u++)
             for (int j = 0; j < N; j++)
                 out[j] += a[u][j];
     }

@@ -17,8 +17,15 @@ Reported beside it, in the same run:
     and the d.q partial fused in), timed with HIP events on the library's
     stream: algorithmic bytes per launch / mean launch time vs the 8 TB/s HBM
     peak
-  * the CPU baseline: the C restatement (oracle) of the reference's plain
-    method, one thread, on a bounded sample of the same cloths
+  * the CPU baseline: the reference's own CPU step (cBaraffCloth::UpdatePos,
+    compiled from its sources -O3 -DNDEBUG into oracle/_ref/ref_cloth), one
+    thread, on a bounded sample of the same cloths, each stepped from the
+    state the GPU reached right after the timed steps and running the PCG
+    iteration count the GPU needed there (the reference's solve runs a fixed
+    count); without that binary, the oracle (the C restatement) in tolerance
+    mode
+  * the per-result error: the GPU's step from that state against the oracle
+    solved to 1e-10 from the same state, over the same sample
 
 Workload "irregular": one cloth of 1.5M seeded points (1 mm minimum
 spacing) in a 2 m x 2 m square, Delaunay-triangulated, top side pinned,
@@ -63,15 +70,28 @@ def kernel_name(lanes, staged, nt):
                      "nontemporal" if nt else "cached")
 
 
-# HBM bytes per launch of the k_cg_spmv variant (lanes per row, staged,
-# nontemporal) the autotuner picks, from rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE over the --roofline-only command, scaled by the same counters'
-# reading for a 256 MiB device-to-device copy (profiles/k_cg_spmv_<workload>.md;
-# the gathered 1- and 2-lane variants time within 1-3% of each other on both
-# workloads, so either may be picked; the counter passes abort inside the
-# profiler on the irregular workload, so it has none)
-TRAFFIC = {"batch64": {(1, False, False): 210387018.0, (2, False, False): 209697107.0},
-           "single257": {(1, False, False): 33077660.0, (2, False, False): 32754087.0}}
+# HBM bytes per launch of a k_cg_spmv variant (lanes per row, staged,
+# nontemporal) on a workload, measured by rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE over the --roofline-only command with that variant fixed, scaled
+# by the same counters' reading for a 256 MiB device-to-device copy; each
+# entry names the profile it comes from.  A bench line reports traffic only
+# when the autotuner picked a variant that has a measured entry here, and
+# says which profile the figure comes from.
+TRAFFIC = {
+    "batch64": {(1, False, False): (210387018.0, "profiles/k_cg_spmv_batch64.md"),
+                (2, False, False): (209697107.0, "profiles/k_cg_spmv_batch64.md")},
+    "single257": {(1, False, False): (33077660.0, "profiles/k_cg_spmv_single257.md"),
+                  (2, False, False): (32754087.0, "profiles/k_cg_spmv_single257.md")},
+}
+
+
+def traffic_for(workload, tuning):
+    """(bytes, source) of the picked variant's counter profile, or (None, why)"""
+    key = (int(tuning[0]), bool(tuning[1]), bool(tuning[2]))
+    hit = TRAFFIC.get(workload, {}).get(key)
+    if hit is None:
+        return None, "no counter profile of the picked variant %s on this workload" % (key,)
+    return hit[0], "%s (variant lanes=%d staged=%s nontemporal=%s)" % (hit[1], *key)
 
 
 def members(workload, precision):
@@ -121,6 +141,8 @@ def time_batch(ms, steps, warmup, graphs=True):
     lat = []
     iters = []
     worst = 0.0
+    per_cloth_iters = np.zeros(b.n)
+    critical = []
     t0 = time.perf_counter()
     for _ in range(steps):
         s0 = time.perf_counter()
@@ -128,13 +150,43 @@ def time_batch(ms, steps, warmup, graphs=True):
         lat.append(time.perf_counter() - s0)
         st = [b.stats(i) for i in range(b.n)]
         iters.append(np.mean([s["pcg_iterations"] for s in st]))
+        critical.append(max(s["pcg_iterations"] + s["outer_iterations"] for s in st))
+        per_cloth_iters += [s["pcg_iterations"] for s in st]
         worst = max(worst, max(s["rel_residual"] for s in st))
     total = time.perf_counter() - t0
+    b.per_cloth_iters = per_cloth_iters
+    b.critical = np.array(critical, dtype=np.float64)
+    b.final_pos = [b.get_pos(i) for i in range(b.n)]
     return b, total, np.array(lat), float(np.mean(iters)), worst
 
 
+def step_bytes(b, fp32=True):
+    """algorithmic bytes of the timed steps' PCG iterations, per iteration of
+    a cloth: its SpMV (matrix block + column index per true block; z, d_prev
+    read, d, q written per vertex) and update (d, q, r, P read, dx32 read and
+    written, r, z written): 40 nnzb + 144 V in the mixed form"""
+    total = 0.0
+    for i in range(b.n):
+        total += b.per_cloth_iters[i] * (40.0 * b.num_blocks(i) + 144.0 * b.nv[i])
+    return total
+
+
+def sample_state(b):
+    """two untimed steps after the timed ones: the state (x, x_pre) before
+    the second and the GPU's result and PCG iteration count of it"""
+    b.step(1)
+    x_pre = [b.get_pos(i) for i in range(b.n)]
+    b.step(1)
+    x = [b.get_pos(i) for i in range(b.n)]
+    b.step(1)
+    x_new = [b.get_pos(i) for i in range(b.n)]
+    iters = [b.stats(i)["pcg_iterations"] for i in range(b.n)]
+    return x, x_pre, x_new, iters
+
+
 def time_plain(ms, steps, warmup):
-    """Every cloth on its own, FP64, direct launches."""
+    """Every cloth on its own, FP64, direct launches.  Returns the time, the
+    worst residual and the final positions."""
     objs = []
     for c, x in ms:
         c = dict(c, precision="fp64")
@@ -150,44 +202,136 @@ def time_plain(ms, steps, warmup):
         for b in objs:
             b.step(1)
             worst = max(worst, b.stats(0)["rel_residual"])
-    return time.perf_counter() - t0, worst
+    t = time.perf_counter() - t0
+    return t, worst, [b.get_pos(0) for b in objs]
 
 
-def cpu_baseline(ms, budget_s=20.0):
-    """The oracle (C restatement of the reference's plain method), one thread,
-    tolerance 1e-6, on every 4th cloth: one warm-up step and one timed step
-    each, stopped early if the budget runs out."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle_ffi as of
+REF_BIN = os.path.join(ROOT, "oracle", "_ref", "ref_cloth")
 
-    sample = ms[::4] if len(ms) > 4 else ms
-    t_steps, n_steps = 0.0, 0
-    start = time.perf_counter()
-    for c, x in sample:
-        oc = of.OracleCloth(c)
-        oc.set_state(x, x)
-        oc.step(of.SOLVE_TOLERANCE, 1e-6)
-        t0 = time.perf_counter()
-        oc.step(of.SOLVE_TOLERANCE, 1e-6)
-        t_steps += time.perf_counter() - t0
-        n_steps += 1
-        if time.perf_counter() - start > budget_s:
-            break
+
+def host_cpu():
     cpu = platform.processor() or platform.machine()
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
                 if line.startswith("model name"):
-                    cpu = line.split(":", 1)[1].strip()
-                    break
+                    return line.split(":", 1)[1].strip()
     except OSError:
         pass
-    return {"value": n_steps / t_steps, "unit": "cloth steps/s", "cores": 1, "kind": "port",
-            "sample": "%d of %d cloths (every 4th), 1 warm-up + 1 timed step each, tol 1e-6, "
-                      "gcc -O2, host %s" % (n_steps, len(ms), cpu)}
+    return cpu
 
 
-def cpu_baseline_extrapolated(ms, gpu_iters):
+def reference_step_seconds(conf, x, x_pre, iterations):
+    """the reference's own cBaraffCloth::UpdatePos (oracle/_ref/ref_cloth) for
+    one step from (x, x_pre) with `iterations` PCG iterations (its solve runs
+    max_iters - 1); returns (seconds of the step alone, positions after it)"""
+    import subprocess
+    import tempfile
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from refdump import read_dump
+
+    c = dict(conf, max_iters=int(iterations) + 1)
+    with tempfile.TemporaryDirectory() as td:
+        paths = [os.path.join(td, n) for n in ("c.json", "x.bin", "xp.bin", "o.bin")]
+        with open(paths[0], "w") as f:
+            json.dump(c, f)
+        np.asarray(x, dtype="<f8").tofile(paths[1])
+        np.asarray(x_pre, dtype="<f8").tofile(paths[2])
+        subprocess.run([REF_BIN, paths[0], paths[1], "1", paths[3], "--xpre", paths[2], "--time"],
+                       check=True)
+        d = read_dump(paths[3])
+    return float(d["step_seconds"][0]), d["x_step1"]
+
+
+def reference_step_extrapolated(conf, x, x_pre, iterations):
+    """For a cloth too large for the budget: the reference's step time with
+    `iterations` PCG iterations, as a + iterations * b from two timed steps
+    of 10 and 50 iterations from the same state"""
+    t10, _ = reference_step_seconds(conf, x, x_pre, 10)
+    t50, _ = reference_step_seconds(conf, x, x_pre, 50)
+    per = (t50 - t10) / 40
+    return (t10 - 10 * per) + iterations * per, per
+
+
+def cpu_baseline(ms, state, budget_s=20.0):
+    """The CPU baseline, one thread, on every 4th cloth until the budget is
+    spent, each stepped once from the GPU's state after the timed steps:
+    the reference's compiled CPU step when oracle/_ref/ref_cloth exists (with
+    the GPU's iteration count), else the oracle in tolerance mode (1e-6).
+    Returns (cpu_baseline, per-result error): the error is the GPU's step
+    from that state against the oracle solved to 1e-10 from it, max over the
+    sample of max |x_gpu - x_oracle| / max |x_oracle - x|."""
+    import concurrent.futures as cf
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_ffi as of
+
+    x, x_pre, x_new, iters = state
+    use_ref = os.path.exists(REF_BIN)
+    order = list(range(0, len(ms), 4)) if len(ms) > 4 else list(range(len(ms)))
+    t_steps, done, ref_out = 0.0, [], {}
+    start = time.perf_counter()
+    extrapolated = None
+    for i in order:
+        c = ms[i][0]
+        if use_ref and len(ms) == 1 and len(x[i]) // 3 * iters[i] > 2e7:
+            # one large cloth: about 0.5 us per vertex-iteration would take
+            # minutes; the step time is extrapolated from 10 and 50 iterations
+            dt, extrapolated = reference_step_extrapolated(c, x[i], x_pre[i], iters[i])
+        elif use_ref:
+            dt, xr = reference_step_seconds(c, x[i], x_pre[i], iters[i])
+            ref_out[i] = xr
+        else:
+            oc = of.OracleCloth(c)
+            oc.set_state(x[i], x_pre[i])
+            t0 = time.perf_counter()
+            oc.step(of.SOLVE_TOLERANCE, 1e-6)
+            dt = time.perf_counter() - t0
+        t_steps += dt
+        done.append(i)
+        if time.perf_counter() - start > budget_s:
+            break
+    if use_ref:
+        base = {"value": len(done) / t_steps, "unit": "cloth steps/s", "cores": 1,
+                "kind": "reference",
+                "sample": "%d of %d cloths (every 4th until %.0f s), one step each from the "
+                          "GPU's state after the timed steps, with the GPU's PCG iteration count "
+                          "there (%d..%d); the reference's cBaraffCloth::UpdatePos compiled from "
+                          "its sources, g++ -O3 -DNDEBUG, serial; host %s"
+                          % (len(done), len(ms), budget_s, min(iters[i] for i in done),
+                             max(iters[i] for i in done), host_cpu())}
+        if extrapolated is not None:
+            base["sample"] += ("; extrapolated from steps of 10 and 50 iterations from the same "
+                               "state (%.3f s per iteration)" % extrapolated)
+    else:
+        base = {"value": len(done) / t_steps, "unit": "cloth steps/s", "cores": 1, "kind": "port",
+                "sample": "%d of %d cloths (every 4th until %.0f s), one step each from the "
+                          "GPU's state after the timed steps, tol 1e-6; the oracle (C restatement "
+                          "of the reference's plain method), gcc -O2, serial; host %s"
+                          % (len(done), len(ms), budget_s, host_cpu())}
+
+    def check(i):
+        oc = of.OracleCloth(ms[i][0])
+        oc.set_state(x[i], x_pre[i])
+        oc.step(of.SOLVE_TOLERANCE, 1e-10)
+        xt = oc.pos()
+        step = np.abs(xt - x[i]).max()
+        e_gpu = np.abs(x_new[i] - xt).max() / step
+        e_ref = np.abs(ref_out[i] - xt).max() / step if i in ref_out else None
+        return e_gpu, e_ref
+
+    with cf.ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 4)) as ex:
+        errs = list(ex.map(check, done))
+    err = {"gpu_max_err_over_step": max(e for e, _ in errs),
+           "sample": "the CPU baseline's %d cloths: the GPU's step from the state after the "
+                     "timed steps vs the oracle solved to 1e-10 from the same state" % len(done)}
+    if ref_out:
+        err["reference_max_err_over_step"] = max(r for _, r in errs)
+    return base, err
+
+
+def cpu_baseline_extrapolated(ms, gpu_iters):  # noqa: C901
     """For a cloth too large to step on the CPU within the budget: the
     oracle's time for a step with k PCG iterations is assembly + k * per
     iteration; both are measured on the same mesh (steps stopped after 6 and
@@ -211,7 +355,35 @@ def cpu_baseline_extrapolated(ms, gpu_iters):
     return {"value": 1.0 / step_s, "unit": "cloth steps/s", "cores": 1, "kind": "port",
             "sample": "extrapolated: oracle assembly %.2f s + %.1f PCG iterations (the GPU's "
                       "mean per step) x %.3f s per iteration, both timed on the same 1.5M-point "
-                      "mesh, gcc -O2, 1 thread" % (base, gpu_iters, per_iter)}
+                      "mesh; the oracle (C restatement), gcc -O2, serial; the reference itself "
+                      "cannot build this mesh (its Delaunay generator is O(V^2)); host %s"
+                      % (base, gpu_iters, per_iter, host_cpu())}
+
+
+def trajectory_error(step, x_mixed, x_fp64):
+    """The 257 x 257 cloth after `step` steps from x0 against the oracle's own
+    FP64 trajectory (tests/golden/grid257_tol200.npz, a strided vertex
+    subset), when the fixture holds that step: max deviation over the
+    fixture's max displacement and the relative L2 deviation, for the mixed
+    and the FP64 GPU form, beside the fixture's drift between two oracle
+    trajectories started 1e-18 m apart (chaos_step)."""
+    path = os.path.join(ROOT, "tests", "golden", "grid257_tol200.npz")
+    if not os.path.exists(path):
+        return {"note": "fixture grid257_tol200.npz not present"}
+    g = np.load(path)
+    if "x_step%d" % step not in g:
+        return {"note": "the fixture keeps steps %s, not %d" % (list(g["steps"]), step)}
+    sel = g["vertex_subset"]
+    xr = g["x_step%d" % step]
+    disp = np.abs(xr - g["x0_subset"]).max()
+    out = {"step": step, "chaos_step": float(g["chaos_step%d" % step])}
+    for name, x in (("mixed", x_mixed), ("fp64", x_fp64)):
+        if x is None:
+            continue
+        xs = x.reshape(-1, 3)[sel].reshape(-1)
+        out[name] = {"max_err_over_displacement": float(np.abs(xs - xr).max() / disp),
+                     "rel_l2": float(np.linalg.norm(xs - xr) / np.linalg.norm(xr))}
+    return out
 
 
 def roofline_only(args, ms):
@@ -406,8 +578,11 @@ def main():
     b, total, lat, iters, worst = time_batch(ms, args.steps, args.warmup)
     n = len(ms)
     value = n * args.steps / total
+    whole_step_gbs = step_bytes(b) / total / 1e9
+    state = None if args.no_cpu or args.workload == "irregular" else sample_state(b)
     us, nbytes = b.time_solver_kernel(reps=200)
     achieved = nbytes / us / 1e3  # GB/s
+    traffic, traffic_src = traffic_for(args.workload, b.tuning)
     out = {
         "metric": METRIC,
         "value": value,
@@ -425,30 +600,44 @@ def main():
         "p50_ms": 1000 * float(np.percentile(lat, 50)),
         "p99_ms": 1000 * float(np.percentile(lat, 99)),
         "pcg_iterations_mean": iters,
+        # the slowest cloth's iterations (and true-residual checks) per step:
+        # a step lasts as long as that cloth's chain of launches
+        "critical_iterations_mean": float(np.mean(b.critical)),
+        "us_per_critical_iteration": 1000.0 * total / float(np.sum(b.critical)),
         "autotune": {"spmv_lanes_per_row": b.tuning[0], "spmv_staged": b.tuning[1],
                      "spmv_nontemporal": b.tuning[2],
                      "spmv_us": {f: {str(k): float(v) for k, v in t.items()}
                                  for f, t in b.tuning[3].items()}},
         "max_rel_residual": worst,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": TRAFFIC.get(args.workload, {}).get(tuple(b.tuning[:3])),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "kernel": kernel_name(*b.tuning[:3]),
                      "launch_us": us, "alg_bytes": nbytes},
+        "whole_step": {"achieved": whole_step_gbs, "unit": "GB/s",
+                       "frac": whole_step_gbs / HBM_PEAK_GBS,
+                       "bytes": "per PCG iteration of a cloth 40 nnzb + 144 V (SpMV and update "
+                                "launches), times its iterations over the timed steps, over "
+                                "the timed wall time (assembly and host gaps included in the "
+                                "time, not in the bytes)"},
     }
     b.close()
+    x_plain = None
     if not args.no_plain:
         print("bench: plain FP64 form", file=sys.stderr, flush=True)
-        tp, worst_p = time_plain(ms, args.steps, args.warmup)
+        tp, worst_p, x_plain = time_plain(ms, args.steps, args.warmup)
         plain = n * args.steps / tp
         out["plain_fp64"] = {"value": plain, "unit": "cloth steps/s",
                              "max_rel_residual": worst_p, "speedup": value / plain}
+    if args.workload == "single257":
+        out["trajectory_error"] = trajectory_error(args.warmup + args.steps, b.final_pos[0],
+                                                   x_plain[0] if x_plain else None)
     if not args.no_cpu:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
         if args.workload == "irregular":
             out["cpu_baseline"] = cpu_baseline_extrapolated(ms, iters)
         else:
-            out["cpu_baseline"] = cpu_baseline(ms)
+            out["cpu_baseline"], out["per_result_error"] = cpu_baseline(ms, state)
     print(json.dumps(out), flush=True)
 
 

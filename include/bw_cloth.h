@@ -248,6 +248,10 @@ int bw_batch_size(const bw_batch *b);
 /* nsteps substeps of every member */
 int bw_batch_step(bw_batch *b, int nsteps);
 int bw_batch_num_vertices(const bw_batch *b, int i);
+/* true 3x3 blocks of member i's matrix (the ELL column structure's entries,
+   bw_get_ell_map's count), for bandwidth accounting; no reference
+   counterpart */
+int bw_batch_num_blocks(const bw_batch *b, int i);
 int bw_batch_num_triangles(const bw_batch *b, int i);
 int bw_batch_get_pos(const bw_batch *b, int i, double *x);
 int bw_batch_set_state(bw_batch *b, int i, const double *x, const double *xpre);

@@ -129,7 +129,11 @@ def test_bad_requests_report_errors():
     bad_type = dict(c, cloth_type="pbd")
     bad_prec = dict(c, precision="fp64")
     bad_mode = dict(c, solver_mode="reference")
-    ids = [s.submit(bad_type, None, 1), s.submit(bad_prec, None, 1), s.submit(bad_mode, None, 1)]
+    # a grid whose vertex count would overflow: refused while the request's
+    # configuration is read, the builder thread and the server unharmed
+    bad_size = dict(c, subdivision=[65536, 65536])
+    ids = [s.submit(bad_type, None, 1), s.submit(bad_prec, None, 1), s.submit(bad_mode, None, 1),
+           s.submit(bad_size, None, 1)]
     good = s.submit(c, None, 1)
     for rid in ids:
         assert s.wait(rid, 60.0)
