@@ -283,15 +283,16 @@ __device__ inline void gathered_columns(const G<int> *__restrict__ cl, int j, in
     }
 }
 
-// slice_product with the column indices of its first chunk already loaded;
-// the same loads otherwise and the same arithmetic in the same order
-template <typename T, int L, bool NT>
+// slice_product with the column indices of its first chunk already loaded,
+// in chunks of U slots per lane: the same arithmetic in the same order for
+// any U (a lane adds its slots j, j + L, ... in turn), so U trades registers
+// for memory round trips (a row of w slots waits on ceil(w / (U L)) of them)
+template <typename T, int L, int U, bool NT>
 __device__ inline void slice_product_from(const G<T> *__restrict__ Wl, const G<int> *__restrict__ cl,
                                           int w, int j, const G<T> *__restrict__ z,
                                           const G<T> *__restrict__ dold, T bt, bool first,
-                                          int (&c)[4 / L], T (&y)[3])
+                                          int (&c)[U], T (&y)[3])
 {
-    constexpr int U = 4 / L;
     constexpr int STEP = U * L;
     y[0] = y[1] = y[2] = 0;
     for (int k0 = j; k0 < w; k0 += STEP)
@@ -483,8 +484,9 @@ __device__ inline void staged_product_from(const DeviceBatch &d, int sl,
         y[a] = phase_sum<L>(y[a]);
 }
 
-// k_cg_spmv on slice sl of batch d; S: the staged variant (vs: its LDS)
-template <typename T, int L, bool S, bool NT>
+// k_cg_spmv on slice sl of batch d; S: the staged variant (vs: its LDS);
+// UG: slots per lane and chunk of the gathered product
+template <typename T, int L, bool S, bool NT, int UG = 4 / L>
 __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> *__restrict__ W,
                                  const G<T> *__restrict__ P, G<T> *__restrict__ r,
                                  G<T> *__restrict__ z, G<T> *__restrict__ q,
@@ -507,7 +509,7 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
     const int w = glob(d.row_w)[p];
     const int inst = si.x;
     const size_t base = (size_t)si.w;
-    constexpr int U = 4 / L;
+    constexpr int U = S ? 4 / L : UG;
     T m0[U][9];
     int c0[U];
     StageRound<T, L> sb;
@@ -528,9 +530,10 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
     G<CgState> *st_out = glob(d.cg) + 2 * inst + 1;
     // the solver sums of the previous update (not used by the first launch
     // of a solve, whose partials are stale but allocated)
-    double sums[3];
-    const int sum_comp[3] = {P_RZ, P_RR, P_BB};
-    solver_sums<3>(d, si, 0, sum_comp, sums);
+    // (b.b only after a true residual, below)
+    double sums[2];
+    const int sum_comp[2] = {P_RZ, P_RR};
+    solver_sums<2>(d, si, 0, sum_comp, sums);
     if (st.done)
     {
         if (lead)
@@ -552,10 +555,13 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
         if (st.pending)
         {
             // the previous k_cg_spmv computed the true residual
+            double bb[1];
+            const int bb_comp[1] = {P_BB};
+            solver_sums<1>(d, si, 0, bb_comp, bb);
             st.pending = 0;
             st.rr_true = rr;
-            st.bb = v[2];
-            if (rr <= tol2 * v[2] || st.final_check || st.total >= ip.max_iter || !isfinite(rr))
+            st.bb = bb[0];
+            if (rr <= tol2 * bb[0] || st.final_check || st.total >= ip.max_iter || !isfinite(rr))
             {
                 st.done = 1;
                 if (lead)
@@ -698,7 +704,8 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
         if constexpr (S)
             slice_product<T, T, L, NT>(W + 9 * base + row, cl, w, j, z, dold, bt, first, y);
         else
-            slice_product_from<T, L, NT>(W + 9 * base + row, cl, w, j, z, dold, bt, first, c0, y);
+            slice_product_from<T, L, U, NT>(W + 9 * base + row, cl, w, j, z, dold, bt, first, c0,
+                                            y);
         y0 = y[0];
         y1 = y[1];
         y2 = y[2];
@@ -872,7 +879,7 @@ __device__ inline void update_body(const DeviceBatch &d, const int blk,
 // sized at launch
 extern __shared__ __align__(16) unsigned char stage_lds[];
 
-template <typename T, int L, bool S, bool NT>
+template <typename T, int L, bool S, bool NT, int UG = 4 / L>
 __global__ void __launch_bounds__(L * 64)
 k_cg_spmv(DeviceBatch d, const T *__restrict__ W, const T *__restrict__ P, T *__restrict__ r,
           T *__restrict__ z, T *__restrict__ q, T *__restrict__ dbuf0, T *__restrict__ dbuf1)
@@ -889,8 +896,8 @@ k_cg_spmv(DeviceBatch d, const T *__restrict__ W, const T *__restrict__ P, T *__
     }
     else
         sl += d.slice_first;
-    spmv_body<T, L, S, NT>(d, sl, glob(W), glob(P), glob(r), glob(z), glob(q), glob(dbuf0),
-                       glob(dbuf1), lds, reinterpret_cast<T *>(stage_lds));
+    spmv_body<T, L, S, NT, UG>(d, sl, glob(W), glob(P), glob(r), glob(z), glob(q), glob(dbuf0),
+                               glob(dbuf1), lds, reinterpret_cast<T *>(stage_lds));
 }
 
 template <typename T, int SB, bool PF>
@@ -1046,11 +1053,29 @@ void launch_cg_reset(const DeviceBatch &d, hipStream_t s)
 // bytes of LDS the staged variant needs for lists of up to halo_max entries
 template <typename T> size_t stage_bytes(int halo_max) { return 4 * sizeof(T) * (size_t)halo_max; }
 
+// Launches of at most this many slices (two wavefronts per SIMD with 1 lane
+// per row) run the gathered product in chunks of 8 / L slots per lane
+// instead of 4 / L: more registers, which such a grid does not need for
+// occupancy, and half the dependent gather rounds per row, which bound it.
+// Same arithmetic in the same order.  FP64 launches of up to half as many
+// (that variant holds one wavefront per SIMD).  BW_ILP_SLICES overrides.
+static int ilp_slices()
+{
+    static const int v = [] {
+        const char *e = std::getenv("BW_ILP_SLICES");
+        return e ? std::max(0, std::atoi(e)) : 2048;
+    }();
+    return v;
+}
+
 template <typename T, int L, bool NT>
 void launch_spmv_nt(const DeviceBatch &d, int slices, const T *W, const T *P, T *r, T *z, T *q,
                     T *d0, T *d1, hipStream_t s)
 {
-    if (d.spmv_stage)
+    if (!d.spmv_stage && slices <= (sizeof(T) == sizeof(float) ? ilp_slices() : ilp_slices() / 2))
+        hipLaunchKernelGGL((k_cg_spmv<T, L, false, NT, 8 / L>), dim3(slices), dim3(L * 64), 0, s,
+                           d, W, P, r, z, q, d0, d1);
+    else if (d.spmv_stage)
         hipLaunchKernelGGL((k_cg_spmv<T, L, true, NT>), dim3(slices), dim3(L * 64),
                            stage_bytes<T>(d.halo_lds), s, d, W, P, r, z, q, d0, d1);
     else

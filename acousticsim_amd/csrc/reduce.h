@@ -62,30 +62,50 @@ __device__ inline void write_partials(PP part, int blk, const int (&comp)[N], co
 // Sum of components comp[] over partial sets b0 .. b0+nb-1, computed by
 // every wavefront on its own (identical in all of them, whatever the
 // workgroup size): lane l adds sets l, l+64, l+128, ... in that order, then
-// the wavefront butterfly.  F sets per lane are loaded at a time (8 for one
-// component: a cloth of up to 512 slices sums in one memory round trip; 4
-// for three, to keep the registers of a kernel's prologue few).
+// the wavefront butterfly.  Up to 64 * ONE sets (ONE = 17 per lane for one
+// or two components: a cloth of 1,088 slices, 263 x 263 vertices) all of a
+// lane's loads go out at once, so the sum costs one memory round trip; more
+// sets go F at a time per lane (4 for three components, 8 for fewer), to
+// bound the registers.  Both paths add in the same order.
 template <int N, typename PP>
 __device__ inline void wave_sets_sum(PP sets, int b0, int nb, const int (&comp)[N], double (&out)[N])
 {
     constexpr int F = N >= 3 ? 4 : 8;
+    constexpr int ONE = N >= 3 ? 8 : 17;
     const int lane = threadIdx.x & 63;
     for (int j = 0; j < N; j++)
         out[j] = 0.0;
-    int i = lane;
-    for (; i + 64 * (F - 1) < nb; i += 64 * F)
+    if (nb <= 64 * ONE)
     {
-        double a[F][N];
-        for (int u = 0; u < F; u++)
-            for (int j = 0; j < N; j++)
-                a[u][j] = sets[PARTS * (size_t)(b0 + i + 64 * u) + comp[j]];
-        for (int u = 0; u < F; u++)
-            for (int j = 0; j < N; j++)
-                out[j] += a[u][j];
+        double a[ONE][N];
+#pragma unroll
+        for (int u = 0; u < ONE; u++)
+            if (lane + 64 * u < nb)
+                for (int j = 0; j < N; j++)
+                    a[u][j] = sets[PARTS * (size_t)(b0 + lane + 64 * u) + comp[j]];
+#pragma unroll
+        for (int u = 0; u < ONE; u++)
+            if (lane + 64 * u < nb)
+                for (int j = 0; j < N; j++)
+                    out[j] += a[u][j];
     }
-    for (; i < nb; i += 64)
-        for (int j = 0; j < N; j++)
-            out[j] += sets[PARTS * (size_t)(b0 + i) + comp[j]];
+    else
+    {
+        int i = lane;
+        for (; i + 64 * (F - 1) < nb; i += 64 * F)
+        {
+            double a[F][N];
+            for (int u = 0; u < F; u++)
+                for (int j = 0; j < N; j++)
+                    a[u][j] = sets[PARTS * (size_t)(b0 + i + 64 * u) + comp[j]];
+            for (int u = 0; u < F; u++)
+                for (int j = 0; j < N; j++)
+                    out[j] += a[u][j];
+        }
+        for (; i < nb; i += 64)
+            for (int j = 0; j < N; j++)
+                out[j] += sets[PARTS * (size_t)(b0 + i) + comp[j]];
+    }
     for (int j = 0; j < N; j++)
         out[j] = wave_sum(out[j]);
 }

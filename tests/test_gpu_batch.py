@@ -87,9 +87,10 @@ def test_autotuner_variants_agree(monkeypatch):
     """The autotuner times the 1-, 2- and 4-lane SpMV variants, each with the
     directions gathered per slot or staged in LDS and the matrix loaded with
     or without the nontemporal hint, and keeps the fastest; every variant
-    gives the same step within the solve tolerance (the staged and gathered
-    forms sum in the same order, but the compiler contracts their products
-    into FMAs differently), and the hint changes no result bit."""
+    gives the same step within the solve tolerance (the lane count changes
+    the order in which a row's products are summed).  For one lane count the
+    staged and gathered forms give the same bits (same slot order, the same
+    explicit FMAs), and the hint changes no result bit."""
     from acousticsim_amd import capi
 
     cs = _confs("mixed")
@@ -119,6 +120,8 @@ def test_autotuner_variants_agree(monkeypatch):
         for stage in ("0", "1"):
             for lpr in ("1", "2", "4"):
                 assert np.array_equal(out["0", stage, lpr][i], out["1", stage, lpr][i])
+        for lpr in ("1", "2", "4"):
+            assert np.array_equal(out["0", "0", lpr][i], out["0", "1", lpr][i]), lpr
 
 
 @pytest.mark.parametrize("graphs", [True, False])
