@@ -93,7 +93,7 @@ EXPORTS = [
     "bw_body_destroy", "bw_body_set_pos", "bw_body_num_edges", "bw_body_get_edges",
     "bw_detect_contacts", "bw_batch_detect_contacts",
     "bw_batch_get_pos", "bw_batch_set_state", "bw_batch_get_stats", "bw_batch_set_graphs",
-    "bw_batch_time_solver_kernel", "bw_batch_autotune", "bw_server_create", "bw_server_destroy", "bw_server_submit",
+    "bw_batch_time_solver_kernel", "bw_batch_time_launch_floor", "bw_batch_autotune", "bw_server_create", "bw_server_destroy", "bw_server_submit",
     "bw_server_wait", "bw_server_get_info", "bw_server_get_pos", "bw_server_release",
     "bw_server_get_counters", "bw_server_now",
 ]
@@ -172,6 +172,7 @@ def lib():
         "bw_batch_get_stats": (i32, [vp, i32, P(StepStats)]),
         "bw_batch_set_graphs": (i32, [vp, i32]),
         "bw_batch_time_solver_kernel": (i32, [vp, i32, dp, dp]),
+        "bw_batch_time_launch_floor": (i32, [vp, i32, dp]),
         "bw_batch_autotune": (i32, [vp, ip, ip, ip, dp]),
         "bw_server_create": (i32, [ctypes.c_double, i32, cp, P(vp)]),
         "bw_server_destroy": (None, [vp]),
@@ -517,6 +518,13 @@ class ClothBatch:
         nbytes = np.zeros(1)
         _check(lib().bw_batch_time_solver_kernel(self._h, reps, _dp(us), _dp(nbytes)))
         return float(us[0]), float(nbytes[0])
+
+    def time_launch_floor(self, reps=200):
+        """mean us per launch of an empty kernel with the grid and workgroup
+        of the solver kernel's launch (bw_batch_time_launch_floor)"""
+        us = np.zeros(1)
+        _check(lib().bw_batch_time_launch_floor(self._h, reps, _dp(us)))
+        return float(us[0])
 
     def autotune(self):
         """(lanes per row chosen, {lanes: mean SpMV launch us}) for the

@@ -821,6 +821,35 @@ double Batch::time_solver_kernel(int reps, double *alg_bytes)
     return 1000.0 * ms / reps;
 }
 
+double Batch::time_launch_floor(int reps)
+{
+    if (reps < 2)
+        throw std::runtime_error("batch: reps must be >= 2");
+    if (kind_ == SolverKind::GpuForm)
+        throw std::runtime_error("batch: kernel timing covers the plain method only");
+    // the grid and workgroup of the dominant kernel's launch with every
+    // cloth active: one workgroup of spmv_lpr wavefronts per slice (the
+    // reference mode's k_pcg_spmv: BW_SB slices per workgroup)
+    const bool ref = kind_ == SolverKind::RefPCG64;
+    const int grid = ref ? d_.n_blocks_wg : d_.n_slices;
+    const int block = ref ? 64 * BW_SB : 64 * d_.spmv_lpr;
+    hipEvent_t e0, e1;
+    check(hipEventCreate(&e0), "hipEventCreate");
+    check(hipEventCreate(&e1), "hipEventCreate");
+    launch_empty(grid, block, stream_); // warm
+    check(hipEventRecord(e0, stream_), "hipEventRecord");
+    for (int k = 0; k < reps; k++)
+        launch_empty(grid, block, stream_);
+    check(hipEventRecord(e1, stream_), "hipEventRecord");
+    check(hipGetLastError(), "empty launch");
+    check(hipEventSynchronize(e1), "hipEventSynchronize");
+    float ms = 0;
+    check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return 1000.0 * ms / reps;
+}
+
 void Batch::read_states()
 {
     if (kind_ == SolverKind::RefPCG64)

@@ -155,6 +155,10 @@ public:
     // mean microseconds per launch of the dominant solver kernel over reps
     // launches, every cloth active; algorithmic bytes per launch
     double time_solver_kernel(int reps, double *alg_bytes);
+    // mean microseconds per launch of an empty kernel with the grid and
+    // workgroup size of that kernel, launched back to back on the batch's
+    // stream: the floor under the solver kernel's launch time
+    double time_launch_floor(int reps);
 
     const DeviceBatch &device() const { return d_; }
 

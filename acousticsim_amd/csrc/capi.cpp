@@ -719,6 +719,16 @@ int bw_batch_time_solver_kernel(bw_batch *b, int reps, double *avg_us, double *a
     });
 }
 
+int bw_batch_time_launch_floor(bw_batch *b, int reps, double *avg_us)
+{
+    return guard([&] {
+        if (!b || !avg_us)
+            return fail(BW_ERR_ARG, "NULL argument");
+        *avg_us = b->batch->time_launch_floor(reps);
+        return BW_OK;
+    });
+}
+
 int bw_batch_autotune(bw_batch *b, int *lanes, int *staged, int *nontemporal, double *us)
 {
     return guard([&] {

@@ -1006,6 +1006,12 @@ __global__ void k_cg_reset(CgState *st, unsigned char *fresh, int n)
 }
 
 // timing set-up: every cloth mid-solve, at a k_cg_spmv with residual 1
+// Does nothing: launched back to back on a stream, each launch waits for the
+// previous one, so the time per launch is what dispatch, completion and the
+// kernel boundary's cache maintenance cost a grid of that shape, the floor of
+// any dependent launch of it.
+__global__ void k_empty() {}
+
 template <typename T> __global__ void k_cg_timing_setup(DeviceBatch d, T *r, T *z)
 {
     const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
@@ -1227,6 +1233,11 @@ void launch_cg_step_multi(const DeviceBatch *table, const int2 *spmv_list, int s
     if (grp_grid > 0)
         hipLaunchKernelGGL(k_cg_group_multi, dim3(grp_grid), dim3(64), 0, s, table, grp_list,
                            phase == 0 ? 1 : 0);
+}
+
+void launch_empty(int grid, int block, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_empty, dim3(grid), dim3(block), 0, s);
 }
 
 void launch_cg_timing_setup(const DeviceBatch &d, bool fp32, hipStream_t s)

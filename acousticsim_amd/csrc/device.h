@@ -241,6 +241,8 @@ void launch_cg_step_multi(const DeviceBatch *table, const int2 *spmv_list, int s
                           bool fp32, int phase, int lpr, int halo_max, bool nt, int xcd_map,
                           hipStream_t s);
 void launch_cg_timing_setup(const DeviceBatch &d, bool fp32, hipStream_t s);
+// an empty kernel of grid x block threads (the dependent-launch floor)
+void launch_empty(int grid, int block, hipStream_t s);
 // group sums of the solver partials in `slot` for cloths that have them
 void launch_group_sums(const DeviceBatch &d, int slot, hipStream_t s);
 // GPU form (gpuform.hip): pins, FP32 elements and rows; then the 100-

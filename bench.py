@@ -59,12 +59,22 @@ from acousticsim_amd import workloads as wl  # noqa: E402
 
 METRIC = "cloth steps/s & p50/p99 step latency at 1e-6 PCG residual; SpMV HBM GB/s"
 HBM_PEAK_GBS = 8000.0
-KERNEL = ("k_cg_spmv<float, %d, %s, %s> (PCG SpMV: d = P r + beta d_prev, q = W d, partial "
+KERNEL = ("k_cg_spmv<float, %d, %s, %s, %d> (PCG SpMV: d = P r + beta d_prev, q = W d, partial "
           "d.q; directions %s; matrix loads %s)")
 
 
-def kernel_name(lanes, staged, nt):
-    return KERNEL % (lanes, "true" if staged else "false", "true" if nt else "false",
+def batch_slices(ms):
+    """the batch's 64-row slices: each cloth's, padded to a multiple of 4"""
+    return sum(-(-(-(-(len(x) // 3) // 64)) // 4) * 4 for _, x in ms)
+
+
+def kernel_name(lanes, staged, nt, slices):
+    """the SpMV instance that a launch over every slice runs: the gathered
+    product takes 8 / lanes slots per chunk on grids of up to 2048 slices
+    (BW_ILP_SLICES overrides), else 4 / lanes"""
+    ilp = int(os.environ.get("BW_ILP_SLICES", "2048"))
+    chunk = (8 if not staged and slices <= ilp else 4) // lanes
+    return KERNEL % (lanes, "true" if staged else "false", "true" if nt else "false", chunk,
                      "staged in LDS per distinct slice column" if staged
                      else "gathered per slot",
                      "nontemporal" if nt else "cached")
@@ -78,10 +88,10 @@ def kernel_name(lanes, staged, nt):
 # when the autotuner picked a variant that has a measured entry here, and
 # says which profile the figure comes from.
 TRAFFIC = {
-    "batch64": {(1, False, False): (210387018.0, "profiles/k_cg_spmv_batch64.md"),
-                (2, False, False): (209697107.0, "profiles/k_cg_spmv_batch64.md")},
-    "single257": {(1, False, False): (33077660.0, "profiles/k_cg_spmv_single257.md"),
-                  (2, False, False): (32754087.0, "profiles/k_cg_spmv_single257.md")},
+    "batch64": {(1, False, False): (211446746.0, "profiles/k_cg_spmv_batch64.md"),
+                (2, False, False): (208552256.0, "profiles/k_cg_spmv_batch64.md")},
+    "single257": {(1, False, False): (33176912.0, "profiles/k_cg_spmv_single257.md"),
+                  (2, False, False): (33178100.0, "profiles/k_cg_spmv_single257.md")},
 }
 
 
@@ -399,15 +409,17 @@ def roofline_only(args, ms):
     b = capi.ClothBatch([c for c, _ in ms], [x for _, x in ms])
     lanes, staged, nt, tune = b.autotune_all()
     us, nbytes = b.time_solver_kernel(reps=200)
+    floor = b.time_launch_floor(reps=200)
     b.close()
     achieved = nbytes / us / 1e3
-    print(json.dumps({"workload": args.workload, "kernel": kernel_name(lanes, staged, nt),
+    print(json.dumps({"workload": args.workload, "kernel": kernel_name(lanes, staged, nt, batch_slices(ms)),
                       "tuning_us": {f: {str(k): float(v) for k, v in t.items()}
                                     for f, t in tune.items()},
                       "launches": 20 * sum(int(v > 0) for t in tune.values()
                                            for v in t.values()) + 201,
                       "launch_us": us, "alg_bytes": nbytes, "achieved_GBs": achieved,
-                      "frac": achieved / HBM_PEAK_GBS}), flush=True)
+                      "frac": achieved / HBM_PEAK_GBS, "launch_floor_us": floor,
+                      "frac_if_at_floor": nbytes / floor / 1e3 / HBM_PEAK_GBS}), flush=True)
 
 
 def serve(args):
@@ -581,6 +593,7 @@ def main():
     whole_step_gbs = step_bytes(b) / total / 1e9
     state = None if args.no_cpu or args.workload == "irregular" else sample_state(b)
     us, nbytes = b.time_solver_kernel(reps=200)
+    floor = b.time_launch_floor(reps=200)
     achieved = nbytes / us / 1e3  # GB/s
     traffic, traffic_src = traffic_for(args.workload, b.tuning)
     out = {
@@ -603,7 +616,7 @@ def main():
         # the slowest cloth's iterations (and true-residual checks) per step:
         # a step lasts as long as that cloth's chain of launches
         "critical_iterations_mean": float(np.mean(b.critical)),
-        "us_per_critical_iteration": 1000.0 * total / float(np.sum(b.critical)),
+        "us_per_critical_iteration": 1e6 * total / float(np.sum(b.critical)),
         "autotune": {"spmv_lanes_per_row": b.tuning[0], "spmv_staged": b.tuning[1],
                      "spmv_nontemporal": b.tuning[2],
                      "spmv_us": {f: {str(k): float(v) for k, v in t.items()}
@@ -612,8 +625,13 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": traffic_src,
-                     "kernel": kernel_name(*b.tuning[:3]),
-                     "launch_us": us, "alg_bytes": nbytes},
+                     "kernel": kernel_name(*b.tuning[:3], batch_slices(ms)),
+                     "launch_us": us, "alg_bytes": nbytes,
+                     # an empty kernel of the same grid and workgroup, launched
+                     # back to back: no launch of that shape is shorter, so
+                     # alg_bytes over it bounds the fraction one launch can reach
+                     "launch_floor_us": floor,
+                     "frac_if_at_floor": nbytes / floor / 1e3 / HBM_PEAK_GBS},
         "whole_step": {"achieved": whole_step_gbs, "unit": "GB/s",
                        "frac": whole_step_gbs / HBM_PEAK_GBS,
                        "bytes": "per PCG iteration of a cloth 40 nnzb + 144 V (SpMV and update "

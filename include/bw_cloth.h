@@ -307,6 +307,12 @@ int bw_batch_set_graphs(bw_batch *b, int enable);
    assembled at the current state first; the solver's vectors are
    overwritten (positions are not). */
 int bw_batch_time_solver_kernel(bw_batch *b, int reps, double *avg_us, double *alg_bytes);
+/* Measurement aid (no reference counterpart): an empty kernel with the grid
+   and workgroup size of that SpMV launch (one workgroup per slice), launched
+   reps times back to back on the batch's stream; avg_us gets the mean time
+   per launch, the floor that dispatch, completion and the kernel boundary
+   put under any launch of that shape which waits for the one before. */
+int bw_batch_time_launch_floor(bw_batch *b, int reps, double *avg_us);
 /* Autotuner (no reference counterpart): time the SpMV variants on the
    batch's own system and keep the fastest; the first bw_batch_step does
    this by itself.  A variant is 1, 2 or 4 lanes per matrix row; the
