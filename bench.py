@@ -216,9 +216,6 @@ def time_plain(ms, steps, warmup):
     return t, worst, [b.get_pos(0) for b in objs]
 
 
-REF_BIN = os.path.join(ROOT, "oracle", "_ref", "ref_cloth")
-
-
 def host_cpu():
     cpu = platform.processor() or platform.machine()
     try:
@@ -231,114 +228,63 @@ def host_cpu():
     return cpu
 
 
-def reference_step_seconds(conf, x, x_pre, iterations):
-    """the reference's own cBaraffCloth::UpdatePos (oracle/_ref/ref_cloth) for
-    one step from (x, x_pre) with `iterations` PCG iterations (its solve runs
-    max_iters - 1); returns (seconds of the step alone, positions after it)"""
-    import subprocess
-    import tempfile
-
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    from refdump import read_dump
-
-    c = dict(conf, max_iters=int(iterations) + 1)
-    with tempfile.TemporaryDirectory() as td:
-        paths = [os.path.join(td, n) for n in ("c.json", "x.bin", "xp.bin", "o.bin")]
-        with open(paths[0], "w") as f:
-            json.dump(c, f)
-        np.asarray(x, dtype="<f8").tofile(paths[1])
-        np.asarray(x_pre, dtype="<f8").tofile(paths[2])
-        subprocess.run([REF_BIN, paths[0], paths[1], "1", paths[3], "--xpre", paths[2], "--time"],
-                       check=True)
-        d = read_dump(paths[3])
-    return float(d["step_seconds"][0]), d["x_step1"]
-
-
-def reference_step_extrapolated(conf, x, x_pre, iterations):
-    """For a cloth too large for the budget: the reference's step time with
-    `iterations` PCG iterations, as a + iterations * b from two timed steps
-    of 10 and 50 iterations from the same state"""
-    t10, _ = reference_step_seconds(conf, x, x_pre, 10)
-    t50, _ = reference_step_seconds(conf, x, x_pre, 50)
-    per = (t50 - t10) / 40
-    return (t10 - 10 * per) + iterations * per, per
-
-
 def cpu_baseline(ms, state, budget_s=20.0):
-    """The CPU baseline, one thread, on every 4th cloth until the budget is
-    spent, each stepped once from the GPU's state after the timed steps:
-    the reference's compiled CPU step when oracle/_ref/ref_cloth exists (with
-    the GPU's iteration count), else the oracle in tolerance mode (1e-6).
-    Returns (cpu_baseline, per-result error): the error is the GPU's step
-    from that state against the oracle solved to 1e-10 from it, max over the
-    sample of max |x_gpu - x_oracle| / max |x_oracle - x|."""
+    """The CPU baseline: the oracle (the C restatement of the reference's
+    plain method, bit-exact to the reference's compiled CPU step on every
+    golden fixture; gcc -O3, no FMA contraction) in tolerance mode (1e-6),
+    every 4th cloth until the budget is spent, each stepped once from the
+    GPU's state after the timed steps -- one thread, and the same sample again
+    with one thread per cloth on up to 16 host cores.
+    Returns (cpu_baseline, all-cores baseline, per-result error): the error
+    is the GPU's step from that state against the oracle solved to 1e-10
+    from it, max over the sample of max |x_gpu - x_oracle| / max |x_oracle - x|."""
     import concurrent.futures as cf
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_ffi as of
 
     x, x_pre, x_new, iters = state
-    use_ref = os.path.exists(REF_BIN)
     order = list(range(0, len(ms), 4)) if len(ms) > 4 else list(range(len(ms)))
-    t_steps, done, ref_out = 0.0, [], {}
+
+    def one_step(i, tol):
+        oc = of.OracleCloth(ms[i][0])
+        oc.set_state(x[i], x_pre[i])
+        t0 = time.perf_counter()
+        oc.step(of.SOLVE_TOLERANCE, tol)
+        return time.perf_counter() - t0, oc.pos()
+
+    t_steps, done = 0.0, []
     start = time.perf_counter()
-    extrapolated = None
     for i in order:
-        c = ms[i][0]
-        if use_ref and len(ms) == 1 and len(x[i]) // 3 * iters[i] > 2e7:
-            # one large cloth: about 0.5 us per vertex-iteration would take
-            # minutes; the step time is extrapolated from 10 and 50 iterations
-            dt, extrapolated = reference_step_extrapolated(c, x[i], x_pre[i], iters[i])
-        elif use_ref:
-            dt, xr = reference_step_seconds(c, x[i], x_pre[i], iters[i])
-            ref_out[i] = xr
-        else:
-            oc = of.OracleCloth(c)
-            oc.set_state(x[i], x_pre[i])
-            t0 = time.perf_counter()
-            oc.step(of.SOLVE_TOLERANCE, 1e-6)
-            dt = time.perf_counter() - t0
-        t_steps += dt
+        t_steps += one_step(i, 1e-6)[0]
         done.append(i)
         if time.perf_counter() - start > budget_s:
             break
-    if use_ref:
-        base = {"value": len(done) / t_steps, "unit": "cloth steps/s", "cores": 1,
-                "kind": "reference",
-                "sample": "%d of %d cloths (every 4th until %.0f s), one step each from the "
-                          "GPU's state after the timed steps, with the GPU's PCG iteration count "
-                          "there (%d..%d); the reference's cBaraffCloth::UpdatePos compiled from "
-                          "its sources, g++ -O3 -DNDEBUG, serial; host %s"
-                          % (len(done), len(ms), budget_s, min(iters[i] for i in done),
-                             max(iters[i] for i in done), host_cpu())}
-        if extrapolated is not None:
-            base["sample"] += ("; extrapolated from steps of 10 and 50 iterations from the same "
-                               "state (%.3f s per iteration)" % extrapolated)
-    else:
-        base = {"value": len(done) / t_steps, "unit": "cloth steps/s", "cores": 1, "kind": "port",
-                "sample": "%d of %d cloths (every 4th until %.0f s), one step each from the "
-                          "GPU's state after the timed steps, tol 1e-6; the oracle (C restatement "
-                          "of the reference's plain method), gcc -O2, serial; host %s"
-                          % (len(done), len(ms), budget_s, host_cpu())}
+    what = ("%d of %d cloths (every 4th until %.0f s), one step each from the GPU's state "
+            "after the timed steps (GPU PCG iterations there %d..%d), tol 1e-6; the oracle "
+            "(C restatement of the reference's plain method), gcc -O3; host %s"
+            % (len(done), len(ms), budget_s, min(iters[i] for i in done),
+               max(iters[i] for i in done), host_cpu()))
+    base = {"value": len(done) / t_steps, "unit": "cloth steps/s", "cores": 1, "kind": "port",
+            "sample": what + "; serial"}
+    cores = min(16, os.cpu_count() or 1, len(done))
+    with cf.ThreadPoolExecutor(max_workers=cores) as ex:
+        t0 = time.perf_counter()
+        list(ex.map(lambda i: one_step(i, 1e-6), done))
+        wall = time.perf_counter() - t0
+    par = {"value": len(done) / wall, "unit": "cloth steps/s", "cores": cores, "kind": "port",
+           "sample": what + "; one thread per cloth, %d threads" % cores}
 
     def check(i):
-        oc = of.OracleCloth(ms[i][0])
-        oc.set_state(x[i], x_pre[i])
-        oc.step(of.SOLVE_TOLERANCE, 1e-10)
-        xt = oc.pos()
-        step = np.abs(xt - x[i]).max()
-        e_gpu = np.abs(x_new[i] - xt).max() / step
-        e_ref = np.abs(ref_out[i] - xt).max() / step if i in ref_out else None
-        return e_gpu, e_ref
+        xt = one_step(i, 1e-10)[1]
+        return np.abs(x_new[i] - xt).max() / np.abs(xt - x[i]).max()
 
     with cf.ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 4)) as ex:
         errs = list(ex.map(check, done))
-    err = {"gpu_max_err_over_step": max(e for e, _ in errs),
+    err = {"gpu_max_err_over_step": max(errs),
            "sample": "the CPU baseline's %d cloths: the GPU's step from the state after the "
                      "timed steps vs the oracle solved to 1e-10 from the same state" % len(done)}
-    if ref_out:
-        err["reference_max_err_over_step"] = max(r for _, r in errs)
-    return base, err
+    return base, par, err
 
 
 def cpu_baseline_extrapolated(ms, gpu_iters):  # noqa: C901
@@ -365,7 +311,7 @@ def cpu_baseline_extrapolated(ms, gpu_iters):  # noqa: C901
     return {"value": 1.0 / step_s, "unit": "cloth steps/s", "cores": 1, "kind": "port",
             "sample": "extrapolated: oracle assembly %.2f s + %.1f PCG iterations (the GPU's "
                       "mean per step) x %.3f s per iteration, both timed on the same 1.5M-point "
-                      "mesh; the oracle (C restatement), gcc -O2, serial; the reference itself "
+                      "mesh; the oracle (C restatement), gcc -O3, serial; the reference itself "
                       "cannot build this mesh (its Delaunay generator is O(V^2)); host %s"
                       % (base, gpu_iters, per_iter, host_cpu())}
 
@@ -655,7 +601,8 @@ def main():
         if args.workload == "irregular":
             out["cpu_baseline"] = cpu_baseline_extrapolated(ms, iters)
         else:
-            out["cpu_baseline"], out["per_result_error"] = cpu_baseline(ms, state)
+            (out["cpu_baseline"], out["cpu_baseline_all_cores"],
+             out["per_result_error"]) = cpu_baseline(ms, state)
     print(json.dumps(out), flush=True)
 
 

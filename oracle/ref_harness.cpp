@@ -3,9 +3,9 @@
 // This file is ours; it is compiled together with the reference's CPU
 // translation units (sources left where they lie under the reference tree, see
 // oracle/Makefile.ref) into oracle/_ref/ref_cloth.  It never ships in the
-// product: tests/golden/make_golden.py runs it to write the golden fixtures,
-// and bench.py's CPU-baseline leg times it (the reference's own CPU step on
-// the states the GPU stepped); the product path never loads or runs it.
+// product and never leaves this container: tests/golden/make_golden.py runs it
+// to write the golden fixtures, and the oracle is timed against it here; the
+// product path never loads or runs it.
 //
 // It follows the reference's own call sequence for one cloth
 // (src/sim/SimObjectBuilder.cpp:40-54 -> cBaraffCloth::Init,
