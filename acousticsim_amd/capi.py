@@ -54,6 +54,7 @@ class RequestInfo(ctypes.Structure):
         ("t_first_step", ctypes.c_double),
         ("t_done", ctypes.c_double),
         ("device_bytes", ctypes.c_double),
+        ("admitted_bytes", ctypes.c_double),
     ]
 
     def as_dict(self):

@@ -830,6 +830,7 @@ extern "C" int bw_server_get_info(bw_server *s, long long id, bw_request_info *i
         info->t_first_step = r.t_first;
         info->t_done = r.t_done;
         info->device_bytes = (double)r.device_bytes;
+        info->admitted_bytes = (double)r.admitted_bytes;
         if (r.status != BW_OK)
             g_error = r.error;
         return BW_OK;

@@ -244,6 +244,7 @@ void Server::builder_loop()
                 ri.t_admit = now();
                 ri.nv = nv;
                 ri.device_bytes = actual;
+                ri.admitted_bytes = est;
                 counters_.queued--;
                 counters_.resident++;
                 counters_.max_resident_seen =

@@ -59,6 +59,7 @@ struct RequestInfo
     // seconds on the server's clock (steady, from server creation)
     double t_submit = 0, t_admit = 0, t_first = 0, t_done = 0;
     size_t device_bytes = 0;
+    size_t admitted_bytes = 0; // the estimate admission reserved
     std::string error;
     std::vector<double> pos; // final positions (3 nv), once done
 };

@@ -69,9 +69,9 @@ def batch_slices(ms):
 
 
 def kernel_name(lanes, staged, nt, slices):
-    """the SpMV instance that a launch over every slice runs: the gathered
-    product takes 8 / lanes slots per chunk on grids of up to 2048 slices
-    (BW_ILP_SLICES overrides), else 4 / lanes"""
+    """the FP32 SpMV instance that a launch over every slice runs: the
+    gathered product takes 8 / lanes slots per lane and chunk on grids of up
+    to 2048 slices (BW_ILP_SLICES overrides), else 4 / lanes"""
     ilp = int(os.environ.get("BW_ILP_SLICES", "2048"))
     chunk = (8 if not staged and slices <= ilp else 4) // lanes
     return KERNEL % (lanes, "true" if staged else "false", "true" if nt else "false", chunk,

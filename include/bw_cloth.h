@@ -346,6 +346,9 @@ typedef struct bw_request_info
     double max_rel_residual;  /* worst final residual over the steps */
     double t_submit, t_admit, t_first_step, t_done; /* seconds, server clock */
     double device_bytes;      /* device memory the request held */
+    double admitted_bytes;    /* what admission reserved against the budget
+                                 before allocating: an upper bound of
+                                 device_bytes computed from the mesh */
 } bw_request_info;
 
 typedef struct bw_server_counters

@@ -241,7 +241,7 @@ def test_serving_slice_matches_batches_and_oracle(monkeypatch):
     stepping that cloth alone with the server's settings (1 lane per row,
     staged directions for 99% of the slices, group sums from 512 slices; the
     nontemporal hint changes no bit), and the two smallest agree with the
-    oracle."""
+    oracle.  Admission reserved at least what each request allocated."""
     from acousticsim_amd import capi
 
     work = wl.serve_requests(10)
@@ -263,6 +263,8 @@ def test_serving_slice_matches_batches_and_oracle(monkeypatch):
         assert s.wait(rid, 600.0), "request did not finish"
         info = s.info(rid)
         assert info["status"] == 0, info
+        # admission reserved an upper bound of what the request allocated
+        assert 0 < info["device_bytes"] <= info["admitted_bytes"], info
         served.append((info, s.get_pos(rid)))
     counters = s.counters()
     s.close()

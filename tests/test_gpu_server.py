@@ -105,9 +105,11 @@ def test_budget_limits_residency():
 
     say("case budget")
     reqs = [_req(40, 1.0, 40 + k) for k in range(3)]
-    # the budget of one request: measure what one takes, then allow 1.5x
+    # the budget of one request: what admission reserves for one (an upper
+    # bound of what it allocates), then allow 1.5x
     (info, _), = _served([reqs[0]])[0]
-    budget = 1.5 * info["device_bytes"] + (2 << 20)
+    assert 0 < info["device_bytes"] <= info["admitted_bytes"]
+    budget = 1.5 * info["admitted_bytes"]
     out, counters = _served(reqs, budget_bytes=budget)
     say("  counters", counters)
     assert counters["max_resident_seen"] == 1
