@@ -382,6 +382,15 @@ void Batch::build_layout()
         P.stall = mixed ? 1e-4 : 0.0;
         if (const char *e = std::getenv("BW_STALL"))
             P.stall = mixed ? std::atof(e) : 0.0;
+        // the switch carries over to the cloth's next solves once `carry`
+        // solves in a row have switched (BW_STALL_CARRY overrides the
+        // count): 1 keeps the FP64 matrix after the first stall, which the
+        // stiffest batch members need (batch64 515 cloth-steps/s against 480
+        // with 2); a cloth whose solves stall only now and then does better
+        // with 2 (single257 19.6 steps/s against 16.1)
+        P.carry = 1;
+        if (const char *e = std::getenv("BW_STALL_CARRY"))
+            P.carry = std::max(1, std::atoi(e));
         P.gf_a = m.cfg.rayleigh_a;
         P.gf_b = m.cfg.rayleigh_b;
         P.blk_begin = slice_base_[i] / BW_SB;

@@ -90,6 +90,7 @@ template <typename SP> __device__ inline CgState load_state(SP p)
     s.force_check = p->force_check;
     s.final_check = p->final_check;
     s.w64 = p->w64;
+    s.stalls = p->stalls;
     return s;
 }
 
@@ -112,6 +113,7 @@ template <typename SP> __device__ inline void store_state(SP p, const CgState &s
     p->force_check = s.force_check;
     p->final_check = s.final_check;
     p->w64 = s.w64;
+    p->stalls = s.stalls;
 }
 
 // SpMV variants: L lanes per row (1, 2 or 4; a workgroup of L wavefronts
@@ -983,24 +985,27 @@ k_cg_group_multi(const DeviceBatch *__restrict__ table, const int2 *__restrict__
     group_body(table[e.x], e.y, slot);
 }
 
-// both slots to the solve start (true residual of the warm start first)
-// Solver states for a new solve, one thread per cloth.  A cloth whose last
-// solve moved to the FP64 matrix starts on it: its stiffness has not
-// changed, so the FP32 matrix would stall again, and finding that out costs
-// every solve up to max_since iterations.  fresh[i] (set by the host when
-// the cloth's state or pins are set from outside) starts it on the FP32
-// matrix instead, as a new cloth.
-__global__ void k_cg_reset(CgState *st, unsigned char *fresh, int n)
+// Solver states for a new solve, one thread per cloth: both slots to the
+// solve start (true residual of the warm start first).  A cloth whose last
+// params[i].carry solves in a row moved to the FP64 matrix starts on it: it
+// stalls step after step, and finding that out again costs every solve up
+// to max_since iterations; one whose solves stall now and then returns to
+// the FP32 matrix.  fresh[i] (set by the host when the cloth's state or pins
+// are set from outside) starts it on the FP32 matrix with no history, as a
+// new cloth.
+__global__ void k_cg_reset(CgState *st, unsigned char *fresh, const InstParams *params, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
-    const int keep = fresh[i] ? 0 : (st[2 * i].w64 | st[2 * i + 1].w64);
+    const CgState &a = st[2 * i], &b = st[2 * i + 1];
+    const int stalls = fresh[i] ? 0 : (a.w64 | b.w64) ? max(a.stalls, b.stalls) + 1 : 0;
     fresh[i] = 0;
     CgState z = {};
     z.start = 1;
     z.first = 1;
-    z.w64 = keep;
+    z.stalls = stalls;
+    z.w64 = stalls >= params[i].carry ? 1 : 0;
     st[2 * i] = z;
     st[2 * i + 1] = z;
 }
@@ -1053,7 +1058,7 @@ constexpr int SB = BW_SB;
 void launch_cg_reset(const DeviceBatch &d, hipStream_t s)
 {
     hipLaunchKernelGGL(k_cg_reset, dim3((d.n_inst + 63) / 64), dim3(64), 0, s, d.cg, d.cg_fresh,
-                       d.n_inst);
+                       d.params, d.n_inst);
 }
 
 // bytes of LDS the staged variant needs for lists of up to halo_max entries

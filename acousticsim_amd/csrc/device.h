@@ -58,6 +58,9 @@ struct InstParams
     // stall * (r.r at the last replacement) switches the cloth's SpMV to the
     // FP64 matrix for the rest of the solve (0: never)
     double stall;
+    // a cloth whose last `carry` solves (consecutively) ended on the FP64
+    // matrix starts its next solve on it
+    int carry;
     float gf_a, gf_b; // GPU form: Rayleigh a, b as the reference's float members
 };
 
@@ -92,6 +95,8 @@ struct CgState
     int force_check; // 1: form a true residual next
     int final_check; // 1: stop after the next true residual
     int w64;         // 1: the SpMV reads the FP64 matrix (mixed form, after a stall)
+    int stalls;      // consecutive solves of the cloth before this one that
+                     // ended on the FP64 matrix
 };
 
 // Per-cloth state of the GPU-form PCG (gpuform.hip); two slots like CgState

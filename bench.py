@@ -153,6 +153,7 @@ def time_batch(ms, steps, warmup, graphs=True):
     worst = 0.0
     per_cloth_iters = np.zeros(b.n)
     critical = []
+    fp64_matrix = 0
     t0 = time.perf_counter()
     for _ in range(steps):
         s0 = time.perf_counter()
@@ -162,10 +163,12 @@ def time_batch(ms, steps, warmup, graphs=True):
         iters.append(np.mean([s["pcg_iterations"] for s in st]))
         critical.append(max(s["pcg_iterations"] + s["outer_iterations"] for s in st))
         per_cloth_iters += [s["pcg_iterations"] for s in st]
+        fp64_matrix += sum(s["matrix_fp64"] for s in st)
         worst = max(worst, max(s["rel_residual"] for s in st))
     total = time.perf_counter() - t0
     b.per_cloth_iters = per_cloth_iters
     b.critical = np.array(critical, dtype=np.float64)
+    b.fp64_matrix_fraction = fp64_matrix / float(steps * b.n)
     b.final_pos = [b.get_pos(i) for i in range(b.n)]
     return b, total, np.array(lat), float(np.mean(iters)), worst
 
@@ -563,6 +566,10 @@ def main():
         # a step lasts as long as that cloth's chain of launches
         "critical_iterations_mean": float(np.mean(b.critical)),
         "us_per_critical_iteration": 1e6 * total / float(np.sum(b.critical)),
+        # cloth-steps of the timed region whose solve read the FP64 matrix
+        # (the mixed form's stall fallback, or its carry-over from an earlier
+        # stalled solve of the same cloth)
+        "fp64_matrix_fraction": b.fp64_matrix_fraction,
         "autotune": {"spmv_lanes_per_row": b.tuning[0], "spmv_staged": b.tuning[1],
                      "spmv_nontemporal": b.tuning[2],
                      "spmv_us": {f: {str(k): float(v) for k, v in t.items()}
