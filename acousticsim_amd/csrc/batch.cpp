@@ -1231,8 +1231,9 @@ void Batch::fixed_variant(int &lpr, int &stage, int &nt) const
 
 // Picks the SpMV variant (lanes per row; directions gathered from memory or
 // staged in LDS; matrix loads with or without the nontemporal hint; see
-// cg.hip) that runs this batch's matrix fastest, timing each on the system
-// assembled at the current state.  BW_SPMV_LPR=1|2|4, BW_SPMV_STAGE=0|1 and
+// cg.hip) that runs this batch's matrix fastest (the first in a fixed order
+// within 3% of the fastest), timing each on the system assembled at the
+// current state.  BW_SPMV_LPR=1|2|4, BW_SPMV_STAGE=0|1 and
 // BW_SPMV_NT=0|1 in the environment fix those choices instead.
 int Batch::autotune()
 {
@@ -1272,12 +1273,16 @@ int Batch::autotune()
                 continue;
             set(v);
             tune_us_[v] = time_solver_kernel(20, nullptr);
-            if (tune_us_[v] < best)
-            {
-                best = tune_us_[v];
-                pick = v;
-            }
+            best = std::min(best, tune_us_[v]);
         }
+    // the first variant, in the order above (fewer lanes, gathered, cached
+    // first), within 3% of the fastest: variants that time alike would
+    // otherwise swap from run to run on timing noise, and the pick sets the
+    // rounding of the product, which a stiff cloth's iteration count follows
+    if (candidates > 1)
+        for (int v = TUNE_VARIANTS - 1; v >= 0; v--)
+            if (tune_us_[v] > 0.0 && tune_us_[v] <= 1.03 * best)
+                pick = v;
     set(pick);
     return d_.spmv_lpr;
 }

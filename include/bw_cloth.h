@@ -314,8 +314,10 @@ int bw_batch_time_solver_kernel(bw_batch *b, int reps, double *avg_us, double *a
    put under any launch of that shape which waits for the one before. */
 int bw_batch_time_launch_floor(bw_batch *b, int reps, double *avg_us);
 /* Autotuner (no reference counterpart): time the SpMV variants on the
-   batch's own system and keep the fastest; the first bw_batch_step does
-   this by itself.  A variant is 1, 2 or 4 lanes per matrix row; the
+   batch's own system and keep the fastest, or the first variant in the
+   order below (fewer lanes, gathered, cached first) that times within 3%
+   of it, so that near-equal variants do not swap between runs; the first
+   bw_batch_step does this by itself.  A variant is 1, 2 or 4 lanes per matrix row; the
    direction vector gathered from memory per slot, or staged in LDS once
    per distinct column of a 64-row slice; matrix loads with or without the
    nontemporal hint.  lanes gets the lanes per row chosen, staged and

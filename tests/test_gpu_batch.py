@@ -86,7 +86,8 @@ def test_batched_step_matches_oracle(precision):
 def test_autotuner_variants_agree(monkeypatch):
     """The autotuner times the 1-, 2- and 4-lane SpMV variants, each with the
     directions gathered per slot or staged in LDS and the matrix loaded with
-    or without the nontemporal hint, and keeps the fastest; every variant
+    or without the nontemporal hint, and keeps the first, in a fixed order,
+    within 3% of the fastest; every variant
     gives the same step within the solve tolerance (the lane count changes
     the order in which a row's products are summed).  For one lane count the
     staged and gathered forms give the same bits (same slot order, the same
@@ -100,7 +101,11 @@ def test_autotuner_variants_agree(monkeypatch):
     timed = [v for f in us.values() for v in f.values()]
     assert len(timed) == 12 and min(timed) > 0
     form = ("staged" if staged else "gathered") + ("_nt" if nt else "")
-    assert lanes in (1, 2, 4) and us[form][lanes] == min(timed)
+    # the first variant, in the order fewer lanes, gathered, cached first,
+    # that times within 3% of the fastest
+    order = [(f, l) for f in ("gathered", "staged", "gathered_nt", "staged_nt") for l in (1, 2, 4)]
+    first = next(v for v in order if us[v[0]][v[1]] <= 1.03 * min(timed))
+    assert lanes in (1, 2, 4) and (form, lanes) == first
     out = {}
     for hint in ("0", "1"):
         for stage in ("0", "1"):
