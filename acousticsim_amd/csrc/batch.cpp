@@ -5,10 +5,12 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <climits>
 #include <cstring>
 #include <stdexcept>
+#include <thread>
 #include <unordered_map>
 
 namespace bw
@@ -169,14 +171,34 @@ Batch::Batch(std::vector<ClothModel> models, const BatchOptions &opt)
     if (kind_ == SolverKind::GpuForm)
         tuned_ = true; // one SpMV form
     build_layout();
+    // lane 0: the batch's own stream, list and state buffers
+    lanes_.resize(1);
+    Lane &L0 = lanes_[0];
+    L0.stream = stream_;
+    L0.spmv_list = d_.spmv_list;
+    L0.upd_list = d_.upd_list;
+    L0.host_spmv = host_spmv_list_;
+    L0.host_upd = host_upd_list_;
+    L0.host_cg = host_cg_;
+    check(hipEventCreateWithFlags(&L0.ev, hipEventDisableTiming), "hipEventCreate");
+    check(hipEventCreateWithFlags(&lanes_start_, hipEventDisableTiming), "hipEventCreate");
 }
 
 Batch::~Batch()
 {
-    for (auto &g : graphs_)
-        (void)hipGraphExecDestroy(g.exec);
+    for (auto &L : lanes_)
+        close_lane(L);
     if (stream_)
         (void)hipStreamSynchronize(stream_);
+    if (!lanes_.empty())
+    {
+        for (auto &g : lanes_[0].graphs)
+            (void)hipGraphExecDestroy(g.exec);
+        if (lanes_[0].ev)
+            (void)hipEventDestroy(lanes_[0].ev);
+    }
+    if (lanes_start_)
+        (void)hipEventDestroy(lanes_start_);
     for (void *p : allocations_)
         (void)hipFree(p);
     if (host_state_)
@@ -614,9 +636,8 @@ void Batch::build_layout()
     host_slice_off_ = slice_off;
     contacts_.assign(models_.size(), Contacts());
     user_force_.assign(models_.size(), std::vector<double>());
-    sched_bounds_.reserve(models_.size());
     sched_pairs_.reserve(models_.size());
-    active_.reserve(models_.size());
+    lane_order_.reserve(models_.size());
     pred_launches_.reserve(models_.size());
     drag_vertex_.assign(models_.size(), -1);
     ydrag_.assign(models_.size(), 0.0f);
@@ -859,43 +880,48 @@ double Batch::time_launch_floor(int reps)
     return 1000.0 * ms / reps;
 }
 
-void Batch::read_states()
+void Batch::read_states(Lane &L, bool sync)
 {
     if (kind_ == SolverKind::RefPCG64)
         check(hipMemcpyAsync(host_state_, d_.state, sizeof(InstState) * models_.size(),
-                             hipMemcpyDeviceToHost, stream_),
+                             hipMemcpyDeviceToHost, L.stream),
               "hipMemcpyAsync state");
     else
-        check(hipMemcpyAsync(host_cg_, d_.cg, 2 * sizeof(CgState) * models_.size(),
-                             hipMemcpyDeviceToHost, stream_),
+        check(hipMemcpyAsync(L.host_cg, d_.cg, 2 * sizeof(CgState) * models_.size(),
+                             hipMemcpyDeviceToHost, L.stream),
               "hipMemcpyAsync state");
-    check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    if (sync)
+        check(hipStreamSynchronize(L.stream), "hipStreamSynchronize");
+    else
+        check(hipEventRecord(L.ev, L.stream), "hipEventRecord");
 }
 
-void Batch::launch_chain(int length, int slices)
+void Batch::launch_chain(Lane &L, int length, int slices)
 {
     if (kind_ == SolverKind::RefPCG64)
         for (int k = 0; k < length; k++)
-            launch_pcg_iter64(d_, stream_);
+            launch_pcg_iter64(d_, L.stream);
     else
     {
         DeviceBatch dl = d_;
-        if (launch_first_ >= 0)
+        dl.spmv_list = L.spmv_list;
+        dl.upd_list = L.upd_list;
+        if (L.launch_first >= 0)
         {
             dl.spmv_list = nullptr;
             dl.upd_list = nullptr;
-            dl.slice_first = launch_first_;
+            dl.slice_first = L.launch_first;
         }
         for (int k = 0; k < length; k++)
-            launch_cg_step(dl, kind_ == SolverKind::CG32, k & 1, slices, stream_);
+            launch_cg_step(dl, kind_ == SolverKind::CG32, k & 1, slices, L.stream);
     }
 }
 
-void Batch::run_chunk(int length, int slices)
+void Batch::run_chunk(Lane &L, int length, int slices)
 {
     if (!use_graphs_)
     {
-        launch_chain(length, slices);
+        launch_chain(L, length, slices);
         check(hipGetLastError(), "solver launch");
         return;
     }
@@ -904,17 +930,17 @@ void Batch::run_chunk(int length, int slices)
     // over and over
     for (int piece = 1 << 30; piece >= 32; piece >>= 1)
         if (length & piece)
-            run_graph(piece, slices);
+            run_graph(L, piece, slices);
     if (length & 31)
-        launch_chain(length & 31, slices);
+        launch_chain(L, length & 31, slices);
 }
 
-void Batch::run_graph(int length, int slices)
+void Batch::run_graph(Lane &L, int length, int slices)
 {
     ++graph_clock_;
     hipGraphExec_t exec = nullptr;
-    for (auto &g : graphs_)
-        if (g.length == length && g.slices == slices && g.first == launch_first_)
+    for (auto &g : L.graphs)
+        if (g.length == length && g.slices == slices && g.first == L.launch_first)
         {
             exec = g.exec;
             g.last_use = graph_clock_;
@@ -923,25 +949,37 @@ void Batch::run_graph(int length, int slices)
     {
         // bounded cache: drop the least recently used graph
         constexpr size_t max_graphs = 128;
-        if (graphs_.size() >= max_graphs)
+        if (L.graphs.size() >= max_graphs)
         {
-            auto lru = std::min_element(graphs_.begin(), graphs_.end(),
+            auto lru = std::min_element(L.graphs.begin(), L.graphs.end(),
                                         [](const GraphEntry &a, const GraphEntry &b) {
                                             return a.last_use < b.last_use;
                                         });
-            check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+            check(hipStreamSynchronize(L.stream), "hipStreamSynchronize");
             (void)hipGraphExecDestroy(lru->exec);
-            graphs_.erase(lru);
+            L.graphs.erase(lru);
         }
         hipGraph_t graph;
-        check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "BeginCapture");
-        launch_chain(length, slices);
-        check(hipStreamEndCapture(stream_, &graph), "EndCapture");
+        check(hipStreamBeginCapture(L.stream, hipStreamCaptureModeThreadLocal), "BeginCapture");
+        launch_chain(L, length, slices);
+        check(hipStreamEndCapture(L.stream, &graph), "EndCapture");
         check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "GraphInstantiate");
         (void)hipGraphDestroy(graph);
-        graphs_.push_back({length, slices, launch_first_, exec, graph_clock_});
+        L.graphs.push_back({length, slices, L.launch_first, exec, graph_clock_});
     }
-    check(hipGraphLaunch(exec, stream_), "GraphLaunch");
+    check(hipGraphLaunch(exec, L.stream), "GraphLaunch");
+}
+
+void Batch::clear_graphs()
+{
+    for (auto &L : lanes_)
+    {
+        if (!L.graphs.empty())
+            check(hipStreamSynchronize(L.stream), "hipStreamSynchronize");
+        for (auto &g : L.graphs)
+            (void)hipGraphExecDestroy(g.exec);
+        L.graphs.clear();
+    }
 }
 
 namespace
@@ -1015,17 +1053,17 @@ void Batch::solve_ref()
     int chunk = chunk_for(last_launches_ > 0 ? last_launches_ : 32);
     for (;;)
     {
-        read_states();
+        read_states(lanes_[0]);
         bool all_done = true;
         for (size_t i = 0; i < models_.size(); i++)
             all_done &= host_state_[i].done != 0;
         if (all_done)
             break;
-        run_chunk(chunk, d_.n_slices);
+        run_chunk(lanes_[0], chunk, d_.n_slices);
         chunk = chunk_for(32);
     }
     launch_residual64(d_, 2, stream_);
-    read_states();
+    read_states(lanes_[0]);
     int most = 0;
     for (size_t i = 0; i < models_.size(); i++)
     {
@@ -1039,21 +1077,21 @@ void Batch::solve_ref()
     last_launches_ = most;
 }
 
-// Chunk boundaries (launch counts) for this solve from the launches every
-// cloth needed in the previous one: a boundary wherever the cloths expected
-// to be still solving own at most half the slices of the previous boundary.
-// (The host buffers are the batch's own, sized at construction, so a solve
-// allocates no host memory either.)
-void Batch::chunk_schedule()
+// Chunk boundaries (launch counts) for this solve of a lane's cloths from
+// the launches every one of them needed in the previous solve: a boundary
+// wherever the cloths expected to be still solving own at most half the
+// slices of the previous boundary.  (The host buffers are the batch's own,
+// sized at construction, so a solve allocates no host memory either.)
+void Batch::chunk_schedule(Lane &L)
 {
-    std::vector<long long> &bounds = sched_bounds_;
+    std::vector<long long> &bounds = L.bounds;
     bounds.clear();
     if (pred_launches_.empty())
         return;
     std::vector<std::pair<long long, int>> &v = sched_pairs_;
     v.clear();
     long long remaining = 0;
-    for (int i = 0; i < size(); i++)
+    for (int i : L.members)
     {
         v.push_back({pred_launches_[i], nslices_[i]});
         remaining += nslices_[i];
@@ -1081,8 +1119,9 @@ void Batch::chunk_schedule()
 // bucket, where a range baked into the launches would need a graph per
 // range.  BW_RANGE=2 also runs any contiguous active range directly, 0 never
 // (measurement runs).  Returns the k_cg_spmv grid.
-int Batch::upload_lists(const std::vector<char> &active)
+int Batch::upload_lists(Lane &L)
 {
+    const std::vector<char> &active = L.active;
     static const int range_rule = [] {
         const char *e = std::getenv("BW_RANGE");
         return e ? std::atoi(e) : 1;
@@ -1101,32 +1140,159 @@ int Batch::upload_lists(const std::vector<char> &active)
     const bool whole = first == 0 && last == d_.n_slices;
     if (first >= 0 && contiguous && (range_rule == 2 || (range_rule == 1 && whole)))
     {
-        launch_first_ = first;
+        L.launch_first = first;
         return last - first;
     }
-    launch_first_ = -1;
+    L.launch_first = -1;
     int n = 0, nb = 0;
     for (int i = 0; i < size(); i++)
         if (active[i])
         {
             for (int s = 0; s < nslices_[i]; s++)
-                host_spmv_list_[n++] = slice_base_[i] + s;
+                L.host_spmv[n++] = slice_base_[i] + s;
             for (int b = 0; b < nslices_[i] / BW_SB; b++)
-                host_upd_list_[nb++] = slice_base_[i] / BW_SB + b;
+                L.host_upd[nb++] = slice_base_[i] / BW_SB + b;
         }
     const int grid = slice_bucket(n, d_.n_slices);
     const int ugrid = (grid + BW_SB - 1) / BW_SB;
     for (int k = n; k < grid; k++)
-        host_spmv_list_[k] = -1;
+        L.host_spmv[k] = -1;
     for (int k = nb; k < ugrid; k++)
-        host_upd_list_[k] = -1;
-    check(hipMemcpyAsync(d_.spmv_list, host_spmv_list_, grid * sizeof(int), hipMemcpyHostToDevice,
-                         stream_),
+        L.host_upd[k] = -1;
+    check(hipMemcpyAsync(L.spmv_list, L.host_spmv, grid * sizeof(int), hipMemcpyHostToDevice,
+                         L.stream),
           "hipMemcpyAsync list");
-    check(hipMemcpyAsync(d_.upd_list, host_upd_list_, ugrid * sizeof(int), hipMemcpyHostToDevice,
-                         stream_),
+    check(hipMemcpyAsync(L.upd_list, L.host_upd, ugrid * sizeof(int), hipMemcpyHostToDevice,
+                         L.stream),
           "hipMemcpyAsync list");
     return grid;
+}
+
+// The cloths that solve in lanes of their own this step: with the previous
+// solve's launch counts known and enough cloths, the ~1/10 of them with the
+// most launches (at least one) form lane 0 on the batch's stream, the next
+// ~1/3 lane 1 and the rest lane 2, each lane a chain of launches on its own
+// stream (BW_LANE_SPLIT="a,b" sets the first two lanes' sizes; on batch64,
+// 6 and 20 cloths ran 543 cloth-steps/s, against 532-539 with 1-3, 2-6,
+// 3-10, 4-16, 6-30 or 10-20, 518-527 with 8-24 or 16-24, and 516 in one
+// chain).  In
+// one chain the stiffest cloths' iterations would wait on every launch of
+// the others while all of them solve; in lanes of their own they run at the
+// pace of their own launches, alongside the others' bandwidth-bound ones,
+// and a step ends when its slowest lane does.  Each cloth's arithmetic is
+// its own, so results are bit-identical to one chain.  One chain when
+// BW_LANES=0, on the first solve, for fewer than 8 cloths, with group sums
+// (whose launch covers every cloth's groups) and in bw_batch_step_noalloc
+// (lanes 1 and 2 allocate streams and buffers on first use).  Returns the
+// number of lanes.
+int Batch::plan_lanes()
+{
+    const char *env = std::getenv("BW_LANES");
+    const bool lanes_rule = env ? std::atoi(env) != 0 : true;
+    const int n = size();
+    lanes_[0].members.clear();
+    if (!lanes_rule || !lanes_allowed_ || n < 8 || d_.n_groups > 0 ||
+        (int)pred_launches_.size() != n)
+    {
+        for (int i = 0; i < n; i++)
+            lanes_[0].members.push_back(i);
+        return 1;
+    }
+    std::vector<int> &order = lane_order_;
+    order.resize(n);
+    for (int i = 0; i < n; i++)
+        order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return pred_launches_[a] > pred_launches_[b]; });
+    int a = std::max(1, n / 10), b = std::max(1, n / 3);
+    if (const char *e = std::getenv("BW_LANE_SPLIT")) // "a,b": lane sizes (tuning)
+    {
+        int ea = 0, eb = 0;
+        if (std::sscanf(e, "%d,%d", &ea, &eb) == 2 && ea >= 1 && eb >= 1 && ea + eb < n)
+        {
+            a = ea;
+            b = eb;
+        }
+    }
+    while ((int)lanes_.size() < 3)
+    {
+        lanes_.emplace_back();
+        open_lane(lanes_.back());
+    }
+    for (int k = 1; k < 3; k++)
+        lanes_[k].members.clear();
+    for (int r = 0; r < n; r++)
+        lanes_[r < a ? 0 : r < a + b ? 1 : 2].members.push_back(order[r]);
+    for (auto &L : lanes_)
+        std::sort(L.members.begin(), L.members.end());
+    return 3;
+}
+
+// stream, event and buffers of a lane beyond the first
+void Batch::open_lane(Lane &L)
+{
+    check(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking), "hipStreamCreate");
+    check(hipEventCreateWithFlags(&L.ev, hipEventDisableTiming), "hipEventCreate");
+    L.owned = true;
+    L.spmv_list = dalloc<int>(d_.n_slices);
+    L.upd_list = dalloc<int>(d_.n_blocks_wg);
+    check(hipHostMalloc((void **)&L.host_spmv, sizeof(int) * (size_t)d_.n_slices), "hipHostMalloc");
+    check(hipHostMalloc((void **)&L.host_upd, sizeof(int) * (size_t)d_.n_blocks_wg), "hipHostMalloc");
+    check(hipHostMalloc((void **)&L.host_cg, 2 * sizeof(CgState) * models_.size()),
+          "hipHostMalloc");
+}
+
+void Batch::close_lane(Lane &L)
+{
+    if (!L.owned)
+        return;
+    if (L.stream)
+        (void)hipStreamSynchronize(L.stream);
+    for (auto &g : L.graphs)
+        (void)hipGraphExecDestroy(g.exec);
+    L.graphs.clear();
+    if (L.host_spmv)
+        (void)hipHostFree(L.host_spmv);
+    if (L.host_upd)
+        (void)hipHostFree(L.host_upd);
+    if (L.host_cg)
+        (void)hipHostFree(L.host_cg);
+    if (L.ev)
+        (void)hipEventDestroy(L.ev);
+    if (L.stream)
+        (void)hipStreamDestroy(L.stream);
+    L = Lane();
+}
+
+// the next chunk of a lane: its length from the lane's schedule, then the
+// launches and the copy of the states after them
+void Batch::lane_chunk(Lane &L, bool sync)
+{
+    while (L.next < L.bounds.size() && L.bounds[L.next] < L.launched + 32)
+        L.next++;
+    long long want;
+    if (L.next < L.bounds.size())
+        want = L.bounds[L.next] - L.launched;
+    else
+        want = 64LL << std::min(L.blind++, 4);
+    const int length = chunk_for(want);
+    run_chunk(L, length, upload_lists(L));
+    L.launched += length;
+    read_states(L, sync);
+}
+
+// after a lane's chunk: which of its cloths still solve; false when none
+bool Batch::lane_update(Lane &L, long long cap)
+{
+    bool any = false;
+    for (int i : L.members)
+    {
+        L.active[i] = L.host_cg[2 * i].done == 0;
+        any |= L.active[i] != 0;
+    }
+    if (any && L.launched > cap)
+        throw std::runtime_error("step: solver did not terminate");
+    return any;
 }
 
 void Batch::solve_cg()
@@ -1135,49 +1301,84 @@ void Batch::solve_cg()
     long long cap = 16;
     for (auto &m : models_)
         cap = std::max<long long>(cap, 4LL * m.cfg.solver_max_iters + 64);
-    chunk_schedule();
-    const std::vector<long long> &bounds = sched_bounds_;
-    std::vector<char> &active = active_;
-    active.assign(size(), 1);
-    long long launched = 0;
-    size_t next = 0;
-    int blind = 0; // chunks run without a predicted boundary
-    for (;;)
+    const int nl = plan_lanes();
+    for (int k = 0; k < nl; k++)
     {
-        while (next < bounds.size() && bounds[next] < launched + 32)
-            next++;
-        long long want;
-        if (next < bounds.size())
-            want = bounds[next] - launched;
-        else
-            want = 64LL << std::min(blind++, 4);
-        const int length = chunk_for(want);
-        run_chunk(length, upload_lists(active));
-        launched += length;
-        read_states();
-        bool any = false;
-        for (int i = 0; i < size(); i++)
+        Lane &L = lanes_[k];
+        chunk_schedule(L);
+        L.active.assign(size(), 0);
+        for (int i : L.members)
+            L.active[i] = 1;
+        L.next = 0;
+        L.launched = 0;
+        L.blind = 0;
+        L.busy = false;
+        L.done = L.members.empty();
+    }
+    if (nl == 1)
+    {
+        Lane &L = lanes_[0];
+        for (;;)
         {
-            active[i] = host_cg_[2 * i].done == 0;
-            any |= active[i] != 0;
+            lane_chunk(L, true);
+            if (!lane_update(L, cap))
+                break;
         }
-        if (!any)
-            break;
-        if (launched > cap)
-            throw std::runtime_error("step: solver did not terminate");
+    }
+    else
+    {
+        // the other lanes start after the reset (and all work before it) on
+        // the batch's stream; the batch's stream continues after all lanes
+        check(hipEventRecord(lanes_start_, stream_), "hipEventRecord");
+        for (int k = 1; k < nl; k++)
+            check(hipStreamWaitEvent(lanes_[k].stream, lanes_start_, 0), "hipStreamWaitEvent");
+        for (;;)
+        {
+            bool all_done = true, launched = false;
+            for (int k = 0; k < nl; k++)
+            {
+                Lane &L = lanes_[k];
+                if (L.done)
+                    continue;
+                all_done = false;
+                if (L.busy)
+                {
+                    const hipError_t q = hipEventQuery(L.ev);
+                    if (q == hipErrorNotReady)
+                        continue;
+                    check(q, "hipEventQuery");
+                    L.busy = false;
+                    if (!lane_update(L, cap))
+                    {
+                        L.done = true;
+                        continue;
+                    }
+                }
+                lane_chunk(L, false);
+                L.busy = true;
+                launched = true;
+            }
+            if (all_done)
+                break;
+            if (!launched)
+                std::this_thread::yield();
+        }
+        for (int k = 1; k < nl; k++)
+            check(hipStreamWaitEvent(stream_, lanes_[k].ev, 0), "hipStreamWaitEvent");
     }
     pred_launches_.assign(size(), 0);
-    for (int i = 0; i < size(); i++)
-    {
-        const CgState &s = host_cg_[2 * i];
-        StepStats &st = stats_[i];
-        st.pcg_iterations = s.total;
-        st.outer_iterations = s.updates;
-        st.matrix_fp64 = s.w64;
-        st.rel_residual = s.bb > 0 ? std::sqrt(s.rr_true / s.bb) : std::sqrt(s.rr_true);
-        // launches this cloth needed: two per iteration and per true residual
-        pred_launches_[i] = 2LL * (s.total + s.updates) + 2;
-    }
+    for (int k = 0; k < nl; k++)
+        for (int i : lanes_[k].members)
+        {
+            const CgState &s = lanes_[k].host_cg[2 * i];
+            StepStats &st = stats_[i];
+            st.pcg_iterations = s.total;
+            st.outer_iterations = s.updates;
+            st.matrix_fp64 = s.w64;
+            st.rel_residual = s.bb > 0 ? std::sqrt(s.rr_true / s.bb) : std::sqrt(s.rr_true);
+            // launches this cloth needed: two per iteration and per true residual
+            pred_launches_[i] = 2LL * (s.total + s.updates) + 2;
+        }
 }
 
 void Batch::begin_step()
@@ -1239,9 +1440,7 @@ int Batch::autotune()
 {
     if (kind_ == SolverKind::RefPCG64 || kind_ == SolverKind::GpuForm)
         return d_.spmv_lpr;
-    for (auto &g : graphs_) // graphs hold the previous choice
-        (void)hipGraphExecDestroy(g.exec);
-    graphs_.clear();
+    clear_graphs(); // graphs hold the previous choice
     tuned_ = true;
     int fixed_lpr = 0, fixed_stage = -1, fixed_nt = -1;
     fixed_variant(fixed_lpr, fixed_stage, fixed_nt);
@@ -2003,6 +2202,7 @@ void Batch::step_noalloc(int nsteps)
             b->d_.cg_partial = partial;
             b->d_.cg_group = group;
             b->use_graphs_ = graphs;
+            b->lanes_allowed_ = true;
         }
     } restore{this, own_cg, own_partial, own_group, own_graphs};
     const size_t st_copy = 2 * (size_t)size() * sizeof(CgState);
@@ -2014,6 +2214,7 @@ void Batch::step_noalloc(int nsteps)
     d_.cg_group = d_.n_groups > 0 ? reinterpret_cast<double *>(base + st_bytes + part_bytes)
                                   : nullptr;
     use_graphs_ = false;
+    lanes_allowed_ = false; // lanes 1 and 2 would allocate on first use
     for (int n = 0; n < nsteps; n++)
     {
         const auto t0 = std::chrono::steady_clock::now();

@@ -168,17 +168,55 @@ private:
     // cloth i (all for -1) starts its next solve on the FP32 matrix
     void mark_fresh(int i);
     void download(int i, const double *src_dev, double *dst) const;
-    void read_states();
+    struct GraphEntry
+    {
+        int length, slices, first;
+        hipGraphExec_t exec;
+        unsigned long long last_use;
+    };
+    // A chain of solver launches on one stream over a set of the cloths (see
+    // plan_lanes).  lanes_[0] runs on the batch's stream with its list and
+    // state buffers; further lanes own theirs.
+    struct Lane
+    {
+        hipStream_t stream = nullptr;
+        hipEvent_t ev = nullptr; // after the lane's last queued state copy
+        bool owned = false;      // stream, event and host buffers are the lane's
+        int *spmv_list = nullptr, *upd_list = nullptr; // device
+        int *host_spmv = nullptr, *host_upd = nullptr; // pinned
+        CgState *host_cg = nullptr;                    // pinned, 2 per cloth
+        // first slice of the contiguous range the next launches cover, or
+        // -1 when they run from the slice lists
+        int launch_first = -1;
+        std::vector<GraphEntry> graphs;
+        std::vector<int> members;      // cloths of this solve
+        std::vector<char> active;      // per cloth: still solving
+        std::vector<long long> bounds; // chunk boundaries (launch counts)
+        size_t next = 0;
+        long long launched = 0;
+        int blind = 0;                 // chunks run without a predicted boundary
+        bool busy = false, done = false;
+    };
+    std::vector<Lane> lanes_;
+    hipEvent_t lanes_start_ = nullptr;
+    bool lanes_allowed_ = true; // false inside bw_batch_step_noalloc
+    std::vector<int> lane_order_;
+    int plan_lanes();
+    void open_lane(Lane &L);
+    void close_lane(Lane &L);
+    void lane_chunk(Lane &L, bool sync);
+    bool lane_update(Lane &L, long long cap);
+    void clear_graphs();
+    // states to the lane's host copy; sync: wait for it, else record L.ev
+    void read_states(Lane &L, bool sync = true);
     // one chunk of solver launches; slices: the k_cg_spmv grid
-    void launch_chain(int length, int slices);
-    void run_chunk(int length, int slices); // as replayed graphs when enabled
-    void run_graph(int length, int slices);
-    // chunk boundaries of this solve into sched_bounds_
-    void chunk_schedule();
-    std::vector<long long> sched_bounds_;
+    void launch_chain(Lane &L, int length, int slices);
+    void run_chunk(Lane &L, int length, int slices); // as replayed graphs when enabled
+    void run_graph(Lane &L, int length, int slices);
+    // chunk boundaries of this solve of the lane's cloths into L.bounds
+    void chunk_schedule(Lane &L);
     std::vector<std::pair<long long, int>> sched_pairs_;
-    std::vector<char> active_;
-    int upload_lists(const std::vector<char> &active);
+    int upload_lists(Lane &L);
     void solve_ref();
     void solve_cg();
     // contacts: device lists for the next step (false if there are none)
@@ -217,17 +255,7 @@ private:
     bool graphs_disabled_ = false; // BW_GRAPHS=0 in the environment
     int last_launches_ = 0;
     std::vector<long long> pred_launches_;      // per cloth, from the last solve
-    int *host_spmv_list_ = nullptr, *host_upd_list_ = nullptr; // pinned
-    // first slice of the contiguous range the next launches cover, or -1
-    // when they run from the slice lists
-    int launch_first_ = -1;
-    struct GraphEntry
-    {
-        int length, slices, first;
-        hipGraphExec_t exec;
-        unsigned long long last_use;
-    };
-    std::vector<GraphEntry> graphs_;
+    int *host_spmv_list_ = nullptr, *host_upd_list_ = nullptr; // pinned (lane 0)
     unsigned long long graph_clock_ = 0;
     std::vector<void *> allocations_;
     template <typename T> T *dalloc(size_t n);

@@ -239,3 +239,42 @@ def test_tiny_and_ragged_cloths_in_one_batch():
         say("  member %d nv %d err/disp %.3e" % (i, oc.nv, err / dd))
         assert b.stats(i)["rel_residual"] <= 1e-6
         assert err <= 1e-4 * dd
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_lanes_equal_one_chain(graphs, monkeypatch):
+    """From its second solve on, a batch of 8 or more cloths solves its
+    stiffest cloths in lanes of their own (chains of launches on streams of
+    their own, concurrently with the rest); every step's positions and
+    iteration counts are bit for bit those of solving all cloths in one
+    chain (BW_LANES=0)."""
+    from acousticsim_amd import capi
+
+    monkeypatch.setenv("BW_SPMV_LPR", "1")
+    monkeypatch.setenv("BW_SPMV_STAGE", "0")
+    monkeypatch.setenv("BW_SPMV_NT", "0")
+    members = [(17 + 4 * k, 10.0 ** (1.0 - 0.2 * k), 40 + k) for k in range(12)]
+    cs = []
+    for side, scale, seed in members:
+        c = wl.grid_config(side, stiffness_scale=scale)
+        c["precision"] = "mixed"
+        cs.append((c, wl.jittered_positions(c, seed)))
+    say("case lanes, graphs %s: %d cloths" % (graphs, len(cs)))
+    out = {}
+    for lanes in ("1", "0"):
+        monkeypatch.setenv("BW_LANES", lanes)
+        b = capi.ClothBatch([c for c, _ in cs], [x for _, x in cs])
+        b.set_graphs(graphs)
+        steps = []
+        for _ in range(4):
+            b.step(1)
+            steps.append(([b.get_pos(i) for i in range(len(cs))],
+                          [b.stats(i)["pcg_iterations"] for i in range(len(cs))]))
+            assert max(b.stats(i)["rel_residual"] for i in range(len(cs))) <= 1e-6
+        out[lanes] = steps
+        b.close()
+    for k in range(4):
+        say("  step %d iterations %s" % (k + 1, out["1"][k][1]))
+        assert out["1"][k][1] == out["0"][k][1], k
+        for i in range(len(cs)):
+            assert np.array_equal(out["1"][k][0][i], out["0"][k][0][i]), (k, i)
