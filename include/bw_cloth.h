@@ -238,7 +238,10 @@ int bw_assemble_system(bw_cloth *c, double *W, double *rhs);
 const char *bw_last_error(void);
 
 /* ---- batched stepping (no reference counterpart: many independent cloths
-   of uneven size stepped together, one launch sequence per phase) -------- */
+   of uneven size stepped together; from the second step on, a batch of 8
+   or more cloths solves the ones that needed the most launches in chains
+   of their own on separate streams, with results bit-identical to one
+   chain) -------------------------------------------------------------- */
 
 /* n cloths from n JSON configs; x0s[i] may be NULL */
 int bw_batch_create(const char *const *json_configs, const double *const *x0s,
