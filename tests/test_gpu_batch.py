@@ -247,7 +247,9 @@ def test_lanes_equal_one_chain(graphs, monkeypatch):
     stiffest cloths in lanes of their own (chains of launches on streams of
     their own, concurrently with the rest); every step's positions and
     iteration counts are bit for bit those of solving all cloths in one
-    chain (BW_LANES=0)."""
+    chain (BW_LANES=0), also with user forces on the stiffest member and on
+    the softest at step 3 (inputs the batch's stream uploads before the
+    lanes start)."""
     from acousticsim_amd import capi
 
     monkeypatch.setenv("BW_SPMV_LPR", "1")
@@ -266,7 +268,10 @@ def test_lanes_equal_one_chain(graphs, monkeypatch):
         b = capi.ClothBatch([c for c, _ in cs], [x for _, x in cs])
         b.set_graphs(graphs)
         steps = []
-        for _ in range(4):
+        for k in range(4):
+            if k == 2:
+                b.apply_perturb(0, 5, [0.0, 0.0, 0.02])
+                b.apply_perturb(11, 40, [0.01, 0.0, -0.01])
             b.step(1)
             steps.append(([b.get_pos(i) for i in range(len(cs))],
                           [b.stats(i)["pcg_iterations"] for i in range(len(cs))]))
