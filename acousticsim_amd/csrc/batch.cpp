@@ -1332,6 +1332,19 @@ void Batch::solve_cg()
         check(hipEventRecord(lanes_start_, stream_), "hipEventRecord");
         for (int k = 1; k < nl; k++)
             check(hipStreamWaitEvent(lanes_[k].stream, lanes_start_, 0), "hipStreamWaitEvent");
+        // on an error, no lane's queued work may outlive this call
+        struct Drain
+        {
+            Batch *b;
+            int nl;
+            bool armed = true;
+            ~Drain()
+            {
+                if (armed)
+                    for (int k = 0; k < nl; k++)
+                        (void)hipStreamSynchronize(b->lanes_[k].stream);
+            }
+        } drain{this, nl};
         for (;;)
         {
             bool all_done = true, launched = false;
@@ -1365,6 +1378,7 @@ void Batch::solve_cg()
         }
         for (int k = 1; k < nl; k++)
             check(hipStreamWaitEvent(stream_, lanes_[k].ev, 0), "hipStreamWaitEvent");
+        drain.armed = false;
     }
     pred_launches_.assign(size(), 0);
     for (int k = 0; k < nl; k++)
