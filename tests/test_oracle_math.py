@@ -160,7 +160,7 @@ def test_W_symmetric_positive_definite_and_solve_matches_dense():
     assert np.allclose(oc.dx(), dx_dense, rtol=0, atol=1e-10 * np.abs(dx_dense).max())
 
 
-# ELL widths: survey's independent recount of the reference GPU rule
+# ELL widths: an independent recount of the reference GPU rule
 # (BaraffClothGpu.cpp:438-500) on the reference grid topology.
 WIDTHS = {
     21: (4521, {6: 44, 9: 216, 11: 4, 12: 32, 13: 145}),
