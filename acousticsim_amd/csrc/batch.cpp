@@ -906,6 +906,7 @@ void Batch::launch_chain(Lane &L, int length, int slices)
         DeviceBatch dl = d_;
         dl.spmv_list = L.spmv_list;
         dl.upd_list = L.upd_list;
+        dl.lds_pad = L.lds_reserve;
         if (L.launch_first >= 0)
         {
             dl.spmv_list = nullptr;
@@ -1228,9 +1229,16 @@ int Batch::plan_lanes()
     return 3;
 }
 
-// stream, event and buffers of a lane beyond the first
+// stream, event and buffers of a lane beyond the first.  Its solver
+// workgroups reserve LDS they do not use (BW_LANE_LDS bytes, default 24 KiB,
+// 0 for none): fewer of them then fit on a CU at once, and the wave slots
+// left over go to lane 0, whose few long-running cloths are bound by the
+// latency of each launch rather than by bandwidth.
 void Batch::open_lane(Lane &L)
 {
+    L.lds_reserve = 24576;
+    if (const char *e = std::getenv("BW_LANE_LDS"))
+        L.lds_reserve = std::min(std::max(std::atoi(e), 0), 32768);
     check(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking), "hipStreamCreate");
     check(hipEventCreateWithFlags(&L.ev, hipEventDisableTiming), "hipEventCreate");
     L.owned = true;

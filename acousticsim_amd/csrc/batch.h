@@ -182,6 +182,7 @@ private:
         hipStream_t stream = nullptr;
         hipEvent_t ev = nullptr; // after the lane's last queued state copy
         bool owned = false;      // stream, event and host buffers are the lane's
+        int lds_reserve = 0;     // dynamic LDS bytes per solver workgroup
         int *spmv_list = nullptr, *upd_list = nullptr; // device
         int *host_spmv = nullptr, *host_upd = nullptr; // pinned
         CgState *host_cg = nullptr;                    // pinned, 2 per cloth

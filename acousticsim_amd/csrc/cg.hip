@@ -1084,14 +1084,14 @@ void launch_spmv_nt(const DeviceBatch &d, int slices, const T *W, const T *P, T 
                     T *d0, T *d1, hipStream_t s)
 {
     if (!d.spmv_stage && slices <= (sizeof(T) == sizeof(float) ? ilp_slices() : ilp_slices() / 2))
-        hipLaunchKernelGGL((k_cg_spmv<T, L, false, NT, 8 / L>), dim3(slices), dim3(L * 64), 0, s,
-                           d, W, P, r, z, q, d0, d1);
+        hipLaunchKernelGGL((k_cg_spmv<T, L, false, NT, 8 / L>), dim3(slices), dim3(L * 64),
+                           d.lds_pad, s, d, W, P, r, z, q, d0, d1);
     else if (d.spmv_stage)
         hipLaunchKernelGGL((k_cg_spmv<T, L, true, NT>), dim3(slices), dim3(L * 64),
                            stage_bytes<T>(d.halo_lds), s, d, W, P, r, z, q, d0, d1);
     else
-        hipLaunchKernelGGL((k_cg_spmv<T, L, false, NT>), dim3(slices), dim3(L * 64), 0, s, d, W,
-                           P, r, z, q, d0, d1);
+        hipLaunchKernelGGL((k_cg_spmv<T, L, false, NT>), dim3(slices), dim3(L * 64), d.lds_pad, s,
+                           d, W, P, r, z, q, d0, d1);
 }
 
 template <typename T, int L>
@@ -1179,10 +1179,10 @@ void launch_update(const DeviceBatch &d, int slices, const T *P, T *r, T *z, con
 {
     const dim3 upd_grid((slices + SB - 1) / SB), upd_block(SB * 64);
     if (slices <= prefetch_slices())
-        hipLaunchKernelGGL((k_cg_update<T, SB, true>), upd_grid, upd_block, 0, s, d, P, r, z, q,
+        hipLaunchKernelGGL((k_cg_update<T, SB, true>), upd_grid, upd_block, d.lds_pad, s, d, P, r, z, q,
                            d0, d1);
     else
-        hipLaunchKernelGGL((k_cg_update<T, SB, false>), upd_grid, upd_block, 0, s, d, P, r, z, q,
+        hipLaunchKernelGGL((k_cg_update<T, SB, false>), upd_grid, upd_block, d.lds_pad, s, d, P, r, z, q,
                            d0, d1);
 }
 

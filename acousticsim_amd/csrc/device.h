@@ -196,6 +196,7 @@ struct DeviceBatch
     // slice_first (a multiple of BW_SB) that the grid covers
     int *spmv_list = nullptr, *upd_list = nullptr;
     int slice_first = 0;
+    int lds_pad = 0; // dynamic LDS bytes the solver launches reserve and leave unused
     // per slice, what the solver kernels need before anything else, in one
     // load: {cloth, first partial set of the cloth's solver sums, their
     // count (negative: -count group sums), the slice's first ELL entry}
