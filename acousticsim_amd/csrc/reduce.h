@@ -20,10 +20,57 @@ enum Part
     P_RR64 = 4, // FP64 true residual r64.r64
 };
 
+// v of the lane that DPP control CTRL selects (every lane active); a
+// double moves as its two halves
+template <int CTRL> __device__ inline double dpp_lane(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// v of lane ^ 32 (M = 32) or lane ^ 16 (M = 16) through gfx950's lane swap
+// of two registers (halves of the wavefront, or pairs of 16-lane rows).
+// Given the same register twice, one of the two results holds the partner
+// lane's value in every lane; which one is read off the same swap of the
+// lane index.
+template <int M> __device__ inline double swap_lane(double v)
+{
+    const int lane = threadIdx.x & 63;
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    if constexpr (M == 32)
+    {
+        const auto li = __builtin_amdgcn_permlane32_swap(lane, lane, false, false);
+        const auto l = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+        const auto h = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+        return (int)li[0] == (lane ^ 32) ? __hiloint2double(h[0], l[0])
+                                         : __hiloint2double(h[1], l[1]);
+    }
+    else
+    {
+        const auto li = __builtin_amdgcn_permlane16_swap(lane, lane, false, false);
+        const auto l = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+        const auto h = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+        return (int)li[0] == (lane ^ 16) ? __hiloint2double(h[0], l[0])
+                                         : __hiloint2double(h[1], l[1]);
+    }
+}
+
+// Sum over the 64 lanes of a wavefront (all active), the same value in every
+// lane: the butterfly v += v[lane ^ m] for m = 32, 16, 8, 4, 2, 1, with the
+// partner values moved by VALU lane operations rather than LDS permutes
+// (the sums sit on every solver launch's dependent chain): the lane swaps
+// for 32 and 16, a rotation by 8 inside each 16-lane row, the mirror inside
+// each 8 lanes followed by the reversal inside each 4 for 4, and quad
+// permutations for 2 and 1.
 __device__ inline double wave_sum(double v)
 {
-    for (int off = 32; off > 0; off >>= 1)
-        v += __shfl_xor(v, off, 64);
+    v += swap_lane<32>(v);
+    v += swap_lane<16>(v);
+    v += dpp_lane<0x128>(v);                 // row_ror:8
+    v += dpp_lane<0x1B>(dpp_lane<0x141>(v)); // row_half_mirror, then quad_perm [3,2,1,0]
+    v += dpp_lane<0x4E>(v);                  // quad_perm [2,3,0,1]
+    v += dpp_lane<0xB1>(v);                  // quad_perm [1,0,3,2]
     return v;
 }
 
