@@ -193,6 +193,51 @@ __device__ inline void solver_sums(const DeviceBatch &d, int4 si, int slot, cons
                          out);
 }
 
+// solver_sums in two phases: sums_issue puts the loads in flight (cloths of
+// up to 64 * E slices or groups: one round, issued before the kernel waits
+// on the cloth's state, so that the two round trips overlap), sums_finish
+// adds them, or sums the larger cloths' sets the usual way.  E costs 2 N E
+// registers while the loads are in flight: the variants for launches of up
+// to 2,048 slices take 17 (1,088 sets), the SpMV variants for whole batches,
+// whose occupancy matters, none, and the update for whole batches, which
+// needs few registers, 5.
+template <int N, int E> struct EarlySums
+{
+    double a[E > 0 ? E : 1][N];
+    int nb;
+    bool one;
+};
+
+template <int N, int E>
+__device__ inline void sums_issue(const DeviceBatch &d, int4 si, int slot, const int (&comp)[N],
+                                  EarlySums<N, E> &e)
+{
+    const bool grp = si.z < 0;
+    e.nb = grp ? -si.z : si.z;
+    e.one = E > 0 && e.nb <= 64 * E;
+    if constexpr (E > 0)
+        if (e.one)
+        {
+            const size_t n = grp ? (size_t)d.n_groups : (size_t)d.n_slices; // sets per slot
+            const G<double> *sets =
+                (grp ? glob(d.cg_group) : glob(d.cg_partial)) + (size_t)slot * PARTS * n;
+            sets_load<N, E>(sets, si.y, e.nb, comp, e.a);
+        }
+}
+
+template <int N, int E>
+__device__ inline void sums_finish(const DeviceBatch &d, int4 si, int slot, const int (&comp)[N],
+                                   const EarlySums<N, E> &e, double (&out)[N])
+{
+    if constexpr (E > 0)
+        if (e.one)
+        {
+            sets_add<N, E>(e.a, e.nb, out);
+            return;
+        }
+    solver_sums<N>(d, si, slot, comp, out);
+}
+
 // Loads of data that one launch reads once (matrix planes, column indices),
 // with the nontemporal hint when NT: a matrix larger than the Infinity Cache
 // then streams past the caches instead of evicting the direction vectors
@@ -511,6 +556,20 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
     const int w = glob(d.row_w)[p];
     const int inst = si.x;
     const size_t base = (size_t)si.w;
+    // the variants for launches of up to 2,048 slices (a cloth solved
+    // alone: latency-bound) also load the row's own z and d_prev (from both
+    // direction buffers: the state that picks one is not loaded yet) now
+    // rather than after the product, and the solver sums with the state
+    constexpr bool SMALL = !S && UG * L == 8;
+    T zo[3], dpo[2][3];
+    if constexpr (SMALL)
+        if (j == 0)
+            for (int a = 0; a < 3; a++)
+            {
+                zo[a] = z[p3 + a];
+                dpo[0][a] = dbuf0[p3 + a];
+                dpo[1][a] = dbuf1[p3 + a];
+            }
     constexpr int U = S ? 4 / L : UG;
     T m0[U][9];
     int c0[U];
@@ -526,16 +585,19 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
     }
     else
         gathered_columns<L, U, NT>(glob(d.col) + base + row, j, w, c0);
+    // the solver sums of the previous update (not used by the first launch
+    // of a solve, whose partials are stale but allocated), in flight with
+    // the state (b.b only after a true residual, below)
+    const int sum_comp[2] = {P_RZ, P_RR};
+    constexpr int E = SMALL ? 17 : 0;
+    EarlySums<2, E> es;
+    sums_issue<2, E>(d, si, 0, sum_comp, es);
     const InstParams ip = glob(d.params)[inst];
     const bool lead = sl == ip.slice_begin && threadIdx.x == 0;
     CgState st = load_state(glob(d.cg) + 2 * inst);
     G<CgState> *st_out = glob(d.cg) + 2 * inst + 1;
-    // the solver sums of the previous update (not used by the first launch
-    // of a solve, whose partials are stale but allocated)
-    // (b.b only after a true residual, below)
     double sums[2];
-    const int sum_comp[2] = {P_RZ, P_RR};
-    solver_sums<2>(d, si, 0, sum_comp, sums);
+    sums_finish<2, E>(d, si, 0, sum_comp, es, sums);
     if (st.done)
     {
         if (lead)
@@ -717,7 +779,10 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
     {
         T dn[3];
         for (int a = 0; a < 3; a++)
-            dn[a] = dir_value<T>(z[p3 + a], dold[p3 + a], bt, first);
+            if constexpr (SMALL)
+                dn[a] = dir_value<T>(zo[a], (st.iter & 1) ? dpo[0][a] : dpo[1][a], bt, first);
+            else
+                dn[a] = dir_value<T>(z[p3 + a], dold[p3 + a], bt, first);
         dnew[p3] = dn[0];
         dnew[p3 + 1] = dn[1];
         dnew[p3 + 2] = dn[2];
@@ -779,6 +844,12 @@ __device__ inline void update_body(const DeviceBatch &d, const int blk,
     const int lane = threadIdx.x & 63;
     const int sl = blk * SB + (threadIdx.x >> 6);
     const size_t p3 = 3 * (size_t)(sl * SLICE + lane);
+    // d.Wd of the preceding k_cg_spmv, in flight with the state (unused when
+    // the cloth is done or that launch formed a true residual)
+    const int dq_comp[1] = {P_DQ};
+    constexpr int E = PF ? 17 : 5;
+    EarlySums<1, E> es;
+    sums_issue<1, E>(d, si, 1, dq_comp, es);
     UpdateRow<T> row;
     if constexpr (PF)
         load_update_row<T>(d, p3, P, r, q, dbuf0, dbuf1, 2, row);
@@ -789,13 +860,8 @@ __device__ inline void update_body(const DeviceBatch &d, const int blk,
     const G<double> *part_in = glob(d.cg_partial) + (size_t)PARTS * d.n_slices;
     G<double> *part_out = glob(d.cg_partial);
     const size_t part0 = (size_t)PARTS * SB * blk; // this workgroup's slices
-    // d.Wd of the preceding k_cg_spmv (loaded with the state; unused when
-    // the cloth is done or that launch formed a true residual)
     double v1[1];
-    {
-        const int comp[1] = {P_DQ};
-        solver_sums<1>(d, si, 1, comp, v1);
-    }
+    sums_finish<1, E>(d, si, 1, dq_comp, es, v1);
     if (st.done)
     {
         if (lead)

@@ -157,6 +157,47 @@ __device__ inline void wave_sets_sum(PP sets, int b0, int nb, const int (&comp)[
         out[j] = wave_sum(out[j]);
 }
 
+// wave_sets_sum's one-round form for up to 64 * E sets (E <= 17) in two
+// phases, so that a kernel can put the loads in flight before it waits on
+// other loads (the cloth's state): sets_load issues them all (lane l: sets
+// l, l + 64, ...; lanes past the end read the last set again, so no load
+// sits behind a per-lane branch), sets_add adds them in wave_sets_sum's
+// order, with the same result bit for bit.
+template <int N, int E, typename PP>
+__device__ inline void sets_load(PP sets, int b0, int nb, const int (&comp)[N], double (&a)[E][N])
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int u = 0; u < E; u++)
+        if (64 * u < nb)
+        {
+            const int s = b0 + min(lane + 64 * u, nb - 1);
+            for (int j = 0; j < N; j++)
+                a[u][j] = sets[PARTS * (size_t)s + comp[j]];
+        }
+}
+
+template <int N, int E>
+__device__ inline void sets_add(const double (&a)[E][N], int nb, double (&out)[N])
+{
+    const int lane = threadIdx.x & 63;
+    for (int j = 0; j < N; j++)
+        out[j] = 0.0;
+#pragma unroll
+    for (int u = 0; u < E; u++)
+        if (64 * u < nb)
+        {
+            const bool in = lane + 64 * u < nb;
+            for (int j = 0; j < N; j++)
+            {
+                const double t = out[j] + a[u][j];
+                out[j] = in ? t : out[j];
+            }
+        }
+    for (int j = 0; j < N; j++)
+        out[j] = wave_sum(out[j]);
+}
+
 // sum the cloth's partials (workgroups b0 .. b0+nb-1), result in every thread
 template <int SB, int N, typename PP>
 __device__ inline void reduce_cloth(PP part, int b0, int nb, const int (&comp)[N],
