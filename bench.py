@@ -90,8 +90,8 @@ def kernel_name(lanes, staged, nt, slices):
 TRAFFIC = {
     "batch64": {(1, False, False): (211446746.0, "profiles/k_cg_spmv_batch64.md"),
                 (2, False, False): (208552256.0, "profiles/k_cg_spmv_batch64.md")},
-    "single257": {(1, False, False): (33176912.0, "profiles/k_cg_spmv_single257.md"),
-                  (2, False, False): (33178100.0, "profiles/k_cg_spmv_single257.md")},
+    "single257": {(1, False, False): (33980746.0, "profiles/k_cg_spmv_single257.md"),
+                  (2, False, False): (33970307.0, "profiles/k_cg_spmv_single257.md")},
 }
 
 
