@@ -260,12 +260,13 @@ template <bool NT, typename V> __device__ inline V stream_load(const G<V> *p)
 // current one, so each chunk waits on one memory round trip (gathers and
 // blocks together) rather than two (indices, then gathers), and the
 // indices' loads, issued last, stay in flight across the chunk's arithmetic.
-template <typename TW, typename T, int L, bool NT>
+template <typename TW, typename T, int L, bool NT, int UW = 1>
 __device__ inline void slice_product(const G<TW> *__restrict__ Wl, const G<int> *__restrict__ cl,
                                      int w, int j, const G<T> *__restrict__ z,
                                      const G<T> *__restrict__ dold, T bt, bool first, TW (&y)[3])
 {
-    constexpr int U = sizeof(TW) == sizeof(T) ? 4 / L : 1; // slots per lane and chunk
+    // slots per lane and chunk: UW for the FP64 matrix under FP32 vectors
+    constexpr int U = sizeof(TW) == sizeof(T) ? 4 / L : UW;
     constexpr int STEP = U * L;
     y[0] = y[1] = y[2] = 0;
     int c[U];
@@ -755,7 +756,12 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
     else if (wide)
     {
         double y[3];
-        slice_product<double, T, L, NT>(glob(d.W64) + 9 * base + row, cl, w, j, z, dold, bt, first, y);
+        // the FP64 matrix in chunks of 4 / L slots per lane in the variants
+        // for launches of up to 2,048 slices (a stiff cloth solved alone
+        // switches to it: each chunk is a round trip), else 1 to keep the
+        // registers of the variants for whole batches
+        slice_product<double, T, L, NT, SMALL ? 4 / L : 1>(glob(d.W64) + 9 * base + row, cl, w, j,
+                                                            z, dold, bt, first, y);
         y0 = (T)y[0];
         y1 = (T)y[1];
         y2 = (T)y[2];
