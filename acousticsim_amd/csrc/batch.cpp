@@ -442,6 +442,7 @@ void Batch::build_layout()
     }
     d_.slice_info = dalloc<int4>(slices);
     upload(d_.slice_info, slice_info);
+    slice_info_ = slice_info;
 
     d_.slice_off = dalloc<int>(slices);
     d_.slice_w = dalloc<int>(slices);
@@ -455,7 +456,7 @@ void Batch::build_layout()
     d_.row_w = dalloc<int>(d_.n_rows);
     d_.row_pinned = dalloc<uint8_t>(d_.n_rows);
     d_.row_mass = dalloc<double>(d_.n_rows);
-    d_.col = dalloc<int>(entries);
+    d_.col = dalloc<int>(entries + COL_PAD);
     // the staged SpMV's lists: the distinct columns of each slice, ascending,
     // and each entry's position in its slice's list
     std::vector<int> halo_off(slices + 1, 0), halo_row;
@@ -619,11 +620,12 @@ void Batch::build_layout()
             d_.d64[1] = dalloc<double>(nv3);
         }
         d_.cg = opt_.cg_states ? opt_.cg_states : dalloc<CgState>(2 * (size_t)n);
-        d_.spmv_list = dalloc<int>(slices);
-        d_.upd_list = dalloc<int>(d_.n_blocks_wg);
-        check(hipHostMalloc((void **)&host_spmv_list_, sizeof(int) * (size_t)slices),
+        d_.spmv_list = dalloc<SolverEntry>(slices);
+        d_.upd_list = dalloc<SolverEntry>(d_.n_blocks_wg);
+        check(hipHostMalloc((void **)&host_spmv_list_, sizeof(SolverEntry) * (size_t)slices),
               "hipHostMalloc");
-        check(hipHostMalloc((void **)&host_upd_list_, sizeof(int) * (size_t)d_.n_blocks_wg),
+        check(hipHostMalloc((void **)&host_upd_list_,
+                            sizeof(SolverEntry) * (size_t)d_.n_blocks_wg),
               "hipHostMalloc");
         d_.cg_partial = dalloc<double>(2 * PARTS * (size_t)d_.n_slices);
         d_.cg_group = dalloc<double>(2 * PARTS * (size_t)d_.n_groups);
@@ -650,6 +652,7 @@ void Batch::build_layout()
     upload(d_.row_pinned, row_pinned);
     upload(d_.row_mass, row_mass);
     upload(d_.col, col);
+    check(hipMemset(d_.col + entries, 0, COL_PAD * sizeof(int)), "hipMemset");
     upload(d_.vert_row, vert_row);
     upload(d_.blk_ptr, blk_ptr);
     upload(d_.blk_col, blk_col);
@@ -1150,21 +1153,28 @@ int Batch::upload_lists(Lane &L)
         if (active[i])
         {
             for (int s = 0; s < nslices_[i]; s++)
-                L.host_spmv[n++] = slice_base_[i] + s;
+            {
+                const int sl = slice_base_[i] + s;
+                L.host_spmv[n++] = {slice_info_[sl], sl, 0, 0, 0};
+            }
             for (int b = 0; b < nslices_[i] / BW_SB; b++)
-                L.host_upd[nb++] = slice_base_[i] / BW_SB + b;
+            {
+                const int blk = slice_base_[i] / BW_SB + b;
+                L.host_upd[nb++] = {slice_info_[(size_t)blk * BW_SB], blk, 0, 0, 0};
+            }
         }
     const int grid = slice_bucket(n, d_.n_slices);
     const int ugrid = (grid + BW_SB - 1) / BW_SB;
+    const SolverEntry pad = {make_int4(-1, -1, INT_MIN, -1), -1, 0, 0, 0};
     for (int k = n; k < grid; k++)
-        L.host_spmv[k] = -1;
+        L.host_spmv[k] = pad;
     for (int k = nb; k < ugrid; k++)
-        L.host_upd[k] = -1;
-    check(hipMemcpyAsync(L.spmv_list, L.host_spmv, grid * sizeof(int), hipMemcpyHostToDevice,
-                         L.stream),
+        L.host_upd[k] = pad;
+    check(hipMemcpyAsync(L.spmv_list, L.host_spmv, grid * sizeof(SolverEntry),
+                         hipMemcpyHostToDevice, L.stream),
           "hipMemcpyAsync list");
-    check(hipMemcpyAsync(L.upd_list, L.host_upd, ugrid * sizeof(int), hipMemcpyHostToDevice,
-                         L.stream),
+    check(hipMemcpyAsync(L.upd_list, L.host_upd, ugrid * sizeof(SolverEntry),
+                         hipMemcpyHostToDevice, L.stream),
           "hipMemcpyAsync list");
     return grid;
 }
@@ -1242,10 +1252,12 @@ void Batch::open_lane(Lane &L)
     check(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking), "hipStreamCreate");
     check(hipEventCreateWithFlags(&L.ev, hipEventDisableTiming), "hipEventCreate");
     L.owned = true;
-    L.spmv_list = dalloc<int>(d_.n_slices);
-    L.upd_list = dalloc<int>(d_.n_blocks_wg);
-    check(hipHostMalloc((void **)&L.host_spmv, sizeof(int) * (size_t)d_.n_slices), "hipHostMalloc");
-    check(hipHostMalloc((void **)&L.host_upd, sizeof(int) * (size_t)d_.n_blocks_wg), "hipHostMalloc");
+    L.spmv_list = dalloc<SolverEntry>(d_.n_slices);
+    L.upd_list = dalloc<SolverEntry>(d_.n_blocks_wg);
+    check(hipHostMalloc((void **)&L.host_spmv, sizeof(SolverEntry) * (size_t)d_.n_slices),
+          "hipHostMalloc");
+    check(hipHostMalloc((void **)&L.host_upd, sizeof(SolverEntry) * (size_t)d_.n_blocks_wg),
+          "hipHostMalloc");
     check(hipHostMalloc((void **)&L.host_cg, 2 * sizeof(CgState) * models_.size()),
           "hipHostMalloc");
 }

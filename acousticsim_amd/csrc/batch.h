@@ -183,8 +183,8 @@ private:
         hipEvent_t ev = nullptr; // after the lane's last queued state copy
         bool owned = false;      // stream, event and host buffers are the lane's
         int lds_reserve = 0;     // dynamic LDS bytes per solver workgroup
-        int *spmv_list = nullptr, *upd_list = nullptr; // device
-        int *host_spmv = nullptr, *host_upd = nullptr; // pinned
+        SolverEntry *spmv_list = nullptr, *upd_list = nullptr; // device
+        SolverEntry *host_spmv = nullptr, *host_upd = nullptr; // pinned
         CgState *host_cg = nullptr;                    // pinned, 2 per cloth
         // first slice of the contiguous range the next launches cover, or
         // -1 when they run from the slice lists
@@ -256,7 +256,8 @@ private:
     bool graphs_disabled_ = false; // BW_GRAPHS=0 in the environment
     int last_launches_ = 0;
     std::vector<long long> pred_launches_;      // per cloth, from the last solve
-    int *host_spmv_list_ = nullptr, *host_upd_list_ = nullptr; // pinned (lane 0)
+    SolverEntry *host_spmv_list_ = nullptr, *host_upd_list_ = nullptr; // pinned (lane 0)
+    std::vector<int4> slice_info_; // host copy of d_.slice_info (list entries carry it)
     unsigned long long graph_clock_ = 0;
     std::vector<void *> allocations_;
     template <typename T> T *dalloc(size_t n);

@@ -58,6 +58,7 @@
 // true residual: ||b - W64 dx|| <= tol ||b||.  "fp64" runs the same machine
 // with T = double.
 #include <algorithm>
+#include <climits>
 #include <cstdlib>
 
 #include "device.h"
@@ -319,16 +320,24 @@ __device__ inline void slice_product(const G<TW> *__restrict__ Wl, const G<int> 
 
 // column indices of the gathered product's first chunk (slots j, j + L, ...,
 // U of them), loaded before the prologue: the chain column -> gather is the
-// product's critical path, the blocks go out with the gathers
+// product's critical path, the blocks go out with the gathers.  The loads
+// do not wait for the row's width: slots past it read the next rows' (or
+// the padding after the last slice: U L <= 8 slots of COL_PAD), and
+// gathered_mask drops them once the width is known.
 template <int L, int U, bool NT>
-__device__ inline void gathered_columns(const G<int> *__restrict__ cl, int j, int w, int (&c)[U])
+__device__ inline void gathered_columns(const G<int> *__restrict__ cl, int j, int (&c)[U])
 {
 #pragma unroll
     for (int u = 0; u < U; u++)
-    {
-        const int k = j + u * L;
-        c[u] = k < w ? stream_load<NT>(cl + k * SLICE) : -1;
-    }
+        c[u] = stream_load<NT>(cl + (j + u * L) * SLICE);
+}
+
+template <int L, int U> __device__ inline void gathered_mask(int j, int w, int (&c)[U])
+{
+#pragma unroll
+    for (int u = 0; u < U; u++)
+        if (j + u * L >= w)
+            c[u] = -1;
 }
 
 // slice_product with the column indices of its first chunk already loaded,
@@ -535,7 +544,8 @@ __device__ inline void staged_product_from(const DeviceBatch &d, int sl,
 // k_cg_spmv on slice sl of batch d; S: the staged variant (vs: its LDS);
 // UG: slots per lane and chunk of the gathered product
 template <typename T, int L, bool S, bool NT, int UG = 4 / L>
-__device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> *__restrict__ W,
+__device__ inline void spmv_body(const DeviceBatch &d, const int sl, const int4 si,
+                                 const G<T> *__restrict__ W,
                                  const G<T> *__restrict__ P, G<T> *__restrict__ r,
                                  G<T> *__restrict__ z, G<T> *__restrict__ q,
                                  G<T> *__restrict__ dbuf0, G<T> *__restrict__ dbuf1, double *lds,
@@ -553,7 +563,6 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
     // blocks; staged: the positions and the first staging round's rows);
     // (3) the direction gathers.  A cloth solved alone is bound by these
     // round trips, not by bandwidth.
-    const int4 si = glob(d.slice_info)[sl];
     const int w = glob(d.row_w)[p];
     const int inst = si.x;
     const size_t base = (size_t)si.w;
@@ -585,7 +594,7 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
         stage_round_rows<T, L>(glob(d.halo_row) + list.x, threadIdx.x, list.y, sb);
     }
     else
-        gathered_columns<L, U, NT>(glob(d.col) + base + row, j, w, c0);
+        gathered_columns<L, U, NT>(glob(d.col) + base + row, j, c0);
     // the solver sums of the previous update (not used by the first launch
     // of a solve, whose partials are stale but allocated), in flight with
     // the state (b.b only after a true residual, below)
@@ -774,8 +783,11 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const G<T> 
         if constexpr (S)
             slice_product<T, T, L, NT>(W + 9 * base + row, cl, w, j, z, dold, bt, first, y);
         else
+        {
+            gathered_mask<L, U>(j, w, c0);
             slice_product_from<T, L, U, NT>(W + 9 * base + row, cl, w, j, z, dold, bt, first, c0,
                                             y);
+        }
         y0 = y[0];
         y1 = y[1];
         y2 = y[2];
@@ -839,13 +851,13 @@ __device__ inline void load_update_row(const DeviceBatch &d, size_t p3, const G<
 
 // k_cg_update on solver workgroup blk (SB slices) of batch d
 template <typename T, int SB, bool PF>
-__device__ inline void update_body(const DeviceBatch &d, const int blk,
+__device__ inline void update_body(const DeviceBatch &d, const int blk, const int4 si,
                                    const G<T> *__restrict__ P, G<T> *__restrict__ r,
                                    G<T> *__restrict__ z, const G<T> *__restrict__ q,
                                    const G<T> *__restrict__ dbuf0, const G<T> *__restrict__ dbuf1)
 {
-    // the SB slices of a workgroup belong to one cloth
-    const int4 si = glob(d.slice_info)[blk * SB];
+    // the SB slices of a workgroup belong to one cloth: si is its first
+    // slice's slice_info entry
     const int inst = si.x;
     const int lane = threadIdx.x & 63;
     const int sl = blk * SB + (threadIdx.x >> 6);
@@ -953,6 +965,15 @@ __device__ inline void update_body(const DeviceBatch &d, const int blk,
 // sized at launch
 extern __shared__ __align__(16) unsigned char stage_lds[];
 
+// A padding entry of a launch list.  Padding has every field negative (the
+// count INT_MIN); the test reads every field of the entry, so that all of
+// its loads go out before the branch, in one round, rather than the rest
+// of the entry after it.
+__device__ inline bool entry_padding(const SolverEntry &e)
+{
+    return (e.id | e.info.x | e.info.y | e.info.w) < 0 || e.info.z == INT_MIN;
+}
+
 template <typename T, int L, bool S, bool NT, int UG = 4 / L>
 __global__ void __launch_bounds__(L * 64)
 k_cg_spmv(DeviceBatch d, const T *__restrict__ W, const T *__restrict__ P, T *__restrict__ r,
@@ -962,16 +983,22 @@ k_cg_spmv(DeviceBatch d, const T *__restrict__ W, const T *__restrict__ P, T *__
     // one slice per workgroup, from the list of active slices when given,
     // else from the contiguous range at slice_first
     int sl = solver_block(d.xcd_map);
+    int4 si;
     if (d.spmv_list)
     {
-        sl = d.spmv_list[sl];
-        if (sl < 0)
+        const SolverEntry e = d.spmv_list[sl];
+        if (entry_padding(e))
             return;
+        sl = e.id;
+        si = e.info;
     }
     else
+    {
         sl += d.slice_first;
-    spmv_body<T, L, S, NT, UG>(d, sl, glob(W), glob(P), glob(r), glob(z), glob(q), glob(dbuf0),
-                               glob(dbuf1), lds, reinterpret_cast<T *>(stage_lds));
+        si = glob(d.slice_info)[sl];
+    }
+    spmv_body<T, L, S, NT, UG>(d, sl, si, glob(W), glob(P), glob(r), glob(z), glob(q),
+                               glob(dbuf0), glob(dbuf1), lds, reinterpret_cast<T *>(stage_lds));
 }
 
 template <typename T, int SB, bool PF>
@@ -980,15 +1007,22 @@ k_cg_update(DeviceBatch d, const T *__restrict__ P, T *__restrict__ r, T *__rest
             const T *__restrict__ q, const T *__restrict__ dbuf0, const T *__restrict__ dbuf1)
 {
     int blk = solver_block(d.xcd_map);
+    int4 si;
     if (d.upd_list)
     {
-        blk = d.upd_list[blk];
-        if (blk < 0)
+        const SolverEntry e = d.upd_list[blk];
+        if (entry_padding(e))
             return;
+        blk = e.id;
+        si = e.info;
     }
     else
+    {
         blk += d.slice_first / SB;
-    update_body<T, SB, PF>(d, blk, glob(P), glob(r), glob(z), glob(q), glob(dbuf0), glob(dbuf1));
+        si = glob(d.slice_info)[blk * SB];
+    }
+    update_body<T, SB, PF>(d, blk, si, glob(P), glob(r), glob(z), glob(q), glob(dbuf0),
+                           glob(dbuf1));
 }
 
 // group sums of the partials in `slot` (just written by the launch before):
@@ -1030,7 +1064,7 @@ k_cg_spmv_multi(const DeviceBatch *__restrict__ table, const int2 *__restrict__ 
     if (e.x < 0)
         return;
     const DeviceBatch &d = table[e.x];
-    spmv_body<T, L, S, NT>(d, e.y, glob(Vecs<T>::W(d)), glob(Vecs<T>::P(d)), glob(Vecs<T>::r(d)),
+    spmv_body<T, L, S, NT>(d, e.y, glob(d.slice_info)[e.y], glob(Vecs<T>::W(d)), glob(Vecs<T>::P(d)), glob(Vecs<T>::r(d)),
                        glob(Vecs<T>::z(d)), glob(Vecs<T>::q(d)), glob(Vecs<T>::d0(d)),
                        glob(Vecs<T>::d1(d)), lds, reinterpret_cast<T *>(stage_lds));
 }
@@ -1044,7 +1078,7 @@ k_cg_update_multi(const DeviceBatch *__restrict__ table, const int2 *__restrict_
     if (e.x < 0)
         return;
     const DeviceBatch &d = table[e.x];
-    update_body<T, SB, PF>(d, e.y, glob(Vecs<T>::P(d)), glob(Vecs<T>::r(d)), glob(Vecs<T>::z(d)),
+    update_body<T, SB, PF>(d, e.y, glob(d.slice_info)[e.y * SB], glob(Vecs<T>::P(d)), glob(Vecs<T>::r(d)), glob(Vecs<T>::z(d)),
                        glob(Vecs<T>::q(d)), glob(Vecs<T>::d0(d)), glob(Vecs<T>::d1(d)));
 }
 

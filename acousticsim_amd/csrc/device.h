@@ -17,6 +17,10 @@ namespace bw
 {
 
 constexpr int SLICE = 64;
+// column-index entries after the last slice's: the gathered SpMV loads the
+// first chunk of a row's column indices (up to 8 slots) before it knows the
+// row's width, so those loads stay inside the allocation
+constexpr int COL_PAD = 8 * SLICE;
 // longest per-slice column list the staged SpMV keeps in LDS (4 values of
 // the solver's precision per entry: 32 KB in FP32, 64 KB in FP64)
 constexpr int HALO_CAP = 2048;
@@ -109,6 +113,16 @@ struct GfState
     int done;
 };
 
+// An entry of a solver launch's list: the slice (k_cg_spmv) or solver
+// workgroup (k_cg_update) to run, with the slice_info entry of that slice
+// (of the workgroup's first slice), so that the workgroup's first load
+// gives it both; padding has every field negative (info.z INT_MIN).
+struct __align__(16) SolverEntry
+{
+    int4 info;
+    int id, pad0, pad1, pad2;
+};
+
 struct DeviceBatch
 {
     int n_inst = 0, n_slices = 0, n_rows = 0;
@@ -194,7 +208,7 @@ struct DeviceBatch
     // optional lists of the slices (k_cg_spmv) and solver workgroups
     // (k_cg_update) a launch runs; null: the contiguous range of slices from
     // slice_first (a multiple of BW_SB) that the grid covers
-    int *spmv_list = nullptr, *upd_list = nullptr;
+    SolverEntry *spmv_list = nullptr, *upd_list = nullptr;
     int slice_first = 0;
     int lds_pad = 0; // dynamic LDS bytes the solver launches reserve and leave unused
     // per slice, what the solver kernels need before anything else, in one

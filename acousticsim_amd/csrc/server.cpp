@@ -32,7 +32,11 @@ size_t estimate_bytes(const ClothModel &m)
     const size_t nie = m.interior.size();
     size_t b = entries * (4 + 72 + 36);                       // col, W64, W32
     b += rows * (3 * 8 * 5 + 3 * 4 + 3 * 8 * 5 + 8 + 4 * 3); // x..P64, P32, solver vectors, mass, row ints
-    b += slices * (3 * 4 + 2 * PARTS * 8 + 2 * 4);
+    // slice ints, slice_info, partials, launch list entries (one per slice
+    // and one per solver workgroup)
+    b += slices * (3 * 4 + 16 + 2 * PARTS * 8 + sizeof(SolverEntry)) +
+         slices / BW_SB * sizeof(SolverEntry);
+    b += COL_PAD * 4;
     b += (size_t)m.nv * 4 * 4 + (size_t)m.nnzb() * 4 * 3;
     b += ((size_t)9 * m.nt + 16 * nie) * 4 + ((size_t)3 * m.nt + 4 * nie) * 4 * 2;
     b += (size_t)m.nt * (3 * 4 + 4 + 12 * 4 + 4 + 25 * 8) + nie * (4 * 4 + 8 + 25 * 8);
