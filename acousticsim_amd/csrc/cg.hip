@@ -558,11 +558,13 @@ __device__ inline void spmv_body(const DeviceBatch &d, const int sl, const int4 
     const int p = sl * SLICE + row;
     const size_t p3 = 3 * (size_t)p;
     // Loads in dependent rounds, as few as the data allow: (1) the slice's
-    // info and the row's width; (2) the cloth's parameters and state, its
-    // solver sums and the first chunk of the matrix (column indices and
-    // blocks; staged: the positions and the first staging round's rows);
-    // (3) the direction gathers.  A cloth solved alone is bound by these
-    // round trips, not by bandwidth.
+    // info (si: the caller's load, from the launch list's entry or from
+    // slice_info); (2) the row's width, the first chunk's column indices
+    // (staged: the positions and blocks and the first staging round's
+    // rows), the cloth's parameters and state and, in the variants for
+    // launches of up to 2,048 slices, its solver sums and the row's own
+    // vectors; (3) the direction gathers with their blocks.  A cloth solved
+    // alone is bound by these round trips, not by bandwidth.
     const int w = glob(d.row_w)[p];
     const int inst = si.x;
     const size_t base = (size_t)si.w;
