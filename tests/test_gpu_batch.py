@@ -241,8 +241,9 @@ def test_tiny_and_ragged_cloths_in_one_batch():
         assert err <= 1e-4 * dd
 
 
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
 @pytest.mark.parametrize("graphs", [True, False])
-def test_lanes_equal_one_chain(graphs, monkeypatch):
+def test_lanes_equal_one_chain(graphs, precision, monkeypatch):
     """From its second solve on, a batch of 8 or more cloths solves its
     stiffest cloths in lanes of their own (chains of launches on streams of
     their own, concurrently with the rest); every step's positions and
@@ -259,9 +260,9 @@ def test_lanes_equal_one_chain(graphs, monkeypatch):
     cs = []
     for side, scale, seed in members:
         c = wl.grid_config(side, stiffness_scale=scale)
-        c["precision"] = "mixed"
+        c["precision"] = precision
         cs.append((c, wl.jittered_positions(c, seed)))
-    say("case lanes, graphs %s: %d cloths" % (graphs, len(cs)))
+    say("case lanes, graphs %s, %s: %d cloths" % (graphs, precision, len(cs)))
     out = {}
     for lanes in ("1", "0"):
         monkeypatch.setenv("BW_LANES", lanes)
