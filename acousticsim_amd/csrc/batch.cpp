@@ -390,9 +390,17 @@ void Batch::build_layout()
         // residual each time r.r drops by rel2, or after max_since
         // iterations; FP64 recurrences are checked only at convergence.
         // BW_REL2 and BW_MAX_SINCE override them for tuning runs.
+        // The forced check's window grows with the cloth's linear size: CG
+        // needs iterations in proportion to the mesh's resolution to reduce
+        // the residual by a given factor, so a fixed window takes a large
+        // cloth's ordinary slow stretches for stalls.  2000 iterations per
+        // 128 vertices of side (sqrt(V / 16384)), at least 2000: the
+        // batch64 cloths (up to 129 x 129) keep 2000-2016, a 257 x 257
+        // cloth gets 4016.
         const bool mixed = kind_ == SolverKind::CG32;
         P.rel2 = mixed ? 1e-2 : 0.0;
-        P.max_since = mixed ? 2000 : (1 << 30);
+        const double side = std::sqrt(std::max(1.0, (double)m.nv / 16384.0));
+        P.max_since = mixed ? (int)std::min(2000.0 * side, 1e9) : (1 << 30);
         if (const char *e = std::getenv("BW_REL2"))
             P.rel2 = std::atof(e);
         if (const char *e = std::getenv("BW_MAX_SINCE"))
