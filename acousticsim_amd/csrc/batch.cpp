@@ -417,7 +417,9 @@ void Batch::build_layout()
         // count): 1 keeps the FP64 matrix after the first stall, which the
         // stiffest batch members need (batch64 515 cloth-steps/s against 480
         // with 2); a cloth whose solves stall only now and then does better
-        // with 2 (single257 19.6 steps/s against 16.1)
+        // with 2 (single257 under a fixed 2000-iteration window: 19.6
+        // steps/s against 16.1; under the size-scaled window it does not
+        // switch at all)
         P.carry = 1;
         if (const char *e = std::getenv("BW_STALL_CARRY"))
             P.carry = std::max(1, std::atoi(e));
